@@ -1,0 +1,162 @@
+/*
+ * louvain_hip.h -- C ABI of the MI355X Louvain engine (liblouvain_hip.so).
+ *
+ * Plain C: pointers and sizes only, integer status codes, no stdout, no
+ * exit().  One lv_graph per device-resident graph; each graph owns its own
+ * HIP stream and device buffers.  Calls on different graphs may run from
+ * different host threads; calls on one graph must not overlap.
+ *
+ * The entry points replace the reference's operator API and its I/O:
+ *   lv_mtx_read                -> parseGraph               (utils.cu:168-229)
+ *   lv_graph_create            -> initDevGraph + copyToDevice + initCommAggr
+ *                                 + setInitComm            (utils.cu:70-98, :66-68)
+ *   lv_modularity_optimization -> modularityOptimization   (modularity.cuh:6,
+ *                                                           modularity.cu:438-520)
+ *   lv_aggregation             -> aggregation              (aggregation.cuh:6,
+ *                                                           aggregation.cu:387-462)
+ *   lv_run                     -> the pass loop of main    (main.cu:41-52)
+ *   lv_get_assignment          -> the init_comm download of printCommunities
+ *                                 (utils.cu:270-275)
+ *   lv_graph_destroy           -> deleteDevGraph + deleteAggrData (utils.cu:26-51)
+ * The finer-grained lv_pass_init / lv_step / lv_sweep / lv_modularity expose the
+ * pieces of modularityOptimization (init :455-463, one bucket step :468-499,
+ * one sweep :467-500, computeModularity :428-436) for step-by-step parity
+ * checks.
+ */
+#ifndef LOUVAIN_HIP_H
+#define LOUVAIN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+    LV_OK = 0,
+    LV_ERR_IO = 1,         /* file cannot be opened (reference: "Error opening file") */
+    LV_ERR_FORMAT = 2,     /* malformed .mtx */
+    LV_ERR_RANGE = 3,      /* weights not representable in the exact fixed-point range */
+    LV_ERR_ARG = 4,        /* bad argument (null pointer, id out of range, size limits) */
+    LV_ERR_NOMEM = 5,      /* host or device allocation failed */
+    LV_ERR_STATE = 6,      /* call out of order (e.g. lv_step before lv_pass_init) */
+    LV_ERR_HIP = 7,        /* HIP runtime error */
+    LV_ERR_NO_DEVICE = 8,  /* no usable GPU */
+    LV_ERR_OVERFLOW = 9    /* internal capacity exceeded (reported, never silent) */
+};
+
+typedef struct lv_graph lv_graph;
+
+/* Host CSR as produced by lv_mtx_read: symmetric rows in file order, each
+ * self-loop stored once, |weight|.  row has n+1 entries. */
+typedef struct lv_host_csr {
+    uint32_t n;
+    uint64_t nnz;  /* edge lines in the file */
+    uint64_t ne;   /* CSR entries = row[n] */
+    uint32_t *row;
+    uint32_t *col;
+    float *w;
+} lv_host_csr;
+
+typedef struct lv_result {
+    float q_final;           /* the modularity the reference prints (modularity.cu:504) */
+    uint32_t passes;         /* passes that improved and were aggregated */
+    uint32_t n_final;        /* vertices of the final graph (= community count) */
+    uint32_t total_iters;    /* sweeps over all passes */
+    uint64_t sum_edges;      /* sum over passes of the pass graph's CSR entries */
+    uint64_t sum_edge_visits;/* sum over passes of (iters+1) * entries */
+    double alg_ms;           /* device time of the whole run (hipEvents) */
+    double bytes_alg;        /* algorithmic HBM bytes of the run (see DESIGN) */
+    int status;
+} lv_result;
+
+/* Per-kernel-family device time, collected with hipEvents on the graph's
+ * stream while profiling is enabled. */
+enum {
+    LV_K_CLASSIFY = 0,
+    LV_K_R1 = 1,          /* k_v / s_v pinned-order sums */
+    LV_K_SUMSQ = 2,       /* sum of a_c^2 */
+    LV_K_MOVE_SMALL = 3,  /* buckets 1-5 */
+    LV_K_MOVE_WAVE = 4,   /* bucket 6 */
+    LV_K_MOVE_BLOCK = 5,  /* bucket 7, LDS hash */
+    LV_K_MOVE_GLOBAL = 6, /* bucket 7, global hash */
+    LV_K_APPLY = 7,
+    LV_K_AGGREGATE = 8,
+    LV_K_COUNT = 9
+};
+
+typedef struct lv_kernel_stats {
+    double ms[LV_K_COUNT];       /* summed launch durations */
+    uint64_t launches[LV_K_COUNT];
+    double bytes[LV_K_COUNT];    /* algorithmic bytes attributed to the family */
+} lv_kernel_stats;
+
+const char *lv_status_string(int status);
+int lv_device_count(int *count);
+
+/* ---- input ---------------------------------------------------------------- */
+int lv_mtx_read(const char *path, lv_host_csr *out);
+void lv_host_csr_free(lv_host_csr *csr);
+
+/* ---- graph lifetime -------------------------------------------------------- */
+/* Uploads the CSR (row: n+1 offsets, col/w: row[n] entries) to `device`.
+ * Limits: n < 2^31, row[n] < 2^32.  Weights must be finite and >= 0. */
+int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t *col,
+                    const float *w, lv_graph **out);
+int lv_graph_destroy(lv_graph *g);
+
+/* ---- the reference operator API ------------------------------------------ */
+/* One Louvain pass of local moves.  *improved is the reference's return value
+ * (false when the first sweep gains less than min_gain); when false,
+ * *q_printed is the modularity the reference prints (old_modularity). */
+int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_printed);
+/* Contract communities into the next pass's graph; composes the assignment. */
+int lv_aggregation(lv_graph *g);
+/* The full pass loop: modularity optimisation then aggregation until a pass
+ * fails its first sweep. */
+int lv_run(lv_graph *g, float min_gain, lv_result *res);
+
+/* ---- results and state --------------------------------------------------- */
+/* original vertex -> vertex of the current graph (length n0) */
+int lv_get_assignment(lv_graph *g, uint32_t *assignment);
+int lv_graph_size(lv_graph *g, uint32_t *n, uint64_t *ne, uint32_t *n0);
+int lv_get_csr(lv_graph *g, uint32_t *row, uint32_t *col, float *w);
+/* k (f32), comm, comm_sz, a (exact fixed point a_c * 2^S); any may be NULL */
+int lv_get_state(lv_graph *g, float *k, uint32_t *comm, uint32_t *comm_sz, int64_t *a);
+int lv_get_scale(lv_graph *g, int *scale_bits, float *m);
+
+/* ---- stepwise pieces of modularityOptimization (parity hooks) ------------- */
+int lv_pass_init(lv_graph *g, float *q0);
+int lv_step(lv_graph *g, int bucket, uint64_t *moved);
+int lv_sweep(lv_graph *g, float *q, uint64_t *moved);
+int lv_modularity(lv_graph *g, float *q);
+
+/* ---- profiling ------------------------------------------------------------ */
+int lv_profile_enable(lv_graph *g, int on);
+int lv_profile_get(lv_graph *g, lv_kernel_stats *out); /* synchronises */
+int lv_profile_reset(lv_graph *g);
+
+/* ---- synthetic inputs (seeded, deterministic) ----------------------------- */
+/* FEM-band analogue of bcsstk16: nodes of 6 degrees of freedom, dense 6x6
+ * blocks, node bandwidth `band`, ~`couple` coupled nodes per node,
+ * w = |N(0,1)| + 0.01, self-loops kept. */
+int lv_gen_fem(uint32_t nodes, uint32_t band, uint32_t couple, uint64_t seed, lv_host_csr *out);
+/* Chung-Lu power-law degrees over a planted partition (unit weights). */
+int lv_gen_chung_lu(uint32_t n, double mean_deg, double exponent, uint32_t max_deg,
+                    uint32_t groups, double p_ratio, uint64_t seed, lv_host_csr *out);
+/* R-MAT (Graph500 style): 2^scale vertices, edge_factor*2^scale edges,
+ * symmetrised, deduplicated, vertex ids permuted, w ~ U(wlo, whi) (wlo==whi:
+ * unit), self-loops kept once. */
+int lv_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                float wlo, float whi, uint64_t seed, lv_host_csr *out);
+/* Stochastic block model: n vertices in `blocks` blocks, expected degree
+ * mean_deg, p_in/p_out = p_ratio, unit weights. */
+int lv_gen_sbm(uint32_t n, uint32_t blocks, double mean_deg, double p_ratio, uint64_t seed,
+               lv_host_csr *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

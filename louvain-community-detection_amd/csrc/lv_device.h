@@ -1,0 +1,139 @@
+// lv_device.h -- device-side constants, state views and launcher declarations
+// shared by the kernel translation units and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lv {
+
+// Reference bucket cut points (modularity.cu:439): bucket b holds
+// cut[b-1] < deg <= cut[b].  These are semantic (they define the synchronous
+// step sets) and are never tuned.
+constexpr uint32_t kCut[8] = {0u, 4u, 8u, 16u, 32u, 84u, 319u, 0x7fffffffu};
+// Virtual lanes per vertex = blockDim.x of the reference kernel serving each
+// bucket (modularity.cu:443-450).  Semantic: they fix each lane's visiting
+// order (per-lane first maximum, bucket 5 round prefixes, last-edge rule).
+constexpr uint32_t kLanes[7] = {4u, 8u, 16u, 32u, 32u, 128u, 128u};
+
+// Kernel classes of the move step (performance choice, not semantic).
+//   0: deg 0 (never moves)      1-4: buckets 1-4, sub-wave groups of 4/8/16/32
+//   5: bucket 5, 32-lane groups 6: bucket 6, one wave64 per vertex
+//   7: bucket 7, deg <= kBlockMax, one workgroup + LDS hash per vertex
+//   8: bucket 7, deg >  kBlockMax, one workgroup + global hash per vertex
+constexpr int kNumClasses = 9;
+constexpr uint32_t kBlockMax = 2048;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+__host__ __device__ inline int vertex_class(uint32_t d) {
+    if (d == 0) return 0;
+    if (d <= 4) return 1;
+    if (d <= 8) return 2;
+    if (d <= 16) return 3;
+    if (d <= 32) return 4;
+    if (d <= 84) return 5;
+    if (d <= 319) return 6;
+    if (d <= kBlockMax) return 7;
+    return 8;
+}
+
+// Aggregation community classes (by total member degree).
+//   0: LDS merge, one wave per community, 256-slot table   (comm_deg <= 128)
+//   1: LDS merge, one workgroup, 4096-slot table           (comm_deg <= 2048)
+//   2: global-hash merge + workgroup radix sort            (comm_deg >  2048)
+constexpr int kNumAggClasses = 3;
+constexpr uint32_t kAggSmallMax = 128;
+constexpr uint32_t kAggMidMax = 2048;
+
+// View of one graph's device state, passed by value to kernels.
+struct DevView {
+    const uint32_t *row;    // n+1
+    const uint32_t *col;    // ne
+    const float *w;         // ne
+    uint32_t *comm;         // n
+    uint32_t *comm_sz;      // n
+    uint32_t *new_comm;     // n
+    unsigned long long *A;  // n: a_c * 2^S (exact)
+    float *k;               // n
+    uint32_t n;
+    int S;          // fixed-point scale bits
+    float scale;    // 2^S
+    float m;        // all_w
+};
+
+// Global-hash scratch for the heavy move class.
+struct HeavyScratch {
+    uint32_t *key;               // arena slots
+    unsigned long long *val;     // arena slots
+    const unsigned long long *slot_off;  // per heavy vertex (index in the class list)
+    uint32_t *dlist;             // ne: distinct-slot lists at [row[v], row[v+1])
+    uint32_t *eslot;             // ne: slot of each entry
+    uint32_t *err;               // overflow flag
+};
+
+// ---- launchers (lv_kernels.hip) ---------------------------------------------
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n);
+void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
+
+// degree classification of the current graph into `order`
+void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n,
+                           uint32_t *counts /*kNumClasses*/,
+                           unsigned long long *degsum /*kNumClasses*/);
+void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n,
+                             uint32_t *cursor /*kNumClasses*/, uint32_t *order);
+
+// pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v.
+// `acc` receives the exact fixed-point total (atomic).
+void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
+               int group, int masked, int init_state, unsigned long long *acc);
+void launch_sumsq(hipStream_t s, const DevView &g, unsigned long long *limbs /*4*/);
+
+void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
+                       uint32_t count, const HeavyScratch &hs);
+void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
+                  unsigned long long *moved);
+void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts,
+                       uint32_t count, unsigned long long *caps);
+
+// exclusive scan with total at out[n]; input functor chosen by `kind`:
+//   0: in[i]   1: in[i] != 0
+void launch_scan_u32(hipStream_t s, const uint32_t *in, int kind, uint32_t n, uint32_t *out,
+                     uint32_t *tmp /* >= n/4096 + 2 */);
+void launch_scan_u64(hipStream_t s, const unsigned long long *in, uint32_t n,
+                     unsigned long long *out, unsigned long long *tmp);
+size_t scan_tmp_elems(uint32_t n);
+
+// aggregation
+struct AggView {
+    uint32_t *new_id;       // n+1 (exclusive scan of comm_sz != 0)
+    uint32_t *comm_deg;     // n
+    uint32_t *epos;         // n+1 (exclusive scan of comm_deg)
+    uint32_t *mstart;       // n+1 (exclusive scan of comm_sz)
+    uint32_t *mcursor;      // n
+    uint32_t *members;      // n
+    uint32_t *cnt;          // n'+1 distinct counts by new id, scanned into new row
+    uint32_t *tkey;         // ne
+    unsigned long long *tval;   // ne
+    uint32_t *skey;         // ne (radix scratch)
+    unsigned long long *sval;   // ne
+    uint32_t *gkey;         // global-hash arena
+    unsigned long long *gval;
+    const unsigned long long *goff;  // per class-2 community
+    uint32_t *err;
+};
+void launch_agg_prep(hipStream_t s, const DevView &g, AggView a);
+void launch_agg_members(hipStream_t s, const DevView &g, AggView a);
+void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts);
+void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uint32_t *cursor,
+                                 uint32_t *order);
+void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+                     unsigned long long *caps);
+void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
+                      uint32_t count, int radix_passes);
+void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
+                     uint32_t count, uint32_t *row2, uint32_t *col2, float *w2);
+void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
+                        uint32_t *init_comm, uint32_t n0);
+
+}  // namespace lv

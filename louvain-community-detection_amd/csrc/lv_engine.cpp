@@ -1,0 +1,881 @@
+// lv_engine.cpp -- host engine behind the C ABI (include/louvain_hip.h).
+//
+// One lv_graph owns a HIP stream (default priority) and all device buffers
+// of one graph, sized once at creation for the original graph.  The pass
+// driver mirrors modularityOptimization (modularity.cu:438-520) and
+// aggregation (aggregation.cu:387-462); the per-iteration host round trip is
+// one small D2H of exact counters (S_in, sum a_c^2 limbs, moved count), from
+// which the FP32 modularity is formed exactly as the reference writes it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/louvain_hip.h"
+#include "lv_device.h"
+
+using lv::kNumAggClasses;
+using lv::kNumClasses;
+typedef unsigned long long ull;
+typedef unsigned __int128 u128;
+
+#define LV_HIP(expr)                                   \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) {                        \
+            (void)hipGetLastError();                   \
+            return LV_ERR_HIP;                         \
+        }                                              \
+    } while (0)
+
+namespace {
+
+constexpr uint64_t kFixedLimit = 1ull << 61;
+constexpr uint32_t kMaxIters = 100000;
+
+// device scratch counters, one allocation
+struct Counters {
+    uint32_t counts[kNumClasses];
+    uint32_t cursor[kNumClasses];
+    uint32_t acounts[kNumAggClasses];
+    uint32_t acursor[kNumAggClasses];
+    uint32_t err;
+    uint32_t pad;
+    ull degsum[kNumClasses];
+    ull acc;       // k/s fixed total
+    ull limbs[4];  // sum a^2 limbs
+    ull moved;
+};
+
+struct ProfRec {
+    int fam;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+}  // namespace
+
+struct lv_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0, n0 = 0;
+    uint64_t ne = 0, ne0 = 0;
+    // current and alternate CSR
+    uint32_t *row = nullptr, *col = nullptr;
+    float *w = nullptr;
+    uint32_t *row2 = nullptr, *col2 = nullptr;
+    float *w2 = nullptr;
+    // pass state
+    float *k = nullptr;
+    uint32_t *comm = nullptr, *comm_sz = nullptr, *new_comm = nullptr;
+    ull *A = nullptr;
+    uint32_t *init_comm = nullptr;
+    uint32_t *order = nullptr;
+    // classification (host copies)
+    uint32_t cls_count[kNumClasses] = {};
+    uint32_t cls_off[kNumClasses + 1] = {};
+    ull cls_deg[kNumClasses] = {};
+    // heavy move scratch
+    ull *heavy_caps = nullptr, *heavy_off = nullptr;
+    uint32_t *hkey = nullptr;
+    ull *hval = nullptr;
+    size_t harena = 0;
+    uint32_t *dlist = nullptr, *eslot = nullptr;
+    // aggregation scratch
+    uint32_t *new_id = nullptr, *comm_deg = nullptr, *epos = nullptr, *mstart = nullptr;
+    uint32_t *mcursor = nullptr, *members = nullptr, *cnt = nullptr, *aorder = nullptr;
+    uint32_t *tkey = nullptr, *skey = nullptr;
+    ull *tval = nullptr, *sval = nullptr;
+    ull *agg_caps = nullptr, *agg_off = nullptr;
+    uint32_t *gkey = nullptr;
+    ull *gval = nullptr;
+    size_t garena = 0;
+    uint32_t *scan_tmp32 = nullptr;
+    ull *scan_tmp64 = nullptr;
+    Counters *dc = nullptr;  // device
+    Counters *hc = nullptr;  // pinned host
+    // scalars
+    int S = 0;
+    float scale = 1.0f, inv_scale = 1.0f;
+    float m = 0.0f;
+    bool pass_ready = false;
+    uint32_t last_iters = 0;
+    // profiling
+    bool prof_on = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    lv_kernel_stats prof = {};
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+int dmalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return LV_ERR_NOMEM;
+    }
+    return LV_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+lv::DevView view(lv_graph *g) {
+    lv::DevView v;
+    v.row = g->row;
+    v.col = g->col;
+    v.w = g->w;
+    v.comm = g->comm;
+    v.comm_sz = g->comm_sz;
+    v.new_comm = g->new_comm;
+    v.A = g->A;
+    v.k = g->k;
+    v.n = g->n;
+    v.S = g->S;
+    v.scale = g->scale;
+    v.m = g->m;
+    return v;
+}
+
+hipEvent_t ev_get(lv_graph *g) {
+    if (!g->ev_pool.empty()) {
+        hipEvent_t e = g->ev_pool.back();
+        g->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+// RAII-free profiling bracket around a launch sequence of one family
+struct ProfScope {
+    lv_graph *g;
+    int fam;
+    double bytes;
+    hipEvent_t a = nullptr;
+    ProfScope(lv_graph *g_, int fam_, double bytes_) : g(g_), fam(fam_), bytes(bytes_) {
+        if (g->prof_on) {
+            a = ev_get(g);
+            if (a) (void)hipEventRecord(a, g->stream);
+        }
+    }
+    ~ProfScope() {
+        if (g->prof_on && a) {
+            hipEvent_t b = ev_get(g);
+            if (b) {
+                (void)hipEventRecord(b, g->stream);
+                g->prof_pending.push_back({fam, a, b, bytes});
+            } else {
+                g->ev_pool.push_back(a);
+            }
+        }
+    }
+};
+
+int sync_counters(lv_graph *g) {
+    LV_HIP(hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
+}
+
+int zero_counters(lv_graph *g) {
+    // small fill kernel instead of a memset node (safe under stream capture)
+    lv::launch_fill_u32(g->stream, (uint32_t *)g->dc, 0u, sizeof(Counters) / 4);
+    return LV_OK;
+}
+
+float fix2f_host(const lv_graph *g, int64_t x) { return (float)x * g->inv_scale; }
+
+// ---------------------------------------------------------------------------
+// pass init (modularity.cu:455-463)
+// ---------------------------------------------------------------------------
+int classify(lv_graph *g) {
+    zero_counters(g);
+    {
+        ProfScope ps(g, LV_K_CLASSIFY, 8.0 * g->n);
+        lv::launch_classify_count(g->stream, g->row, g->n, g->dc->counts, g->dc->degsum);
+    }
+    int rc = sync_counters(g);
+    if (rc) return rc;
+    g->cls_off[0] = 0;
+    for (int c = 0; c < kNumClasses; ++c) {
+        g->cls_count[c] = g->hc->counts[c];
+        g->cls_deg[c] = g->hc->degsum[c];
+        g->cls_off[c + 1] = g->cls_off[c] + g->cls_count[c];
+    }
+    uint32_t cur[kNumClasses];
+    for (int c = 0; c < kNumClasses; ++c) cur[c] = g->cls_off[c];
+    LV_HIP(hipMemcpyAsync(g->dc->cursor, cur, sizeof(cur), hipMemcpyHostToDevice, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));  // `cur` is pageable stack memory
+    {
+        ProfScope ps(g, LV_K_CLASSIFY, 12.0 * g->n);
+        lv::launch_classify_scatter(g->stream, g->row, g->n, g->dc->cursor, g->order);
+    }
+    // heavy-class table arena: pow2 >= 2*deg slots per vertex
+    const uint32_t nh = g->cls_count[8];
+    if (nh) {
+        const uint32_t *hv = g->order + g->cls_off[8];
+        lv::launch_heavy_caps(g->stream, g->row, hv, nh, g->heavy_caps);
+        lv::launch_scan_u64(g->stream, g->heavy_caps, nh, g->heavy_off, g->scan_tmp64);
+        ull total = 0;
+        LV_HIP(hipMemcpyAsync(&total, g->heavy_off + nh, sizeof(ull), hipMemcpyDeviceToHost,
+                              g->stream));
+        LV_HIP(hipStreamSynchronize(g->stream));
+        if (total > g->harena) {
+            dfree(g->hkey);
+            dfree(g->hval);
+            size_t cap = (size_t)total;
+            if (dmalloc(&g->hkey, cap) || dmalloc(&g->hval, cap)) return LV_ERR_NOMEM;
+            g->harena = cap;
+            lv::launch_fill_u32(g->stream, g->hkey, lv::kEmpty, cap);
+            lv::launch_fill_u64(g->stream, g->hval, 0ull, cap);
+        }
+    }
+    return LV_OK;
+}
+
+// groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
+// 16 for 3, 32 for 4-8 (contiguous in `order`)
+int r1_all(lv_graph *g, int masked, int init_state) {
+    lv::DevView v = view(g);
+    const struct {
+        int G, c0, c1;
+    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 8}};
+    for (const auto &q : grp) {
+        uint32_t a = g->cls_off[q.c0], b = g->cls_off[q.c1 + 1];
+        double ent = 0;
+        for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
+        ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
+        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, &g->dc->acc);
+    }
+    return LV_OK;
+}
+
+// FP32 modularity from the exact counters (modularity.cu:428-436)
+float q_from_counters(const lv_graph *g) {
+    const Counters *c = g->hc;
+    float s_in = fix2f_host(g, (int64_t)c->acc);
+    u128 sq = (u128)c->limbs[0] + ((u128)c->limbs[1] << 32) + ((u128)c->limbs[2] << 64) +
+              ((u128)c->limbs[3] << 96);
+    float a2 = ldexpf((float)sq, -2 * g->S);
+    float m = g->m;
+    return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
+}
+
+// launches the S_in and sum a^2 reductions into zeroed counters
+void modularity_launch(lv_graph *g) {
+    r1_all(g, 1, 0);
+    lv::DevView v = view(g);
+    ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
+    lv::launch_sumsq(g->stream, v, g->dc->limbs);
+}
+
+int pass_init(lv_graph *g, float *q0) {
+    int rc = classify(g);
+    if (rc) return rc;
+    zero_counters(g);
+    r1_all(g, 0, 1);
+    rc = sync_counters(g);
+    if (rc) return rc;
+    g->m = fix2f_host(g, (int64_t)g->hc->acc) / 2.0f;
+    if (!(g->m > 0.0f)) {
+        // zero total weight: the reference divides by zero and never stops
+        return LV_ERR_RANGE;
+    }
+    zero_counters(g);
+    modularity_launch(g);
+    rc = sync_counters(g);
+    if (rc) return rc;
+    g->pass_ready = true;
+    if (q0) *q0 = q_from_counters(g);
+    return LV_OK;
+}
+
+// one bucket step: decide from the snapshot, then apply
+int step(lv_graph *g, int b) {
+    lv::DevView v = view(g);
+    lv::HeavyScratch hs;
+    hs.key = g->hkey;
+    hs.val = g->hval;
+    hs.slot_off = g->heavy_off;
+    hs.dlist = g->dlist;
+    hs.eslot = g->eslot;
+    hs.err = &g->dc->err;
+    const int c0 = b, c1 = (b == 7) ? 8 : b;
+    static const int fam[kNumClasses] = {0,
+                                         LV_K_MOVE_SMALL,
+                                         LV_K_MOVE_SMALL,
+                                         LV_K_MOVE_SMALL,
+                                         LV_K_MOVE_SMALL,
+                                         LV_K_MOVE_SMALL,
+                                         LV_K_MOVE_WAVE,
+                                         LV_K_MOVE_BLOCK,
+                                         LV_K_MOVE_GLOBAL};
+    for (int c = c0; c <= c1; ++c) {
+        if (!g->cls_count[c]) continue;
+        ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c]);
+        lv::launch_move_class(g->stream, v, c, g->order + g->cls_off[c], g->cls_count[c], hs);
+    }
+    const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
+    if (e > a) {
+        ProfScope ps(g, LV_K_APPLY, 16.0 * (e - a));
+        lv::launch_apply(g->stream, v, g->order + a, e - a, &g->dc->moved);
+    }
+    return LV_OK;
+}
+
+int sweep(lv_graph *g, float *q, uint64_t *moved) {
+    zero_counters(g);
+    for (int b = 1; b <= 7; ++b) step(g, b);
+    modularity_launch(g);
+    int rc = sync_counters(g);
+    if (rc) return rc;
+    if (g->hc->err) return LV_ERR_OVERFLOW;
+    if (q) *q = q_from_counters(g);
+    if (moved) *moved = g->hc->moved;
+    return LV_OK;
+}
+
+int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_print,
+                            uint32_t *iters) {
+    float act = 0.0f, old = 0.0f;
+    int rc = pass_init(g, &act);
+    if (rc) return rc;
+    bool first = true, ret = true;
+    uint32_t it = 0;
+    for (;;) {
+        uint64_t moved = 0;
+        old = act;
+        rc = sweep(g, &act, &moved);
+        if (rc) return rc;
+        ++it;
+        // the loop condition of modularity.cu:508; a sweep without moves is a
+        // fixed point and also ends the loop
+        bool stop = !(act - old >= min_gain) || moved == 0 || it >= kMaxIters;
+        if (first && (act - old < min_gain || moved == 0)) {
+            if (q_print) *q_print = old;
+            ret = false;
+        }
+        first = false;
+        if (stop) break;
+    }
+    g->last_iters = it;
+    if (improved) *improved = ret ? 1 : 0;
+    if (iters) *iters = it;
+    return LV_OK;
+}
+
+lv::AggView aview(lv_graph *g) {
+    lv::AggView a;
+    a.new_id = g->new_id;
+    a.comm_deg = g->comm_deg;
+    a.epos = g->epos;
+    a.mstart = g->mstart;
+    a.mcursor = g->mcursor;
+    a.members = g->members;
+    a.cnt = g->cnt;
+    a.tkey = g->tkey;
+    a.tval = g->tval;
+    a.skey = g->skey;
+    a.sval = g->sval;
+    a.gkey = g->gkey;
+    a.gval = g->gval;
+    a.goff = g->agg_off;
+    a.err = &g->dc->err;
+    return a;
+}
+
+// aggregation (aggregation.cu:387-462) + composition of init_comm
+// (modularity.cu:515-517, aggregation.cu:459)
+int aggregation(lv_graph *g) {
+    if (!g->pass_ready) return LV_ERR_STATE;
+    hipStream_t s = g->stream;
+    lv::DevView v = view(g);
+    const uint32_t n = g->n;
+    ProfScope ps(g, LV_K_AGGREGATE, 12.0 * g->ne + 24.0 * n);
+    zero_counters(g);
+    lv::launch_fill_u32(s, g->comm_deg, 0u, n);
+    lv::launch_fill_u32(s, g->mcursor, 0u, n);
+    lv::launch_agg_prep(s, v, aview(g));
+    lv::launch_scan_u32(s, g->comm_sz, 1, n, g->new_id, g->scan_tmp32);
+    lv::launch_scan_u32(s, g->comm_sz, 0, n, g->mstart, g->scan_tmp32);
+    lv::launch_scan_u32(s, g->comm_deg, 0, n, g->epos, g->scan_tmp32);
+    lv::launch_agg_members(s, v, aview(g));
+    lv::launch_agg_classify_count(s, v, aview(g), g->dc->acounts);
+    uint32_t nn = 0;
+    LV_HIP(hipMemcpyAsync(&nn, g->new_id + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    int rc = sync_counters(g);
+    if (rc) return rc;
+    uint32_t aoff[kNumAggClasses + 1];
+    aoff[0] = 0;
+    for (int c = 0; c < kNumAggClasses; ++c) aoff[c + 1] = aoff[c] + g->hc->acounts[c];
+    if (aoff[kNumAggClasses] != nn) return LV_ERR_OVERFLOW;
+    LV_HIP(hipMemcpyAsync(g->dc->acursor, aoff, sizeof(uint32_t) * kNumAggClasses,
+                          hipMemcpyHostToDevice, s));
+    LV_HIP(hipStreamSynchronize(s));  // `aoff` is pageable stack memory
+    lv::launch_agg_classify_scatter(s, v, aview(g), g->dc->acursor, g->aorder);
+    // global-hash arena for the large communities
+    const uint32_t nl = g->hc->acounts[2];
+    if (nl) {
+        const uint32_t *lc = g->aorder + aoff[2];
+        lv::launch_agg_caps(s, aview(g), lc, nl, g->agg_caps);
+        lv::launch_scan_u64(s, g->agg_caps, nl, g->agg_off, g->scan_tmp64);
+        ull total = 0;
+        LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        if (total > g->garena) {
+            dfree(g->gkey);
+            dfree(g->gval);
+            if (dmalloc(&g->gkey, (size_t)total) || dmalloc(&g->gval, (size_t)total))
+                return LV_ERR_NOMEM;
+            g->garena = (size_t)total;
+            lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
+            lv::launch_fill_u64(s, g->gval, 0ull, total);
+        }
+    }
+    int passes = 1;
+    {
+        uint32_t bits = 0;
+        while (bits < 32 && (n > 1) && ((uint64_t)(n - 1) >> bits)) ++bits;
+        passes = std::max(1, (int)((bits + 7) / 8));
+    }
+    lv::AggView av = aview(g);
+    for (int c = 0; c < kNumAggClasses; ++c)
+        lv::launch_agg_merge(s, v, av, c, g->aorder + aoff[c], aoff[c + 1] - aoff[c], passes);
+    // new row pointers from distinct counts
+    lv::launch_scan_u32(s, g->cnt, 0, nn, g->row2, g->scan_tmp32);
+    uint32_t ne2 = 0;
+    LV_HIP(hipMemcpyAsync(&ne2, g->row2 + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    lv::launch_agg_emit(s, v, av, g->aorder, nn, g->row2, g->col2, g->w2);
+    lv::launch_agg_compose(s, g->comm, g->new_id, g->init_comm, g->n0);
+    rc = sync_counters(g);
+    if (rc) return rc;
+    if (g->hc->err) return LV_ERR_OVERFLOW;
+    std::swap(g->row, g->row2);
+    std::swap(g->col, g->col2);
+    std::swap(g->w, g->w2);
+    g->n = nn;
+    g->ne = ne2;
+    g->pass_ready = false;
+    return LV_OK;
+}
+
+// exact host-side validation and fixed-point scale (same rule as the oracle)
+int scale_bits(uint64_t ne, const float *w, int *S_out) {
+    int S = 0;
+    u128 tot = 0;
+    for (uint64_t j = 0; j < ne; ++j) {
+        float x = w[j];
+        if (!(x >= 0.0f) || std::isinf(x)) return LV_ERR_RANGE;
+        if (x == 0.0f) continue;
+        uint32_t bits;
+        memcpy(&bits, &x, 4);
+        int ef = (int)((bits >> 23) & 0xff);
+        uint32_t man = bits & 0x7fffffu;
+        int e;
+        if (ef == 0) {
+            e = -149;
+        } else {
+            man |= 0x800000u;
+            e = ef - 150;
+        }
+        while ((man & 1u) == 0) {
+            man >>= 1;
+            ++e;
+        }
+        if (-e > S) S = -e;
+    }
+    if (S > 100) return LV_ERR_RANGE;
+    for (uint64_t j = 0; j < ne; ++j) {
+        if (w[j] == 0.0f) continue;
+        if (S > 62) return LV_ERR_RANGE;
+        tot += (u128)(uint64_t)ldexpf(w[j], S);
+        if (tot >= (u128)kFixedLimit) return LV_ERR_RANGE;
+    }
+    *S_out = S;
+    return LV_OK;
+}
+
+void free_all(lv_graph *g) {
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto &r : g->prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : g->ev_pool) (void)hipEventDestroy(e);
+    dfree(g->row);
+    dfree(g->col);
+    dfree(g->w);
+    dfree(g->row2);
+    dfree(g->col2);
+    dfree(g->w2);
+    dfree(g->k);
+    dfree(g->comm);
+    dfree(g->comm_sz);
+    dfree(g->new_comm);
+    dfree(g->A);
+    dfree(g->init_comm);
+    dfree(g->order);
+    dfree(g->heavy_caps);
+    dfree(g->heavy_off);
+    dfree(g->hkey);
+    dfree(g->hval);
+    dfree(g->dlist);
+    dfree(g->eslot);
+    dfree(g->new_id);
+    dfree(g->comm_deg);
+    dfree(g->epos);
+    dfree(g->mstart);
+    dfree(g->mcursor);
+    dfree(g->members);
+    dfree(g->cnt);
+    dfree(g->aorder);
+    dfree(g->tkey);
+    dfree(g->skey);
+    dfree(g->tval);
+    dfree(g->sval);
+    dfree(g->agg_caps);
+    dfree(g->agg_off);
+    dfree(g->gkey);
+    dfree(g->gval);
+    dfree(g->scan_tmp32);
+    dfree(g->scan_tmp64);
+    dfree(g->dc);
+    if (g->hc) (void)hipHostFree(g->hc);
+    g->hc = nullptr;
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    g->stream = nullptr;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *lv_status_string(int st) {
+    switch (st) {
+        case LV_OK: return "ok";
+        case LV_ERR_IO: return "Error opening file";
+        case LV_ERR_FORMAT: return "malformed .mtx input";
+        case LV_ERR_RANGE: return "weights outside the exact fixed-point range";
+        case LV_ERR_ARG: return "invalid argument";
+        case LV_ERR_NOMEM: return "out of memory";
+        case LV_ERR_STATE: return "call out of order";
+        case LV_ERR_HIP: return "HIP runtime error";
+        case LV_ERR_NO_DEVICE: return "no GPU device";
+        case LV_ERR_OVERFLOW: return "internal capacity exceeded";
+        default: return "unknown status";
+    }
+}
+
+int lv_device_count(int *count) {
+    if (!count) return LV_ERR_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return LV_ERR_NO_DEVICE;
+    }
+    *count = c;
+    return LV_OK;
+}
+
+int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t *col,
+                    const float *w, lv_graph **out) {
+    if (!out || !row) return LV_ERR_ARG;
+    *out = nullptr;
+    if (n >= 0x80000000u) return LV_ERR_ARG;
+    if (row[0] != 0) return LV_ERR_ARG;
+    for (uint32_t v = 0; v < n; ++v)
+        if (row[v + 1] < row[v]) return LV_ERR_ARG;
+    const uint64_t ne = row[n];
+    if (ne && (!col || !w)) return LV_ERR_ARG;
+    for (uint64_t j = 0; j < ne; ++j)
+        if (col[j] >= n) return LV_ERR_ARG;
+    int S = 0;
+    int rc = scale_bits(ne, w, &S);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        (void)hipGetLastError();
+        return LV_ERR_NO_DEVICE;
+    }
+    lv_graph *g = new (std::nothrow) lv_graph();
+    if (!g) return LV_ERR_NOMEM;
+    g->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        delete g;
+        return LV_ERR_HIP;
+    }
+    g->n = g->n0 = n;
+    g->ne = g->ne0 = ne;
+    g->S = S;
+    g->scale = ldexpf(1.0f, S);
+    g->inv_scale = ldexpf(1.0f, -S);
+    const size_t N = (size_t)n + 1, E = (size_t)ne;
+    int e = 0;
+    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E) | dmalloc(&g->w, E);
+    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E) | dmalloc(&g->w2, E);
+    e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
+    e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
+    e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
+    e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
+    e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
+    e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);
+    e |= dmalloc(&g->cnt, N) | dmalloc(&g->aorder, N);
+    e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |
+         dmalloc(&g->sval, E);
+    e |= dmalloc(&g->agg_caps, N) | dmalloc(&g->agg_off, N + 1);
+    e |= dmalloc(&g->scan_tmp32, lv::scan_tmp_elems(n + 1) + 8);
+    e |= dmalloc(&g->scan_tmp64, lv::scan_tmp_elems(n + 1) + 8);
+    e |= dmalloc(&g->dc, 1);
+    if (!e && hipHostMalloc((void **)&g->hc, sizeof(Counters)) != hipSuccess) {
+        (void)hipGetLastError();
+        e = LV_ERR_NOMEM;
+    }
+    if (e) {
+        free_all(g);
+        delete g;
+        return LV_ERR_NOMEM;
+    }
+    hipStream_t s = g->stream;
+    if (hipMemcpyAsync(g->row, row, N * sizeof(uint32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+        (E && hipMemcpyAsync(g->col, col, E * sizeof(uint32_t), hipMemcpyHostToDevice, s) !=
+                  hipSuccess) ||
+        (E && hipMemcpyAsync(g->w, w, E * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)) {
+        (void)hipGetLastError();
+        free_all(g);
+        delete g;
+        return LV_ERR_HIP;
+    }
+    lv::launch_iota_u32(s, g->init_comm, n);  // setInitComm (utils.cu:66-68)
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        free_all(g);
+        delete g;
+        return LV_ERR_HIP;
+    }
+    *out = g;
+    return LV_OK;
+}
+
+int lv_graph_destroy(lv_graph *g) {
+    if (!g) return LV_ERR_ARG;
+    (void)hipSetDevice(g->device);
+    free_all(g);
+    delete g;
+    return LV_OK;
+}
+
+int lv_pass_init(lv_graph *g, float *q0) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    return pass_init(g, q0);
+}
+
+int lv_step(lv_graph *g, int bucket, uint64_t *moved) {
+    if (!g || bucket < 1 || bucket > 7) return LV_ERR_ARG;
+    if (!g->pass_ready) return LV_ERR_STATE;
+    LV_HIP(hipSetDevice(g->device));
+    zero_counters(g);
+    step(g, bucket);
+    int rc = sync_counters(g);
+    if (rc) return rc;
+    if (g->hc->err) return LV_ERR_OVERFLOW;
+    if (moved) *moved = g->hc->moved;
+    return LV_OK;
+}
+
+int lv_sweep(lv_graph *g, float *q, uint64_t *moved) {
+    if (!g) return LV_ERR_ARG;
+    if (!g->pass_ready) return LV_ERR_STATE;
+    LV_HIP(hipSetDevice(g->device));
+    return sweep(g, q, moved);
+}
+
+int lv_modularity(lv_graph *g, float *q) {
+    if (!g || !q) return LV_ERR_ARG;
+    if (!g->pass_ready) return LV_ERR_STATE;
+    LV_HIP(hipSetDevice(g->device));
+    zero_counters(g);
+    modularity_launch(g);
+    int rc = sync_counters(g);
+    if (rc) return rc;
+    *q = q_from_counters(g);
+    return LV_OK;
+}
+
+int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_printed) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    float qp = 0.0f;
+    int imp = 0;
+    int rc = modularity_optimization(g, min_gain, &imp, &qp, nullptr);
+    if (rc) return rc;
+    if (improved) *improved = imp;
+    if (q_printed && !imp) *q_printed = qp;
+    return LV_OK;
+}
+
+int lv_aggregation(lv_graph *g) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    return aggregation(g);
+}
+
+int lv_run(lv_graph *g, float min_gain, lv_result *res) {
+    if (!g || !res) return LV_ERR_ARG;
+    memset(res, 0, sizeof(*res));
+    LV_HIP(hipSetDevice(g->device));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    LV_HIP(hipEventCreate(&t0));
+    LV_HIP(hipEventCreate(&t1));
+    LV_HIP(hipEventRecord(t0, g->stream));
+    int rc = LV_OK;
+    double bytes = 0;
+    for (;;) {
+        const double V = g->n, E = (double)g->ne;
+        int imp = 0;
+        uint32_t it = 0;
+        float qp = 0.0f;
+        rc = modularity_optimization(g, min_gain, &imp, &qp, &it);
+        if (rc) break;
+        res->total_iters += it;
+        res->sum_edges += g->ne;
+        res->sum_edge_visits += (uint64_t)(it + 1) * g->ne;
+        bytes += 8 * E + 8 * V + it * (24 * E + 44 * V);
+        if (!imp) {
+            res->q_final = qp;
+            break;
+        }
+        rc = aggregation(g);
+        if (rc) break;
+        bytes += 12 * E + 8 * (double)g->ne + 24 * V + 4 * (double)g->n;
+        res->passes++;
+    }
+    (void)hipEventRecord(t1, g->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    res->alg_ms = ms;
+    res->bytes_alg = bytes;
+    res->n_final = g->n;
+    res->status = rc;
+    return rc;
+}
+
+int lv_get_assignment(lv_graph *g, uint32_t *assignment) {
+    if (!g || !assignment) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    LV_HIP(hipMemcpyAsync(assignment, g->init_comm, (size_t)g->n0 * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
+}
+
+int lv_graph_size(lv_graph *g, uint32_t *n, uint64_t *ne, uint32_t *n0) {
+    if (!g) return LV_ERR_ARG;
+    if (n) *n = g->n;
+    if (ne) *ne = g->ne;
+    if (n0) *n0 = g->n0;
+    return LV_OK;
+}
+
+int lv_get_csr(lv_graph *g, uint32_t *row, uint32_t *col, float *w) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    if (row)
+        LV_HIP(hipMemcpyAsync(row, g->row, ((size_t)g->n + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (col && g->ne)
+        LV_HIP(hipMemcpyAsync(col, g->col, (size_t)g->ne * 4, hipMemcpyDeviceToHost, s));
+    if (w && g->ne) LV_HIP(hipMemcpyAsync(w, g->w, (size_t)g->ne * 4, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    return LV_OK;
+}
+
+int lv_get_state(lv_graph *g, float *k, uint32_t *comm, uint32_t *comm_sz, int64_t *a) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    const size_t n = g->n;
+    if (k) LV_HIP(hipMemcpyAsync(k, g->k, n * 4, hipMemcpyDeviceToHost, s));
+    if (comm) LV_HIP(hipMemcpyAsync(comm, g->comm, n * 4, hipMemcpyDeviceToHost, s));
+    if (comm_sz) LV_HIP(hipMemcpyAsync(comm_sz, g->comm_sz, n * 4, hipMemcpyDeviceToHost, s));
+    if (a) LV_HIP(hipMemcpyAsync(a, g->A, n * 8, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    return LV_OK;
+}
+
+int lv_get_scale(lv_graph *g, int *scale_bits_out, float *m) {
+    if (!g) return LV_ERR_ARG;
+    if (scale_bits_out) *scale_bits_out = g->S;
+    if (m) *m = g->m;
+    return LV_OK;
+}
+
+int lv_profile_enable(lv_graph *g, int on) {
+    if (!g) return LV_ERR_ARG;
+    g->prof_on = on != 0;
+    return LV_OK;
+}
+
+int lv_profile_reset(lv_graph *g) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipStreamSynchronize(g->stream));
+    for (auto &r : g->prof_pending) {
+        g->ev_pool.push_back(r.a);
+        g->ev_pool.push_back(r.b);
+    }
+    g->prof_pending.clear();
+    memset(&g->prof, 0, sizeof(g->prof));
+    return LV_OK;
+}
+
+int lv_profile_get(lv_graph *g, lv_kernel_stats *out) {
+    if (!g || !out) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    for (auto &r : g->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            g->prof.ms[r.fam] += ms;
+            g->prof.launches[r.fam] += 1;
+            g->prof.bytes[r.fam] += r.bytes;
+        } else {
+            (void)hipGetLastError();
+        }
+        g->ev_pool.push_back(r.a);
+        g->ev_pool.push_back(r.b);
+    }
+    g->prof_pending.clear();
+    *out = g->prof;
+    return LV_OK;
+}
+
+}  // extern "C"

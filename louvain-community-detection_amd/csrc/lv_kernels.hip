@@ -1,0 +1,1370 @@
+// lv_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the Louvain engine.
+//
+// Everything here is integer/byte work plus scalar FP32 gain arithmetic:
+// there is no dense contraction, so no MFMA.  Kernels are written for
+// 64-wide wavefronts; sub-wave groups of 4/8/16/32 lanes serve low-degree
+// vertices, one wave or one workgroup serves a heavier vertex.
+//
+// Arithmetic contract (shared bit-for-bit with the CPU oracle):
+//   * weights, k_v, hash sums, a_c and merged weights are exact integers at
+//     a per-graph scale 2^S (int64), rounded once to FP32 when used;
+//   * k_v and s_v use the reference's pinned FP32 order: 32 strided
+//     sequential partials, then the shuffle-down tree 16,8,4,2,1
+//     (modularity.cu:27-49, 400-426);
+//   * the gain, move test and modularity are evaluated in FP32 exactly as
+//     written in the reference (modularity.cu:149-152, :182, :435), compiled
+//     with -ffp-contract=off and IEEE division.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lv_device.h"
+
+namespace lv {
+
+// ---------------------------------------------------------------------------
+// arithmetic helpers
+// ---------------------------------------------------------------------------
+
+// Correctly rounded (nearest-even) u64 -> f32.
+__device__ __forceinline__ float u64_to_f32_rn(unsigned long long u) {
+    if (u < (1ull << 24)) return (float)(uint32_t)u;
+    int lz = __clzll(u);
+    int sh = 40 - lz;  // keep the top 24 bits
+    unsigned long long mant = u >> sh;
+    unsigned long long rem = u & ((1ull << sh) - 1ull);
+    unsigned long long half = 1ull << (sh - 1);
+    if (rem > half || (rem == half && (mant & 1ull))) mant++;
+    return ldexpf((float)(uint32_t)mant, sh);
+}
+
+// exact fixed point (x * 2^-S) rounded once to f32
+__device__ __forceinline__ float fix2f(unsigned long long x, int S) {
+    return ldexpf(u64_to_f32_rn(x), -S);
+}
+
+// f32 (a multiple of 2^-S, >= 0) -> exact fixed point
+__device__ __forceinline__ unsigned long long f2fix(float x, float scale) {
+    return (unsigned long long)(x * scale);
+}
+
+// Gain of moving to community C (modularity.cu:149-152): FP32, this
+// association, no contraction.
+__device__ __forceinline__ float gain_of(float hw, float m, float kv, float acv, float ac) {
+    float t1 = hw / m;
+    float t3 = (acv - kv) - ac;
+    float t4 = kv * t3;
+    float t6 = (2.0f * m) * m;
+    return t1 + t4 / t6;
+}
+
+// reduction order of the reference's arg-max (modularity.cu:169-179, 363)
+__device__ __forceinline__ bool better(float ga, uint32_t ca, float gb, uint32_t cb) {
+    return ga > gb || (ga == gb && ca < cb);
+}
+
+template <int LOG2CAP>
+__device__ __forceinline__ uint32_t hslot(uint32_t key) {
+    return (key * 0x9E3779B1u) >> (32 - LOG2CAP);
+}
+
+__device__ __forceinline__ uint32_t hslot_dyn(uint32_t key, int log2cap) {
+    return (key * 0x9E3779B1u) >> (32 - log2cap);
+}
+
+template <int N>
+struct Log2 {
+    static constexpr int value = 1 + Log2<N / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
+// wave-level sum of u64
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// fills
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+static inline unsigned grid_for(size_t n, unsigned threads, unsigned cap = 65536) {
+    size_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+__global__ void k_iota_u32(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = (uint32_t)i;
+}
+void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+// ---------------------------------------------------------------------------
+// degree classification: once per pass (the reference re-partitions the
+// whole vertex set 7 times per sweep, modularity.cu:468-469)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uint32_t n,
+                                                        uint32_t *counts,
+                                                        unsigned long long *degsum) {
+    __shared__ uint32_t hc[kNumClasses];
+    __shared__ unsigned long long hd[kNumClasses];
+    if (threadIdx.x < kNumClasses) {
+        hc[threadIdx.x] = 0;
+        hd[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < n; v += gridDim.x * 256) {
+        uint32_t d = row[v + 1] - row[v];
+        int c = vertex_class(d);
+        atomicAdd(&hc[c], 1u);
+        atomicAdd(&hd[c], (unsigned long long)d);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumClasses) {
+        if (hc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hc[threadIdx.x]);
+        if (hd[threadIdx.x]) atomicAdd(&degsum[threadIdx.x], hd[threadIdx.x]);
+    }
+}
+
+// cursor[c] starts at the class offset; order within a class is not
+// semantic (every vertex of a step decides from the same snapshot)
+__global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, uint32_t n,
+                                                          uint32_t *cursor, uint32_t *order) {
+    __shared__ uint32_t hc[kNumClasses];
+    __shared__ uint32_t base[kNumClasses];
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    if (threadIdx.x < kNumClasses) hc[threadIdx.x] = 0;
+    __syncthreads();
+    int c = -1;
+    uint32_t r = 0;
+    if (v < n) {
+        c = vertex_class(row[v + 1] - row[v]);
+        r = atomicAdd(&hc[c], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumClasses && hc[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hc[threadIdx.x]);
+    __syncthreads();
+    if (c >= 0) order[base[c] + r] = v;
+}
+
+void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t *counts,
+                           unsigned long long *degsum) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_classify_count, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, row, n,
+                       counts, degsum);
+}
+void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t *cursor,
+                             uint32_t *order) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 255) / 256), dim3(256), 0, s, row, n, cursor,
+                       order);
+}
+
+// ---------------------------------------------------------------------------
+// pinned-order sums k_v / s_v (modularity.cu:27-49 and 400-426)
+//
+// G lanes per vertex.  For G < 32 the class guarantees deg <= G, and the
+// reference's 32-lane tree restricted to G lanes gives the same bits (the
+// missing partials are +0).  Lane l sums entries l, l+32, ... sequentially;
+// loads are batched 8 rounds ahead, adds stay in order.
+// ---------------------------------------------------------------------------
+template <int G, bool MASKED>
+__global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, uint32_t count,
+                                            int init_state, unsigned long long *acc) {
+    constexpr int GPB = 256 / G;
+    const int lane = threadIdx.x % G;
+    const uint32_t vi = blockIdx.x * GPB + threadIdx.x / G;
+    unsigned long long fixed = 0;
+    if (vi < count) {
+        const uint32_t v = verts[vi];
+        const uint32_t base = g.row[v];
+        const uint32_t d = g.row[v + 1] - base;
+        const uint32_t cv = MASKED ? g.comm[v] : 0u;
+        const float *wr = g.w + base;
+        const uint32_t *cr = g.col + base;
+        float p = 0.0f;
+        uint32_t j = lane;
+        if (G == 32) {
+            for (; j + 7 * 32 < d; j += 8 * 32) {
+                float x[8];
+                bool keep[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    x[u] = wr[j + 32 * u];
+                    keep[u] = MASKED ? (g.comm[cr[j + 32 * u]] == cv) : true;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (keep[u]) p += x[u];
+            }
+        }
+        for (; j < d; j += 32) {
+            float x = wr[j];
+            if (!MASKED || g.comm[cr[j]] == cv) p += x;
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) p += __shfl_down(p, o, G);
+        if (lane == 0) {
+            fixed = f2fix(p, g.scale);
+            if (!MASKED) {
+                g.k[v] = p;
+                if (init_state) {
+                    g.comm[v] = v;
+                    g.comm_sz[v] = 1u;
+                    g.A[v] = fixed;
+                }
+            }
+        }
+    }
+    // block total (exact, order-free)
+    __shared__ unsigned long long part[4];
+    unsigned long long t = wave_sum_u64(fixed);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = part[0] + part[1] + part[2] + part[3];
+        if (s) atomicAdd(acc, s);
+    }
+}
+
+void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
+               int masked, int init_state, unsigned long long *acc) {
+    if (!count) return;
+    const unsigned gpb = 256 / group;
+    dim3 grid((count + gpb - 1) / gpb), block(256);
+#define LV_R1(GG)                                                                            \
+    if (group == GG) {                                                                       \
+        if (masked)                                                                          \
+            hipLaunchKernelGGL((k_r1<GG, true>), grid, block, 0, s, g, verts, count, init_state, \
+                               acc);                                                         \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_r1<GG, false>), grid, block, 0, s, g, verts, count,           \
+                               init_state, acc);                                             \
+        return;                                                                              \
+    }
+    LV_R1(4) LV_R1(8) LV_R1(16) LV_R1(32)
+#undef LV_R1
+}
+
+// ---------------------------------------------------------------------------
+// sum over communities of round32(a_c)^2, exact at scale 2^2S, as four
+// 32-bit limbs accumulated in u64 (modularity.cu:431-432, square<float>)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sumsq(DevView g, unsigned long long *limbs) {
+    unsigned long long l0 = 0, l1 = 0, l2 = 0, l3 = 0;
+    const int twoS = 2 * g.S;
+    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < g.n; c += gridDim.x * 256) {
+        unsigned long long a = g.A[c];
+        if (!a) continue;
+        float af = fix2f(a, g.S);
+        float a2 = af * af;
+        uint32_t u = __float_as_uint(a2);
+        int ef = (int)((u >> 23) & 0xff);
+        uint32_t man = u & 0x7fffffu;
+        int e;
+        if (ef == 0) {
+            e = -149;
+        } else {
+            man |= 0x800000u;
+            e = ef - 150;
+        }
+        int sh = e + twoS;  // >= 0: a2 is a multiple of 2^-2S
+        unsigned long long lo, hi;
+        if (sh <= 0) {  // sh < 0 cannot occur: a2 is a multiple of 2^-2S
+            lo = (unsigned long long)man >> (-sh);
+            hi = 0ull;
+        } else if (sh < 64) {
+            lo = (unsigned long long)man << sh;
+            hi = (unsigned long long)man >> (64 - sh);
+        } else {
+            lo = 0ull;
+            hi = (unsigned long long)man << (sh - 64);
+        }
+        l0 += lo & 0xffffffffull;
+        l1 += lo >> 32;
+        l2 += hi & 0xffffffffull;
+        l3 += hi >> 32;
+    }
+    __shared__ unsigned long long part[4][4];
+    unsigned long long t0 = wave_sum_u64(l0), t1 = wave_sum_u64(l1), t2 = wave_sum_u64(l2),
+                       t3 = wave_sum_u64(l3);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[w][0] = t0;
+        part[w][1] = t1;
+        part[w][2] = t2;
+        part[w][3] = t3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                               part[3][threadIdx.x];
+        if (s) atomicAdd(&limbs[threadIdx.x], s);
+    }
+}
+
+void launch_sumsq(hipStream_t s, const DevView &g, unsigned long long *limbs) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, s, g, limbs);
+}
+
+// ---------------------------------------------------------------------------
+// move kernels (computeMoveWarp / computeMoveBlock / computeMoveGlobal,
+// modularity.cu:95-380), canonical semantics:
+//   * each vertex of the step decides from the step-start snapshot;
+//   * virtual lane l of T visits entries l, l+T, ... ; a lane keeps its FIRST
+//     maximum (strict >);
+//   * buckets 1-5: the gain of an entry in round r sees the sums of entries
+//     of rounds <= r; destination = best (gain desc, comm asc) lane's comm;
+//   * buckets 6-7: complete sums; the 128-lane tree of modularity.cu:265-275
+//     picks the winner lane; destination = comm of that lane's LAST visited
+//     entry (graph.comm[e], modularity.cu:183/279);
+//   * move iff G - comm_e_w / m > 0 (FP32), else stay; no candidate: stay.
+// ---------------------------------------------------------------------------
+
+// buckets 1-5: T lanes per vertex (T = reference lanes), ROUNDS rounds,
+// per-group LDS open-addressing table of CAP slots with exact u64 sums.
+template <int T, int ROUNDS, int CAP>
+__global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *verts,
+                                                    uint32_t count) {
+    constexpr int GPB = 256 / T;
+    constexpr int LC = Log2<CAP>::value;
+    __shared__ uint32_t skey[GPB * CAP];
+    __shared__ unsigned long long sval[GPB * CAP];
+    const int grp = threadIdx.x / T, lane = threadIdx.x % T;
+    uint32_t *key = skey + grp * CAP;
+    unsigned long long *val = sval + grp * CAP;
+    for (int i = lane; i < CAP; i += T) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    const uint32_t vi = blockIdx.x * GPB + grp;
+    const bool active = vi < count;
+    uint32_t v = 0, base = 0, d = 0, cv = 0, szv = 0;
+    float kv = 0.0f, acv = 0.0f;
+    if (active) {
+        v = verts[vi];
+        base = g.row[v];
+        d = g.row[v + 1] - base;
+        cv = g.comm[v];
+        szv = g.comm_sz[cv];
+        kv = g.k[v];
+        acv = fix2f(g.A[cv], g.S);
+    }
+    __syncthreads();
+    float best_g = -INFINITY;
+    uint32_t best_c = kEmpty;
+    unsigned long long cew = 0;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const uint32_t j = (uint32_t)(r * T + lane);
+        const bool has = active && j < d;
+        uint32_t e = 0, ce = 0, slot = 0;
+        if (has) {
+            e = g.col[base + j];
+            ce = g.comm[e];
+            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            slot = hslot<LC>(ce);
+            for (int probe = 0; probe < CAP; ++probe) {
+                uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+                if (prev == kEmpty || prev == ce) break;
+                slot = (slot + 1) & (CAP - 1);
+            }
+            atomicAdd(&val[slot], wf);
+            if (ce == cv && e != v) cew += wf;
+        }
+        __syncthreads();
+        if (has) {
+            bool cand = true;  // isLowerIdx (modularity.cu:85-93)
+            if (szv == 1u && g.comm_sz[ce] == 1u) cand = ce < cv;
+            if (cand) {
+                float gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[ce], g.S));
+                if (gg > best_g) {
+                    best_g = gg;
+                    best_c = ce;
+                }
+            }
+        }
+        if (ROUNDS > 1) __syncthreads();
+    }
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) {
+        float og = __shfl_xor(best_g, o, T);
+        uint32_t oc = __shfl_xor(best_c, o, T);
+        unsigned long long ow = __shfl_xor(cew, o, T);
+        if (better(og, oc, best_g, best_c)) {
+            best_g = og;
+            best_c = oc;
+        }
+        cew += ow;
+    }
+    if (active && lane == 0) {
+        uint32_t nc = cv;
+        if (best_g != -INFINITY) {
+            float cewf = fix2f(cew, g.S);
+            if (best_g - (cewf / g.m) > 0.0f) nc = best_c;
+        }
+        g.new_comm[v] = nc;
+    }
+}
+
+// The 128-virtual-lane arg-max tree of computeMoveBlock/Global
+// (modularity.cu:260-275) on one wave: lane p holds virtual lanes p and
+// p+64.  Returns the winner (gain, comm, virtual lane) in lane 0.
+__device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_t c1, float &bg,
+                                        uint32_t &bc, uint32_t &bp) {
+    const uint32_t p = __lane_id();
+    bg = g0;
+    bc = c0;
+    bp = p;
+    if (better(g1, c1, g0, c0)) {
+        bg = g1;
+        bc = c1;
+        bp = p + 64;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float og = __shfl_down(bg, o, 64);
+        uint32_t oc = __shfl_down(bc, o, 64);
+        uint32_t op = __shfl_down(bp, o, 64);
+        if (better(og, oc, bg, bc)) {
+            bg = og;
+            bc = oc;
+            bp = op;
+        }
+    }
+}
+
+// last entry visited by virtual lane wl of 128 on a row of d entries
+__device__ __forceinline__ uint32_t last_entry(uint32_t wl, uint32_t d) {
+    return wl + 128u * ((d - 1u - wl) / 128u);
+}
+
+// final decision for buckets 6-7
+__device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, uint32_t v, uint32_t base,
+                                                    uint32_t d, uint32_t cv, float bg,
+                                                    uint32_t bp, unsigned long long cew) {
+    if (bg == -INFINITY || bp >= d) return cv;
+    uint32_t dest = g.comm[g.col[base + last_entry(bp, d)]];
+    float cewf = fix2f(cew, g.S);
+    if (bg - (cewf / g.m) > 0.0f) return dest;
+    return cv;
+}
+
+// bucket 6 (84 < deg <= 319): one wave64 per vertex, LDS table per wave.
+// Physical lane p visits entries p, p+64, ...: even rounds belong to virtual
+// lane p, odd rounds to virtual lane p+64.
+constexpr int kWaveCap = 512;
+constexpr int kWaveMaxPer = 5;  // ceil(319 / 64)
+__global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *verts,
+                                                   uint32_t count) {
+    constexpr int LC = Log2<kWaveCap>::value;
+    __shared__ uint32_t skey[4 * kWaveCap];
+    __shared__ unsigned long long sval[4 * kWaveCap];
+    __shared__ uint32_t sdist[4][320];
+    __shared__ uint32_t sndist[4];
+    __shared__ unsigned long long scew[4];
+    const int wv = threadIdx.x >> 6, p = threadIdx.x & 63;
+    uint32_t *key = skey + wv * kWaveCap;
+    unsigned long long *val = sval + wv * kWaveCap;
+    for (int i = p; i < kWaveCap; i += 64) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    if (p == 0) {
+        sndist[wv] = 0;
+        scew[wv] = 0;
+    }
+    const uint32_t vi = blockIdx.x * 4 + wv;
+    const bool active = vi < count;
+    uint32_t v = 0, base = 0, d = 0, cv = 0, szv = 0;
+    float kv = 0.0f, acv = 0.0f;
+    if (active) {
+        v = verts[vi];
+        base = g.row[v];
+        d = g.row[v + 1] - base;
+        cv = g.comm[v];
+        szv = g.comm_sz[cv];
+        kv = g.k[v];
+        acv = fix2f(g.A[cv], g.S);
+    }
+    __syncthreads();
+    uint32_t eslot[kWaveMaxPer];
+    unsigned long long cew = 0;
+#pragma unroll
+    for (int i = 0; i < kWaveMaxPer; ++i) {
+        const uint32_t j = p + 64u * i;
+        eslot[i] = kEmpty;
+        if (active && j < d) {
+            uint32_t e = g.col[base + j];
+            uint32_t ce = g.comm[e];
+            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            uint32_t slot = hslot<LC>(ce);
+            for (int probe = 0; probe < kWaveCap; ++probe) {
+                uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+                if (prev == kEmpty) {
+                    uint32_t idx = atomicAdd(&sndist[wv], 1u);
+                    sdist[wv][idx] = slot;
+                    break;
+                }
+                if (prev == ce) break;
+                slot = (slot + 1) & (kWaveCap - 1);
+            }
+            atomicAdd(&val[slot], wf);
+            if (ce == cv && e != v) cew += wf;
+            eslot[i] = slot;
+        }
+    }
+    cew = wave_sum_u64(cew);
+    __syncthreads();
+    // one gain per distinct community (complete sums), stored over the sum
+    const uint32_t nd = active ? sndist[wv] : 0u;
+    for (uint32_t i = p; i < nd; i += 64) {
+        uint32_t slot = sdist[wv][i];
+        uint32_t c = key[slot];
+        float gg = -INFINITY;
+        bool cand = true;
+        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        val[slot] = (unsigned long long)__float_as_uint(gg);
+    }
+    __syncthreads();
+    float g0 = -INFINITY, g1 = -INFINITY;
+    uint32_t c0 = kEmpty, c1 = kEmpty;
+#pragma unroll
+    for (int i = 0; i < kWaveMaxPer; ++i) {
+        if (eslot[i] == kEmpty) continue;
+        float gg = __uint_as_float((uint32_t)val[eslot[i]]);
+        uint32_t c = key[eslot[i]];
+        if (i & 1) {
+            if (gg > g1) {
+                g1 = gg;
+                c1 = c;
+            }
+        } else {
+            if (gg > g0) {
+                g0 = gg;
+                c0 = c;
+            }
+        }
+    }
+    float bg;
+    uint32_t bc, bp;
+    tree128(g0, c0, g1, c1, bg, bc, bp);
+    if (active && p == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, cew);
+}
+
+// Combine per-thread lane states into the 128 virtual lanes and run the
+// tree.  Thread t of P serves virtual lane t % 128 on entries t, t+P, ...;
+// a virtual lane's first maximum is (max gain, then smallest entry index).
+template <int P>
+__device__ __forceinline__ void block_argmax(float tg, uint32_t tc, uint32_t tj, float *lg,
+                                             uint32_t *lc, uint32_t *lj, float &bg, uint32_t &bc,
+                                             uint32_t &bp) {
+    lg[threadIdx.x] = tg;
+    lc[threadIdx.x] = tc;
+    lj[threadIdx.x] = tj;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float gv[2];
+        uint32_t cvv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t vl = threadIdx.x + 64u * h;
+            float g = lg[vl];
+            uint32_t c = lc[vl], j = lj[vl];
+#pragma unroll
+            for (int q = 1; q < P / 128; ++q) {
+                const uint32_t t2 = vl + 128u * q;
+                float g2 = lg[t2];
+                uint32_t j2 = lj[t2];
+                if (g2 > g || (g2 == g && j2 < j)) {
+                    g = g2;
+                    c = lc[t2];
+                    j = j2;
+                }
+            }
+            gv[h] = g;
+            cvv[h] = c;
+        }
+        tree128(gv[0], cvv[0], gv[1], cvv[1], bg, bc, bp);
+    }
+}
+
+// bucket 7 with deg <= kBlockMax: one workgroup of P threads per vertex,
+// LDS table of CAP slots.
+template <int P, int CAP>
+__global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *verts,
+                                                  uint32_t count) {
+    constexpr int LC = Log2<CAP>::value;
+    constexpr int MAXPER = (kBlockMax + P - 1) / P;
+    __shared__ uint32_t key[CAP];
+    __shared__ unsigned long long val[CAP];
+    __shared__ uint32_t dist[kBlockMax];
+    __shared__ uint32_t ndist;
+    __shared__ unsigned long long scew;
+    __shared__ float lg[P];
+    __shared__ uint32_t lc[P], lj[P];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < CAP; i += P) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    if (t == 0) {
+        ndist = 0;
+        scew = 0;
+    }
+    const uint32_t v = verts[blockIdx.x];
+    const uint32_t base = g.row[v];
+    const uint32_t d = g.row[v + 1] - base;
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
+    __syncthreads();
+    uint32_t eslot[MAXPER];
+    unsigned long long cew = 0;
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i) {
+        const uint32_t j = t + (uint32_t)P * i;
+        eslot[i] = kEmpty;
+        if (j < d) {
+            uint32_t e = g.col[base + j];
+            uint32_t ce = g.comm[e];
+            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            uint32_t slot = hslot<LC>(ce);
+            for (int probe = 0; probe < CAP; ++probe) {
+                uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+                if (prev == kEmpty) {
+                    dist[atomicAdd(&ndist, 1u)] = slot;
+                    break;
+                }
+                if (prev == ce) break;
+                slot = (slot + 1) & (CAP - 1);
+            }
+            atomicAdd(&val[slot], wf);
+            if (ce == cv && e != v) cew += wf;
+            eslot[i] = slot;
+        }
+    }
+    cew = wave_sum_u64(cew);
+    if ((t & 63) == 0 && cew) atomicAdd(&scew, cew);
+    __syncthreads();
+    const uint32_t nd = ndist;
+    for (uint32_t i = t; i < nd; i += P) {
+        uint32_t slot = dist[i];
+        uint32_t c = key[slot];
+        float gg = -INFINITY;
+        bool cand = true;
+        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        val[slot] = (unsigned long long)__float_as_uint(gg);
+    }
+    __syncthreads();
+    float tg = -INFINITY;
+    uint32_t tc = kEmpty, tj = kEmpty;
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i) {
+        if (eslot[i] == kEmpty) continue;
+        float gg = __uint_as_float((uint32_t)val[eslot[i]]);
+        if (gg > tg) {
+            tg = gg;
+            tc = key[eslot[i]];
+            tj = t + (uint32_t)P * i;
+        }
+    }
+    float bg = -INFINITY;
+    uint32_t bc = kEmpty, bp = kEmpty;
+    block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp);
+    if (t == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, scew);
+}
+
+// bucket 7 with deg > kBlockMax: one workgroup of P threads per vertex,
+// table of cap = pow2 >= 2*deg slots in a global arena (kept clean between
+// uses: every touched slot is reset at the end), distinct-slot list and
+// per-entry slots in E-sized scratch at the vertex's row range.
+template <int P>
+__global__ __launch_bounds__(P) void k_move_global(DevView g, const uint32_t *verts,
+                                                   uint32_t count, HeavyScratch hs) {
+    __shared__ uint32_t ndist;
+    __shared__ unsigned long long scew;
+    __shared__ float lg[P];
+    __shared__ uint32_t lc[P], lj[P];
+    const uint32_t t = threadIdx.x;
+    const uint32_t vi = blockIdx.x;
+    const uint32_t v = verts[vi];
+    const uint32_t base = g.row[v];
+    const uint32_t d = g.row[v + 1] - base;
+    const unsigned long long off = hs.slot_off[vi];
+    const uint32_t cap = (uint32_t)(hs.slot_off[vi + 1] - off);
+    const int lcap = 31 - __clz(cap);
+    uint32_t *key = hs.key + off;
+    unsigned long long *val = hs.val + off;
+    uint32_t *dist = hs.dlist + base;
+    uint32_t *eslot = hs.eslot + base;
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
+    if (t == 0) {
+        ndist = 0;
+        scew = 0;
+    }
+    __syncthreads();
+    unsigned long long cew = 0;
+    for (uint32_t j = t; j < d; j += P) {
+        uint32_t e = g.col[base + j];
+        uint32_t ce = g.comm[e];
+        unsigned long long wf = f2fix(g.w[base + j], g.scale);
+        uint32_t slot = hslot_dyn(ce, lcap);
+        uint32_t probe = 0;
+        for (; probe < cap; ++probe) {
+            uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+            if (prev == kEmpty) {
+                dist[atomicAdd(&ndist, 1u)] = slot;
+                break;
+            }
+            if (prev == ce) break;
+            slot = (slot + 1) & (cap - 1);
+        }
+        if (probe == cap) {  // cannot happen: cap >= 2*deg
+            atomicOr(hs.err, 1u);
+            slot = 0;
+        }
+        atomicAdd(&val[slot], wf);
+        if (ce == cv && e != v) cew += wf;
+        eslot[j] = slot;
+    }
+    cew = wave_sum_u64(cew);
+    if ((t & 63) == 0 && cew) atomicAdd(&scew, cew);
+    __syncthreads();
+    const uint32_t nd = ndist;
+    for (uint32_t i = t; i < nd; i += P) {
+        uint32_t slot = dist[i];
+        uint32_t c = key[slot];
+        float gg = -INFINITY;
+        bool cand = true;
+        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        val[slot] = (unsigned long long)__float_as_uint(gg);
+    }
+    __syncthreads();
+    float tg = -INFINITY;
+    uint32_t tc = kEmpty, tj = kEmpty;
+    for (uint32_t j = t; j < d; j += P) {
+        uint32_t slot = eslot[j];
+        float gg = __uint_as_float((uint32_t)val[slot]);
+        if (gg > tg) {
+            tg = gg;
+            tc = key[slot];
+            tj = j;
+        }
+    }
+    float bg = -INFINITY;
+    uint32_t bc = kEmpty, bp = kEmpty;
+    block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp);
+    if (t == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, scew);
+    __syncthreads();
+    // leave the arena clean for the next use
+    for (uint32_t i = t; i < nd; i += P) {
+        uint32_t slot = dist[i];
+        key[slot] = kEmpty;
+        val[slot] = 0ull;
+    }
+}
+
+void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
+                       uint32_t count, const HeavyScratch &hs) {
+    if (!count) return;
+    switch (cls) {
+        case 1:
+            hipLaunchKernelGGL((k_move_small<4, 1, 8>), dim3((count + 63) / 64), dim3(256), 0, s, g,
+                               verts, count);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_move_small<8, 1, 16>), dim3((count + 31) / 32), dim3(256), 0, s,
+                               g, verts, count);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_move_small<16, 1, 32>), dim3((count + 15) / 16), dim3(256), 0, s,
+                               g, verts, count);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_move_small<32, 1, 64>), dim3((count + 7) / 8), dim3(256), 0, s,
+                               g, verts, count);
+            break;
+        case 5:
+            hipLaunchKernelGGL((k_move_small<32, 3, 256>), dim3((count + 7) / 8), dim3(256), 0, s,
+                               g, verts, count);
+            break;
+        case 6:
+            hipLaunchKernelGGL(k_move_wave, dim3((count + 3) / 4), dim3(256), 0, s, g, verts,
+                               count);
+            break;
+        case 7:
+            hipLaunchKernelGGL((k_move_block<256, 4096>), dim3(count), dim3(256), 0, s, g, verts,
+                               count);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_move_global<512>), dim3(count), dim3(512), 0, s, g, verts, count,
+                               hs);
+            break;
+        default:
+            break;
+    }
+}
+
+// heavy-class table capacities: pow2 >= 2*deg
+__global__ void k_heavy_caps(const uint32_t *row, const uint32_t *verts, uint32_t count,
+                             unsigned long long *caps) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        uint32_t v = verts[i];
+        unsigned long long d = row[v + 1] - row[v];
+        unsigned long long c = 1;
+        while (c < 2 * d) c <<= 1;
+        caps[i] = c;
+    }
+}
+void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
+                       unsigned long long *caps) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_heavy_caps, dim3((count + 255) / 256), dim3(256), 0, s, row, verts,
+                       count, caps);
+}
+
+// ---------------------------------------------------------------------------
+// apply one step's moves together (assignCommunity, modularity.cu:382-391)
+// and update the community weights incrementally and exactly (equal to the
+// from-scratch recompute of computeNewCommWeights, modularity.cu:393-398)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts, uint32_t count,
+                                               unsigned long long *moved) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long mv = 0;
+    if (i < count) {
+        const uint32_t v = verts[i];
+        const uint32_t o = g.comm[v], nc = g.new_comm[v];
+        if (o != nc) {
+            const unsigned long long K = f2fix(g.k[v], g.scale);
+            atomicSub(&g.comm_sz[o], 1u);
+            atomicAdd(&g.comm_sz[nc], 1u);
+            atomicAdd(&g.A[o], (unsigned long long)(-(long long)K));
+            atomicAdd(&g.A[nc], K);
+            g.comm[v] = nc;
+            mv = 1;
+        }
+    }
+    mv = wave_sum_u64(mv);
+    if ((threadIdx.x & 63) == 0 && mv) atomicAdd(moved, mv);
+}
+void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
+                  unsigned long long *moved) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_apply, dim3((count + 255) / 256), dim3(256), 0, s, g, verts, count,
+                       moved);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (3 phases: tile sums, scan of tile sums, tile scan)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+size_t scan_tmp_elems(uint32_t n) { return (size_t)n / kScanTile + 2; }
+
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T &total) {
+    __shared__ T wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        T s = wsum[i];
+        if (i < wv) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + v - x;
+}
+
+template <typename T>
+__device__ __forceinline__ T scan_in(const T *in, int kind, size_t i) {
+    T x = in[i];
+    return kind == 1 ? (T)(x != 0) : x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const T *in, int kind, uint32_t n,
+                                                             T *tsum) {
+    const size_t b = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (b + i < n) s += scan_in(in, kind, b + i);
+    T tot;
+    block_excl_scan<T>(s, tot);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(T *tsum, uint32_t nt, T *out,
+                                                            uint32_t n) {
+    T carry = 0;
+    for (uint32_t b = 0; b < nt; b += kScanThreads) {
+        uint32_t i = b + threadIdx.x;
+        T x = i < nt ? tsum[i] : (T)0;
+        T tot;
+        T e = block_excl_scan<T>(x, tot);
+        if (i < nt) tsum[i] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(const T *in, int kind, uint32_t n,
+                                                             const T *tsum, T *out) {
+    const size_t b = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    T x[kScanItems];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        x[i] = (b + i < n) ? scan_in(in, kind, b + i) : (T)0;
+        s += x[i];
+    }
+    T tot;
+    T e = block_excl_scan<T>(s, tot) + tsum[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (b + i < n) out[b + i] = e;
+        e += x[i];
+    }
+}
+
+template <typename T>
+static void scan_impl(hipStream_t s, const T *in, int kind, uint32_t n, T *out, T *tmp) {
+    const uint32_t nt = (uint32_t)(((size_t)n + kScanTile - 1) / kScanTile);
+    if (n == 0) {
+        hipLaunchKernelGGL((k_scan_sums<T>), dim3(1), dim3(kScanThreads), 0, s, tmp, 0u, out, 0u);
+        return;
+    }
+    hipLaunchKernelGGL((k_scan_tiles<T>), dim3(nt), dim3(kScanThreads), 0, s, in, kind, n, tmp);
+    hipLaunchKernelGGL((k_scan_sums<T>), dim3(1), dim3(kScanThreads), 0, s, tmp, nt, out, n);
+    hipLaunchKernelGGL((k_scan_final<T>), dim3(nt), dim3(kScanThreads), 0, s, in, kind, n, tmp,
+                       out);
+}
+void launch_scan_u32(hipStream_t s, const uint32_t *in, int kind, uint32_t n, uint32_t *out,
+                     uint32_t *tmp) {
+    scan_impl<uint32_t>(s, in, kind, n, out, tmp);
+}
+void launch_scan_u64(hipStream_t s, const unsigned long long *in, uint32_t n,
+                     unsigned long long *out, unsigned long long *tmp) {
+    scan_impl<unsigned long long>(s, in, 0, n, out, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// aggregation (aggregation.cu:387-462), canonical form:
+//   new_id(c) = number of non-empty c' < c (scan of comm_sz != 0);
+//   row of new_id(c) = distinct neighbour communities d of c's members,
+//   W_cd exact then rounded once, in ascending d (== ascending new_id(d));
+//   communities without entries get an empty row.
+// ---------------------------------------------------------------------------
+
+// comm_deg[c] = sum of member degrees (setCommData, aggregation.cu:21-36)
+__global__ void k_agg_prep(DevView g, AggView a) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < g.n) {
+        uint32_t d = g.row[v + 1] - g.row[v];
+        if (d) atomicAdd(&a.comm_deg[g.comm[v]], d);
+    }
+}
+void launch_agg_prep(hipStream_t s, const DevView &g, AggView a) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_agg_prep, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a);
+}
+
+// members grouped by community (vertByCommOrder, aggregation.cu:38-45); the
+// order inside a community is not observable in the canonical output
+__global__ void k_agg_members(DevView g, AggView a) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < g.n) {
+        uint32_t c = g.comm[v];
+        uint32_t r = atomicAdd(&a.mcursor[c], 1u);
+        a.members[a.mstart[c] + r] = v;
+    }
+}
+void launch_agg_members(hipStream_t s, const DevView &g, AggView a) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_agg_members, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a);
+}
+
+__device__ __forceinline__ int agg_class(uint32_t cdeg) {
+    if (cdeg <= kAggSmallMax) return 0;
+    if (cdeg <= kAggMidMax) return 1;
+    return 2;
+}
+
+__global__ __launch_bounds__(256) void k_agg_class_count(DevView g, AggView a, uint32_t *counts) {
+    __shared__ uint32_t hc[kNumAggClasses];
+    if (threadIdx.x < kNumAggClasses) hc[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < g.n; c += gridDim.x * 256)
+        if (g.comm_sz[c]) atomicAdd(&hc[agg_class(a.comm_deg[c])], 1u);
+    __syncthreads();
+    if (threadIdx.x < kNumAggClasses && hc[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], hc[threadIdx.x]);
+}
+__global__ __launch_bounds__(256) void k_agg_class_scatter(DevView g, AggView a, uint32_t *cursor,
+                                                           uint32_t *order) {
+    __shared__ uint32_t hc[kNumAggClasses], base[kNumAggClasses];
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (threadIdx.x < kNumAggClasses) hc[threadIdx.x] = 0;
+    __syncthreads();
+    int k = -1;
+    uint32_t r = 0;
+    if (c < g.n && g.comm_sz[c]) {
+        k = agg_class(a.comm_deg[c]);
+        r = atomicAdd(&hc[k], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumAggClasses && hc[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hc[threadIdx.x]);
+    __syncthreads();
+    if (k >= 0) order[base[k] + r] = c;
+}
+void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_agg_class_count, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, g, a,
+                       counts);
+}
+void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uint32_t *cursor,
+                                 uint32_t *order) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_agg_class_scatter, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a,
+                       cursor, order);
+}
+
+__global__ void k_agg_caps(AggView a, const uint32_t *comms, uint32_t count,
+                           unsigned long long *caps) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        unsigned long long d = a.comm_deg[comms[i]];
+        unsigned long long c = 1;
+        while (c < 2 * d) c <<= 1;
+        caps[i] = c;
+    }
+}
+void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+                     unsigned long long *caps) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_agg_caps, dim3((count + 255) / 256), dim3(256), 0, s, a, comms, count,
+                       caps);
+}
+
+// Bitonic sort of n2 (pow2) (key, slot) pairs in LDS, ascending by key.
+template <int P>
+__device__ void lds_bitonic(uint32_t *k, uint32_t *sl, uint32_t n2) {
+    for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += P) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    bool up = (i & kk) == 0;
+                    uint32_t a = k[i], b = k[ixj];
+                    if ((a > b) == up) {
+                        k[i] = b;
+                        k[ixj] = a;
+                        uint32_t t = sl[i];
+                        sl[i] = sl[ixj];
+                        sl[ixj] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// LDS merge: one workgroup of P threads per community; waves walk members,
+// lanes walk a member's row.  Output: sorted (d, W) at tkey/tval[epos[c]..],
+// distinct count at cnt[new_id[c]].
+template <int P, int CAP>
+__global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const uint32_t *comms,
+                                                     uint32_t count) {
+    constexpr int LC = Log2<CAP>::value;
+    constexpr int NW = P / 64;
+    __shared__ uint32_t key[CAP];
+    __shared__ unsigned long long val[CAP];
+    __shared__ uint32_t skeys[CAP / 2];
+    __shared__ uint32_t sslot[CAP / 2];
+    __shared__ uint32_t ndist;
+    const uint32_t c = comms[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < CAP; i += P) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    if (threadIdx.x == 0) ndist = 0;
+    __syncthreads();
+    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t mi = wv; mi < msz; mi += NW) {
+        const uint32_t v = a.members[m0 + mi];
+        const uint32_t b = g.row[v], e = g.row[v + 1];
+        for (uint32_t j = b + lane; j < e; j += 64) {
+            uint32_t dd = g.comm[g.col[j]];
+            unsigned long long wf = f2fix(g.w[j], g.scale);
+            uint32_t slot = hslot<LC>(dd);
+            for (int probe = 0; probe < CAP; ++probe) {
+                uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
+                if (prev == kEmpty) {
+                    uint32_t idx = atomicAdd(&ndist, 1u);
+                    skeys[idx] = dd;
+                    sslot[idx] = slot;
+                    break;
+                }
+                if (prev == dd) break;
+                slot = (slot + 1) & (CAP - 1);
+            }
+            atomicAdd(&val[slot], wf);
+        }
+    }
+    __syncthreads();
+    const uint32_t nd = ndist;
+    uint32_t n2 = 1;
+    while (n2 < nd) n2 <<= 1;
+    for (uint32_t i = nd + threadIdx.x; i < n2; i += P) {
+        skeys[i] = kEmpty;
+        sslot[i] = 0;
+    }
+    __syncthreads();
+    lds_bitonic<P>(skeys, sslot, n2);
+    const uint32_t out = a.epos[c];
+    for (uint32_t i = threadIdx.x; i < nd; i += P) {
+        a.tkey[out + i] = skeys[i];
+        a.tval[out + i] = val[sslot[i]];
+    }
+    if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
+}
+
+// global-hash merge for large communities, then an in-workgroup stable LSD
+// radix sort of the distinct keys (8-bit digits) with scratch skey/sval.
+constexpr int kRadixThreads = 1024;
+__global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, AggView a,
+                                                                    const uint32_t *comms,
+                                                                    uint32_t count, int passes) {
+    constexpr int P = kRadixThreads;
+    constexpr int NW = P / 64;
+    __shared__ uint32_t ndist;
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t binbase[256];
+    __shared__ uint32_t ctot[256];
+    __shared__ uint32_t wcnt[NW][256];
+    const uint32_t ci = blockIdx.x;
+    const uint32_t c = comms[ci];
+    const unsigned long long off = a.goff[ci];
+    const uint32_t cap = (uint32_t)(a.goff[ci + 1] - off);
+    const int lcap = 31 - __clz(cap);
+    uint32_t *key = a.gkey + off;
+    unsigned long long *val = a.gval + off;
+    const uint32_t out = a.epos[c];
+    uint32_t *ok = a.tkey + out;
+    unsigned long long *ov = a.tval + out;
+    if (threadIdx.x == 0) ndist = 0;
+    __syncthreads();
+    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t mi = wv; mi < msz; mi += NW) {
+        const uint32_t v = a.members[m0 + mi];
+        const uint32_t b = g.row[v], e = g.row[v + 1];
+        for (uint32_t j = b + lane; j < e; j += 64) {
+            uint32_t dd = g.comm[g.col[j]];
+            unsigned long long wf = f2fix(g.w[j], g.scale);
+            uint32_t slot = hslot_dyn(dd, lcap);
+            uint32_t probe = 0;
+            for (; probe < cap; ++probe) {
+                uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
+                if (prev == kEmpty) {
+                    uint32_t idx = atomicAdd(&ndist, 1u);
+                    ok[idx] = dd;
+                    ov[idx] = slot;
+                    break;
+                }
+                if (prev == dd) break;
+                slot = (slot + 1) & (cap - 1);
+            }
+            if (probe == cap) {
+                atomicOr(a.err, 1u);
+                continue;
+            }
+            atomicAdd(&val[slot], wf);
+        }
+    }
+    __syncthreads();
+    const uint32_t nd = ndist;
+    // gather sums and clean the arena
+    for (uint32_t i = threadIdx.x; i < nd; i += P) {
+        uint32_t slot = (uint32_t)ov[i];
+        ov[i] = val[slot];
+        val[slot] = 0ull;
+        key[slot] = kEmpty;
+    }
+    __syncthreads();
+    uint32_t *kin = ok, *kout = a.skey + out;
+    unsigned long long *vin = ov, *vout = a.sval + out;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int shift = 8 * ps;
+        for (int i = threadIdx.x; i < 256; i += P) hist[i] = 0;
+        for (int i = threadIdx.x; i < NW * 256; i += P) (&wcnt[0][0])[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nd; i += P) atomicAdd(&hist[(kin[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int b = 0; b < 256; ++b) {
+                binbase[b] = run;
+                run += hist[b];
+            }
+        }
+        __syncthreads();
+        for (uint32_t ch = 0; ch < nd; ch += P) {
+            const uint32_t i = ch + threadIdx.x;
+            const bool has = i < nd;
+            const uint32_t kk = has ? kin[i] : 0u;
+            const uint32_t dig = (kk >> shift) & 255u;
+            unsigned long long eq = __ballot(has);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                unsigned long long bb = __ballot(has && ((dig >> bit) & 1u));
+                eq &= ((dig >> bit) & 1u) ? bb : ~bb;
+            }
+            const uint32_t rank = __popcll(eq & ((1ull << lane) - 1ull));
+            if (has && rank == 0) wcnt[wv][dig] = __popcll(eq);
+            __syncthreads();
+            if (threadIdx.x < 256) {
+                uint32_t run = 0;
+                for (int w2 = 0; w2 < NW; ++w2) {
+                    uint32_t x = wcnt[w2][threadIdx.x];
+                    wcnt[w2][threadIdx.x] = run;
+                    run += x;
+                }
+                ctot[threadIdx.x] = run;
+            }
+            __syncthreads();
+            if (has) {
+                uint32_t pos = binbase[dig] + wcnt[wv][dig] + rank;
+                kout[pos] = kk;
+                vout[pos] = vin[i];
+            }
+            __syncthreads();
+            if (threadIdx.x < 256) {
+                binbase[threadIdx.x] += ctot[threadIdx.x];
+                for (int w2 = 0; w2 < NW; ++w2) wcnt[w2][threadIdx.x] = 0;
+            }
+            __syncthreads();
+        }
+        uint32_t *tk = kin;
+        kin = kout;
+        kout = tk;
+        unsigned long long *tv = vin;
+        vin = vout;
+        vout = tv;
+    }
+    if (kin != ok) {
+        for (uint32_t i = threadIdx.x; i < nd; i += P) {
+            ok[i] = kin[i];
+            ov[i] = vin[i];
+        }
+    }
+    if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
+}
+
+void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
+                      uint32_t count, int radix_passes) {
+    if (!count) return;
+    if (cls == 0)
+        hipLaunchKernelGGL((k_agg_merge_lds<64, 256>), dim3(count), dim3(64), 0, s, g, a, comms,
+                           count);
+    else if (cls == 1)
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 4096>), dim3(count), dim3(256), 0, s, g, a, comms,
+                           count);
+    else
+        hipLaunchKernelGGL(k_agg_merge_global, dim3(count), dim3(kRadixThreads), 0, s, g, a, comms,
+                           count, radix_passes);
+}
+
+// compressEdges (aggregation.cu:359-378): one wave per community copies its
+// sorted distinct list into the new CSR with renumbered ids and FP32 weights
+__global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const uint32_t *comms,
+                                                  uint32_t count, uint32_t *row2, uint32_t *col2,
+                                                  float *w2) {
+    const uint32_t ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (ci >= count) return;
+    const uint32_t c = comms[ci];
+    const uint32_t nid = a.new_id[c];
+    const uint32_t o = row2[nid], nd = row2[nid + 1] - o;
+    const uint32_t src = a.epos[c];
+    for (uint32_t i = lane; i < nd; i += 64) {
+        col2[o + i] = a.new_id[a.tkey[src + i]];
+        w2[o + i] = fix2f(a.tval[src + i], g.S);
+    }
+}
+void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
+                     uint32_t count, uint32_t *row2, uint32_t *col2, float *w2) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_agg_emit, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms, count,
+                       row2, col2, w2);
+}
+
+// graphVertToComm + graphVertToNewComm (modularity.cu:51-56,
+// aggregation.cu:380-385) fused: init_comm <- new_id[comm[init_comm]]
+__global__ void k_agg_compose(const uint32_t *comm, const uint32_t *new_id, uint32_t *init_comm,
+                              uint32_t n0) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n0) init_comm[i] = new_id[comm[init_comm[i]]];
+}
+void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
+                        uint32_t *init_comm, uint32_t n0) {
+    if (!n0) return;
+    hipLaunchKernelGGL(k_agg_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, comm, new_id,
+                       init_comm, n0);
+}
+
+}  // namespace lv

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bit-exact for every integer and FP32 quantity: k_v, m, community assignment,
+community sizes and exact community weights after every bucket step, the
+FP32 modularity after every sweep, renumbering and the contracted CSR after
+every aggregation, and the final printed modularity and assignment.  The
+final modularity is also checked against an FP64 recompute on the original
+graph (|dQ| <= 1e-6).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+Q64_TOL = 1e-6
+
+
+def _graphs(lv, orc, text):
+    out = {}
+    out["bcsstk16"] = orc.parse_mtx(text=text)
+    out["fem"] = lv.gen_fem(814, 25, 9, 1)
+    out["chung_lu_20k"] = lv.gen_chung_lu(20000, 9.4, 2.5, 420, 100, 20.0, 5)
+    out["rmat12"] = lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 7)
+    out["rmat14_unit"] = lv.gen_rmat(14, 16, 0.57, 0.19, 0.19, 1.0, 1.0, 9)
+    out["sbm_5k"] = lv.gen_sbm(5000, 20, 12.0, 10.0, 11)
+    return out
+
+
+@pytest.fixture(scope="module")
+def graphs(lv, orc, bcsstk16_text):
+    if lv.device_count() < 1:
+        pytest.fail("no GPU visible: GPU tests must run on a GPU box")
+    return _graphs(lv, orc, bcsstk16_text)
+
+
+def _u32(a):
+    return np.asarray(a).astype(np.uint32)
+
+
+def _cmp_state(g, o, where):
+    k, comm, sz, a = g.state()
+    assert np.array_equal(comm, o.comm()), f"comm differs {where}"
+    assert np.array_equal(sz, o.comm_sz()), f"comm_sz differs {where}"
+    assert np.array_equal(a, o.a_fixed()), f"a_c differs {where}"
+    assert np.array_equal(k.view(np.uint32), o.k().view(np.uint32)), f"k differs {where}"
+
+
+def _cmp_csr(g, o, where):
+    row, col, w = g.csr()
+    orow, ocol, ow = o.csr()
+    assert np.array_equal(row.astype(np.uint64), orow), f"row_ptr differs {where}"
+    assert np.array_equal(col, ocol), f"col differs {where}"
+    assert np.array_equal(w.view(np.uint32), ow.view(np.uint32)), f"weights differ {where}"
+
+
+@pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
+                                  "sbm_5k"])
+def test_stepwise_first_pass(lv, orc, graphs, name):
+    row, col, w = graphs[name]
+    print(f"[case] stepwise {name} n={row.size - 1} ne={col.size}", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    with lv.Graph(_u32(row), col, w) as g:
+        s, m = g.scale()
+        assert s == o.scale
+        q0 = g.pass_init()
+        oq0 = o.pass_init()
+        assert g.scale()[1] == o.m
+        assert q0.view(np.uint32) == oq0.view(np.uint32), (q0, oq0)
+        _cmp_state(g, o, "after pass init")
+        for it in range(3):
+            for b in range(1, 8):
+                mv = g.step(b)
+                omv = o.step(b)
+                assert mv == omv, f"moved count differs it={it} b={b}"
+                _cmp_state(g, o, f"it={it} bucket={b}")
+            q = g.modularity()
+            oq = o.modularity()
+            assert q.view(np.uint32) == oq.view(np.uint32), (it, q, oq)
+
+
+@pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
+                                  "sbm_5k"])
+def test_passes_and_aggregation(lv, orc, graphs, name):
+    row, col, w = graphs[name]
+    print(f"[case] passes {name}", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    gain = np.float32(1e-6)
+    with lv.Graph(_u32(row), col, w) as g:
+        for p in range(20):
+            imp, qp = g.modularity_optimization(gain)
+            oimp, oqp, _ = o.modularity_optimization(gain)
+            assert imp == oimp, f"pass {p}: improved differs"
+            _cmp_state(g, o, f"end of pass {p}")
+            if not imp:
+                assert qp.view(np.uint32) == oqp.view(np.uint32)
+                break
+            g.aggregation()
+            o.aggregation()
+            _cmp_csr(g, o, f"after aggregation {p}")
+            assert np.array_equal(g.assignment(), o.init_comm()), f"assignment pass {p}"
+
+
+@pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
+                                  "sbm_5k"])
+def test_full_run(lv, orc, graphs, name):
+    row, col, w = graphs[name]
+    print(f"[case] run {name}", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    ores = o.run(1e-6)
+    with lv.Graph(_u32(row), col, w) as g:
+        res = g.run(1e-6)
+        assign = g.assignment()
+        n_final = g.size()[0]
+    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)
+    assert res.passes == ores.passes and res.n_final == ores.n_final == n_final
+    assert res.total_iters == ores.total_iters
+    assert res.sum_edges == ores.sum_edges
+    assert np.array_equal(assign, o.init_comm())
+    q64 = orc.q64(row.astype(np.uint64), col, w, assign)
+    assert abs(float(res.q_final) - q64) <= Q64_TOL, (res.q_final, q64)
+
+
+def test_determinism(lv, graphs):
+    row, col, w = graphs["rmat12"]
+    outs = []
+    for _ in range(3):
+        with lv.Graph(_u32(row), col, w) as g:
+            r = g.run(1e-6)
+            outs.append((np.float32(r.q_final).view(np.uint32), g.assignment().tobytes()))
+    assert all(x == outs[0] for x in outs)
+
+
+def test_edge_cases(lv, orc):
+    # two triangles joined by an edge; isolated vertices; self-loops; star
+    cases = {
+        "two_triangles": "6 6 7\n1 2 1\n2 3 1\n1 3 1\n4 5 1\n5 6 1\n4 6 1\n3 4 1\n",
+        "isolated_and_loops": "8 8 6\n1 1 2\n1 2 1\n2 3 1\n3 1 1\n5 5 3\n6 7 0.5\n",
+        "star": "40 40 39\n" + "".join(f"1 {i} 1\n" for i in range(2, 41)),
+    }
+    # high-degree vertices through duplicate edges (buckets 6 and 7, heavy class)
+    rng = np.random.default_rng(5)
+    lines = []
+    for hub, deg in [(1, 300), (2, 1500), (3, 5000)]:
+        for _ in range(deg):
+            lines.append(f"{hub} {int(rng.integers(4, 60))} {1 + int(rng.integers(0, 3))}\n")
+    cases["dup_hubs"] = f"60 60 {len(lines)}\n" + "".join(lines)
+    for name, text in cases.items():
+        print(f"[case] edge {name}", flush=True)
+        row, col, w = orc.parse_mtx(text=text)
+        o = orc.Oracle(row, col, w)
+        ores = o.run(1e-6)
+        with lv.Graph(_u32(row), col, w) as g:
+            res = g.run(1e-6)
+            assign = g.assignment()
+        assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32), name
+        assert np.array_equal(assign, o.init_comm()), name
