@@ -106,6 +106,10 @@ void lv_host_csr_free(lv_host_csr *csr);
 int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t *col,
                     const float *w, lv_graph **out);
 int lv_graph_destroy(lv_graph *g);
+/* Restore the graph as uploaded (device-to-device copy) and the identity
+ * assignment, so the same resident input can be processed again.  No
+ * counterpart in the reference, which runs one graph per process. */
+int lv_graph_reset(lv_graph *g);
 
 /* ---- the reference operator API ------------------------------------------ */
 /* One Louvain pass of local moves.  *improved is the reference's return value

@@ -58,7 +58,7 @@ class KernelStats(ctypes.Structure):
 # every symbol declared in include/louvain_hip.h
 EXPORTS = [
     "lv_status_string", "lv_device_count", "lv_mtx_read", "lv_host_csr_free",
-    "lv_graph_create", "lv_graph_destroy", "lv_modularity_optimization", "lv_aggregation",
+    "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_modularity_optimization", "lv_aggregation",
     "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
@@ -91,6 +91,7 @@ def lib():
     L.lv_graph_create.argtypes = [c.c_int, c.c_uint32, _u32p, _u32p, _f32p,
                                   c.POINTER(c.c_void_p)]
     L.lv_graph_destroy.argtypes = [c.c_void_p]
+    L.lv_graph_reset.argtypes = [c.c_void_p]
     L.lv_modularity_optimization.argtypes = [c.c_void_p, c.c_float, c.POINTER(c.c_int),
                                              c.POINTER(c.c_float)]
     L.lv_aggregation.argtypes = [c.c_void_p]
@@ -205,6 +206,9 @@ class Graph:
 
     def __exit__(self, *a):
         self.close()
+
+    def reset(self):
+        _chk(self.L.lv_graph_reset(self.h), "lv_graph_reset")
 
     def size(self):
         n, ne, n0 = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32()

@@ -71,6 +71,9 @@ struct lv_graph {
     float *w = nullptr;
     uint32_t *row2 = nullptr, *col2 = nullptr;
     float *w2 = nullptr;
+    // pristine copy of the uploaded graph (lv_graph_reset)
+    uint32_t *row0 = nullptr, *col0 = nullptr;
+    float *w0 = nullptr;
     // pass state
     float *k = nullptr;
     uint32_t *comm = nullptr, *comm_sz = nullptr, *new_comm = nullptr;
@@ -529,6 +532,9 @@ void free_all(lv_graph *g) {
     dfree(g->row2);
     dfree(g->col2);
     dfree(g->w2);
+    dfree(g->row0);
+    dfree(g->col0);
+    dfree(g->w0);
     dfree(g->k);
     dfree(g->comm);
     dfree(g->comm_sz);
@@ -639,6 +645,7 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     int e = 0;
     e |= dmalloc(&g->row, N) | dmalloc(&g->col, E) | dmalloc(&g->w, E);
     e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E) | dmalloc(&g->w2, E);
+    e |= dmalloc(&g->row0, N) | dmalloc(&g->col0, E) | dmalloc(&g->w0, E);
     e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
     e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
     e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
@@ -671,6 +678,17 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
         delete g;
         return LV_ERR_HIP;
     }
+    if (hipMemcpyAsync(g->row0, g->row, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess ||
+        (E && hipMemcpyAsync(g->col0, g->col, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
+                  hipSuccess) ||
+        (E && hipMemcpyAsync(g->w0, g->w, E * sizeof(float), hipMemcpyDeviceToDevice, s) !=
+                  hipSuccess)) {
+        (void)hipGetLastError();
+        free_all(g);
+        delete g;
+        return LV_ERR_HIP;
+    }
     lv::launch_iota_u32(s, g->init_comm, n);  // setInitComm (utils.cu:66-68)
     if (hipStreamSynchronize(s) != hipSuccess) {
         (void)hipGetLastError();
@@ -679,6 +697,24 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
         return LV_ERR_HIP;
     }
     *out = g;
+    return LV_OK;
+}
+
+int lv_graph_reset(lv_graph *g) {
+    if (!g) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    const size_t N = (size_t)g->n0 + 1, E = (size_t)g->ne0;
+    LV_HIP(hipMemcpyAsync(g->row, g->row0, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    if (E) {
+        LV_HIP(hipMemcpyAsync(g->col, g->col0, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        LV_HIP(hipMemcpyAsync(g->w, g->w0, E * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    lv::launch_iota_u32(s, g->init_comm, g->n0);
+    LV_HIP(hipStreamSynchronize(s));
+    g->n = g->n0;
+    g->ne = g->ne0;
+    g->pass_ready = false;
     return LV_OK;
 }
 
