@@ -61,14 +61,22 @@ struct DevView {
     float m;        // all_w
 };
 
-// Global-hash scratch for the heavy move class.
-struct HeavyScratch {
-    uint32_t *key;               // arena slots
-    unsigned long long *val;     // arena slots
-    const unsigned long long *slot_off;  // per heavy vertex (index in the class list)
-    uint32_t *dlist;             // ne: distinct-slot lists at [row[v], row[v+1])
-    uint32_t *eslot;             // ne: slot of each entry
-    uint32_t *err;               // overflow flag
+// Heavy-vertex state for the chunked kernels (lv_heavy.hip).
+constexpr uint32_t kHeavyChunk = 4096;  // entries per workgroup; multiple of 128
+struct HeavySplit {
+    const uint32_t *verts;               // class-8 vertices
+    uint32_t nh;                         // their count
+    uint32_t nchunks;                    // total chunks = coff[nh]
+    const uint32_t *coff;                // nh+1 chunk offsets
+    const unsigned long long *slot_off;  // nh+1 table offsets in the arena
+    uint32_t *key;                       // arena
+    unsigned long long *val;             // arena
+    uint32_t *dlist;                     // ne: distinct slots at [row[v], row[v+1])
+    uint32_t *eslot;                     // ne: slot of each entry
+    unsigned long long *hcnt;            // nh distinct counts, followed by
+    unsigned long long *hcew;            // nh comm_e_w sums and
+    unsigned long long *hlane;           // nh*128 packed lane maxima (one allocation)
+    uint32_t *err;
 };
 
 // ---- launchers (lv_kernels.hip) ---------------------------------------------
@@ -89,12 +97,20 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
                int group, int masked, int init_state, unsigned long long *acc);
 void launch_sumsq(hipStream_t s, const DevView &g, unsigned long long *limbs /*4*/);
 
+// classes 1-7 (class 8 goes through launch_heavy_move)
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
-                       uint32_t count, const HeavyScratch &hs);
+                       uint32_t count);
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved);
 void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts,
                        uint32_t count, unsigned long long *caps);
+
+// ---- launchers (lv_heavy.hip) -----------------------------------------------
+void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *verts,
+                         uint32_t count, uint32_t *nchunks);
+void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h);
+void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float *stage,
+                     int masked, int init_state, unsigned long long *acc);
 
 // exclusive scan with total at out[n]; input functor chosen by `kind`:
 //   0: in[i]   1: in[i] != 0

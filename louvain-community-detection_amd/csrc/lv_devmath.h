@@ -1,0 +1,121 @@
+// lv_devmath.h -- device arithmetic shared by the kernel translation units:
+// exact fixed point <-> FP32, the reference's gain expression and arg-max
+// order, hashing, wave reductions and the 128-lane winner tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lv_device.h"
+
+namespace lv {
+
+// ---------------------------------------------------------------------------
+// arithmetic helpers
+// ---------------------------------------------------------------------------
+
+// Correctly rounded (nearest-even) u64 -> f32.
+__device__ __forceinline__ float u64_to_f32_rn(unsigned long long u) {
+    if (u < (1ull << 24)) return (float)(uint32_t)u;
+    int lz = __clzll(u);
+    int sh = 40 - lz;  // keep the top 24 bits
+    unsigned long long mant = u >> sh;
+    unsigned long long rem = u & ((1ull << sh) - 1ull);
+    unsigned long long half = 1ull << (sh - 1);
+    if (rem > half || (rem == half && (mant & 1ull))) mant++;
+    return ldexpf((float)(uint32_t)mant, sh);
+}
+
+// exact fixed point (x * 2^-S) rounded once to f32
+__device__ __forceinline__ float fix2f(unsigned long long x, int S) {
+    return ldexpf(u64_to_f32_rn(x), -S);
+}
+
+// f32 (a multiple of 2^-S, >= 0) -> exact fixed point
+__device__ __forceinline__ unsigned long long f2fix(float x, float scale) {
+    return (unsigned long long)(x * scale);
+}
+
+// Gain of moving to community C (modularity.cu:149-152): FP32, this
+// association, no contraction.
+__device__ __forceinline__ float gain_of(float hw, float m, float kv, float acv, float ac) {
+    float t1 = hw / m;
+    float t3 = (acv - kv) - ac;
+    float t4 = kv * t3;
+    float t6 = (2.0f * m) * m;
+    return t1 + t4 / t6;
+}
+
+// reduction order of the reference's arg-max (modularity.cu:169-179, 363)
+__device__ __forceinline__ bool better(float ga, uint32_t ca, float gb, uint32_t cb) {
+    return ga > gb || (ga == gb && ca < cb);
+}
+
+template <int LOG2CAP>
+__device__ __forceinline__ uint32_t hslot(uint32_t key) {
+    return (key * 0x9E3779B1u) >> (32 - LOG2CAP);
+}
+
+__device__ __forceinline__ uint32_t hslot_dyn(uint32_t key, int log2cap) {
+    return (key * 0x9E3779B1u) >> (32 - log2cap);
+}
+
+template <int N>
+struct Log2 {
+    static constexpr int value = 1 + Log2<N / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
+// wave-level sum of u64
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// The 128-virtual-lane arg-max tree of computeMoveBlock/Global
+// (modularity.cu:260-275) on one wave: lane p holds virtual lanes p and
+// p+64.  Returns the winner (gain, comm, virtual lane) in lane 0.
+__device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_t c1, float &bg,
+                                        uint32_t &bc, uint32_t &bp) {
+    const uint32_t p = __lane_id();
+    bg = g0;
+    bc = c0;
+    bp = p;
+    if (better(g1, c1, g0, c0)) {
+        bg = g1;
+        bc = c1;
+        bp = p + 64;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float og = __shfl_down(bg, o, 64);
+        uint32_t oc = __shfl_down(bc, o, 64);
+        uint32_t op = __shfl_down(bp, o, 64);
+        if (better(og, oc, bg, bc)) {
+            bg = og;
+            bc = oc;
+            bp = op;
+        }
+    }
+}
+
+// last entry visited by virtual lane wl of 128 on a row of d entries
+__device__ __forceinline__ uint32_t last_entry(uint32_t wl, uint32_t d) {
+    return wl + 128u * ((d - 1u - wl) / 128u);
+}
+
+// final decision for buckets 6-7
+__device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, uint32_t v, uint32_t base,
+                                                    uint32_t d, uint32_t cv, float bg,
+                                                    uint32_t bp, unsigned long long cew) {
+    if (bg == -INFINITY || bp >= d) return cv;
+    uint32_t dest = g.comm[g.col[base + last_entry(bp, d)]];
+    float cewf = fix2f(cew, g.S);
+    if (bg - (cewf / g.m) > 0.0f) return dest;
+    return cv;
+}
+
+}  // namespace lv

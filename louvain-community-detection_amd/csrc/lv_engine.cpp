@@ -84,11 +84,15 @@ struct lv_graph {
     uint32_t cls_count[kNumClasses] = {};
     uint32_t cls_off[kNumClasses + 1] = {};
     ull cls_deg[kNumClasses] = {};
-    // heavy move scratch
+    // heavy move scratch (class 8)
     ull *heavy_caps = nullptr, *heavy_off = nullptr;
+    uint32_t *heavy_nch = nullptr, *heavy_coff = nullptr;
+    uint32_t heavy_chunks = 0;
     uint32_t *hkey = nullptr;
     ull *hval = nullptr;
     size_t harena = 0;
+    ull *hscr = nullptr;
+    size_t hscr_cap = 0;
     uint32_t *dlist = nullptr, *eslot = nullptr;
     // aggregation scratch
     uint32_t *new_id = nullptr, *comm_deg = nullptr, *epos = nullptr, *mstart = nullptr;
@@ -234,16 +238,24 @@ int classify(lv_graph *g) {
         ProfScope ps(g, LV_K_CLASSIFY, 12.0 * g->n);
         lv::launch_classify_scatter(g->stream, g->row, g->n, g->dc->cursor, g->order);
     }
-    // heavy-class table arena: pow2 >= 2*deg slots per vertex
+    // heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk
+    // offsets of the split kernels, per-vertex scratch
     const uint32_t nh = g->cls_count[8];
+    g->heavy_chunks = 0;
     if (nh) {
         const uint32_t *hv = g->order + g->cls_off[8];
         lv::launch_heavy_caps(g->stream, g->row, hv, nh, g->heavy_caps);
         lv::launch_scan_u64(g->stream, g->heavy_caps, nh, g->heavy_off, g->scan_tmp64);
+        lv::launch_heavy_chunks(g->stream, g->row, hv, nh, g->heavy_nch);
+        lv::launch_scan_u32(g->stream, g->heavy_nch, 0, nh, g->heavy_coff, g->scan_tmp32);
         ull total = 0;
+        uint32_t chunks = 0;
         LV_HIP(hipMemcpyAsync(&total, g->heavy_off + nh, sizeof(ull), hipMemcpyDeviceToHost,
                               g->stream));
+        LV_HIP(hipMemcpyAsync(&chunks, g->heavy_coff + nh, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, g->stream));
         LV_HIP(hipStreamSynchronize(g->stream));
+        g->heavy_chunks = chunks;
         if (total > g->harena) {
             dfree(g->hkey);
             dfree(g->hval);
@@ -253,8 +265,32 @@ int classify(lv_graph *g) {
             lv::launch_fill_u32(g->stream, g->hkey, lv::kEmpty, cap);
             lv::launch_fill_u64(g->stream, g->hval, 0ull, cap);
         }
+        if ((size_t)nh * 130 > g->hscr_cap) {
+            dfree(g->hscr);
+            if (dmalloc(&g->hscr, (size_t)nh * 130)) return LV_ERR_NOMEM;
+            g->hscr_cap = (size_t)nh * 130;
+        }
     }
     return LV_OK;
+}
+
+lv::HeavySplit heavy_view(lv_graph *g) {
+    lv::HeavySplit h;
+    const uint32_t nh = g->cls_count[8];
+    h.verts = g->order + g->cls_off[8];
+    h.nh = nh;
+    h.nchunks = g->heavy_chunks;
+    h.coff = g->heavy_coff;
+    h.slot_off = g->heavy_off;
+    h.key = g->hkey;
+    h.val = g->hval;
+    h.dlist = g->dlist;
+    h.eslot = g->eslot;
+    h.hcnt = g->hscr;
+    h.hcew = g->hscr + nh;
+    h.hlane = g->hscr + 2 * (size_t)nh;
+    h.err = &g->dc->err;
+    return h;
 }
 
 // groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
@@ -263,13 +299,18 @@ int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
     const struct {
         int G, c0, c1;
-    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 8}};
+    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 7}};
     for (const auto &q : grp) {
         uint32_t a = g->cls_off[q.c0], b = g->cls_off[q.c1 + 1];
         double ent = 0;
         for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
         lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, &g->dc->acc);
+    }
+    if (g->cls_count[8]) {  // heavy rows: staged, LDS-streamed pinned sums
+        ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[8]);
+        lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
+                            &g->dc->acc);
     }
     return LV_OK;
 }
@@ -317,13 +358,6 @@ int pass_init(lv_graph *g, float *q0) {
 // one bucket step: decide from the snapshot, then apply
 int step(lv_graph *g, int b) {
     lv::DevView v = view(g);
-    lv::HeavyScratch hs;
-    hs.key = g->hkey;
-    hs.val = g->hval;
-    hs.slot_off = g->heavy_off;
-    hs.dlist = g->dlist;
-    hs.eslot = g->eslot;
-    hs.err = &g->dc->err;
     const int c0 = b, c1 = (b == 7) ? 8 : b;
     static const int fam[kNumClasses] = {0,
                                          LV_K_MOVE_SMALL,
@@ -337,7 +371,10 @@ int step(lv_graph *g, int b) {
     for (int c = c0; c <= c1; ++c) {
         if (!g->cls_count[c]) continue;
         ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c]);
-        lv::launch_move_class(g->stream, v, c, g->order + g->cls_off[c], g->cls_count[c], hs);
+        if (c == 8)
+            lv::launch_heavy_move(g->stream, v, heavy_view(g));
+        else
+            lv::launch_move_class(g->stream, v, c, g->order + g->cls_off[c], g->cls_count[c]);
     }
     const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
     if (e > a) {
@@ -544,6 +581,9 @@ void free_all(lv_graph *g) {
     dfree(g->order);
     dfree(g->heavy_caps);
     dfree(g->heavy_off);
+    dfree(g->heavy_nch);
+    dfree(g->heavy_coff);
+    dfree(g->hscr);
     dfree(g->hkey);
     dfree(g->hval);
     dfree(g->dlist);
@@ -649,6 +689,7 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
     e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
     e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
+    e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
     e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
     e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
     e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);

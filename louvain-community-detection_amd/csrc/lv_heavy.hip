@@ -1,0 +1,319 @@
+// lv_heavy.hip -- heavy-tail vertices (bucket 7, deg > kBlockMax) spread over
+// many workgroups.
+//
+// A vertex with hundreds of thousands of entries (R-MAT hubs) would make one
+// workgroup the critical path of its step.  Its row is cut into chunks of
+// kHeavyChunk entries; every kernel below runs one workgroup per chunk, and
+// the vertex's state lives in global memory:
+//   H1 insert   exact u64 hash sums into the vertex's global table, distinct
+//               slot list, per-entry slot, comm_e_w
+//   H2 gain     one FP32 gain per distinct community (complete sums)
+//   H3 lanes    per virtual lane (128 lanes, entries l, l+128, ...) the first
+//               maximum (max gain, then smallest entry index) as one packed
+//               u64 combined with atomicMax
+//   H4 decide   one wave per vertex: 128-lane tree of computeMoveBlock/Global
+//               (modularity.cu:165-187), last-edge destination
+//   H5 clean    reset the touched table slots
+// The result is the same as one workgroup walking the whole row,
+// because every combination step is exact or order-free.
+//
+// The pinned FP32 sums (k_v, s_v) of these rows are sequential per lane by
+// definition (modularity.cu:34-43): one workgroup per vertex streams the row
+// through LDS in a lane-transposed layout so 32 summing lanes read LDS while
+// three loader waves fetch the next chunk.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lv_device.h"
+#include "lv_devmath.h"
+
+namespace lv {
+
+__device__ __forceinline__ uint32_t heavy_find(const uint32_t *coff, uint32_t nh, uint32_t b) {
+    uint32_t lo = 0, hi = nh;  // largest i with coff[i] <= b
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (coff[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t f32_ord(float x) {
+    uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float f32_unord(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+
+struct HeavyCtx {
+    uint32_t i, v, base, d, c;
+    unsigned long long off;
+    uint32_t cap;
+    int lcap;
+};
+
+__device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit &h) {
+    HeavyCtx x;
+    x.i = heavy_find(h.coff, h.nh, blockIdx.x);
+    x.c = blockIdx.x - h.coff[x.i];
+    x.v = h.verts[x.i];
+    x.base = g.row[x.v];
+    x.d = g.row[x.v + 1] - x.base;
+    x.off = h.slot_off[x.i];
+    x.cap = (uint32_t)(h.slot_off[x.i + 1] - x.off);
+    x.lcap = 31 - __clz(x.cap);
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
+    const HeavyCtx x = heavy_ctx(g, h);
+    uint32_t *key = h.key + x.off;
+    unsigned long long *val = h.val + x.off;
+    const uint32_t cv = g.comm[x.v];
+    const uint32_t j0 = x.c * kHeavyChunk;
+    const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    unsigned long long cew = 0;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+        const uint32_t e = g.col[x.base + j];
+        const uint32_t ce = g.comm[e];
+        const unsigned long long wf = f2fix(g.w[x.base + j], g.scale);
+        uint32_t slot = hslot_dyn(ce, x.lcap);
+        uint32_t probe = 0;
+        for (; probe < x.cap; ++probe) {
+            uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+            if (prev == kEmpty) {
+                unsigned long long idx = atomicAdd(&h.hcnt[x.i], 1ull);
+                h.dlist[x.base + idx] = slot;
+                break;
+            }
+            if (prev == ce) break;
+            slot = (slot + 1) & (x.cap - 1);
+        }
+        if (probe == x.cap) {  // cannot happen: cap >= 2*deg
+            atomicOr(h.err, 1u);
+            slot = 0;
+        }
+        atomicAdd(&val[slot], wf);
+        if (ce == cv && e != x.v) cew += wf;
+        h.eslot[x.base + j] = slot;
+    }
+    __shared__ unsigned long long part[4];
+    cew = wave_sum_u64(cew);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = part[0] + part[1] + part[2] + part[3];
+        if (s) atomicAdd(&h.hcew[x.i], s);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
+    const HeavyCtx x = heavy_ctx(g, h);
+    uint32_t *key = h.key + x.off;
+    unsigned long long *val = h.val + x.off;
+    const uint32_t nd = (uint32_t)h.hcnt[x.i];
+    const uint32_t i0 = x.c * kHeavyChunk;
+    if (i0 >= nd) return;
+    const uint32_t i1 = min(nd, i0 + kHeavyChunk);
+    const uint32_t cv = g.comm[x.v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[x.v];
+    const float acv = fix2f(g.A[cv], g.S);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const uint32_t slot = h.dlist[x.base + i];
+        const uint32_t c = key[slot];
+        float gg = -INFINITY;
+        bool cand = true;  // isLowerIdx (modularity.cu:85-93)
+        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        val[slot] = (unsigned long long)__float_as_uint(gg);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
+    const HeavyCtx x = heavy_ctx(g, h);
+    const unsigned long long *val = h.val + x.off;
+    const uint32_t j0 = x.c * kHeavyChunk;  // multiple of 128: thread t serves lane t % 128
+    const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    float tg = -INFINITY;
+    uint32_t tj = 0xffffffffu;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+        const float gg = __uint_as_float((uint32_t)val[h.eslot[x.base + j]]);
+        if (gg > tg) {  // entries visited in increasing order: first maximum
+            tg = gg;
+            tj = j;
+        }
+    }
+    __shared__ float sg[256];
+    __shared__ uint32_t sj[256];
+    sg[threadIdx.x] = tg;
+    sj[threadIdx.x] = tj;
+    __syncthreads();
+    if (threadIdx.x < 128) {
+        const float g2 = sg[threadIdx.x + 128];
+        const uint32_t j2 = sj[threadIdx.x + 128];
+        if (g2 > tg || (g2 == tg && j2 < tj)) {
+            tg = g2;
+            tj = j2;
+        }
+        if (tg != -INFINITY) {
+            const unsigned long long packed =
+                ((unsigned long long)f32_ord(tg) << 32) | (unsigned long long)(0xffffffffu - tj);
+            atomicMax(&h.hlane[(size_t)x.i * 128 + threadIdx.x], packed);
+        }
+    }
+}
+
+// one wave per heavy vertex
+__global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= h.nh) return;
+    const uint32_t p = threadIdx.x & 63;
+    const uint32_t v = h.verts[i];
+    const uint32_t base = g.row[v];
+    const uint32_t d = g.row[v + 1] - base;
+    const uint32_t *key = h.key + h.slot_off[i];
+    float gl[2];
+    uint32_t cl[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const unsigned long long k = h.hlane[(size_t)i * 128 + p + 64 * q];
+        if (k == 0ull) {
+            gl[q] = -INFINITY;
+            cl[q] = kEmpty;
+        } else {
+            gl[q] = f32_unord((uint32_t)(k >> 32));
+            const uint32_t j = 0xffffffffu - (uint32_t)(k & 0xffffffffull);
+            cl[q] = key[h.eslot[base + j]];
+        }
+    }
+    float bg;
+    uint32_t bc, bp;
+    tree128(gl[0], cl[0], gl[1], cl[1], bg, bc, bp);
+    if (p == 0) {
+        const uint32_t cv = g.comm[v];
+        g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, h.hcew[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heavy_clean(DevView g, HeavySplit h) {
+    const HeavyCtx x = heavy_ctx(g, h);
+    uint32_t *key = h.key + x.off;
+    unsigned long long *val = h.val + x.off;
+    const uint32_t nd = (uint32_t)h.hcnt[x.i];
+    const uint32_t i0 = x.c * kHeavyChunk;
+    if (i0 >= nd) return;
+    const uint32_t i1 = min(nd, i0 + kHeavyChunk);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const uint32_t slot = h.dlist[x.base + i];
+        key[slot] = kEmpty;
+        val[slot] = 0ull;
+    }
+}
+
+__global__ void k_heavy_chunks(const uint32_t *row, const uint32_t *verts, uint32_t count,
+                               uint32_t *nchunks) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const uint32_t v = verts[i];
+        nchunks[i] = (row[v + 1] - row[v] + kHeavyChunk - 1) / kHeavyChunk;
+    }
+}
+
+void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *verts,
+                         uint32_t count, uint32_t *nchunks) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_heavy_chunks, dim3((count + 255) / 256), dim3(256), 0, s, row, verts,
+                       count, nchunks);
+}
+
+void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h) {
+    if (!h.nh || !h.nchunks) return;
+    launch_fill_u64(s, h.hcnt, 0ull, (size_t)h.nh * 130);  // hcnt | hcew | hlane
+    const dim3 grid(h.nchunks), block(256);
+    hipLaunchKernelGGL(k_heavy_insert, grid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_gain, grid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_lanes, grid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_decide, dim3((h.nh + 3) / 4), block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_clean, grid, block, 0, s, g, h);
+}
+
+// ---------------------------------------------------------------------------
+// pinned-order sums of heavy rows
+// ---------------------------------------------------------------------------
+
+// masked weights of the heavy rows, staged contiguously: w if the neighbour
+// is in v's community, else +0 (adding +0 to a partial >= +0 changes nothing)
+__global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, float *stage) {
+    const HeavyCtx x = heavy_ctx(g, h);
+    const uint32_t cv = g.comm[x.v];
+    const uint32_t j0 = x.c * kHeavyChunk;
+    const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+        const float wv = g.w[x.base + j];
+        stage[x.base + j] = (g.comm[g.col[x.base + j]] == cv) ? wv : 0.0f;
+    }
+}
+
+constexpr int kR1K = 128;             // entries per lane per chunk
+constexpr int kR1Chunk = 32 * kR1K;   // 4096 entries
+
+__global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const float *stage,
+                                                  int masked, int init_state,
+                                                  unsigned long long *acc) {
+    __shared__ float buf[2][32][kR1K + 1];
+    const uint32_t v = h.verts[blockIdx.x];
+    const uint32_t base = g.row[v];
+    const uint32_t d = g.row[v + 1] - base;
+    const float *src = masked ? stage + base : g.w + base;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t nch = (d + kR1Chunk - 1) / kR1Chunk;
+    auto load = [&](uint32_t c, int b) {
+        for (int idx = (int)threadIdx.x - 64; idx < kR1Chunk; idx += 192) {
+            const uint32_t j = c * kR1Chunk + (uint32_t)idx;
+            buf[b][idx & 31][idx >> 5] = j < d ? src[j] : 0.0f;
+        }
+    };
+    if (wv != 0) load(0, 0);
+    __syncthreads();
+    float p = 0.0f;
+    for (uint32_t c = 0; c < nch; ++c) {
+        if (wv != 0) {
+            if (c + 1 < nch) load(c + 1, (c + 1) & 1);
+        } else if (lane < 32) {
+            const float *r = &buf[c & 1][lane][0];
+            for (int i = 0; i < kR1K; ++i) p += r[i];
+        }
+        __syncthreads();
+    }
+    if (wv == 0) {
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) p += __shfl_down(p, o, 32);
+        if (lane == 0) {
+            const unsigned long long fixed = f2fix(p, g.scale);
+            if (!masked) {
+                g.k[v] = p;
+                if (init_state) {
+                    g.comm[v] = v;
+                    g.comm_sz[v] = 1u;
+                    g.A[v] = fixed;
+                }
+            }
+            if (fixed) atomicAdd(acc, fixed);
+        }
+    }
+}
+
+void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float *stage,
+                     int masked, int init_state, unsigned long long *acc) {
+    if (!h.nh) return;
+    if (masked)
+        hipLaunchKernelGGL(k_heavy_stage, dim3(h.nchunks), dim3(256), 0, s, g, h, stage);
+    hipLaunchKernelGGL(k_heavy_r1, dim3(h.nh), dim3(256), 0, s, g, h, stage, masked, init_state,
+                       acc);
+}
+
+}  // namespace lv

@@ -18,73 +18,9 @@
 #include <stdint.h>
 
 #include "lv_device.h"
+#include "lv_devmath.h"
 
 namespace lv {
-
-// ---------------------------------------------------------------------------
-// arithmetic helpers
-// ---------------------------------------------------------------------------
-
-// Correctly rounded (nearest-even) u64 -> f32.
-__device__ __forceinline__ float u64_to_f32_rn(unsigned long long u) {
-    if (u < (1ull << 24)) return (float)(uint32_t)u;
-    int lz = __clzll(u);
-    int sh = 40 - lz;  // keep the top 24 bits
-    unsigned long long mant = u >> sh;
-    unsigned long long rem = u & ((1ull << sh) - 1ull);
-    unsigned long long half = 1ull << (sh - 1);
-    if (rem > half || (rem == half && (mant & 1ull))) mant++;
-    return ldexpf((float)(uint32_t)mant, sh);
-}
-
-// exact fixed point (x * 2^-S) rounded once to f32
-__device__ __forceinline__ float fix2f(unsigned long long x, int S) {
-    return ldexpf(u64_to_f32_rn(x), -S);
-}
-
-// f32 (a multiple of 2^-S, >= 0) -> exact fixed point
-__device__ __forceinline__ unsigned long long f2fix(float x, float scale) {
-    return (unsigned long long)(x * scale);
-}
-
-// Gain of moving to community C (modularity.cu:149-152): FP32, this
-// association, no contraction.
-__device__ __forceinline__ float gain_of(float hw, float m, float kv, float acv, float ac) {
-    float t1 = hw / m;
-    float t3 = (acv - kv) - ac;
-    float t4 = kv * t3;
-    float t6 = (2.0f * m) * m;
-    return t1 + t4 / t6;
-}
-
-// reduction order of the reference's arg-max (modularity.cu:169-179, 363)
-__device__ __forceinline__ bool better(float ga, uint32_t ca, float gb, uint32_t cb) {
-    return ga > gb || (ga == gb && ca < cb);
-}
-
-template <int LOG2CAP>
-__device__ __forceinline__ uint32_t hslot(uint32_t key) {
-    return (key * 0x9E3779B1u) >> (32 - LOG2CAP);
-}
-
-__device__ __forceinline__ uint32_t hslot_dyn(uint32_t key, int log2cap) {
-    return (key * 0x9E3779B1u) >> (32 - log2cap);
-}
-
-template <int N>
-struct Log2 {
-    static constexpr int value = 1 + Log2<N / 2>::value;
-};
-template <>
-struct Log2<1> {
-    static constexpr int value = 0;
-};
-
-// wave-level sum of u64
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
 
 // ---------------------------------------------------------------------------
 // fills
@@ -434,49 +370,6 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     }
 }
 
-// The 128-virtual-lane arg-max tree of computeMoveBlock/Global
-// (modularity.cu:260-275) on one wave: lane p holds virtual lanes p and
-// p+64.  Returns the winner (gain, comm, virtual lane) in lane 0.
-__device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_t c1, float &bg,
-                                        uint32_t &bc, uint32_t &bp) {
-    const uint32_t p = __lane_id();
-    bg = g0;
-    bc = c0;
-    bp = p;
-    if (better(g1, c1, g0, c0)) {
-        bg = g1;
-        bc = c1;
-        bp = p + 64;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        float og = __shfl_down(bg, o, 64);
-        uint32_t oc = __shfl_down(bc, o, 64);
-        uint32_t op = __shfl_down(bp, o, 64);
-        if (better(og, oc, bg, bc)) {
-            bg = og;
-            bc = oc;
-            bp = op;
-        }
-    }
-}
-
-// last entry visited by virtual lane wl of 128 on a row of d entries
-__device__ __forceinline__ uint32_t last_entry(uint32_t wl, uint32_t d) {
-    return wl + 128u * ((d - 1u - wl) / 128u);
-}
-
-// final decision for buckets 6-7
-__device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, uint32_t v, uint32_t base,
-                                                    uint32_t d, uint32_t cv, float bg,
-                                                    uint32_t bp, unsigned long long cew) {
-    if (bg == -INFINITY || bp >= d) return cv;
-    uint32_t dest = g.comm[g.col[base + last_entry(bp, d)]];
-    float cewf = fix2f(cew, g.S);
-    if (bg - (cewf / g.m) > 0.0f) return dest;
-    return cv;
-}
-
 // bucket 6 (84 < deg <= 319): one wave64 per vertex, LDS table per wave.
 // Physical lane p visits entries p, p+64, ...: even rounds belong to virtual
 // lane p, odd rounds to virtual lane p+64.
@@ -705,102 +598,8 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     if (t == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, scew);
 }
 
-// bucket 7 with deg > kBlockMax: one workgroup of P threads per vertex,
-// table of cap = pow2 >= 2*deg slots in a global arena (kept clean between
-// uses: every touched slot is reset at the end), distinct-slot list and
-// per-entry slots in E-sized scratch at the vertex's row range.
-template <int P>
-__global__ __launch_bounds__(P) void k_move_global(DevView g, const uint32_t *verts,
-                                                   uint32_t count, HeavyScratch hs) {
-    __shared__ uint32_t ndist;
-    __shared__ unsigned long long scew;
-    __shared__ float lg[P];
-    __shared__ uint32_t lc[P], lj[P];
-    const uint32_t t = threadIdx.x;
-    const uint32_t vi = blockIdx.x;
-    const uint32_t v = verts[vi];
-    const uint32_t base = g.row[v];
-    const uint32_t d = g.row[v + 1] - base;
-    const unsigned long long off = hs.slot_off[vi];
-    const uint32_t cap = (uint32_t)(hs.slot_off[vi + 1] - off);
-    const int lcap = 31 - __clz(cap);
-    uint32_t *key = hs.key + off;
-    unsigned long long *val = hs.val + off;
-    uint32_t *dist = hs.dlist + base;
-    uint32_t *eslot = hs.eslot + base;
-    const uint32_t cv = g.comm[v];
-    const uint32_t szv = g.comm_sz[cv];
-    const float kv = g.k[v];
-    const float acv = fix2f(g.A[cv], g.S);
-    if (t == 0) {
-        ndist = 0;
-        scew = 0;
-    }
-    __syncthreads();
-    unsigned long long cew = 0;
-    for (uint32_t j = t; j < d; j += P) {
-        uint32_t e = g.col[base + j];
-        uint32_t ce = g.comm[e];
-        unsigned long long wf = f2fix(g.w[base + j], g.scale);
-        uint32_t slot = hslot_dyn(ce, lcap);
-        uint32_t probe = 0;
-        for (; probe < cap; ++probe) {
-            uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
-            if (prev == kEmpty) {
-                dist[atomicAdd(&ndist, 1u)] = slot;
-                break;
-            }
-            if (prev == ce) break;
-            slot = (slot + 1) & (cap - 1);
-        }
-        if (probe == cap) {  // cannot happen: cap >= 2*deg
-            atomicOr(hs.err, 1u);
-            slot = 0;
-        }
-        atomicAdd(&val[slot], wf);
-        if (ce == cv && e != v) cew += wf;
-        eslot[j] = slot;
-    }
-    cew = wave_sum_u64(cew);
-    if ((t & 63) == 0 && cew) atomicAdd(&scew, cew);
-    __syncthreads();
-    const uint32_t nd = ndist;
-    for (uint32_t i = t; i < nd; i += P) {
-        uint32_t slot = dist[i];
-        uint32_t c = key[slot];
-        float gg = -INFINITY;
-        bool cand = true;
-        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
-        val[slot] = (unsigned long long)__float_as_uint(gg);
-    }
-    __syncthreads();
-    float tg = -INFINITY;
-    uint32_t tc = kEmpty, tj = kEmpty;
-    for (uint32_t j = t; j < d; j += P) {
-        uint32_t slot = eslot[j];
-        float gg = __uint_as_float((uint32_t)val[slot]);
-        if (gg > tg) {
-            tg = gg;
-            tc = key[slot];
-            tj = j;
-        }
-    }
-    float bg = -INFINITY;
-    uint32_t bc = kEmpty, bp = kEmpty;
-    block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp);
-    if (t == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, scew);
-    __syncthreads();
-    // leave the arena clean for the next use
-    for (uint32_t i = t; i < nd; i += P) {
-        uint32_t slot = dist[i];
-        key[slot] = kEmpty;
-        val[slot] = 0ull;
-    }
-}
-
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
-                       uint32_t count, const HeavyScratch &hs) {
+                       uint32_t count) {
     if (!count) return;
     switch (cls) {
         case 1:
@@ -830,10 +629,6 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
         case 7:
             hipLaunchKernelGGL((k_move_block<256, 4096>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
-            break;
-        case 8:
-            hipLaunchKernelGGL((k_move_global<512>), dim3(count), dim3(512), 0, s, g, verts, count,
-                               hs);
             break;
         default:
             break;
