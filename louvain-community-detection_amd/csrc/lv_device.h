@@ -24,6 +24,10 @@ constexpr uint32_t kLanes[7] = {4u, 8u, 16u, 32u, 32u, 128u, 128u};
 constexpr int kNumClasses = 9;
 constexpr uint32_t kBlockMax = 2048;
 constexpr uint32_t kEmpty = 0xffffffffu;
+// Exact integer totals are accumulated into kSpread slots (block b adds to
+// slot b % kSpread) and summed on the host: same-address atomics from
+// thousands of blocks serialise in L2.
+constexpr uint32_t kSpread = 64;
 
 __host__ __device__ inline int vertex_class(uint32_t d) {
     if (d == 0) return 0;

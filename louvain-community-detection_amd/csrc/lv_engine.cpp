@@ -48,10 +48,16 @@ struct Counters {
     uint32_t err;
     uint32_t pad;
     ull degsum[kNumClasses];
-    ull acc;       // k/s fixed total
-    ull limbs[4];  // sum a^2 limbs
-    ull moved;
+    ull acc[lv::kSpread];       // k/s fixed total, spread
+    ull limbs[4][lv::kSpread];  // sum a^2 limbs, spread
+    ull moved[lv::kSpread];
 };
+
+ull sum_spread(const ull *x) {
+    ull s = 0;
+    for (uint32_t i = 0; i < lv::kSpread; ++i) s += x[i];
+    return s;
+}
 
 struct ProfRec {
     int fam;
@@ -305,12 +311,12 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         double ent = 0;
         for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
-        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, &g->dc->acc);
+        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, g->dc->acc);
     }
     if (g->cls_count[8]) {  // heavy rows: staged, LDS-streamed pinned sums
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[8]);
         lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
-                            &g->dc->acc);
+                            g->dc->acc);
     }
     return LV_OK;
 }
@@ -318,9 +324,9 @@ int r1_all(lv_graph *g, int masked, int init_state) {
 // FP32 modularity from the exact counters (modularity.cu:428-436)
 float q_from_counters(const lv_graph *g) {
     const Counters *c = g->hc;
-    float s_in = fix2f_host(g, (int64_t)c->acc);
-    u128 sq = (u128)c->limbs[0] + ((u128)c->limbs[1] << 32) + ((u128)c->limbs[2] << 64) +
-              ((u128)c->limbs[3] << 96);
+    float s_in = fix2f_host(g, (int64_t)sum_spread(c->acc));
+    u128 sq = 0;
+    for (int l = 0; l < 4; ++l) sq += (u128)sum_spread(c->limbs[l]) << (32 * l);
     float a2 = ldexpf((float)sq, -2 * g->S);
     float m = g->m;
     return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
@@ -331,7 +337,7 @@ void modularity_launch(lv_graph *g) {
     r1_all(g, 1, 0);
     lv::DevView v = view(g);
     ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
-    lv::launch_sumsq(g->stream, v, g->dc->limbs);
+    lv::launch_sumsq(g->stream, v, &g->dc->limbs[0][0]);
 }
 
 int pass_init(lv_graph *g, float *q0) {
@@ -341,7 +347,7 @@ int pass_init(lv_graph *g, float *q0) {
     r1_all(g, 0, 1);
     rc = sync_counters(g);
     if (rc) return rc;
-    g->m = fix2f_host(g, (int64_t)g->hc->acc) / 2.0f;
+    g->m = fix2f_host(g, (int64_t)sum_spread(g->hc->acc)) / 2.0f;
     if (!(g->m > 0.0f)) {
         // zero total weight: the reference divides by zero and never stops
         return LV_ERR_RANGE;
@@ -379,7 +385,7 @@ int step(lv_graph *g, int b) {
     const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
     if (e > a) {
         ProfScope ps(g, LV_K_APPLY, 16.0 * (e - a));
-        lv::launch_apply(g->stream, v, g->order + a, e - a, &g->dc->moved);
+        lv::launch_apply(g->stream, v, g->order + a, e - a, g->dc->moved);
     }
     return LV_OK;
 }
@@ -392,7 +398,7 @@ int sweep(lv_graph *g, float *q, uint64_t *moved) {
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;
     if (q) *q = q_from_counters(g);
-    if (moved) *moved = g->hc->moved;
+    if (moved) *moved = sum_spread(g->hc->moved);
     return LV_OK;
 }
 
@@ -782,7 +788,7 @@ int lv_step(lv_graph *g, int bucket, uint64_t *moved) {
     int rc = sync_counters(g);
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;
-    if (moved) *moved = g->hc->moved;
+    if (moved) *moved = sum_spread(g->hc->moved);
     return LV_OK;
 }
 

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
                     g.A[v] = fixed;
                 }
             }
-            if (fixed) atomicAdd(acc, fixed);
+            if (fixed) atomicAdd(&acc[blockIdx.x % kSpread], fixed);
         }
     }
 }

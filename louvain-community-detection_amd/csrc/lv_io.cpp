@@ -420,15 +420,21 @@ int lv_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double
         }
         return x & mask;
     };
+    // quadrant thresholds on 16-bit uniforms: one 64-bit hash serves 4 levels
+    const uint32_t ta = (uint32_t)std::lround(a * 65536.0), tab = (uint32_t)std::lround(ab * 65536.0),
+                   tabc = (uint32_t)std::lround(abc * 65536.0);
     auto gen = [=](uint64_t k, uint32_t *u, uint32_t *v) {
         uint32_t x = 0, y = 0;
+        uint64_t r = 0;
         for (uint32_t l = 0; l < scale; ++l) {
-            double r = u01(rng(seed, k, l));
+            if ((l & 3) == 0) r = mix64(seed ^ (k * 0x9E3779B97F4A7C15ull + (l >> 2)));
+            const uint32_t q = (uint32_t)(r & 0xffffu);
+            r >>= 16;
             uint32_t bx = 0, by = 0;
-            if (r < a) {
-            } else if (r < ab) {
+            if (q < ta) {
+            } else if (q < tab) {
                 by = 1;
-            } else if (r < abc) {
+            } else if (q < tabc) {
                 bx = 1;
             } else {
                 bx = 1;

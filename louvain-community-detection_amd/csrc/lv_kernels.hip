@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long s = part[0] + part[1] + part[2] + part[3];
-        if (s) atomicAdd(acc, s);
+        if (s) atomicAdd(&acc[blockIdx.x % kSpread], s);
     }
 }
 
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_sumsq(DevView g, unsigned long long *li
     if (threadIdx.x < 4) {
         unsigned long long s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
                                part[3][threadIdx.x];
-        if (s) atomicAdd(&limbs[threadIdx.x], s);
+        if (s) atomicAdd(&limbs[threadIdx.x * kSpread + blockIdx.x % kSpread], s);
     }
 }
 
@@ -676,8 +676,14 @@ __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts,
             mv = 1;
         }
     }
+    __shared__ unsigned long long part[4];
     mv = wave_sum_u64(mv);
-    if ((threadIdx.x & 63) == 0 && mv) atomicAdd(moved, mv);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = mv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long s = part[0] + part[1] + part[2] + part[3];
+        if (s) atomicAdd(&moved[blockIdx.x % kSpread], s);
+    }
 }
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved) {

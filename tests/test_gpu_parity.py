@@ -23,6 +23,7 @@ def _graphs(lv, orc, text):
     out["rmat12"] = lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 7)
     out["rmat14_unit"] = lv.gen_rmat(14, 16, 0.57, 0.19, 0.19, 1.0, 1.0, 9)
     out["sbm_5k"] = lv.gen_sbm(5000, 20, 12.0, 10.0, 11)
+    out["rmat16"] = lv.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 13)
     return out
 
 
@@ -54,7 +55,7 @@ def _cmp_csr(g, o, where):
 
 
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
-                                  "sbm_5k"])
+                                  "sbm_5k", "rmat16"])
 def test_stepwise_first_pass(lv, orc, graphs, name):
     row, col, w = graphs[name]
     print(f"[case] stepwise {name} n={row.size - 1} ne={col.size}", flush=True)
@@ -79,7 +80,7 @@ def test_stepwise_first_pass(lv, orc, graphs, name):
 
 
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
-                                  "sbm_5k"])
+                                  "sbm_5k", "rmat16"])
 def test_passes_and_aggregation(lv, orc, graphs, name):
     row, col, w = graphs[name]
     print(f"[case] passes {name}", flush=True)
@@ -101,7 +102,7 @@ def test_passes_and_aggregation(lv, orc, graphs, name):
 
 
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
-                                  "sbm_5k"])
+                                  "sbm_5k", "rmat16"])
 def test_full_run(lv, orc, graphs, name):
     row, col, w = graphs[name]
     print(f"[case] run {name}", flush=True)
@@ -140,7 +141,7 @@ def test_edge_cases(lv, orc):
     # high-degree vertices through duplicate edges (buckets 6 and 7, heavy class)
     rng = np.random.default_rng(5)
     lines = []
-    for hub, deg in [(1, 300), (2, 1500), (3, 5000)]:
+    for hub, deg in [(1, 300), (2, 1500), (3, 5000), (4, 21000)]:
         for _ in range(deg):
             lines.append(f"{hub} {int(rng.integers(4, 60))} {1 + int(rng.integers(0, 3))}\n")
     cases["dup_hubs"] = f"60 60 {len(lines)}\n" + "".join(lines)
@@ -154,3 +155,40 @@ def test_edge_cases(lv, orc):
             assign = g.assignment()
         assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32), name
         assert np.array_equal(assign, o.init_comm()), name
+
+
+def test_copurchase_full_size(lv, orc):
+    """Full-size co-purchase workload (262 111 vertices) against the oracle."""
+    row, col, w = lv.gen_chung_lu(262111, 9.4, 2.5, 420, 2000, 20.0, 2)
+    print(f"[case] copurchase full n={row.size - 1} ne={col.size}", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    ores = o.run(1e-6)
+    with lv.Graph(row, col, w) as g:
+        res = g.run(1e-6)
+        assign = g.assignment()
+    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)
+    assert np.array_equal(assign, o.init_comm())
+    assert res.total_iters == ores.total_iters and res.n_final == ores.n_final
+
+
+def test_rmat24_properties(lv, orc):
+    """R-MAT scale 24 (16.8M vertices, ~521M entries): size-independent checks
+    -- identical results on two runs from the resident copy, a valid dense
+    assignment, and the printed modularity within 1e-6 of an FP64 recompute."""
+    row, col, w = lv.gen_rmat(24, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)
+    n = row.size - 1
+    print(f"[case] rmat24 n={n} ne={col.size}", flush=True)
+    with lv.Graph(row, col, w) as g:
+        r1 = g.run(1e-6)
+        a1 = g.assignment()
+        g.reset()
+        r2 = g.run(1e-6)
+        a2 = g.assignment()
+    print(f"[case] rmat24 Q={r1.q_final} passes={r1.passes} sweeps={r1.total_iters} "
+          f"communities={r1.n_final} ms={r1.alg_ms:.1f}", flush=True)
+    assert np.float32(r1.q_final).view(np.uint32) == np.float32(r2.q_final).view(np.uint32)
+    assert np.array_equal(a1, a2)
+    assert a1.max() + 1 == r1.n_final
+    assert np.unique(a1).size == r1.n_final
+    q64 = orc.q64(row.astype(np.uint64), col, w, a1)
+    assert abs(float(r1.q_final) - q64) <= Q64_TOL, (r1.q_final, q64)
