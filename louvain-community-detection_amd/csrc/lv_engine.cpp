@@ -390,15 +390,54 @@ int step(lv_graph *g, int b) {
     return LV_OK;
 }
 
-int sweep(lv_graph *g, float *q, uint64_t *moved) {
+// the device work of one sweep, ending with the counters' D2H; no host sync
+// (this sequence is what a pass captures into a HIP graph)
+void sweep_enqueue(lv_graph *g) {
     zero_counters(g);
     for (int b = 1; b <= 7; ++b) step(g, b);
     modularity_launch(g);
-    int rc = sync_counters(g);
-    if (rc) return rc;
+    (void)hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream);
+}
+
+int sweep_finish(lv_graph *g, float *q, uint64_t *moved) {
+    LV_HIP(hipStreamSynchronize(g->stream));
     if (g->hc->err) return LV_ERR_OVERFLOW;
     if (q) *q = q_from_counters(g);
     if (moved) *moved = sum_spread(g->hc->moved);
+    return LV_OK;
+}
+
+int sweep(lv_graph *g, float *q, uint64_t *moved) {
+    sweep_enqueue(g);
+    return sweep_finish(g, q, moved);
+}
+
+// One pass's sweep as a HIP graph: the launch sequence is fixed within a
+// pass (same classes, same m), so it is captured once and replayed per sweep.
+struct SweepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~SweepGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+bool graphs_enabled(const lv_graph *g) {
+    static const bool off = getenv("LV_NO_GRAPH") != nullptr;
+    return !off && !g->prof_on;
+}
+
+int capture_sweep(lv_graph *g, SweepGraph &sg) {
+    LV_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    sweep_enqueue(g);
+    hipError_t e = hipStreamEndCapture(g->stream, &sg.graph);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        sg.graph = nullptr;
+        return LV_ERR_HIP;
+    }
+    LV_HIP(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
     return LV_OK;
 }
 
@@ -409,10 +448,21 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
     if (rc) return rc;
     bool first = true, ret = true;
     uint32_t it = 0;
+    SweepGraph sg;
+    const bool use_graph = graphs_enabled(g);
+    if (use_graph) {
+        rc = capture_sweep(g, sg);
+        if (rc) return rc;
+    }
     for (;;) {
         uint64_t moved = 0;
         old = act;
-        rc = sweep(g, &act, &moved);
+        if (use_graph) {
+            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            rc = sweep_finish(g, &act, &moved);
+        } else {
+            rc = sweep(g, &act, &moved);
+        }
         if (rc) return rc;
         ++it;
         // the loop condition of modularity.cu:508; a sweep without moves is a

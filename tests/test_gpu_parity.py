@@ -192,3 +192,29 @@ def test_rmat24_properties(lv, orc):
     assert np.unique(a1).size == r1.n_final
     q64 = orc.q64(row.astype(np.uint64), col, w, a1)
     assert abs(float(r1.q_final) - q64) <= Q64_TOL, (r1.q_final, q64)
+
+
+def test_cli_output_matches_reference_format(lv, orc, bcsstk16_path):
+    """`louvain -f bcsstk16.mtx -g 1e-6 -v`: line 1 the final modularity as the
+    reference's default ostream prints it (modularity.cu:504), line 2 two
+    times (main.cu:56), then the community listing of printCommunities
+    (utils.cu:270-293), all from the oracle's result."""
+    import subprocess
+    row, col, w = orc.parse_mtx(path=bcsstk16_path)
+    o = orc.Oracle(row, col, w)
+    res = o.run(1e-6)
+    assign = o.init_comm()
+    r = subprocess.run([lv.CLI_PATH, "-f", bcsstk16_path, "-g", "0.000001", "-v"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")
+    assert lines[0] == "%g" % float(np.float32(res.q_final))
+    t = lines[1].split(" ")
+    assert len(t) == 2 and float(t[0]) > 0 and float(t[1]) >= float(t[0])
+    assert lines[2] == str(res.n_final)
+    expect = []
+    for c in range(res.n_final):
+        members = np.nonzero(assign == c)[0] + 1
+        expect.append(f"{c + 1} " + "".join(f"{m} " for m in members))
+    assert lines[3:3 + res.n_final] == expect
+    assert lines[3 + res.n_final:] == [""]
