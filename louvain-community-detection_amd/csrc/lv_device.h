@@ -19,9 +19,11 @@ constexpr uint32_t kLanes[7] = {4u, 8u, 16u, 32u, 32u, 128u, 128u};
 // Kernel classes of the move step (performance choice, not semantic).
 //   0: deg 0 (never moves)      1-4: buckets 1-4, sub-wave groups of 4/8/16/32
 //   5: bucket 5, 32-lane groups 6: bucket 6, one wave64 per vertex
-//   7: bucket 7, deg <= kBlockMax, one workgroup + LDS hash per vertex
-//   8: bucket 7, deg >  kBlockMax, one workgroup + global hash per vertex
-constexpr int kNumClasses = 9;
+//   7-9: bucket 7 with deg <= 512 / 1024 / 2048: one workgroup per vertex,
+//        LDS table of 1024 / 2048 / 4096 slots
+//   10: bucket 7, deg > kBlockMax: chunked kernels, global tables
+constexpr int kNumClasses = 11;
+constexpr int kClassHeavy = 10;
 constexpr uint32_t kBlockMax = 2048;
 constexpr uint32_t kEmpty = 0xffffffffu;
 // Exact integer totals are accumulated into kSpread slots (block b adds to
@@ -37,15 +39,19 @@ __host__ __device__ inline int vertex_class(uint32_t d) {
     if (d <= 32) return 4;
     if (d <= 84) return 5;
     if (d <= 319) return 6;
-    if (d <= kBlockMax) return 7;
-    return 8;
+    if (d <= 512) return 7;
+    if (d <= 1024) return 8;
+    if (d <= kBlockMax) return 9;
+    return kClassHeavy;
 }
 
 // Aggregation community classes (by total member degree).
-//   0: LDS merge, one wave per community, 256-slot table   (comm_deg <= 128)
-//   1: LDS merge, one workgroup, 4096-slot table           (comm_deg <= 2048)
-//   2: global-hash merge + workgroup radix sort            (comm_deg >  2048)
-constexpr int kNumAggClasses = 3;
+//   0: no entries (empty row, no kernel)                   (comm_deg == 0)
+//   1: LDS merge, one wave per community, 256-slot table   (comm_deg <= 128)
+//   2: LDS merge, one workgroup, 4096-slot table           (comm_deg <= 2048)
+//   3: global-hash merge + workgroup radix sort            (comm_deg >  2048)
+constexpr int kNumAggClasses = 4;
+constexpr int kAggGlobalClass = 3;
 constexpr uint32_t kAggSmallMax = 128;
 constexpr uint32_t kAggMidMax = 2048;
 
@@ -66,7 +72,7 @@ struct DevView {
 };
 
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
-constexpr uint32_t kHeavyChunk = 4096;  // entries per workgroup; multiple of 128
+constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
 struct HeavySplit {
     const uint32_t *verts;               // class-8 vertices
     uint32_t nh;                         // their count

@@ -246,10 +246,10 @@ int classify(lv_graph *g) {
     }
     // heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk
     // offsets of the split kernels, per-vertex scratch
-    const uint32_t nh = g->cls_count[8];
+    const uint32_t nh = g->cls_count[lv::kClassHeavy];
     g->heavy_chunks = 0;
     if (nh) {
-        const uint32_t *hv = g->order + g->cls_off[8];
+        const uint32_t *hv = g->order + g->cls_off[lv::kClassHeavy];
         lv::launch_heavy_caps(g->stream, g->row, hv, nh, g->heavy_caps);
         lv::launch_scan_u64(g->stream, g->heavy_caps, nh, g->heavy_off, g->scan_tmp64);
         lv::launch_heavy_chunks(g->stream, g->row, hv, nh, g->heavy_nch);
@@ -282,8 +282,8 @@ int classify(lv_graph *g) {
 
 lv::HeavySplit heavy_view(lv_graph *g) {
     lv::HeavySplit h;
-    const uint32_t nh = g->cls_count[8];
-    h.verts = g->order + g->cls_off[8];
+    const uint32_t nh = g->cls_count[lv::kClassHeavy];
+    h.verts = g->order + g->cls_off[lv::kClassHeavy];
     h.nh = nh;
     h.nchunks = g->heavy_chunks;
     h.coff = g->heavy_coff;
@@ -305,7 +305,7 @@ int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
     const struct {
         int G, c0, c1;
-    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 7}};
+    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, lv::kClassHeavy - 1}};
     for (const auto &q : grp) {
         uint32_t a = g->cls_off[q.c0], b = g->cls_off[q.c1 + 1];
         double ent = 0;
@@ -313,8 +313,8 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
         lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, g->dc->acc);
     }
-    if (g->cls_count[8]) {  // heavy rows: staged, LDS-streamed pinned sums
-        ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[8]);
+    if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
+        ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy]);
         lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
                             g->dc->acc);
     }
@@ -364,7 +364,7 @@ int pass_init(lv_graph *g, float *q0) {
 // one bucket step: decide from the snapshot, then apply
 int step(lv_graph *g, int b) {
     lv::DevView v = view(g);
-    const int c0 = b, c1 = (b == 7) ? 8 : b;
+    const int c0 = b, c1 = (b == 7) ? lv::kClassHeavy : b;
     static const int fam[kNumClasses] = {0,
                                          LV_K_MOVE_SMALL,
                                          LV_K_MOVE_SMALL,
@@ -373,11 +373,13 @@ int step(lv_graph *g, int b) {
                                          LV_K_MOVE_SMALL,
                                          LV_K_MOVE_WAVE,
                                          LV_K_MOVE_BLOCK,
+                                         LV_K_MOVE_BLOCK,
+                                         LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_GLOBAL};
     for (int c = c0; c <= c1; ++c) {
         if (!g->cls_count[c]) continue;
         ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c]);
-        if (c == 8)
+        if (c == lv::kClassHeavy)
             lv::launch_heavy_move(g->stream, v, heavy_view(g));
         else
             lv::launch_move_class(g->stream, v, c, g->order + g->cls_off[c], g->cls_count[c]);
@@ -531,9 +533,10 @@ int aggregation(lv_graph *g) {
     LV_HIP(hipStreamSynchronize(s));  // `aoff` is pageable stack memory
     lv::launch_agg_classify_scatter(s, v, aview(g), g->dc->acursor, g->aorder);
     // global-hash arena for the large communities
-    const uint32_t nl = g->hc->acounts[2];
+    lv::launch_fill_u32(s, g->cnt, 0u, nn);  // edgeless communities keep 0
+    const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
     if (nl) {
-        const uint32_t *lc = g->aorder + aoff[2];
+        const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
         lv::launch_agg_caps(s, aview(g), lc, nl, g->agg_caps);
         lv::launch_scan_u64(s, g->agg_caps, nl, g->agg_off, g->scan_tmp64);
         ull total = 0;

@@ -68,25 +68,78 @@ __device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit
     return x;
 }
 
+// H1: the chunk's entries are first merged in an LDS table (exact u64 sums),
+// then each distinct community of the chunk is added to the vertex's global
+// table once: global atomics per chunk = distinct communities, not entries.
+constexpr int kLocalCap = 2 * kHeavyChunk;
+constexpr int kPerThread = kHeavyChunk / 256;
+
 __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
+    constexpr int LC = Log2<kLocalCap>::value;
+    __shared__ uint32_t lkey[kLocalCap];
+    __shared__ unsigned long long lval[kLocalCap];
+    __shared__ uint32_t lglob[kLocalCap];
+    __shared__ uint32_t ldist[kHeavyChunk];
+    __shared__ uint32_t nld, nnew, newbase;
+    __shared__ unsigned long long part[4];
     const HeavyCtx x = heavy_ctx(g, h);
     uint32_t *key = h.key + x.off;
     unsigned long long *val = h.val + x.off;
+    for (int i = threadIdx.x; i < kLocalCap; i += 256) {
+        lkey[i] = kEmpty;
+        lval[i] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        nld = 0;
+        nnew = 0;
+    }
     const uint32_t cv = g.comm[x.v];
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    __syncthreads();
+    uint32_t lslot[kPerThread];
     unsigned long long cew = 0;
-    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
-        const uint32_t e = g.col[x.base + j];
-        const uint32_t ce = g.comm[e];
-        const unsigned long long wf = f2fix(g.w[x.base + j], g.scale);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        lslot[q] = kEmpty;
+        if (j < j1) {
+            const uint32_t e = g.col[x.base + j];
+            const uint32_t ce = g.comm[e];
+            const unsigned long long wf = f2fix(g.w[x.base + j], g.scale);
+            uint32_t slot = hslot<LC>(ce);
+            for (int probe = 0; probe < kLocalCap; ++probe) {
+                const uint32_t prev = atomicCAS(&lkey[slot], kEmpty, ce);
+                if (prev == kEmpty) {
+                    ldist[atomicAdd(&nld, 1u)] = slot;
+                    break;
+                }
+                if (prev == ce) break;
+                slot = (slot + 1) & (kLocalCap - 1);
+            }
+            atomicAdd(&lval[slot], wf);
+            if (ce == cv && e != x.v) cew += wf;
+            lslot[q] = slot;
+        }
+    }
+    cew = wave_sum_u64(cew);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
+    __syncthreads();
+    // flush the chunk's distinct communities into the vertex's global table
+    const uint32_t nd = nld;
+    uint32_t my_new[kPerThread * 2];
+    uint32_t my_rank[kPerThread * 2];
+    int nmine = 0;
+    for (uint32_t i = threadIdx.x; i < nd; i += 256) {
+        const uint32_t ls = ldist[i];
+        const uint32_t ce = lkey[ls];
         uint32_t slot = hslot_dyn(ce, x.lcap);
         uint32_t probe = 0;
+        bool claimed = false;
         for (; probe < x.cap; ++probe) {
-            uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
             if (prev == kEmpty) {
-                unsigned long long idx = atomicAdd(&h.hcnt[x.i], 1ull);
-                h.dlist[x.base + idx] = slot;
+                claimed = true;
                 break;
             }
             if (prev == ce) break;
@@ -96,17 +149,26 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
             atomicOr(h.err, 1u);
             slot = 0;
         }
-        atomicAdd(&val[slot], wf);
-        if (ce == cv && e != x.v) cew += wf;
-        h.eslot[x.base + j] = slot;
+        atomicAdd(&val[slot], lval[ls]);
+        lglob[ls] = slot;
+        if (claimed) {
+            my_new[nmine] = slot;
+            my_rank[nmine] = atomicAdd(&nnew, 1u);
+            ++nmine;
+        }
     }
-    __shared__ unsigned long long part[4];
-    cew = wave_sum_u64(cew);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long s = part[0] + part[1] + part[2] + part[3];
+        const unsigned long long s = part[0] + part[1] + part[2] + part[3];
         if (s) atomicAdd(&h.hcew[x.i], s);
+        newbase = nnew ? (uint32_t)atomicAdd(&h.hcnt[x.i], (unsigned long long)nnew) : 0u;
+    }
+    __syncthreads();
+    for (int q = 0; q < nmine; ++q) h.dlist[x.base + newbase + my_rank[q]] = my_new[q];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        if (lslot[q] != kEmpty) h.eslot[x.base + j] = lglob[lslot[q]];
     }
 }
 

@@ -516,10 +516,11 @@ template <int P, int CAP>
 __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *verts,
                                                   uint32_t count) {
     constexpr int LC = Log2<CAP>::value;
-    constexpr int MAXPER = (kBlockMax + P - 1) / P;
+    constexpr int MAXDEG = CAP / 2;
+    constexpr int MAXPER = (MAXDEG + P - 1) / P;
     __shared__ uint32_t key[CAP];
     __shared__ unsigned long long val[CAP];
-    __shared__ uint32_t dist[kBlockMax];
+    __shared__ uint32_t dist[MAXDEG];
     __shared__ uint32_t ndist;
     __shared__ unsigned long long scew;
     __shared__ float lg[P];
@@ -627,6 +628,14 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                count);
             break;
         case 7:
+            hipLaunchKernelGGL((k_move_block<256, 1024>), dim3(count), dim3(256), 0, s, g, verts,
+                               count);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_move_block<256, 2048>), dim3(count), dim3(256), 0, s, g, verts,
+                               count);
+            break;
+        case 9:
             hipLaunchKernelGGL((k_move_block<256, 4096>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
@@ -837,9 +846,10 @@ void launch_agg_members(hipStream_t s, const DevView &g, AggView a) {
 }
 
 __device__ __forceinline__ int agg_class(uint32_t cdeg) {
-    if (cdeg <= kAggSmallMax) return 0;
-    if (cdeg <= kAggMidMax) return 1;
-    return 2;
+    if (cdeg == 0) return 0;
+    if (cdeg <= kAggSmallMax) return 1;
+    if (cdeg <= kAggMidMax) return 2;
+    return 3;
 }
 
 __global__ __launch_bounds__(256) void k_agg_class_count(DevView g, AggView a, uint32_t *counts) {
@@ -984,6 +994,89 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
 }
 
+// Small communities (comm_deg <= 128): one wave per community, four per
+// workgroup; every wave sorts a fixed 128-element array so the block-wide
+// barriers of the bitonic sort stay uniform.
+__global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
+                                                         const uint32_t *comms, uint32_t count) {
+    constexpr int CAP = 256, LC = 8, N = 128;
+    __shared__ uint32_t skey_all[4][CAP];
+    __shared__ unsigned long long sval_all[4][CAP];
+    __shared__ uint32_t sk_all[4][N];
+    __shared__ uint32_t ss_all[4][N];
+    __shared__ uint32_t nd_all[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *key = skey_all[wv];
+    unsigned long long *val = sval_all[wv];
+    uint32_t *sk = sk_all[wv], *ss = ss_all[wv];
+    for (int i = lane; i < CAP; i += 64) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    for (int i = lane; i < N; i += 64) {
+        sk[i] = kEmpty;
+        ss[i] = 0;
+    }
+    if (lane == 0) nd_all[wv] = 0;
+    const uint32_t ci = blockIdx.x * 4 + wv;
+    const bool active = ci < count;
+    const uint32_t c = active ? comms[ci] : 0u;
+    __syncthreads();
+    if (active) {
+        const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
+        for (uint32_t mi = 0; mi < msz; ++mi) {
+            const uint32_t v = a.members[m0 + mi];
+            const uint32_t b = g.row[v], e = g.row[v + 1];
+            for (uint32_t j = b + lane; j < e; j += 64) {
+                const uint32_t dd = g.comm[g.col[j]];
+                const unsigned long long wf = f2fix(g.w[j], g.scale);
+                uint32_t slot = hslot<LC>(dd);
+                for (int probe = 0; probe < CAP; ++probe) {
+                    const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
+                    if (prev == kEmpty) {
+                        const uint32_t idx = atomicAdd(&nd_all[wv], 1u);
+                        sk[idx] = dd;
+                        ss[idx] = slot;
+                        break;
+                    }
+                    if (prev == dd) break;
+                    slot = (slot + 1) & (CAP - 1);
+                }
+                atomicAdd(&val[slot], wf);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= (uint32_t)N; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < (uint32_t)N; i += 64) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & kk) == 0;
+                    const uint32_t x = sk[i], y = sk[ixj];
+                    if ((x > y) == up) {
+                        sk[i] = y;
+                        sk[ixj] = x;
+                        const uint32_t t = ss[i];
+                        ss[i] = ss[ixj];
+                        ss[ixj] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (active) {
+        const uint32_t nd = nd_all[wv];
+        const uint32_t out = a.epos[c];
+        for (uint32_t i = lane; i < nd; i += 64) {
+            a.tkey[out + i] = sk[i];
+            a.tval[out + i] = val[ss[i]];
+        }
+        if (lane == 0) a.cnt[a.new_id[c]] = nd;
+    }
+}
+
 // global-hash merge for large communities, then an in-workgroup stable LSD
 // radix sort of the distinct keys (8-bit digits) with scratch skey/sval.
 constexpr int kRadixThreads = 1024;
@@ -1118,11 +1211,11 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
 
 void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
                       uint32_t count, int radix_passes) {
-    if (!count) return;
-    if (cls == 0)
-        hipLaunchKernelGGL((k_agg_merge_lds<64, 256>), dim3(count), dim3(64), 0, s, g, a, comms,
+    if (!count || cls == 0) return;  // class 0: no entries, cnt stays 0
+    if (cls == 1)
+        hipLaunchKernelGGL(k_agg_merge_small, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms,
                            count);
-    else if (cls == 1)
+    else if (cls == 2)
         hipLaunchKernelGGL((k_agg_merge_lds<256, 4096>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
     else
