@@ -145,7 +145,13 @@ struct AggView {
     unsigned long long *sval;   // ne
     uint32_t *gkey;         // global-hash arena
     unsigned long long *gval;
-    const unsigned long long *goff;  // per class-2 community
+    const unsigned long long *goff;  // per large community: table offsets (count+1)
+    uint32_t *mdeg;         // n: degree of members[i]
+    uint32_t *moff;         // n+1: exclusive scan of mdeg
+    uint32_t *cnk;          // per large community: chunk count
+    uint32_t *coff;         // per large community: chunk offsets (count+1)
+    uint32_t *dcnt;         // per large community: distinct neighbour communities
+    uint32_t nchunks;       // total chunks of the large communities
     uint32_t *err;
 };
 void launch_agg_prep(hipStream_t s, const DevView &g, AggView a);
@@ -157,6 +163,8 @@ void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t c
                      unsigned long long *caps);
 void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
                       uint32_t count, int radix_passes);
+void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
+                           uint32_t count);
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
                      uint32_t count, uint32_t *row2, uint32_t *col2, float *w2);
 void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,

@@ -106,6 +106,9 @@ struct lv_graph {
     uint32_t *tkey = nullptr, *skey = nullptr;
     ull *tval = nullptr, *sval = nullptr;
     ull *agg_caps = nullptr, *agg_off = nullptr;
+    uint32_t *agg_mdeg = nullptr, *agg_moff = nullptr, *agg_cnk = nullptr, *agg_coff = nullptr;
+    uint32_t *agg_dcnt = nullptr;
+    uint32_t agg_nchunks = 0;
     uint32_t *gkey = nullptr;
     ull *gval = nullptr;
     size_t garena = 0;
@@ -121,6 +124,7 @@ struct lv_graph {
     uint32_t last_iters = 0;
     // profiling
     bool prof_on = false;
+    std::vector<ProfRec> *capture_recs = nullptr;  // non-null while capturing a sweep
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> ev_pool;
     lv_kernel_stats prof = {};
@@ -197,7 +201,10 @@ struct ProfScope {
             hipEvent_t b = ev_get(g);
             if (b) {
                 (void)hipEventRecord(b, g->stream);
-                g->prof_pending.push_back({fam, a, b, bytes});
+                // inside a captured sweep the pair becomes event-record nodes,
+                // read back after every replay
+                (g->capture_recs ? *g->capture_recs : g->prof_pending)
+                    .push_back({fam, a, b, bytes});
             } else {
                 g->ev_pool.push_back(a);
             }
@@ -417,30 +424,60 @@ int sweep(lv_graph *g, float *q, uint64_t *moved) {
 // One pass's sweep as a HIP graph: the launch sequence is fixed within a
 // pass (same classes, same m), so it is captured once and replayed per sweep.
 struct SweepGraph {
+    lv_graph *g = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<ProfRec> recs;  // profiling event pairs recorded inside the graph
     ~SweepGraph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (g)
+            for (auto &r : recs) {
+                g->ev_pool.push_back(r.a);
+                g->ev_pool.push_back(r.b);
+            }
+    }
+    // accumulate the kernel times of the replay that just finished
+    void collect() {
+        for (auto &r : recs) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                g->prof.ms[r.fam] += ms;
+                g->prof.launches[r.fam] += 1;
+                g->prof.bytes[r.fam] += r.bytes;
+            } else {
+                (void)hipGetLastError();
+            }
+        }
     }
 };
 
-bool graphs_enabled(const lv_graph *g) {
+bool graphs_enabled(const lv_graph *) {
     static const bool off = getenv("LV_NO_GRAPH") != nullptr;
-    return !off && !g->prof_on;
+    return !off;
 }
 
-int capture_sweep(lv_graph *g, SweepGraph &sg) {
-    LV_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+// Captures one sweep; on failure the caller falls back to eager launches.
+bool capture_sweep(lv_graph *g, SweepGraph &sg) {
+    sg.g = g;
+    if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    g->capture_recs = &sg.recs;
     sweep_enqueue(g);
-    hipError_t e = hipStreamEndCapture(g->stream, &sg.graph);
-    if (e != hipSuccess) {
+    g->capture_recs = nullptr;
+    if (hipStreamEndCapture(g->stream, &sg.graph) != hipSuccess) {
         (void)hipGetLastError();
         sg.graph = nullptr;
-        return LV_ERR_HIP;
+        return false;
     }
-    LV_HIP(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
-    return LV_OK;
+    if (hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        sg.exec = nullptr;
+        return false;
+    }
+    return true;
 }
 
 int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_print,
@@ -451,17 +488,14 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
     bool first = true, ret = true;
     uint32_t it = 0;
     SweepGraph sg;
-    const bool use_graph = graphs_enabled(g);
-    if (use_graph) {
-        rc = capture_sweep(g, sg);
-        if (rc) return rc;
-    }
+    const bool use_graph = graphs_enabled(g) && capture_sweep(g, sg);
     for (;;) {
         uint64_t moved = 0;
         old = act;
         if (use_graph) {
             LV_HIP(hipGraphLaunch(sg.exec, g->stream));
             rc = sweep_finish(g, &act, &moved);
+            if (g->prof_on) sg.collect();
         } else {
             rc = sweep(g, &act, &moved);
         }
@@ -499,6 +533,12 @@ lv::AggView aview(lv_graph *g) {
     a.gkey = g->gkey;
     a.gval = g->gval;
     a.goff = g->agg_off;
+    a.mdeg = g->agg_mdeg;
+    a.moff = g->agg_moff;
+    a.cnk = g->agg_cnk;
+    a.coff = g->agg_coff;
+    a.dcnt = g->agg_dcnt;
+    a.nchunks = g->agg_nchunks;
     a.err = &g->dc->err;
     return a;
 }
@@ -535,13 +575,22 @@ int aggregation(lv_graph *g) {
     // global-hash arena for the large communities
     lv::launch_fill_u32(s, g->cnt, 0u, nn);  // edgeless communities keep 0
     const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
+    g->agg_nchunks = 0;
     if (nl) {
         const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
         lv::launch_agg_caps(s, aview(g), lc, nl, g->agg_caps);
         lv::launch_scan_u64(s, g->agg_caps, nl, g->agg_off, g->scan_tmp64);
+        lv::launch_agg_large_prep(s, v, aview(g), lc, nl);
+        lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
+        lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
+        lv::launch_fill_u32(s, g->agg_dcnt, 0u, nl);
         ull total = 0;
+        uint32_t chunks = 0;
         LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync(&chunks, g->agg_coff + nl, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              s));
         LV_HIP(hipStreamSynchronize(s));
+        g->agg_nchunks = chunks;
         if (total > g->garena) {
             dfree(g->gkey);
             dfree(g->gval);
@@ -661,6 +710,11 @@ void free_all(lv_graph *g) {
     dfree(g->sval);
     dfree(g->agg_caps);
     dfree(g->agg_off);
+    dfree(g->agg_mdeg);
+    dfree(g->agg_moff);
+    dfree(g->agg_cnk);
+    dfree(g->agg_coff);
+    dfree(g->agg_dcnt);
     dfree(g->gkey);
     dfree(g->gval);
     dfree(g->scan_tmp32);
@@ -756,6 +810,8 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |
          dmalloc(&g->sval, E);
     e |= dmalloc(&g->agg_caps, N) | dmalloc(&g->agg_off, N + 1);
+    e |= dmalloc(&g->agg_mdeg, N) | dmalloc(&g->agg_moff, N + 1) | dmalloc(&g->agg_cnk, N);
+    e |= dmalloc(&g->agg_coff, N + 1) | dmalloc(&g->agg_dcnt, N);
     e |= dmalloc(&g->scan_tmp32, lv::scan_tmp_elems(n + 1) + 8);
     e |= dmalloc(&g->scan_tmp64, lv::scan_tmp_elems(n + 1) + 8);
     e |= dmalloc(&g->dc, 1);

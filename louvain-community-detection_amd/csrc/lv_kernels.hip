@@ -1077,6 +1077,142 @@ __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
     }
 }
 
+// Large communities (comm_deg > 2048): their member entries form the flat
+// range [epos[c], epos[c] + comm_deg[c]) of the member-ordered entry sequence
+// (moff = scan of member degrees).  One workgroup per chunk of kAggChunk flat
+// entries merges its chunk in LDS, then adds each distinct neighbour
+// community once to the community's global table and lists newly claimed
+// slots; k_agg_merge_global then sorts the list.
+constexpr uint32_t kAggChunk = 2048;
+
+__global__ void k_agg_mdeg(DevView g, AggView a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) {
+        const uint32_t v = a.members[i];
+        a.mdeg[i] = g.row[v + 1] - g.row[v];
+    }
+}
+__global__ void k_agg_chunk_counts(AggView a, const uint32_t *comms, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a.cnk[i] = (a.comm_deg[comms[i]] + kAggChunk - 1) / kAggChunk;
+}
+
+__global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
+                                                           const uint32_t *comms, uint32_t count) {
+    constexpr int LCAP = 2 * kAggChunk, LC = Log2<LCAP>::value, PER = kAggChunk / 256;
+    __shared__ uint32_t lkey[LCAP];
+    __shared__ unsigned long long lval[LCAP];
+    __shared__ uint32_t ldist[kAggChunk];
+    __shared__ uint32_t mb[kAggChunk + 1];
+    __shared__ uint32_t nld, nnew, newbase, i0s, nmb;
+    // chunk -> community (largest ci with coff[ci] <= blockIdx.x)
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.coff[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t ci = lo, c = comms[ci];
+    const uint32_t k = blockIdx.x - a.coff[ci];
+    const uint32_t cb = a.epos[c], ce = cb + a.comm_deg[c];
+    const uint32_t f0 = cb + k * kAggChunk, f1 = min(ce, f0 + kAggChunk);
+    const uint32_t m0 = a.mstart[c], m1 = m0 + g.comm_sz[c];
+    for (int i = threadIdx.x; i < LCAP; i += 256) {
+        lkey[i] = kEmpty;
+        lval[i] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        nld = 0;
+        nnew = 0;
+        uint32_t l = m0, h = m1;  // last member with moff <= f0
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (a.moff[mid] <= f0) l = mid;
+            else h = mid;
+        }
+        i0s = l;
+        // members of a large community have degree >= 1, so at most
+        // kAggChunk of them cover the chunk
+        nmb = min(m1 - l, (uint32_t)kAggChunk);
+    }
+    __syncthreads();
+    const uint32_t i0 = i0s, nm = nmb;
+    for (uint32_t q = threadIdx.x; q <= nm; q += 256) mb[q] = a.moff[i0 + q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t f = f0 + threadIdx.x + 256u * q;
+        if (f < f1) {
+            uint32_t l = 0, h = nm;  // member of flat entry f
+            while (h - l > 1) {
+                const uint32_t mid = (l + h) >> 1;
+                if (mb[mid] <= f) l = mid;
+                else h = mid;
+            }
+            const uint32_t v = a.members[i0 + l];
+            const uint32_t j = g.row[v] + (f - mb[l]);
+            const uint32_t dd = g.comm[g.col[j]];
+            const unsigned long long wf = f2fix(g.w[j], g.scale);
+            uint32_t slot = hslot<LC>(dd);
+            for (int probe = 0; probe < LCAP; ++probe) {
+                const uint32_t prev = atomicCAS(&lkey[slot], kEmpty, dd);
+                if (prev == kEmpty) {
+                    ldist[atomicAdd(&nld, 1u)] = slot;
+                    break;
+                }
+                if (prev == dd) break;
+                slot = (slot + 1) & (LCAP - 1);
+            }
+            atomicAdd(&lval[slot], wf);
+        }
+    }
+    __syncthreads();
+    const unsigned long long off = a.goff[ci];
+    const uint32_t cap = (uint32_t)(a.goff[ci + 1] - off);
+    const int lcap = 31 - __clz(cap);
+    uint32_t *key = a.gkey + off;
+    unsigned long long *val = a.gval + off;
+    const uint32_t nd = nld;
+    uint32_t my_new[PER], my_rank[PER], my_key[PER];
+    int nmine = 0;
+    for (uint32_t i = threadIdx.x; i < nd; i += 256) {
+        const uint32_t ls = ldist[i];
+        const uint32_t dd = lkey[ls];
+        uint32_t slot = hslot_dyn(dd, lcap);
+        uint32_t probe = 0;
+        bool claimed = false;
+        for (; probe < cap; ++probe) {
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
+            if (prev == kEmpty) {
+                claimed = true;
+                break;
+            }
+            if (prev == dd) break;
+            slot = (slot + 1) & (cap - 1);
+        }
+        if (probe == cap) {  // cannot happen: cap >= 2*comm_deg
+            atomicOr(a.err, 1u);
+            continue;
+        }
+        atomicAdd(&val[slot], lval[ls]);
+        if (claimed) {
+            my_new[nmine] = slot;
+            my_key[nmine] = dd;
+            my_rank[nmine] = atomicAdd(&nnew, 1u);
+            ++nmine;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) newbase = nnew ? atomicAdd(&a.dcnt[ci], nnew) : 0u;
+    __syncthreads();
+    uint32_t *ok = a.tkey + cb;
+    unsigned long long *ov = a.tval + cb;
+    for (int q = 0; q < nmine; ++q) {
+        ok[newbase + my_rank[q]] = my_key[q];
+        ov[newbase + my_rank[q]] = my_new[q];
+    }
+}
+
 // global-hash merge for large communities, then an in-workgroup stable LSD
 // radix sort of the distinct keys (8-bit digits) with scratch skey/sval.
 constexpr int kRadixThreads = 1024;
@@ -1085,7 +1221,6 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
                                                                     uint32_t count, int passes) {
     constexpr int P = kRadixThreads;
     constexpr int NW = P / 64;
-    __shared__ uint32_t ndist;
     __shared__ uint32_t hist[256];
     __shared__ uint32_t binbase[256];
     __shared__ uint32_t ctot[256];
@@ -1093,45 +1228,15 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     const uint32_t ci = blockIdx.x;
     const uint32_t c = comms[ci];
     const unsigned long long off = a.goff[ci];
-    const uint32_t cap = (uint32_t)(a.goff[ci + 1] - off);
-    const int lcap = 31 - __clz(cap);
     uint32_t *key = a.gkey + off;
     unsigned long long *val = a.gval + off;
     const uint32_t out = a.epos[c];
     uint32_t *ok = a.tkey + out;
     unsigned long long *ov = a.tval + out;
-    if (threadIdx.x == 0) ndist = 0;
-    __syncthreads();
-    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t mi = wv; mi < msz; mi += NW) {
-        const uint32_t v = a.members[m0 + mi];
-        const uint32_t b = g.row[v], e = g.row[v + 1];
-        for (uint32_t j = b + lane; j < e; j += 64) {
-            uint32_t dd = g.comm[g.col[j]];
-            unsigned long long wf = f2fix(g.w[j], g.scale);
-            uint32_t slot = hslot_dyn(dd, lcap);
-            uint32_t probe = 0;
-            for (; probe < cap; ++probe) {
-                uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
-                if (prev == kEmpty) {
-                    uint32_t idx = atomicAdd(&ndist, 1u);
-                    ok[idx] = dd;
-                    ov[idx] = slot;
-                    break;
-                }
-                if (prev == dd) break;
-                slot = (slot + 1) & (cap - 1);
-            }
-            if (probe == cap) {
-                atomicOr(a.err, 1u);
-                continue;
-            }
-            atomicAdd(&val[slot], wf);
-        }
-    }
-    __syncthreads();
-    const uint32_t nd = ndist;
+    // entries were inserted by k_agg_insert_chunks; the nd distinct keys are
+    // listed at ok[0..nd) with their table slots in ov[0..nd)
+    const uint32_t nd = a.dcnt[ci];
     // gather sums and clean the arena
     for (uint32_t i = threadIdx.x; i < nd; i += P) {
         uint32_t slot = (uint32_t)ov[i];
@@ -1218,9 +1323,21 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
     else if (cls == 2)
         hipLaunchKernelGGL((k_agg_merge_lds<256, 4096>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
-    else
+    else {
+        hipLaunchKernelGGL(k_agg_insert_chunks, dim3(a.nchunks), dim3(256), 0, s, g, a, comms,
+                           count);
         hipLaunchKernelGGL(k_agg_merge_global, dim3(count), dim3(kRadixThreads), 0, s, g, a, comms,
                            count, radix_passes);
+    }
+}
+
+void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
+                           uint32_t count) {
+    if (g.n)
+        hipLaunchKernelGGL(k_agg_mdeg, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a);
+    if (count)
+        hipLaunchKernelGGL(k_agg_chunk_counts, dim3((count + 255) / 256), dim3(256), 0, s, a,
+                           comms, count);
 }
 
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
