@@ -68,8 +68,14 @@ struct DevView {
     uint32_t n;
     int S;          // fixed-point scale bits
     float scale;    // 2^S
-    float m;        // all_w
+    float m;        // all_w (single-graph mode)
+    // batch mode (gid != nullptr): many graphs in one segmented CSR
+    const uint32_t *gid;     // vertex -> graph
+    const float *gm;         // all_w per graph
+    const uint32_t *gstate;  // per graph; only kGraphActive graphs move
+    uint32_t G;              // graphs (stride of the per-graph accumulators)
 };
+constexpr uint32_t kGraphActive = 1;
 
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
 constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
@@ -79,8 +85,7 @@ struct HeavySplit {
     uint32_t nchunks;                    // total chunks = coff[nh]
     const uint32_t *coff;                // nh+1 chunk offsets
     const unsigned long long *slot_off;  // nh+1 table offsets in the arena
-    uint32_t *key;                       // arena
-    unsigned long long *val;             // arena
+    unsigned long long *tab;             // arena of 16-byte slots {key, pad, sum}
     uint32_t *dlist;                     // ne: distinct slots at [row[v], row[v+1])
     uint32_t *eslot;                     // ne: slot of each entry
     unsigned long long *hcnt;            // nh distinct counts, followed by
@@ -105,7 +110,10 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n,
 // `acc` receives the exact fixed-point total (atomic).
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                int group, int masked, int init_state, unsigned long long *acc);
-void launch_sumsq(hipStream_t s, const DevView &g, unsigned long long *limbs /*4*/);
+// sum of round32(a_c)^2 over the communities in `verts`; limbs: 4 arrays of
+// kSpread (single graph) or G (batch) slots
+void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
+                  unsigned long long *limbs);
 
 // classes 1-7 (class 8 goes through launch_heavy_move)
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
@@ -118,6 +126,7 @@ void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts
 // ---- launchers (lv_heavy.hip) -----------------------------------------------
 void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *verts,
                          uint32_t count, uint32_t *nchunks);
+void launch_fill_slots(hipStream_t s, unsigned long long *tab, size_t cap);
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h);
 void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float *stage,
                      int masked, int init_state, unsigned long long *acc);

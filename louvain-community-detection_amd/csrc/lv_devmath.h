@@ -108,14 +108,47 @@ __device__ __forceinline__ uint32_t last_entry(uint32_t wl, uint32_t d) {
 }
 
 // final decision for buckets 6-7
-__device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, uint32_t v, uint32_t base,
+__device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, float m, uint32_t base,
                                                     uint32_t d, uint32_t cv, float bg,
                                                     uint32_t bp, unsigned long long cew) {
     if (bg == -INFINITY || bp >= d) return cv;
     uint32_t dest = g.comm[g.col[base + last_entry(bp, d)]];
     float cewf = fix2f(cew, g.S);
-    if (bg - (cewf / g.m) > 0.0f) return dest;
+    if (bg - (cewf / m) > 0.0f) return dest;
     return cv;
+}
+
+// ---- single-graph / batch mode --------------------------------------------
+// a vertex takes part in the current sweep (batch: its graph is active)
+__device__ __forceinline__ bool v_live(const DevView &g, uint32_t v) {
+    return !g.gid || g.gstate[g.gid[v]] == kGraphActive;
+}
+// the modularity normaliser m of the vertex's graph
+__device__ __forceinline__ float v_m(const DevView &g, uint32_t v) {
+    return g.gid ? g.gm[g.gid[v]] : g.m;
+}
+// accumulator slot for an exact total: the vertex's graph in batch mode, a
+// spread slot (block % kSpread) in single-graph mode
+__device__ __forceinline__ uint32_t acc_slot(const DevView &g, uint32_t v) {
+    return g.gid ? g.gid[v] : (blockIdx.x % kSpread);
+}
+__device__ __forceinline__ uint32_t acc_stride(const DevView &g) {
+    return g.gid ? g.G : kSpread;
+}
+// base[key] += x over the valid lanes of a wave: one atomic when the valid
+// lanes share the key, else one per lane.  Call from every lane of the wave.
+__device__ __forceinline__ void wave_add(unsigned long long *base, uint32_t key,
+                                         unsigned long long x, bool valid) {
+    const unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    const int first = __ffsll((long long)vm) - 1;
+    const uint32_t k0 = __shfl(key, first, 64);
+    if (__ballot(valid && key != k0) == 0ull) {
+        const unsigned long long s = wave_sum_u64(valid ? x : 0ull);
+        if ((int)__lane_id() == first && s) atomicAdd(base + k0, s);
+    } else if (valid && x) {
+        atomicAdd(base + key, x);
+    }
 }
 
 }  // namespace lv

@@ -94,8 +94,7 @@ struct lv_graph {
     ull *heavy_caps = nullptr, *heavy_off = nullptr;
     uint32_t *heavy_nch = nullptr, *heavy_coff = nullptr;
     uint32_t heavy_chunks = 0;
-    uint32_t *hkey = nullptr;
-    ull *hval = nullptr;
+    ull *htab = nullptr;  // heavy arena: 2 words per slot
     size_t harena = 0;
     ull *hscr = nullptr;
     size_t hscr_cap = 0;
@@ -122,6 +121,18 @@ struct lv_graph {
     float m = 0.0f;
     bool pass_ready = false;
     uint32_t last_iters = 0;
+    // batch mode (G > 0): G graphs in one segmented CSR, vertex ids global
+    uint32_t G = 0;
+    uint32_t *gid = nullptr;       // device: vertex -> graph
+    float *gm_d = nullptr;         // device: m per graph
+    uint32_t *gstate_d = nullptr;  // device: graph state
+    uint32_t *voff_d = nullptr;    // device: G+1 current vertex offsets
+    uint32_t *vgather = nullptr;   // device: G+1 scratch
+    ull *gcnt = nullptr;           // device: per-graph acc[G] | limbs[4][G] | moved[G]
+    ull *gcnt_h = nullptr;         // pinned host mirror
+    std::vector<uint32_t> voff, voff0;  // host: current / original vertex offsets
+    std::vector<float> gm_h;
+    std::vector<uint32_t> gstate_h;
     // profiling
     bool prof_on = false;
     std::vector<ProfRec> *capture_recs = nullptr;  // non-null while capturing a sweep
@@ -167,8 +178,17 @@ lv::DevView view(lv_graph *g) {
     v.S = g->S;
     v.scale = g->scale;
     v.m = g->m;
+    v.gid = g->G ? g->gid : nullptr;
+    v.gm = g->gm_d;
+    v.gstate = g->gstate_d;
+    v.G = g->G;
     return v;
 }
+
+// exact-total accumulators: spread slots (single graph) or per graph (batch)
+ull *acc_ptr(lv_graph *g) { return g->G ? g->gcnt : g->dc->acc; }
+ull *limbs_ptr(lv_graph *g) { return g->G ? g->gcnt + g->G : &g->dc->limbs[0][0]; }
+ull *moved_ptr(lv_graph *g) { return g->G ? g->gcnt + 5ull * g->G : g->dc->moved; }
 
 hipEvent_t ev_get(lv_graph *g) {
     if (!g->ev_pool.empty()) {
@@ -212,16 +232,26 @@ struct ProfScope {
     }
 };
 
+void counters_d2h(lv_graph *g);
 int sync_counters(lv_graph *g) {
-    LV_HIP(hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+    counters_d2h(g);
     LV_HIP(hipStreamSynchronize(g->stream));
     return LV_OK;
 }
 
 int zero_counters(lv_graph *g) {
-    // small fill kernel instead of a memset node (safe under stream capture)
+    // small fill kernels instead of memset nodes (safe under stream capture)
     lv::launch_fill_u32(g->stream, (uint32_t *)g->dc, 0u, sizeof(Counters) / 4);
+    if (g->G) lv::launch_fill_u64(g->stream, g->gcnt, 0ull, 6ull * g->G);
     return LV_OK;
+}
+
+// D2H of the counters (and the per-graph counters in batch mode)
+void counters_d2h(lv_graph *g) {
+    (void)hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream);
+    if (g->G)
+        (void)hipMemcpyAsync(g->gcnt_h, g->gcnt, 6ull * g->G * sizeof(ull), hipMemcpyDeviceToHost,
+                             g->stream);
 }
 
 float fix2f_host(const lv_graph *g, int64_t x) { return (float)x * g->inv_scale; }
@@ -270,13 +300,11 @@ int classify(lv_graph *g) {
         LV_HIP(hipStreamSynchronize(g->stream));
         g->heavy_chunks = chunks;
         if (total > g->harena) {
-            dfree(g->hkey);
-            dfree(g->hval);
+            dfree(g->htab);
             size_t cap = (size_t)total;
-            if (dmalloc(&g->hkey, cap) || dmalloc(&g->hval, cap)) return LV_ERR_NOMEM;
+            if (dmalloc(&g->htab, 2 * cap)) return LV_ERR_NOMEM;
             g->harena = cap;
-            lv::launch_fill_u32(g->stream, g->hkey, lv::kEmpty, cap);
-            lv::launch_fill_u64(g->stream, g->hval, 0ull, cap);
+            lv::launch_fill_slots(g->stream, g->htab, cap);
         }
         if ((size_t)nh * 130 > g->hscr_cap) {
             dfree(g->hscr);
@@ -295,8 +323,7 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.nchunks = g->heavy_chunks;
     h.coff = g->heavy_coff;
     h.slot_off = g->heavy_off;
-    h.key = g->hkey;
-    h.val = g->hval;
+    h.tab = g->htab;
     h.dlist = g->dlist;
     h.eslot = g->eslot;
     h.hcnt = g->hscr;
@@ -318,12 +345,12 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         double ent = 0;
         for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
-        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, g->dc->acc);
+        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, acc_ptr(g));
     }
     if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy]);
         lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
-                            g->dc->acc);
+                            acc_ptr(g));
     }
     return LV_OK;
 }
@@ -344,7 +371,7 @@ void modularity_launch(lv_graph *g) {
     r1_all(g, 1, 0);
     lv::DevView v = view(g);
     ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
-    lv::launch_sumsq(g->stream, v, &g->dc->limbs[0][0]);
+    lv::launch_sumsq(g->stream, v, g->order, g->n, limbs_ptr(g));
 }
 
 int pass_init(lv_graph *g, float *q0) {
@@ -394,7 +421,7 @@ int step(lv_graph *g, int b) {
     const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
     if (e > a) {
         ProfScope ps(g, LV_K_APPLY, 16.0 * (e - a));
-        lv::launch_apply(g->stream, v, g->order + a, e - a, g->dc->moved);
+        lv::launch_apply(g->stream, v, g->order + a, e - a, moved_ptr(g));
     }
     return LV_OK;
 }
@@ -405,7 +432,7 @@ void sweep_enqueue(lv_graph *g) {
     zero_counters(g);
     for (int b = 1; b <= 7; ++b) step(g, b);
     modularity_launch(g);
-    (void)hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream);
+    counters_d2h(g);
 }
 
 int sweep_finish(lv_graph *g, float *q, uint64_t *moved) {
@@ -692,8 +719,7 @@ void free_all(lv_graph *g) {
     dfree(g->heavy_nch);
     dfree(g->heavy_coff);
     dfree(g->hscr);
-    dfree(g->hkey);
-    dfree(g->hval);
+    dfree(g->htab);
     dfree(g->dlist);
     dfree(g->eslot);
     dfree(g->new_id);

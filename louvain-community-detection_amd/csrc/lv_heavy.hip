@@ -48,11 +48,21 @@ __device__ __forceinline__ float f32_unord(uint32_t o) {
     return __uint_as_float(u);
 }
 
+// Table slots are 16 bytes {u32 key, u32 pad, u64 sum} so a probe and its sum
+// share one cache line.
+__device__ __forceinline__ uint32_t *tkey(unsigned long long *tab, uint32_t s) {
+    return reinterpret_cast<uint32_t *>(tab + 2ull * s);
+}
+__device__ __forceinline__ unsigned long long *tval(unsigned long long *tab, uint32_t s) {
+    return tab + 2ull * s + 1ull;
+}
+
 struct HeavyCtx {
     uint32_t i, v, base, d, c;
     unsigned long long off;
     uint32_t cap;
     int lcap;
+    bool live;
 };
 
 __device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit &h) {
@@ -65,6 +75,7 @@ __device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit
     x.off = h.slot_off[x.i];
     x.cap = (uint32_t)(h.slot_off[x.i + 1] - x.off);
     x.lcap = 31 - __clz(x.cap);
+    x.live = v_live(g, x.v);
     return x;
 }
 
@@ -83,8 +94,8 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
     __shared__ uint32_t nld, nnew, newbase;
     __shared__ unsigned long long part[4];
     const HeavyCtx x = heavy_ctx(g, h);
-    uint32_t *key = h.key + x.off;
-    unsigned long long *val = h.val + x.off;
+    if (!x.live) return;  // uniform over the workgroup
+    unsigned long long *tab = h.tab + 2ull * x.off;
     for (int i = threadIdx.x; i < kLocalCap; i += 256) {
         lkey[i] = kEmpty;
         lval[i] = 0ull;
@@ -137,7 +148,7 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
         uint32_t probe = 0;
         bool claimed = false;
         for (; probe < x.cap; ++probe) {
-            const uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+            const uint32_t prev = atomicCAS(tkey(tab, slot), kEmpty, ce);
             if (prev == kEmpty) {
                 claimed = true;
                 break;
@@ -149,7 +160,7 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
             atomicOr(h.err, 1u);
             slot = 0;
         }
-        atomicAdd(&val[slot], lval[ls]);
+        atomicAdd(tval(tab, slot), lval[ls]);
         lglob[ls] = slot;
         if (claimed) {
             my_new[nmine] = slot;
@@ -174,8 +185,8 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
 
 __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
-    uint32_t *key = h.key + x.off;
-    unsigned long long *val = h.val + x.off;
+    if (!x.live) return;  // uniform over the workgroup
+    unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
     if (i0 >= nd) return;
@@ -186,24 +197,26 @@ __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
     const float acv = fix2f(g.A[cv], g.S);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
         const uint32_t slot = h.dlist[x.base + i];
-        const uint32_t c = key[slot];
+        const uint32_t c = *tkey(tab, slot);
         float gg = -INFINITY;
         bool cand = true;  // isLowerIdx (modularity.cu:85-93)
         if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
-        val[slot] = (unsigned long long)__float_as_uint(gg);
+        if (cand)
+            gg = gain_of(fix2f(*tval(tab, slot), g.S), v_m(g, x.v), kv, acv, fix2f(g.A[c], g.S));
+        *tval(tab, slot) = (unsigned long long)__float_as_uint(gg);
     }
 }
 
 __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
-    const unsigned long long *val = h.val + x.off;
+    if (!x.live) return;  // uniform over the workgroup
+    unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t j0 = x.c * kHeavyChunk;  // multiple of 128: thread t serves lane t % 128
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
     float tg = -INFINITY;
     uint32_t tj = 0xffffffffu;
     for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
-        const float gg = __uint_as_float((uint32_t)val[h.eslot[x.base + j]]);
+        const float gg = __uint_as_float((uint32_t)*tval(tab, h.eslot[x.base + j]));
         if (gg > tg) {  // entries visited in increasing order: first maximum
             tg = gg;
             tj = j;
@@ -235,9 +248,10 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
     if (i >= h.nh) return;
     const uint32_t p = threadIdx.x & 63;
     const uint32_t v = h.verts[i];
+    if (!v_live(g, v)) return;  // uniform over the wave
     const uint32_t base = g.row[v];
     const uint32_t d = g.row[v + 1] - base;
-    const uint32_t *key = h.key + h.slot_off[i];
+    unsigned long long *tab = h.tab + 2ull * h.slot_off[i];
     float gl[2];
     uint32_t cl[2];
 #pragma unroll
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
         } else {
             gl[q] = f32_unord((uint32_t)(k >> 32));
             const uint32_t j = 0xffffffffu - (uint32_t)(k & 0xffffffffull);
-            cl[q] = key[h.eslot[base + j]];
+            cl[q] = *tkey(tab, h.eslot[base + j]);
         }
     }
     float bg;
@@ -257,22 +271,22 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
     tree128(gl[0], cl[0], gl[1], cl[1], bg, bc, bp);
     if (p == 0) {
         const uint32_t cv = g.comm[v];
-        g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, h.hcew[i]);
+        g.new_comm[v] = decide_lastedge(g, v_m(g, v), base, d, cv, bg, bp, h.hcew[i]);
     }
 }
 
 __global__ __launch_bounds__(256) void k_heavy_clean(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
-    uint32_t *key = h.key + x.off;
-    unsigned long long *val = h.val + x.off;
+    if (!x.live) return;  // uniform over the workgroup
+    unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
     if (i0 >= nd) return;
     const uint32_t i1 = min(nd, i0 + kHeavyChunk);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
         const uint32_t slot = h.dlist[x.base + i];
-        key[slot] = kEmpty;
-        val[slot] = 0ull;
+        tab[2ull * slot] = (unsigned long long)kEmpty;
+        tab[2ull * slot + 1ull] = 0ull;
     }
 }
 
@@ -283,6 +297,21 @@ __global__ void k_heavy_chunks(const uint32_t *row, const uint32_t *verts, uint3
         const uint32_t v = verts[i];
         nchunks[i] = (row[v + 1] - row[v] + kHeavyChunk - 1) / kHeavyChunk;
     }
+}
+
+__global__ void k_fill_slots(unsigned long long *tab, size_t cap) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < cap; i += stride) {
+        tab[2 * i] = (unsigned long long)kEmpty;
+        tab[2 * i + 1] = 0ull;
+    }
+}
+void launch_fill_slots(hipStream_t s, unsigned long long *tab, size_t cap) {
+    if (!cap) return;
+    size_t gsz = (cap + 255) / 256;
+    if (gsz > 65536) gsz = 65536;
+    hipLaunchKernelGGL(k_fill_slots, dim3((unsigned)gsz), dim3(256), 0, s, tab, cap);
 }
 
 void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *verts,
@@ -311,6 +340,7 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h) {
 // is in v's community, else +0 (adding +0 to a partial >= +0 changes nothing)
 __global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, float *stage) {
     const HeavyCtx x = heavy_ctx(g, h);
+    if (!x.live) return;  // uniform over the workgroup
     const uint32_t cv = g.comm[x.v];
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
@@ -328,6 +358,7 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
                                                   unsigned long long *acc) {
     __shared__ float buf[2][32][kR1K + 1];
     const uint32_t v = h.verts[blockIdx.x];
+    if (!v_live(g, v)) return;  // uniform over the workgroup
     const uint32_t base = g.row[v];
     const uint32_t d = g.row[v + 1] - base;
     const float *src = masked ? stage + base : g.w + base;
@@ -364,7 +395,7 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
                     g.A[v] = fixed;
                 }
             }
-            if (fixed) atomicAdd(&acc[blockIdx.x % kSpread], fixed);
+            if (fixed) atomicAdd(&acc[acc_slot(g, v)], fixed);
         }
     }
 }

@@ -137,8 +137,10 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
     const int lane = threadIdx.x % G;
     const uint32_t vi = blockIdx.x * GPB + threadIdx.x / G;
     unsigned long long fixed = 0;
-    if (vi < count) {
-        const uint32_t v = verts[vi];
+    uint32_t slot = 0;
+    bool valid = false;
+    const uint32_t v = vi < count ? verts[vi] : 0u;
+    if (vi < count && v_live(g, v)) {
         const uint32_t base = g.row[v];
         const uint32_t d = g.row[v + 1] - base;
         const uint32_t cv = MASKED ? g.comm[v] : 0u;
@@ -168,6 +170,8 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
         for (int o = G / 2; o > 0; o >>= 1) p += __shfl_down(p, o, G);
         if (lane == 0) {
             fixed = f2fix(p, g.scale);
+            slot = acc_slot(g, v);
+            valid = true;
             if (!MASKED) {
                 g.k[v] = p;
                 if (init_state) {
@@ -178,15 +182,7 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
             }
         }
     }
-    // block total (exact, order-free)
-    __shared__ unsigned long long part[4];
-    unsigned long long t = wave_sum_u64(fixed);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long s = part[0] + part[1] + part[2] + part[3];
-        if (s) atomicAdd(&acc[blockIdx.x % kSpread], s);
-    }
+    wave_add(acc, slot, fixed, valid);  // exact, order-free
 }
 
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
@@ -212,12 +208,20 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
 // sum over communities of round32(a_c)^2, exact at scale 2^2S, as four
 // 32-bit limbs accumulated in u64 (modularity.cu:431-432, square<float>)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sumsq(DevView g, unsigned long long *limbs) {
-    unsigned long long l0 = 0, l1 = 0, l2 = 0, l3 = 0;
+__global__ __launch_bounds__(256) void k_sumsq(DevView g, const uint32_t *verts, uint32_t count,
+                                               unsigned long long *limbs) {
     const int twoS = 2 * g.S;
-    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < g.n; c += gridDim.x * 256) {
-        unsigned long long a = g.A[c];
-        if (!a) continue;
+    const uint32_t stride = acc_stride(g);
+    for (uint32_t base_i = blockIdx.x * 256; base_i < count; base_i += gridDim.x * 256) {
+        const uint32_t i = base_i + threadIdx.x;
+        unsigned long long l0 = 0, l1 = 0, l2 = 0, l3 = 0;
+        uint32_t slot = 0;
+        bool valid = false;
+        const uint32_t c = i < count ? verts[i] : 0u;
+        const unsigned long long a = (i < count && v_live(g, c)) ? g.A[c] : 0ull;
+        if (a) {
+        valid = true;
+        slot = acc_slot(g, c);
         float af = fix2f(a, g.S);
         float a2 = af * af;
         uint32_t u = __float_as_uint(a2);
@@ -242,32 +246,23 @@ __global__ __launch_bounds__(256) void k_sumsq(DevView g, unsigned long long *li
             lo = 0ull;
             hi = (unsigned long long)man << (sh - 64);
         }
-        l0 += lo & 0xffffffffull;
-        l1 += lo >> 32;
-        l2 += hi & 0xffffffffull;
-        l3 += hi >> 32;
-    }
-    __shared__ unsigned long long part[4][4];
-    unsigned long long t0 = wave_sum_u64(l0), t1 = wave_sum_u64(l1), t2 = wave_sum_u64(l2),
-                       t3 = wave_sum_u64(l3);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        part[w][0] = t0;
-        part[w][1] = t1;
-        part[w][2] = t2;
-        part[w][3] = t3;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        unsigned long long s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                               part[3][threadIdx.x];
-        if (s) atomicAdd(&limbs[threadIdx.x * kSpread + blockIdx.x % kSpread], s);
+        l0 = lo & 0xffffffffull;
+        l1 = lo >> 32;
+        l2 = hi & 0xffffffffull;
+        l3 = hi >> 32;
+        }
+        wave_add(limbs, slot, l0, valid);
+        wave_add(limbs + stride, slot, l1, valid);
+        wave_add(limbs + 2 * stride, slot, l2, valid);
+        wave_add(limbs + 3 * stride, slot, l3, valid);
     }
 }
 
-void launch_sumsq(hipStream_t s, const DevView &g, unsigned long long *limbs) {
-    if (!g.n) return;
-    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, s, g, limbs);
+void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
+                  unsigned long long *limbs) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, g, verts, count,
+                       limbs);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,11 +296,12 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         val[i] = 0ull;
     }
     const uint32_t vi = blockIdx.x * GPB + grp;
-    const bool active = vi < count;
-    uint32_t v = 0, base = 0, d = 0, cv = 0, szv = 0;
-    float kv = 0.0f, acv = 0.0f;
+    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const bool active = vi < count && v_live(g, v);
+    uint32_t base = 0, d = 0, cv = 0, szv = 0;
+    float kv = 0.0f, acv = 0.0f, m = 1.0f;
     if (active) {
-        v = verts[vi];
+        m = v_m(g, v);
         base = g.row[v];
         d = g.row[v + 1] - base;
         cv = g.comm[v];
@@ -340,7 +336,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
             bool cand = true;  // isLowerIdx (modularity.cu:85-93)
             if (szv == 1u && g.comm_sz[ce] == 1u) cand = ce < cv;
             if (cand) {
-                float gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[ce], g.S));
+                float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[ce], g.S));
                 if (gg > best_g) {
                     best_g = gg;
                     best_c = ce;
@@ -364,7 +360,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         uint32_t nc = cv;
         if (best_g != -INFINITY) {
             float cewf = fix2f(cew, g.S);
-            if (best_g - (cewf / g.m) > 0.0f) nc = best_c;
+            if (best_g - (cewf / m) > 0.0f) nc = best_c;
         }
         g.new_comm[v] = nc;
     }
@@ -395,11 +391,12 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         scew[wv] = 0;
     }
     const uint32_t vi = blockIdx.x * 4 + wv;
-    const bool active = vi < count;
-    uint32_t v = 0, base = 0, d = 0, cv = 0, szv = 0;
-    float kv = 0.0f, acv = 0.0f;
+    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const bool active = vi < count && v_live(g, v);
+    uint32_t base = 0, d = 0, cv = 0, szv = 0;
+    float kv = 0.0f, acv = 0.0f, m = 1.0f;
     if (active) {
-        v = verts[vi];
+        m = v_m(g, v);
         base = g.row[v];
         d = g.row[v + 1] - base;
         cv = g.comm[v];
@@ -444,7 +441,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         float gg = -INFINITY;
         bool cand = true;
         if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
     __syncthreads();
@@ -470,7 +467,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
     float bg;
     uint32_t bc, bp;
     tree128(g0, c0, g1, c1, bg, bc, bp);
-    if (active && p == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, cew);
+    if (active && p == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, cew);
 }
 
 // Combine per-thread lane states into the 128 virtual lanes and run the
@@ -535,6 +532,8 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
         scew = 0;
     }
     const uint32_t v = verts[blockIdx.x];
+    if (!v_live(g, v)) return;  // uniform over the workgroup
+    const float m = v_m(g, v);
     const uint32_t base = g.row[v];
     const uint32_t d = g.row[v + 1] - base;
     const uint32_t cv = g.comm[v];
@@ -577,7 +576,7 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
         float gg = -INFINITY;
         bool cand = true;
         if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), g.m, kv, acv, fix2f(g.A[c], g.S));
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
     __syncthreads();
@@ -596,7 +595,7 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     float bg = -INFINITY;
     uint32_t bc = kEmpty, bp = kEmpty;
     block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp);
-    if (t == 0) g.new_comm[v] = decide_lastedge(g, v, base, d, cv, bg, bp, scew);
+    if (t == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, scew);
 }
 
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
@@ -672,8 +671,9 @@ __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts,
                                                unsigned long long *moved) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long mv = 0;
-    if (i < count) {
-        const uint32_t v = verts[i];
+    const uint32_t v = i < count ? verts[i] : 0u;
+    const bool valid = i < count && v_live(g, v);
+    if (valid) {
         const uint32_t o = g.comm[v], nc = g.new_comm[v];
         if (o != nc) {
             const unsigned long long K = f2fix(g.k[v], g.scale);
@@ -685,14 +685,7 @@ __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts,
             mv = 1;
         }
     }
-    __shared__ unsigned long long part[4];
-    mv = wave_sum_u64(mv);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = mv;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long s = part[0] + part[1] + part[2] + part[3];
-        if (s) atomicAdd(&moved[blockIdx.x % kSpread], s);
-    }
+    wave_add(moved, valid ? acc_slot(g, v) : 0u, mv, valid);
 }
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved) {
