@@ -34,6 +34,9 @@ CONFIGS = {
                args=(262111, 9.4, 2.5, 420, 2000, 20.0, 2)),
     # R-MAT scale 24, edge factor 16, w ~ U(0.5, 1.5)
     "c3": dict(workload="rmat-24-ef16", gen="rmat", args=(24, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)),
+    # 4096 stochastic-block-model graphs in one segmented CSR
+    "c4": dict(workload="batch-4096-sbm", gen="batch", args=(4096, 100, 50000, 8, 64, 8.0, 32.0,
+                                                            10.0, 4)),
 }
 # the same recipe at a size the single-core oracle finishes in ~10-30 s
 CPU_SAMPLES = {
@@ -44,7 +47,26 @@ CPU_SAMPLES = {
 }
 
 
+def batch_specs(G, vmin, vmax, bmin, bmax, dmin, dmax, p_ratio, seed):
+    """Per-graph SBM parameters: V log-uniform, blocks and mean degree uniform."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(vmin), np.log(vmax), G)).astype(np.int64)
+    blocks = rng.integers(bmin, bmax + 1, G)
+    deg = rng.uniform(dmin, dmax, G)
+    return [(int(n[i]), int(min(blocks[i], n[i])), float(deg[i]), p_ratio, seed * 100003 + i)
+            for i in range(G)]
+
+
+def make_batch(lv, args):
+    from concurrent.futures import ThreadPoolExecutor
+    specs = batch_specs(*args)
+    with ThreadPoolExecutor(max_workers=16) as ex:  # generators release the GIL
+        return list(ex.map(lambda sp: lv.gen_sbm(*sp), specs))
+
+
 def make_graph(lv, cfg):
+    if cfg["gen"] == "batch":
+        return make_batch(lv, cfg["args"])
     return getattr(lv, "gen_" + cfg["gen"])(*cfg["args"])
 
 
@@ -85,6 +107,8 @@ def main():
     if lv.device_count() < 1:
         raise RuntimeError("no GPU visible")
     cfg = CONFIGS[args.config]
+    if cfg["gen"] == "batch":
+        return bench_batch(lv, cfg, args)
     t0 = time.perf_counter()
     graph = make_graph(lv, cfg)
     row, col, w = graph
@@ -152,6 +176,79 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lv, args.config, graph)
     g.close()
+    print(json.dumps(line))
+
+
+def bench_batch(lv, cfg, args):
+    """graphs/s of the batched engine vs one-graph-at-a-time plain runs on the
+    same GPU, with identical per-graph results."""
+    t0 = time.perf_counter()
+    graphs = make_graph(lv, cfg)
+    nv = sum(int(r.size) - 1 for r, _, _ in graphs)
+    ne = sum(int(c.size) for _, c, _ in graphs)
+    print(f"[bench] {cfg['workload']}: {len(graphs)} graphs, {nv} vertices, {ne} entries "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    b = lv.Batch(graphs)
+    for i in range(args.warmup):
+        b.reset()
+        per, tot = b.run(MIN_GAIN)
+        print(f"[bench] warmup {i}: {tot.alg_ms:.1f} ms", file=sys.stderr, flush=True)
+    times = []
+    for i in range(args.steps):
+        b.reset()
+        t = time.perf_counter()
+        per, tot = b.run(MIN_GAIN)
+        times.append(time.perf_counter() - t)
+        print(f"[bench] step {i}: {tot.alg_ms:.1f} ms ({times[-1] * 1e3:.1f} ms wall) "
+              f"passes={tot.passes} sweeps={tot.total_iters}", file=sys.stderr, flush=True)
+    b.close()
+    batch_gps = len(graphs) / (sum(times) / len(times))
+    # plain path, one graph at a time, on a strided sample; same results
+    sample = list(range(0, len(graphs), max(1, len(graphs) // 256)))
+    plain_ms = 0.0
+    same = True
+    for i in sample:
+        r, c, w = graphs[i]
+        with lv.Graph(r, c, w) as g:
+            res = g.run(MIN_GAIN)
+        plain_ms += res.alg_ms
+        same &= (np.float32(res.q_final).view(np.uint32) ==
+                 np.float32(per[i].q_final).view(np.uint32)) and res.passes == per[i].passes
+    plain_gps = len(sample) / (plain_ms * 1e-3)
+    print(f"[bench] plain: {len(sample)} graphs in {plain_ms:.1f} ms -> {plain_gps:.1f} graphs/s; "
+          f"identical results: {same}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "Louvain graphs/s, batched vs plain (1 GPU)",
+        "value": batch_gps,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sum(times) / len(times) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 gains, exact i64 fixed-point sums",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg["workload"], "graphs": len(graphs), "vertices": nv,
+                   "entries": ne, "min_gain": MIN_GAIN},
+        "plain": {"graphs_per_s": plain_gps, "sample_graphs": len(sample),
+                  "identical_results": bool(same)},
+        "batched_over_plain": batch_gps / plain_gps,
+        "edges_per_s": sum(p.sum_edges for p in per) / (sum(times) / len(times)),
+    }
+    if not args.no_cpu:
+        import oracle
+        cs = sample[:8]
+        t = time.perf_counter()
+        for i in cs:
+            r, c, w = graphs[i]
+            oracle.Oracle(r.astype(np.uint64), c, w).run(MIN_GAIN)
+        dt = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": len(cs) / dt, "unit": "graphs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"{len(cs)} graphs of the batch, single-core C oracle, "
+                                          f"{dt:.2f} s"}
     print(json.dumps(line))
 
 

@@ -137,6 +137,29 @@ int lv_step(lv_graph *g, int bucket, uint64_t *moved);
 int lv_sweep(lv_graph *g, float *q, uint64_t *moved);
 int lv_modularity(lv_graph *g, float *q);
 
+/* ---- batches of independent graphs --------------------------------------- */
+/* Many graphs in one segmented CSR: graph g owns vertices
+ * [graph_off[g], graph_off[g+1]) (64-bit segment offsets), row/col/w index
+ * the concatenation with global vertex ids, and every entry stays inside its
+ * graph.  Each graph's result is bit-identical to an lv_run of that graph
+ * alone.  No counterpart in the reference, which processes one graph per
+ * process (main.cu). */
+typedef struct lv_batch_graph_result {
+    float q_final;
+    uint32_t passes;
+    uint32_t n_final;
+    uint32_t total_iters;
+    uint64_t sum_edges;
+    int status;           /* LV_OK, or LV_ERR_RANGE for a graph without weight */
+} lv_batch_graph_result;
+
+int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, const uint32_t *row,
+                    const uint32_t *col, const float *w, lv_graph **out);
+int lv_batch_run(lv_graph *batch, float min_gain, lv_batch_graph_result *per_graph,
+                 lv_result *total);
+/* per original vertex: its community as a 0-based id local to its graph */
+int lv_batch_get_assignment(lv_graph *batch, uint32_t *local_assignment);
+
 /* ---- profiling ------------------------------------------------------------ */
 int lv_profile_enable(lv_graph *g, int on);
 int lv_profile_get(lv_graph *g, lv_kernel_stats *out); /* synchronises */

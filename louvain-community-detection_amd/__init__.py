@@ -50,6 +50,12 @@ class Result(ctypes.Structure):
                 ("status", ctypes.c_int)]
 
 
+class BatchGraphResult(ctypes.Structure):
+    _fields_ = [("q_final", ctypes.c_float), ("passes", ctypes.c_uint32),
+                ("n_final", ctypes.c_uint32), ("total_iters", ctypes.c_uint32),
+                ("sum_edges", ctypes.c_uint64), ("status", ctypes.c_int)]
+
+
 class KernelStats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 9), ("launches", ctypes.c_uint64 * 9),
                 ("bytes", ctypes.c_double * 9)]
@@ -62,7 +68,8 @@ EXPORTS = [
     "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
-    "lv_gen_chung_lu", "lv_gen_rmat", "lv_gen_sbm",
+    "lv_gen_chung_lu", "lv_gen_rmat", "lv_gen_sbm", "lv_batch_create", "lv_batch_run",
+    "lv_batch_get_assignment",
 ]
 
 _LIB = None
@@ -116,6 +123,12 @@ def lib():
                               c.c_float, c.c_float, c.c_uint64, c.POINTER(HostCSR)]
     L.lv_gen_sbm.argtypes = [c.c_uint32, c.c_uint32, c.c_double, c.c_double, c.c_uint64,
                              c.POINTER(HostCSR)]
+    _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+    L.lv_batch_create.argtypes = [c.c_int, c.c_uint32, _u64p, _u32p, _u32p, _f32p,
+                                  c.POINTER(c.c_void_p)]
+    L.lv_batch_run.argtypes = [c.c_void_p, c.c_float, c.POINTER(BatchGraphResult),
+                               c.POINTER(Result)]
+    L.lv_batch_get_assignment.argtypes = [c.c_void_p, _u32p]
     _LIB = L
     return L
 
@@ -284,6 +297,87 @@ class Graph:
 
     def profile_reset(self):
         _chk(self.L.lv_profile_reset(self.h), "lv_profile_reset")
+
+    def profile_get(self):
+        st = KernelStats()
+        _chk(self.L.lv_profile_get(self.h, ctypes.byref(st)), "lv_profile_get")
+        return {f: {"ms": st.ms[i], "launches": st.launches[i], "bytes": st.bytes[i]}
+                for i, f in enumerate(KERNEL_FAMILIES)}
+
+
+def concat_graphs(graphs):
+    """Concatenate [(row, col, w), ...] into one segmented CSR with global ids:
+    returns (graph_off u64[G+1], row u32[n+1], col u32[ne], w f32[ne])."""
+    sizes = [int(r.size) - 1 for r, _, _ in graphs]
+    graph_off = np.zeros(len(graphs) + 1, np.uint64)
+    graph_off[1:] = np.cumsum(sizes)
+    rows, cols, ws = [np.zeros(1, np.uint64)], [], []
+    ebase = 0
+    for (r, c, w), vo in zip(graphs, graph_off[:-1]):
+        r = r.astype(np.uint64)
+        rows.append(r[1:] + ebase)
+        cols.append(c.astype(np.uint64) + vo)
+        ws.append(w.astype(np.float32))
+        ebase += int(r[-1])
+    row = np.concatenate(rows)
+    if row[-1] >= 2 ** 32:
+        raise ValueError("batch exceeds 2^32 entries")
+    col = np.concatenate(cols) if cols else np.zeros(0, np.uint64)
+    return (graph_off, row.astype(np.uint32), col.astype(np.uint32),
+            np.concatenate(ws) if ws else np.zeros(0, np.float32))
+
+
+class Batch:
+    """Many graphs processed together (lv_batch_*); results per graph equal
+    single-graph runs bit for bit."""
+
+    def __init__(self, graphs, device=0):
+        L = lib()
+        self.L = L
+        self.graph_off, row, col, w = concat_graphs(graphs)
+        self.G = len(graphs)
+        h = ctypes.c_void_p()
+        _chk(L.lv_batch_create(device, self.G, self.graph_off, row,
+                               col if col.size else np.zeros(1, np.uint32),
+                               w if w.size else np.zeros(1, np.float32), ctypes.byref(h)),
+             "lv_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lv_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def reset(self):
+        _chk(self.L.lv_graph_reset(self.h), "lv_graph_reset")
+
+    def run(self, min_gain):
+        per = (BatchGraphResult * self.G)()
+        tot = Result()
+        _chk(self.L.lv_batch_run(self.h, ctypes.c_float(min_gain), per, ctypes.byref(tot)),
+             "lv_batch_run")
+        return list(per), tot
+
+    def assignments(self):
+        n0 = int(self.graph_off[-1])
+        a = np.zeros(max(n0, 1), np.uint32)
+        _chk(self.L.lv_batch_get_assignment(self.h, a), "lv_batch_get_assignment")
+        return [a[int(self.graph_off[i]):int(self.graph_off[i + 1])] for i in range(self.G)]
+
+    def profile(self, on=True):
+        _chk(self.L.lv_profile_enable(self.h, 1 if on else 0), "lv_profile_enable")
 
     def profile_get(self):
         st = KernelStats()

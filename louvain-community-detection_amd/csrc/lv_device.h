@@ -99,6 +99,14 @@ void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n);
 void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
 
+// batch mode
+void launch_batch_gid(hipStream_t s, const uint32_t *voff, uint32_t G, uint32_t *gid,
+                      uint32_t n);
+void launch_batch_identity(hipStream_t s, const uint32_t *gid, const uint32_t *gstate,
+                           uint32_t keep, uint32_t *comm, uint32_t *comm_sz, uint32_t n);
+void launch_gather_u32(hipStream_t s, const uint32_t *src, const uint32_t *idx, uint32_t *out,
+                       uint32_t count);
+
 // degree classification of the current graph into `order`
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n,
                            uint32_t *counts /*kNumClasses*/,

@@ -65,6 +65,19 @@ struct ProfRec {
     double bytes;
 };
 
+// graph states in batch mode
+enum : uint32_t { kGsActive = lv::kGraphActive, kGsWait = 2, kGsDone = 3 };
+
+// host-side state of one graph of a batch (the per-graph copy of the loop
+// variables of modularityOptimization and main, modularity.cu:464-508)
+struct BatchGraph {
+    float act = 0.0f, old = 0.0f, q_final = 0.0f;
+    uint32_t iters = 0, passes = 0, pass_iters = 0;
+    bool first = true, ret = true;
+    int status = 0;
+    uint64_t ne = 0, sum_edges = 0, sum_visits = 0;
+};
+
 }  // namespace
 
 struct lv_graph {
@@ -133,6 +146,7 @@ struct lv_graph {
     std::vector<uint32_t> voff, voff0;  // host: current / original vertex offsets
     std::vector<float> gm_h;
     std::vector<uint32_t> gstate_h;
+    std::vector<BatchGraph> bg;
     // profiling
     bool prof_on = false;
     std::vector<ProfRec> *capture_recs = nullptr;  // non-null while capturing a sweep
@@ -655,6 +669,161 @@ int aggregation(lv_graph *g) {
     return LV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batch mode: many independent graphs in one segmented CSR
+//
+// Every graph keeps the exact arithmetic and stop rules of a single run: its
+// own m, its own per-sweep modularity and its own decision to continue, stop
+// after an improving pass (aggregate) or finish.  All graphs sweep together;
+// a graph that has stopped is frozen (its vertices are skipped by every
+// kernel) until the union of the graphs that improved is contracted at once.
+// The contraction of the union is the union of the contractions: renumbering
+// by rank keeps each graph's vertices contiguous and its rows ordered.  The
+// fixed-point scale is the largest over the graphs; exact values and their
+// FP32 roundings do not depend on the scale.
+// ---------------------------------------------------------------------------
+float q_graph(const lv_graph *g, uint32_t gi) {
+    const ull *c = g->gcnt_h;
+    const uint32_t G = g->G;
+    const float s_in = fix2f_host(g, (int64_t)c[gi]);
+    u128 sq = 0;
+    for (int l = 0; l < 4; ++l) sq += (u128)c[G + (size_t)l * G + gi] << (32 * l);
+    const float a2 = ldexpf((float)sq, -2 * g->S);
+    const float m = g->gm_h[gi];
+    return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
+}
+
+int push_gstate(lv_graph *g) {
+    LV_HIP(hipMemcpyAsync(g->gstate_d, g->gstate_h.data(), g->G * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
+}
+
+int batch_pass_init(lv_graph *g) {
+    const uint32_t G = g->G;
+    int rc = classify(g);
+    if (rc) return rc;
+    // entries per graph of this pass (row pointer at the graph boundaries)
+    lv::launch_gather_u32(g->stream, g->row, g->voff_d, g->vgather, G + 1);
+    std::vector<uint32_t> rb(G + 1);
+    LV_HIP(hipMemcpyAsync(rb.data(), g->vgather, (G + 1) * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, g->stream));
+    zero_counters(g);
+    r1_all(g, 0, 1);
+    rc = sync_counters(g);
+    if (rc) return rc;
+    bool changed = false;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        BatchGraph &b = g->bg[gi];
+        b.ne = rb[gi + 1] - rb[gi];
+        if (g->gstate_h[gi] != kGsActive) continue;
+        const float m = fix2f_host(g, (int64_t)g->gcnt_h[gi]) / 2.0f;
+        g->gm_h[gi] = m;
+        if (!(m > 0.0f)) {  // no weight: the reference would divide by zero
+            b.status = LV_ERR_RANGE;
+            g->gstate_h[gi] = kGsDone;
+            changed = true;
+        }
+    }
+    LV_HIP(hipMemcpyAsync(g->gm_d, g->gm_h.data(), G * sizeof(float), hipMemcpyHostToDevice,
+                          g->stream));
+    if (changed) {
+        rc = push_gstate(g);
+        if (rc) return rc;
+    }
+    zero_counters(g);
+    modularity_launch(g);
+    rc = sync_counters(g);
+    if (rc) return rc;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        if (g->gstate_h[gi] != kGsActive) continue;
+        BatchGraph &b = g->bg[gi];
+        b.act = q_graph(g, gi);
+        b.first = true;
+        b.ret = true;
+        b.pass_iters = 0;
+    }
+    g->pass_ready = true;
+    return LV_OK;
+}
+
+// One pass for every active graph; returns in *more whether any graph
+// improved (and was contracted).
+int batch_pass(lv_graph *g, float min_gain, bool *more) {
+    *more = false;
+    const uint32_t G = g->G;
+    int rc = batch_pass_init(g);
+    if (rc) return rc;
+    SweepGraph sg;
+    const bool use_graph = graphs_enabled(g) && capture_sweep(g, sg);
+    for (;;) {
+        bool any_active = false;
+        for (uint32_t gi = 0; gi < G && !any_active; ++gi) any_active = g->gstate_h[gi] == kGsActive;
+        if (!any_active) break;
+        if (use_graph) {
+            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            rc = sweep_finish(g, nullptr, nullptr);
+            if (g->prof_on) sg.collect();
+        } else {
+            rc = sweep(g, nullptr, nullptr);
+        }
+        if (rc) return rc;
+        bool changed = false;
+        for (uint32_t gi = 0; gi < G; ++gi) {
+            if (g->gstate_h[gi] != kGsActive) continue;
+            BatchGraph &b = g->bg[gi];
+            const uint64_t moved = g->gcnt_h[5ull * G + gi];
+            b.old = b.act;
+            b.act = q_graph(g, gi);
+            ++b.pass_iters;
+            // modularity.cu:503-508 per graph, plus the fixed-point stop
+            const bool stop =
+                !(b.act - b.old >= min_gain) || moved == 0 || b.pass_iters >= kMaxIters;
+            if (b.first && (b.act - b.old < min_gain || moved == 0)) {
+                b.q_final = b.old;
+                b.ret = false;
+            }
+            b.first = false;
+            if (stop) {
+                b.iters += b.pass_iters;
+                b.sum_edges += b.ne;
+                b.sum_visits += (uint64_t)(b.pass_iters + 1) * b.ne;
+                g->gstate_h[gi] = b.ret ? kGsWait : kGsDone;
+                changed = true;
+            }
+        }
+        if (changed) {
+            rc = push_gstate(g);
+            if (rc) return rc;
+        }
+    }
+    bool any_wait = false;
+    for (uint32_t gi = 0; gi < G; ++gi) any_wait |= g->gstate_h[gi] == kGsWait;
+    if (!any_wait) return LV_OK;
+    // contract the graphs that improved; every other graph stays as it is
+    lv::launch_batch_identity(g->stream, g->gid, g->gstate_d, kGsWait, g->comm, g->comm_sz, g->n);
+    rc = aggregation(g);
+    if (rc) return rc;
+    // new vertex ranges: rank of each graph's first old vertex
+    lv::launch_gather_u32(g->stream, g->new_id, g->voff_d, g->vgather, G + 1);
+    LV_HIP(hipMemcpyAsync(g->voff.data(), g->vgather, (G + 1) * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    LV_HIP(hipMemcpyAsync(g->voff_d, g->voff.data(), (G + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, g->stream));
+    lv::launch_batch_gid(g->stream, g->voff_d, G, g->gid, g->n);
+    for (uint32_t gi = 0; gi < G; ++gi)
+        if (g->gstate_h[gi] == kGsWait) {
+            g->gstate_h[gi] = kGsActive;
+            g->bg[gi].passes++;
+        }
+    rc = push_gstate(g);
+    if (rc) return rc;
+    *more = true;
+    return LV_OK;
+}
+
 // exact host-side validation and fixed-point scale (same rule as the oracle)
 int scale_bits(uint64_t ne, const float *w, int *S_out) {
     int S = 0;
@@ -748,6 +917,14 @@ void free_all(lv_graph *g) {
     dfree(g->dc);
     if (g->hc) (void)hipHostFree(g->hc);
     g->hc = nullptr;
+    dfree(g->gid);
+    dfree(g->gm_d);
+    dfree(g->gstate_d);
+    dfree(g->voff_d);
+    dfree(g->vgather);
+    dfree(g->gcnt);
+    if (g->gcnt_h) (void)hipHostFree(g->gcnt_h);
+    g->gcnt_h = nullptr;
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
 }
@@ -893,6 +1070,12 @@ int lv_graph_reset(lv_graph *g) {
         LV_HIP(hipMemcpyAsync(g->w, g->w0, E * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
     lv::launch_iota_u32(s, g->init_comm, g->n0);
+    if (g->G) {
+        g->voff = g->voff0;
+        LV_HIP(hipMemcpyAsync(g->voff_d, g->voff.data(), (g->G + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        lv::launch_batch_gid(s, g->voff_d, g->G, g->gid, g->n0);
+    }
     LV_HIP(hipStreamSynchronize(s));
     g->n = g->n0;
     g->ne = g->ne0;
@@ -1054,6 +1237,118 @@ int lv_get_scale(lv_graph *g, int *scale_bits_out, float *m) {
     if (!g) return LV_ERR_ARG;
     if (scale_bits_out) *scale_bits_out = g->S;
     if (m) *m = g->m;
+    return LV_OK;
+}
+
+int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, const uint32_t *row,
+                    const uint32_t *col, const float *w, lv_graph **out) {
+    if (!out || !graph_off || !row || ngraphs == 0) return LV_ERR_ARG;
+    *out = nullptr;
+    if (graph_off[0] != 0) return LV_ERR_ARG;
+    for (uint32_t gi = 0; gi < ngraphs; ++gi)
+        if (graph_off[gi + 1] < graph_off[gi]) return LV_ERR_ARG;
+    const uint64_t n64 = graph_off[ngraphs];
+    if (n64 >= 0x80000000ull) return LV_ERR_ARG;
+    const uint32_t n = (uint32_t)n64;
+    if (row[0] != 0) return LV_ERR_ARG;
+    for (uint32_t v = 0; v < n; ++v)
+        if (row[v + 1] < row[v]) return LV_ERR_ARG;
+    // every entry stays inside its graph's vertex range
+    for (uint32_t gi = 0; gi < ngraphs; ++gi)
+        for (uint64_t j = row[graph_off[gi]]; j < row[graph_off[gi + 1]]; ++j)
+            if (col[j] < graph_off[gi] || col[j] >= graph_off[gi + 1]) return LV_ERR_ARG;
+    lv_graph *g = nullptr;
+    int rc = lv_graph_create(device, n, row, col, w, &g);
+    if (rc) return rc;
+    const uint32_t G = ngraphs;
+    g->G = G;
+    int e = 0;
+    e |= dmalloc(&g->gid, (size_t)n + 1) | dmalloc(&g->gm_d, G) | dmalloc(&g->gstate_d, G);
+    e |= dmalloc(&g->voff_d, (size_t)G + 1) | dmalloc(&g->vgather, (size_t)G + 1);
+    e |= dmalloc(&g->gcnt, 6ull * G);
+    if (!e && hipHostMalloc((void **)&g->gcnt_h, 6ull * G * sizeof(ull)) != hipSuccess) {
+        (void)hipGetLastError();
+        e = LV_ERR_NOMEM;
+    }
+    if (e) {
+        lv_graph_destroy(g);
+        return LV_ERR_NOMEM;
+    }
+    g->voff0.resize(G + 1);
+    for (uint32_t gi = 0; gi <= G; ++gi) g->voff0[gi] = (uint32_t)graph_off[gi];
+    g->voff = g->voff0;
+    g->gm_h.assign(G, 0.0f);
+    g->gstate_h.assign(G, kGsActive);
+    g->bg.assign(G, BatchGraph());
+    if (hipMemcpy(g->voff_d, g->voff.data(), (G + 1) * sizeof(uint32_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        lv_graph_destroy(g);
+        return LV_ERR_HIP;
+    }
+    lv::launch_batch_gid(g->stream, g->voff_d, G, g->gid, n);
+    if (hipStreamSynchronize(g->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        lv_graph_destroy(g);
+        return LV_ERR_HIP;
+    }
+    *out = g;
+    return LV_OK;
+}
+
+int lv_batch_run(lv_graph *g, float min_gain, lv_batch_graph_result *per_graph,
+                 lv_result *total) {
+    if (!g || !g->G) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    const uint32_t G = g->G;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    LV_HIP(hipEventCreate(&t0));
+    LV_HIP(hipEventCreate(&t1));
+    LV_HIP(hipEventRecord(t0, g->stream));
+    g->bg.assign(G, BatchGraph());
+    for (uint32_t gi = 0; gi < G; ++gi)
+        g->gstate_h[gi] = g->voff[gi + 1] > g->voff[gi] ? kGsActive : kGsDone;
+    int rc = push_gstate(g);
+    bool more = true;
+    while (!rc && more) rc = batch_pass(g, min_gain, &more);
+    (void)hipEventRecord(t1, g->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (total) {
+        memset(total, 0, sizeof(*total));
+        total->alg_ms = ms;
+        total->status = rc;
+        total->n_final = g->n;
+    }
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        const BatchGraph &b = g->bg[gi];
+        if (per_graph) {
+            per_graph[gi].q_final = b.q_final;
+            per_graph[gi].passes = b.passes;
+            per_graph[gi].n_final = g->voff[gi + 1] - g->voff[gi];
+            per_graph[gi].total_iters = b.iters;
+            per_graph[gi].sum_edges = b.sum_edges;
+            per_graph[gi].status = b.status;
+        }
+        if (total) {
+            total->passes += b.passes;
+            total->total_iters += b.iters;
+            total->sum_edges += b.sum_edges;
+            total->sum_edge_visits += b.sum_visits;
+        }
+    }
+    return rc;
+}
+
+int lv_batch_get_assignment(lv_graph *g, uint32_t *local_assignment) {
+    if (!g || !g->G || !local_assignment) return LV_ERR_ARG;
+    int rc = lv_get_assignment(g, local_assignment);
+    if (rc) return rc;
+    for (uint32_t gi = 0; gi < g->G; ++gi)
+        for (uint32_t v = g->voff0[gi]; v < g->voff0[gi + 1]; ++v) local_assignment[v] -= g->voff[gi];
     return LV_OK;
 }
 

@@ -61,6 +61,54 @@ void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// batch mode helpers
+// ---------------------------------------------------------------------------
+// gid[v] = graph whose vertex range [voff[g], voff[g+1]) holds v
+__global__ void k_batch_gid(const uint32_t *voff, uint32_t G, uint32_t *gid, uint32_t n) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (voff[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    gid[v] = lo;
+}
+void launch_batch_gid(hipStream_t s, const uint32_t *voff, uint32_t G, uint32_t *gid,
+                      uint32_t n) {
+    if (!n || !G) return;
+    hipLaunchKernelGGL(k_batch_gid, dim3((n + 255) / 256), dim3(256), 0, s, voff, G, gid, n);
+}
+// vertices of graphs that do not aggregate now become their own community, so
+// the union contraction leaves those graphs unchanged up to renumbering
+__global__ void k_batch_identity(const uint32_t *gid, const uint32_t *gstate, uint32_t keep,
+                                 uint32_t *comm, uint32_t *comm_sz, uint32_t n) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && gstate[gid[v]] != keep) {
+        comm[v] = v;
+        comm_sz[v] = 1u;
+    }
+}
+void launch_batch_identity(hipStream_t s, const uint32_t *gid, const uint32_t *gstate,
+                           uint32_t keep, uint32_t *comm, uint32_t *comm_sz, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_batch_identity, dim3((n + 255) / 256), dim3(256), 0, s, gid, gstate, keep,
+                       comm, comm_sz, n);
+}
+__global__ void k_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *out,
+                             uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = src[idx[i]];
+}
+void launch_gather_u32(hipStream_t s, const uint32_t *src, const uint32_t *idx, uint32_t *out,
+                       uint32_t count) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_gather_u32, dim3((count + 255) / 256), dim3(256), 0, s, src, idx, out,
+                       count);
+}
+
+// ---------------------------------------------------------------------------
 // degree classification: once per pass (the reference re-partitions the
 // whole vertex set 7 times per sweep, modularity.cu:468-469)
 // ---------------------------------------------------------------------------

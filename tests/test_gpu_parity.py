@@ -218,3 +218,54 @@ def test_cli_output_matches_reference_format(lv, orc, bcsstk16_path):
         expect.append(f"{c + 1} " + "".join(f"{m} " for m in members))
     assert lines[3:3 + res.n_final] == expect
     assert lines[3 + res.n_final:] == [""]
+
+
+def _mixed_batch(lv, orc, bcsstk16_text):
+    rng = np.random.default_rng(21)
+    graphs = [orc.parse_mtx(text=bcsstk16_text), lv.gen_fem(200, 25, 9, 3),
+              lv.gen_chung_lu(3000, 9.4, 2.5, 200, 30, 20.0, 4),
+              lv.gen_rmat(11, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 5),
+              orc.parse_mtx(text="5 5 0\n"),                       # no edges
+              orc.parse_mtx(text="6 6 7\n1 2 1\n2 3 1\n1 3 1\n4 5 1\n5 6 1\n4 6 1\n3 4 1\n")]
+    for i in range(24):
+        n = int(np.exp(rng.uniform(np.log(100), np.log(4000))))
+        blocks = int(rng.integers(8, 33))
+        deg = float(rng.uniform(8, 32))
+        graphs.append(lv.gen_sbm(n, min(blocks, n), deg, 10.0, 100 + i))
+    return [(r.astype(np.uint32), c, w) for r, c, w in graphs]
+
+
+def test_batch_matches_single_runs(lv, orc, bcsstk16_text):
+    """Every graph of a mixed batch gets the result of its own single run."""
+    graphs = _mixed_batch(lv, orc, bcsstk16_text)
+    print(f"[case] batch of {len(graphs)} graphs", flush=True)
+    with lv.Batch(graphs) as b:
+        per, tot = b.run(1e-6)
+        assigns = b.assignments()
+    for i, (r, c, w) in enumerate(graphs):
+        o = orc.Oracle(r.astype(np.uint64), c, w)
+        try:
+            ores = o.run(1e-6)
+        except orc.OracleError:
+            assert per[i].status != 0, i  # graph without weight
+            continue
+        assert per[i].status == 0
+        assert np.float32(per[i].q_final).view(np.uint32) == \
+            np.float32(ores.q_final).view(np.uint32), (i, per[i].q_final, ores.q_final)
+        assert per[i].passes == ores.passes and per[i].n_final == ores.n_final, i
+        assert per[i].total_iters == ores.total_iters, i
+        assert per[i].sum_edges == ores.sum_edges, i
+        assert np.array_equal(assigns[i], o.init_comm()), i
+
+
+def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
+    graphs = _mixed_batch(lv, orc, bcsstk16_text)[:10]
+    with lv.Batch(graphs) as b:
+        p1, _ = b.run(1e-6)
+        a1 = b.assignments()
+        b.reset()
+        p2, _ = b.run(1e-6)
+        a2 = b.assignments()
+    for x, y in zip(p1, p2):
+        assert np.float32(x.q_final).view(np.uint32) == np.float32(y.q_final).view(np.uint32)
+    assert all(np.array_equal(x, y) for x, y in zip(a1, a2))
