@@ -79,6 +79,8 @@ constexpr uint32_t kGraphActive = 1;
 
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
 constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
+constexpr uint32_t kBucketTarget = 1024;  // heavy-row entries per hash bucket (target)
+constexpr uint32_t kMaxBuckets = 1024;    // buckets per heavy vertex (max)
 struct HeavySplit {
     const uint32_t *verts;               // class-8 vertices
     uint32_t nh;                         // their count
@@ -91,6 +93,17 @@ struct HeavySplit {
     unsigned long long *hcnt;            // nh distinct counts, followed by
     unsigned long long *hcew;            // nh comm_e_w sums and
     unsigned long long *hlane;           // nh*128 packed lane maxima (one allocation)
+    // partitioned path
+    const uint32_t *nbuck;               // nh: buckets per vertex (pow2)
+    const uint32_t *bstart;              // nh+1: first bucket of each vertex
+    uint32_t nbuckets;                   // total buckets = bstart[nh]
+    uint32_t *bcnt, *bpos, *bcur;        // nbuckets / nbuckets+1 / nbuckets
+    uint32_t *pc, *pj;                   // heavy-entry space: community, entry index
+    float *pw;                           //                    weight
+    uint32_t *hflag;                     // nh: LDS overflow, redo on global tables
+    const uint32_t *only;                // non-null: kernels run for flagged vertices only
+    int force_global;                    // test hook: skip the partitioned path
+    uint32_t *scan_tmp;
     uint32_t *err;
 };
 
@@ -135,6 +148,8 @@ void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts
 void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *verts,
                          uint32_t count, uint32_t *nchunks);
 void launch_fill_slots(hipStream_t s, unsigned long long *tab, size_t cap);
+void launch_hp_nbuck(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
+                     uint32_t *nbuck);
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h);
 void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float *stage,
                      int masked, int init_state, unsigned long long *acc);

@@ -111,6 +111,11 @@ struct lv_graph {
     size_t harena = 0;
     ull *hscr = nullptr;
     size_t hscr_cap = 0;
+    // partitioned heavy path
+    uint32_t *hp_nbuck = nullptr, *hp_bstart = nullptr, *hp_flag = nullptr;
+    uint32_t *hp_bcnt = nullptr, *hp_bpos = nullptr, *hp_bcur = nullptr, *hp_btmp = nullptr;
+    size_t hp_bcap = 0;
+    uint32_t hp_nbuckets = 0;
     uint32_t *dlist = nullptr, *eslot = nullptr;
     // aggregation scratch
     uint32_t *new_id = nullptr, *comm_deg = nullptr, *epos = nullptr, *mstart = nullptr;
@@ -305,14 +310,29 @@ int classify(lv_graph *g) {
         lv::launch_scan_u64(g->stream, g->heavy_caps, nh, g->heavy_off, g->scan_tmp64);
         lv::launch_heavy_chunks(g->stream, g->row, hv, nh, g->heavy_nch);
         lv::launch_scan_u32(g->stream, g->heavy_nch, 0, nh, g->heavy_coff, g->scan_tmp32);
+        lv::launch_hp_nbuck(g->stream, g->row, hv, nh, g->hp_nbuck);
+        lv::launch_scan_u32(g->stream, g->hp_nbuck, 0, nh, g->hp_bstart, g->scan_tmp32);
         ull total = 0;
-        uint32_t chunks = 0;
+        uint32_t chunks = 0, nbuckets = 0;
         LV_HIP(hipMemcpyAsync(&total, g->heavy_off + nh, sizeof(ull), hipMemcpyDeviceToHost,
                               g->stream));
         LV_HIP(hipMemcpyAsync(&chunks, g->heavy_coff + nh, sizeof(uint32_t),
                               hipMemcpyDeviceToHost, g->stream));
+        LV_HIP(hipMemcpyAsync(&nbuckets, g->hp_bstart + nh, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, g->stream));
         LV_HIP(hipStreamSynchronize(g->stream));
         g->heavy_chunks = chunks;
+        g->hp_nbuckets = nbuckets;
+        if (nbuckets > g->hp_bcap) {
+            dfree(g->hp_bcnt);
+            dfree(g->hp_bpos);
+            dfree(g->hp_bcur);
+            dfree(g->hp_btmp);
+            if (dmalloc(&g->hp_bcnt, nbuckets) || dmalloc(&g->hp_bpos, (size_t)nbuckets + 1) ||
+                dmalloc(&g->hp_bcur, nbuckets) || dmalloc(&g->hp_btmp, lv::scan_tmp_elems(nbuckets) + 8))
+                return LV_ERR_NOMEM;
+            g->hp_bcap = nbuckets;
+        }
         if (total > g->harena) {
             dfree(g->htab);
             size_t cap = (size_t)total;
@@ -343,6 +363,22 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.hcnt = g->hscr;
     h.hcew = g->hscr + nh;
     h.hlane = g->hscr + 2 * (size_t)nh;
+    h.nbuck = g->hp_nbuck;
+    h.bstart = g->hp_bstart;
+    h.nbuckets = g->hp_nbuckets;
+    h.bcnt = g->hp_bcnt;
+    h.bpos = g->hp_bpos;
+    h.bcur = g->hp_bcur;
+    // heavy-entry space buffers share memory with the aggregation scratch
+    // (never live at the same time)
+    h.pc = g->tkey;
+    h.pj = g->skey;
+    h.pw = reinterpret_cast<float *>(g->tval);
+    h.hflag = g->hp_flag;
+    h.only = nullptr;
+    static const bool force_global = getenv("LV_HEAVY_GLOBAL") != nullptr;
+    h.force_global = force_global ? 1 : 0;
+    h.scan_tmp = g->hp_btmp;
     h.err = &g->dc->err;
     return h;
 }
@@ -888,6 +924,13 @@ void free_all(lv_graph *g) {
     dfree(g->heavy_nch);
     dfree(g->heavy_coff);
     dfree(g->hscr);
+    dfree(g->hp_nbuck);
+    dfree(g->hp_bstart);
+    dfree(g->hp_flag);
+    dfree(g->hp_bcnt);
+    dfree(g->hp_bpos);
+    dfree(g->hp_bcur);
+    dfree(g->hp_btmp);
     dfree(g->htab);
     dfree(g->dlist);
     dfree(g->eslot);
@@ -1006,6 +1049,7 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
     e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
     e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
+    e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
     e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
     e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
     e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);

@@ -95,6 +95,7 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
     __shared__ unsigned long long part[4];
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
+    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
     unsigned long long *tab = h.tab + 2ull * x.off;
     for (int i = threadIdx.x; i < kLocalCap; i += 256) {
         lkey[i] = kEmpty;
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
 __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
+    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
@@ -210,6 +212,7 @@ __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
 __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
+    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t j0 = x.c * kHeavyChunk;  // multiple of 128: thread t serves lane t % 128
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
@@ -251,7 +254,6 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
     if (!v_live(g, v)) return;  // uniform over the wave
     const uint32_t base = g.row[v];
     const uint32_t d = g.row[v + 1] - base;
-    unsigned long long *tab = h.tab + 2ull * h.slot_off[i];
     float gl[2];
     uint32_t cl[2];
 #pragma unroll
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
         } else {
             gl[q] = f32_unord((uint32_t)(k >> 32));
             const uint32_t j = 0xffffffffu - (uint32_t)(k & 0xffffffffull);
-            cl[q] = *tkey(tab, h.eslot[base + j]);
+            cl[q] = g.comm[g.col[base + j]];
         }
     }
     float bg;
@@ -278,6 +280,7 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
 __global__ __launch_bounds__(256) void k_heavy_clean(DevView g, HeavySplit h) {
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
+    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
@@ -297,6 +300,219 @@ __global__ void k_heavy_chunks(const uint32_t *row, const uint32_t *verts, uint3
         const uint32_t v = verts[i];
         nchunks[i] = (row[v + 1] - row[v] + kHeavyChunk - 1) / kHeavyChunk;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned heavy path (the default): a heavy row is split into hash
+// buckets of its neighbour communities, sized so a bucket's distinct
+// communities fit an LDS table; every bucket is then accumulated, scored and
+// reduced to per-virtual-lane maxima entirely in LDS.
+//   P0 count    community of every entry (kept in eslot), bucket histogram,
+//               comm_e_w
+//   (scan of the bucket counts -> bucket positions in heavy-entry space)
+//   P1 scatter  (community, weight, entry index) into the bucket's range
+//   P2 bucket   LDS table of exact sums, gains per distinct community, per
+//               virtual lane first maximum via an LDS atomicMax on
+//               (gain, -entry), merged into the vertex's lanes with one
+//               global atomicMax per lane
+// A bucket whose distinct communities overflow the LDS table flags its
+// vertex; flagged vertices are redone by the global-table kernels above.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bucket_of(uint32_t c, int lb) {
+    return lb ? ((c * 0x9E3779B1u) >> (32 - lb)) : 0u;
+}
+__device__ __forceinline__ uint32_t pslot(uint32_t c, int lcap) {
+    return (c * 0x85EBCA6Bu) >> (32 - lcap);
+}
+
+__global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
+    __shared__ uint32_t hist[kMaxBuckets];
+    __shared__ unsigned long long part[4];
+    const HeavyCtx x = heavy_ctx(g, h);
+    if (!x.live) return;  // uniform over the workgroup
+    const uint32_t nb = h.nbuck[x.i];
+    const int lb = 31 - __clz(nb);
+    for (uint32_t b = threadIdx.x; b < nb; b += 256) hist[b] = 0;
+    __syncthreads();
+    const uint32_t cv = g.comm[x.v];
+    const uint32_t j0 = x.c * kHeavyChunk;
+    const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    unsigned long long cew = 0;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+        const uint32_t e = g.col[x.base + j];
+        const uint32_t ce = g.comm[e];
+        h.eslot[x.base + j] = ce;
+        atomicAdd(&hist[bucket_of(ce, lb)], 1u);
+        if (ce == cv && e != x.v) cew += f2fix(g.w[x.base + j], g.scale);
+    }
+    cew = wave_sum_u64(cew);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
+    __syncthreads();
+    const uint32_t b0 = h.bstart[x.i];
+    for (uint32_t b = threadIdx.x; b < nb; b += 256)
+        if (hist[b]) atomicAdd(&h.bcnt[b0 + b], hist[b]);
+    if (threadIdx.x == 0) {
+        const unsigned long long s = part[0] + part[1] + part[2] + part[3];
+        if (s) atomicAdd(&h.hcew[x.i], s);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
+    __shared__ uint32_t lcnt[kMaxBuckets];
+    __shared__ uint32_t lbase[kMaxBuckets];
+    const HeavyCtx x = heavy_ctx(g, h);
+    if (!x.live) return;
+    const uint32_t nb = h.nbuck[x.i];
+    const int lb = 31 - __clz(nb);
+    for (uint32_t b = threadIdx.x; b < nb; b += 256) lcnt[b] = 0;
+    __syncthreads();
+    const uint32_t j0 = x.c * kHeavyChunk;
+    const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    uint32_t rank[kPerThread], bk[kPerThread], cc[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        bk[q] = kEmpty;
+        if (j < j1) {
+            cc[q] = h.eslot[x.base + j];
+            bk[q] = bucket_of(cc[q], lb);
+            rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t b0 = h.bstart[x.i];
+    for (uint32_t b = threadIdx.x; b < nb; b += 256)
+        if (lcnt[b]) lbase[b] = atomicAdd(&h.bcur[b0 + b], lcnt[b]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        if (bk[q] == kEmpty) continue;
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        const uint32_t pos = lbase[bk[q]] + rank[q];
+        h.pc[pos] = cc[q];
+        h.pw[pos] = g.w[x.base + j];
+        h.pj[pos] = j;
+    }
+}
+
+constexpr int kBucketCap = 4096;  // LDS table slots of one bucket
+
+__global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
+    constexpr int LC = Log2<kBucketCap>::value;
+    __shared__ uint32_t key[kBucketCap];
+    __shared__ unsigned long long val[kBucketCap];
+    __shared__ uint32_t dist[kBucketCap / 2];
+    __shared__ unsigned long long lanes[128];
+    __shared__ uint32_t nd_s, over;
+    // bucket -> heavy vertex (largest i with bstart[i] <= bucket)
+    uint32_t lo = 0, hi = h.nh;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (h.bstart[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t i = lo;
+    const uint32_t v = h.verts[i];
+    if (!v_live(g, v)) return;
+    const uint32_t p0 = h.bpos[blockIdx.x], p1 = h.bpos[blockIdx.x + 1];
+    if (p0 == p1) return;
+    for (int s2 = threadIdx.x; s2 < kBucketCap; s2 += 256) {
+        key[s2] = kEmpty;
+        val[s2] = 0ull;
+    }
+    if (threadIdx.x < 128) lanes[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        nd_s = 0;
+        over = 0;
+    }
+    __syncthreads();
+    for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
+        const uint32_t c = h.pc[q];
+        uint32_t slot = pslot(c, LC);
+        int probe = 0;
+        for (; probe < kBucketCap; ++probe) {
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, c);
+            if (prev == kEmpty) {
+                const uint32_t idx = atomicAdd(&nd_s, 1u);
+                if (idx < kBucketCap / 2) dist[idx] = slot;
+                else over = 1u;  // more distinct communities than the table allows
+                break;
+            }
+            if (prev == c) break;
+            slot = (slot + 1) & (kBucketCap - 1);
+        }
+        if (probe == kBucketCap) {
+            over = 1u;
+            continue;
+        }
+        atomicAdd(&val[slot], f2fix(h.pw[q], g.scale));
+    }
+    __syncthreads();
+    if (over) {  // uniform: redo this vertex with the global-table kernels
+        if (threadIdx.x == 0) atomicOr(&h.hflag[i], 1u);
+        return;
+    }
+    const uint32_t nd = nd_s;
+    const float m = v_m(g, v);
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
+    for (uint32_t q = threadIdx.x; q < nd; q += 256) {
+        const uint32_t slot = dist[q];
+        const uint32_t c = key[slot];
+        float gg = -INFINITY;
+        bool cand = true;  // isLowerIdx (modularity.cu:85-93)
+        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
+        val[slot] = (unsigned long long)__float_as_uint(gg);
+    }
+    __syncthreads();
+    for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
+        const uint32_t c = h.pc[q];
+        uint32_t slot = pslot(c, LC);
+        while (key[slot] != c) slot = (slot + 1) & (kBucketCap - 1);  // present
+        const float gg = __uint_as_float((uint32_t)val[slot]);
+        if (gg != -INFINITY) {
+            const uint32_t j = h.pj[q];
+            const unsigned long long packed =
+                ((unsigned long long)f32_ord(gg) << 32) | (unsigned long long)(0xffffffffu - j);
+            atomicMax(&lanes[j & 127u], packed);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 128 && lanes[threadIdx.x])
+        atomicMax(&h.hlane[(size_t)i * 128 + threadIdx.x], lanes[threadIdx.x]);
+}
+
+// per heavy vertex: bucket count pow2 >= ceil(deg / kBucketTarget)
+__global__ void k_hp_nbuck(const uint32_t *row, const uint32_t *verts, uint32_t count,
+                           uint32_t *nbuck) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const uint32_t v = verts[i];
+        const uint32_t d = row[v + 1] - row[v];
+        uint32_t b = 1;
+        while (b < kMaxBuckets && (uint64_t)b * kBucketTarget < d) b <<= 1;
+        nbuck[i] = b;
+    }
+}
+void launch_hp_nbuck(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
+                     uint32_t *nbuck) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_hp_nbuck, dim3((count + 255) / 256), dim3(256), 0, s, row, verts, count,
+                       nbuck);
+}
+
+// flagged vertices restart on the global-table kernels
+__global__ void k_hp_reset_flagged(HeavySplit h) {
+    const uint32_t i = blockIdx.x;
+    if (i >= h.nh || !h.hflag[i]) return;
+    if (threadIdx.x == 0) {
+        h.hcnt[i] = 0ull;
+        h.hcew[i] = 0ull;
+    }
+    if (threadIdx.x < 128) h.hlane[(size_t)i * 128 + threadIdx.x] = 0ull;
 }
 
 __global__ void k_fill_slots(unsigned long long *tab, size_t cap) {
@@ -321,15 +537,33 @@ void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *ver
                        count, nchunks);
 }
 
-void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h) {
-    if (!h.nh || !h.nchunks) return;
-    launch_fill_u64(s, h.hcnt, 0ull, (size_t)h.nh * 130);  // hcnt | hcew | hlane
+void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
+    if (!hin.nh || !hin.nchunks) return;
+    HeavySplit h = hin;
+    h.only = nullptr;
     const dim3 grid(h.nchunks), block(256);
-    hipLaunchKernelGGL(k_heavy_insert, grid, block, 0, s, g, h);
-    hipLaunchKernelGGL(k_heavy_gain, grid, block, 0, s, g, h);
-    hipLaunchKernelGGL(k_heavy_lanes, grid, block, 0, s, g, h);
+    launch_fill_u64(s, h.hcnt, 0ull, (size_t)h.nh * 130);  // hcnt | hcew | hlane
+    if (h.force_global) {  // every heavy vertex on the global tables
+        launch_fill_u32(s, h.hflag, 1u, h.nh);
+    } else {
+        launch_fill_u32(s, h.hflag, 0u, h.nh);
+        launch_fill_u32(s, h.bcnt, 0u, h.nbuckets);
+        hipLaunchKernelGGL(k_hp_count, grid, block, 0, s, g, h);
+        launch_scan_u32(s, h.bcnt, 0, h.nbuckets, h.bpos, h.scan_tmp);
+        (void)hipMemcpyAsync(h.bcur, h.bpos, (size_t)h.nbuckets * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, s);
+        hipLaunchKernelGGL(k_hp_scatter, grid, block, 0, s, g, h);
+        hipLaunchKernelGGL(k_hp_bucket, dim3(h.nbuckets), block, 0, s, g, h);
+    }
+    // overflowing vertices (if any) redo the step on the global tables
+    hipLaunchKernelGGL(k_hp_reset_flagged, dim3(h.nh), dim3(128), 0, s, h);
+    HeavySplit f = h;
+    f.only = h.hflag;
+    hipLaunchKernelGGL(k_heavy_insert, grid, block, 0, s, g, f);
+    hipLaunchKernelGGL(k_heavy_gain, grid, block, 0, s, g, f);
+    hipLaunchKernelGGL(k_heavy_lanes, grid, block, 0, s, g, f);
     hipLaunchKernelGGL(k_heavy_decide, dim3((h.nh + 3) / 4), block, 0, s, g, h);
-    hipLaunchKernelGGL(k_heavy_clean, grid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_clean, grid, block, 0, s, g, f);
 }
 
 // ---------------------------------------------------------------------------

@@ -520,6 +520,9 @@ int or_pass_init(or_state *st, float *q0) {
     /* all_w = reduce(neigh_w)/2 (modularity.cu:457-458): exact sum, one
      * rounding, exact halving */
     st->m = fix2f(st, tot) / 2.0f;
+    /* a graph without weight would divide by zero in every gain and in Q;
+     * it is reported as out of range instead */
+    if (!(st->m > 0.0f)) return OR_ERR_RANGE;
     for (uint32_t v = 0; v < st->n; v++) {
         st->comm[v] = v;
         st->comm_sz[v] = 1;

@@ -269,3 +269,28 @@ def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
     for x, y in zip(p1, p2):
         assert np.float32(x.q_final).view(np.uint32) == np.float32(y.q_final).view(np.uint32)
     assert all(np.array_equal(x, y) for x, y in zip(a1, a2))
+
+
+def test_heavy_global_table_path(lv, orc):
+    """The global-table kernels that take over heavy vertices whose hash
+    buckets overflow LDS, forced for every heavy vertex (LV_HEAVY_GLOBAL),
+    in a child process so the setting does not leak."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from loader import load_package; import oracle\n"
+        "lv = load_package()\n"
+        "r, c, w = lv.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 13)\n"
+        "o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "g = lv.Graph(r, c, w); res = g.run(1e-6)\n"
+        "assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)\n"
+        "assert np.array_equal(g.assignment(), o.init_comm())\n"
+        "print('global-table path ok')\n"
+    ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
+    env = dict(os.environ, LV_HEAVY_GLOBAL="1")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "global-table path ok" in r.stdout
