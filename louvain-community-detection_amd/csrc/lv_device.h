@@ -29,7 +29,7 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 // Exact integer totals are accumulated into kSpread slots (block b adds to
 // slot b % kSpread) and summed on the host: same-address atomics from
 // thousands of blocks serialise in L2.
-constexpr uint32_t kSpread = 64;
+constexpr uint32_t kSpread = 256;
 
 __host__ __device__ inline int vertex_class(uint32_t d) {
     if (d == 0) return 0;
@@ -111,6 +111,8 @@ struct HeavySplit {
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n);
 void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
+void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
+                     unsigned long long *out);
 
 // batch mode
 void launch_batch_gid(hipStream_t s, const uint32_t *voff, uint32_t G, uint32_t *gid,

@@ -135,6 +135,30 @@ __device__ __forceinline__ uint32_t acc_slot(const DevView &g, uint32_t v) {
 __device__ __forceinline__ uint32_t acc_stride(const DevView &g) {
     return g.gid ? g.G : kSpread;
 }
+// Exact total contribution of a workgroup.  Single-graph mode: the block is
+// reduced in LDS and adds once to spread slot block % kSpread.  Batch mode:
+// wave_add to the per-graph slot.  Every thread of the block must call it.
+__device__ void wave_add(unsigned long long *base, uint32_t key, unsigned long long x,
+                         bool valid);
+__device__ __forceinline__ void block_add(const DevView &g, unsigned long long *base,
+                                          uint32_t key, unsigned long long x, bool valid) {
+    if (g.gid) {
+        wave_add(base, key, x, valid);
+        return;
+    }
+    __shared__ unsigned long long part[16];
+    const unsigned long long s = wave_sum_u64(valid ? x : 0ull);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        const uint32_t nw = (blockDim.x + 63) / 64;
+        for (uint32_t i = 0; i < nw; ++i) tot += part[i];
+        if (tot) atomicAdd(base + blockIdx.x % kSpread, tot);
+    }
+    __syncthreads();
+}
+
 // base[key] += x over the valid lanes of a wave: one atomic when the valid
 // lanes share the key, else one per lane.  Call from every lane of the wave.
 __device__ __forceinline__ void wave_add(unsigned long long *base, uint32_t key,

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -40,6 +41,8 @@ constexpr uint64_t kFixedLimit = 1ull << 61;
 constexpr uint32_t kMaxIters = 100000;
 
 // device scratch counters, one allocation
+// Device counters.  The head is what the host reads back each sweep; the
+// spread slots behind it are collapsed into `tot` on the device first.
 struct Counters {
     uint32_t counts[kNumClasses];
     uint32_t cursor[kNumClasses];
@@ -48,16 +51,15 @@ struct Counters {
     uint32_t err;
     uint32_t pad;
     ull degsum[kNumClasses];
-    ull acc[lv::kSpread];       // k/s fixed total, spread
-    ull limbs[4][lv::kSpread];  // sum a^2 limbs, spread
+    ull tot[6];                 // acc | limbs[4] | moved, collapsed
+    // spread slots (not copied back)
+    ull acc[lv::kSpread];       // k/s fixed total
+    ull limbs[4][lv::kSpread];  // sum a^2 limbs
     ull moved[lv::kSpread];
 };
-
-ull sum_spread(const ull *x) {
-    ull s = 0;
-    for (uint32_t i = 0; i < lv::kSpread; ++i) s += x[i];
-    return s;
-}
+constexpr size_t kCountersHead = offsetof(Counters, acc);
+static_assert(offsetof(Counters, limbs) == offsetof(Counters, acc) + lv::kSpread * 8, "layout");
+static_assert(offsetof(Counters, moved) == offsetof(Counters, acc) + 5 * lv::kSpread * 8, "layout");
 
 struct ProfRec {
     int fam;
@@ -265,9 +267,11 @@ int zero_counters(lv_graph *g) {
     return LV_OK;
 }
 
-// D2H of the counters (and the per-graph counters in batch mode)
+// D2H of the counters' head (and the per-graph counters in batch mode),
+// after collapsing the spread slots on the device
 void counters_d2h(lv_graph *g) {
-    (void)hipMemcpyAsync(g->hc, g->dc, sizeof(Counters), hipMemcpyDeviceToHost, g->stream);
+    lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
+    (void)hipMemcpyAsync(g->hc, g->dc, kCountersHead, hipMemcpyDeviceToHost, g->stream);
     if (g->G)
         (void)hipMemcpyAsync(g->gcnt_h, g->gcnt, 6ull * g->G * sizeof(ull), hipMemcpyDeviceToHost,
                              g->stream);
@@ -408,9 +412,9 @@ int r1_all(lv_graph *g, int masked, int init_state) {
 // FP32 modularity from the exact counters (modularity.cu:428-436)
 float q_from_counters(const lv_graph *g) {
     const Counters *c = g->hc;
-    float s_in = fix2f_host(g, (int64_t)sum_spread(c->acc));
+    float s_in = fix2f_host(g, (int64_t)c->tot[0]);
     u128 sq = 0;
-    for (int l = 0; l < 4; ++l) sq += (u128)sum_spread(c->limbs[l]) << (32 * l);
+    for (int l = 0; l < 4; ++l) sq += (u128)c->tot[1 + l] << (32 * l);
     float a2 = ldexpf((float)sq, -2 * g->S);
     float m = g->m;
     return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
@@ -431,7 +435,7 @@ int pass_init(lv_graph *g, float *q0) {
     r1_all(g, 0, 1);
     rc = sync_counters(g);
     if (rc) return rc;
-    g->m = fix2f_host(g, (int64_t)sum_spread(g->hc->acc)) / 2.0f;
+    g->m = fix2f_host(g, (int64_t)g->hc->tot[0]) / 2.0f;
     if (!(g->m > 0.0f)) {
         // zero total weight: the reference divides by zero and never stops
         return LV_ERR_RANGE;
@@ -489,7 +493,7 @@ int sweep_finish(lv_graph *g, float *q, uint64_t *moved) {
     LV_HIP(hipStreamSynchronize(g->stream));
     if (g->hc->err) return LV_ERR_OVERFLOW;
     if (q) *q = q_from_counters(g);
-    if (moved) *moved = sum_spread(g->hc->moved);
+    if (moved) *moved = g->hc->tot[5];
     return LV_OK;
 }
 
@@ -1150,7 +1154,7 @@ int lv_step(lv_graph *g, int bucket, uint64_t *moved) {
     int rc = sync_counters(g);
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;
-    if (moved) *moved = sum_spread(g->hc->moved);
+    if (moved) *moved = g->hc->tot[5];
     return LV_OK;
 }
 

@@ -41,6 +41,25 @@ static inline unsigned grid_for(size_t n, unsigned threads, unsigned cap = 65536
     if (g > cap) g = cap;
     return (unsigned)g;
 }
+// sums `groups` arrays of kSpread slots into out[0..groups)
+__global__ __launch_bounds__(256) void k_collapse(const unsigned long long *spread, int groups,
+                                                  unsigned long long *out) {
+    for (int gi = 0; gi < groups; ++gi) {
+        unsigned long long x = 0;
+        for (uint32_t i = threadIdx.x; i < kSpread; i += 256) x += spread[gi * kSpread + i];
+        x = wave_sum_u64(x);
+        __shared__ unsigned long long part[4];
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) out[gi] = part[0] + part[1] + part[2] + part[3];
+        __syncthreads();
+    }
+}
+void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
+                     unsigned long long *out) {
+    hipLaunchKernelGGL(k_collapse, dim3(1), dim3(256), 0, s, spread, groups, out);
+}
+
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
@@ -230,7 +249,7 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
             }
         }
     }
-    wave_add(acc, slot, fixed, valid);  // exact, order-free
+    block_add(g, acc, slot, fixed, valid);  // exact, order-free
 }
 
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
@@ -299,10 +318,10 @@ __global__ __launch_bounds__(256) void k_sumsq(DevView g, const uint32_t *verts,
         l2 = hi & 0xffffffffull;
         l3 = hi >> 32;
         }
-        wave_add(limbs, slot, l0, valid);
-        wave_add(limbs + stride, slot, l1, valid);
-        wave_add(limbs + 2 * stride, slot, l2, valid);
-        wave_add(limbs + 3 * stride, slot, l3, valid);
+        block_add(g, limbs, slot, l0, valid);
+        block_add(g, limbs + stride, slot, l1, valid);
+        block_add(g, limbs + 2 * stride, slot, l2, valid);
+        block_add(g, limbs + 3 * stride, slot, l3, valid);
     }
 }
 
@@ -733,7 +752,7 @@ __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts,
             mv = 1;
         }
     }
-    wave_add(moved, valid ? acc_slot(g, v) : 0u, mv, valid);
+    block_add(g, moved, valid ? acc_slot(g, v) : 0u, mv, valid);
 }
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved) {
