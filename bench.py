@@ -34,6 +34,8 @@ CONFIGS = {
                args=(262111, 9.4, 2.5, 420, 2000, 20.0, 2)),
     # R-MAT scale 24, edge factor 16, w ~ U(0.5, 1.5)
     "c3": dict(workload="rmat-24-ef16", gen="rmat", args=(24, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)),
+    # Poisson request stream of SBM and R-MAT graphs through the serving loop
+    "c5": dict(workload="serve-poisson-200rps", gen="serve", args=(400, 200.0, 5)),
     # 4096 stochastic-block-model graphs in one segmented CSR
     "c4": dict(workload="batch-4096-sbm", gen="batch", args=(4096, 100, 50000, 8, 64, 8.0, 32.0,
                                                             10.0, 4)),
@@ -109,6 +111,8 @@ def main():
     cfg = CONFIGS[args.config]
     if cfg["gen"] == "batch":
         return bench_batch(lv, cfg, args)
+    if cfg["gen"] == "serve":
+        return bench_serve(lv, cfg, args)
     t0 = time.perf_counter()
     graph = make_graph(lv, cfg)
     row, col, w = graph
@@ -249,6 +253,103 @@ def bench_batch(lv, cfg, args):
                                 "kind": "port",
                                 "sample": f"{len(cs)} graphs of the batch, single-core C oracle, "
                                           f"{dt:.2f} s"}
+    print(json.dumps(line))
+
+
+def sym_weights(row, col, lo, hi, seed):
+    """w ~ U(lo, hi) per undirected pair, equal on both directions."""
+    n = row.size - 1
+    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(row.astype(np.int64)))
+    a = np.minimum(src, col.astype(np.uint64))
+    b = np.maximum(src, col.astype(np.uint64))
+    x = (a * np.uint64(0x9E3779B97F4A7C15) + b * np.uint64(0xBF58476D1CE4E5B9) +
+         np.uint64(seed)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x ^= x >> np.uint64(31)
+    x = (x * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x ^= x >> np.uint64(29)
+    u = (x >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    return (np.float32(lo) + np.float32(hi - lo) * u.astype(np.float32)).astype(np.float32)
+
+
+def serve_pool(lv, seed):
+    """SBM graphs with 1k-200k vertices and R-MAT scale 14-21, w ~ U(0.5, 1.5)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    sbm_specs = []
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        sbm_specs.append((n, int(rng.integers(8, 65)), float(rng.uniform(8, 32)), 10.0,
+                          seed * 1000 + i))
+    rmat_specs = [(sc, 16, 0.57, 0.19, 0.19, 0.5, 1.5, seed * 1000 + 100 + sc)
+                  for sc in range(14, 22)]
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        sbm = list(ex.map(lambda sp: lv.gen_sbm(*sp), sbm_specs))
+        rmat = list(ex.map(lambda sp: lv.gen_rmat(*sp), rmat_specs))
+    sbm = [(r, c, sym_weights(r, c, 0.5, 1.5, 7 + i)) for i, (r, c, _) in enumerate(sbm)]
+    return sbm, rmat
+
+
+def bench_serve(lv, cfg, args):
+    """Poisson request stream through the serving loop: p50/p99 latency and
+    graphs/s, with every result checked against a solo run."""
+    nreq, rate, seed = cfg["args"]
+    t0 = time.perf_counter()
+    sbm, rmat = serve_pool(lv, seed)
+    pool = sbm + rmat
+    print(f"[bench] pool: {len(sbm)} SBM + {len(rmat)} R-MAT graphs "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    solo = []
+    for r, c, w in pool:
+        with lv.Graph(r, c, w) as g:
+            res = g.run(MIN_GAIN)
+        solo.append((np.float32(res.q_final).view(np.uint32), res.passes, res.n_final,
+                     res.alg_ms))
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    kinds = rng.integers(0, 2, nreq)
+    picks = [int(rng.integers(0, len(sbm))) if k == 0 else len(sbm) + int(rng.integers(0, len(rmat)))
+             for k in kinds]
+    results = []
+    with lv.Server(slots=8, hbm_budget=200 << 30, min_gain=MIN_GAIN) as s:
+        start = time.perf_counter()
+        for rid in range(nreq):
+            delay = arrivals[rid] - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            s.submit(rid, *pool[picks[rid]])
+            results += s.poll()
+        s.drain()
+        results += s.poll(4096)
+    assert len(results) == nreq
+    lat = np.array([r.t_end_ms - r.t_submit_ms for r in results])
+    span_s = (max(r.t_end_ms for r in results) - min(r.t_submit_ms for r in results)) / 1e3
+    same = all(r.status == 0 and np.float32(r.q_final).view(np.uint32) == solo[picks[r.id]][0] and
+               r.passes == solo[picks[r.id]][1] for r in results)
+    solo_gps = nreq / (sum(solo[p][3] for p in picks) * 1e-3)
+    print(f"[bench] {nreq} requests over {span_s:.2f} s; p50 {np.percentile(lat, 50):.1f} ms "
+          f"p99 {np.percentile(lat, 99):.1f} ms; identical to solo runs: {same}",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": "Louvain served graphs/s (Poisson stream, 1 GPU)",
+        "value": nreq / span_s,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": span_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 gains, exact i64 fixed-point sums",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg["workload"], "requests": nreq, "rate_rps": rate,
+                   "pool": "12 SBM (1k-200k vertices) + R-MAT scale 14-21, w ~ U(0.5,1.5)",
+                   "slots": 8, "hbm_budget_gb": 200},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean())},
+        "one_at_a_time_graphs_per_s": solo_gps,
+        "identical_to_solo": bool(same),
+    }
     print(json.dumps(line))
 
 

@@ -160,6 +160,41 @@ int lv_batch_run(lv_graph *batch, float min_gain, lv_batch_graph_result *per_gra
 /* per original vertex: its community as a 0-based id local to its graph */
 int lv_batch_get_assignment(lv_graph *batch, uint32_t *local_assignment);
 
+/* ---- serving --------------------------------------------------------------- */
+/* A request loop on one GPU: graphs are admitted as worker slots and HBM
+ * budget free up, run with device-resident state on their own streams (so
+ * they overlap on the device), and their results are published as they
+ * finish.  Each result equals lv_run of that graph alone.  No counterpart in
+ * the reference (one graph per process, main.cu). */
+typedef struct lv_server lv_server;
+
+typedef struct lv_request_result {
+    uint64_t id;
+    int status;
+    float q_final;
+    uint32_t passes;
+    uint32_t n_final;
+    uint32_t total_iters;
+    uint64_t sum_edges;
+    double t_submit_ms;  /* server clock: submission, start on a slot, finish */
+    double t_start_ms;
+    double t_end_ms;
+} lv_request_result;
+
+/* device bytes a graph of n vertices / ne entries may occupy while running */
+uint64_t lv_graph_footprint(uint32_t n, uint64_t ne);
+int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, float min_gain,
+                     int keep_assignments, lv_server **out);
+/* copies the CSR; LV_ERR_NOMEM if the graph can never fit the budget */
+int lv_server_submit(lv_server *server, uint64_t id, uint32_t n, const uint32_t *row,
+                     const uint32_t *col, const float *w);
+/* non-blocking: moves up to max finished results into out, *count set */
+int lv_server_poll(lv_server *server, lv_request_result *out, uint32_t max, uint32_t *count);
+/* with keep_assignments: the finished request's assignment (once) */
+int lv_server_get_assignment(lv_server *server, uint64_t id, uint32_t *assignment, uint32_t n);
+int lv_server_drain(lv_server *server);  /* waits until every submitted request finished */
+int lv_server_destroy(lv_server *server);
+
 /* ---- profiling ------------------------------------------------------------ */
 int lv_profile_enable(lv_graph *g, int on);
 int lv_profile_get(lv_graph *g, lv_kernel_stats *out); /* synchronises */

@@ -56,6 +56,14 @@ class BatchGraphResult(ctypes.Structure):
                 ("sum_edges", ctypes.c_uint64), ("status", ctypes.c_int)]
 
 
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("q_final", ctypes.c_float),
+                ("passes", ctypes.c_uint32), ("n_final", ctypes.c_uint32),
+                ("total_iters", ctypes.c_uint32), ("sum_edges", ctypes.c_uint64),
+                ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double),
+                ("t_end_ms", ctypes.c_double)]
+
+
 class KernelStats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 9), ("launches", ctypes.c_uint64 * 9),
                 ("bytes", ctypes.c_double * 9)]
@@ -69,7 +77,8 @@ EXPORTS = [
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
     "lv_gen_chung_lu", "lv_gen_rmat", "lv_gen_sbm", "lv_batch_create", "lv_batch_run",
-    "lv_batch_get_assignment",
+    "lv_batch_get_assignment", "lv_graph_footprint", "lv_server_create", "lv_server_submit",
+    "lv_server_poll", "lv_server_get_assignment", "lv_server_drain", "lv_server_destroy",
 ]
 
 _LIB = None
@@ -129,6 +138,16 @@ def lib():
     L.lv_batch_run.argtypes = [c.c_void_p, c.c_float, c.POINTER(BatchGraphResult),
                                c.POINTER(Result)]
     L.lv_batch_get_assignment.argtypes = [c.c_void_p, _u32p]
+    L.lv_graph_footprint.argtypes = [c.c_uint32, c.c_uint64]
+    L.lv_graph_footprint.restype = c.c_uint64
+    L.lv_server_create.argtypes = [c.c_int, c.c_uint32, c.c_uint64, c.c_float, c.c_int,
+                                   c.POINTER(c.c_void_p)]
+    L.lv_server_submit.argtypes = [c.c_void_p, c.c_uint64, c.c_uint32, _u32p, _u32p, _f32p]
+    L.lv_server_poll.argtypes = [c.c_void_p, c.POINTER(RequestResult), c.c_uint32,
+                                 c.POINTER(c.c_uint32)]
+    L.lv_server_get_assignment.argtypes = [c.c_void_p, c.c_uint64, _u32p, c.c_uint32]
+    L.lv_server_drain.argtypes = [c.c_void_p]
+    L.lv_server_destroy.argtypes = [c.c_void_p]
     _LIB = L
     return L
 
@@ -384,3 +403,57 @@ class Batch:
         _chk(self.L.lv_profile_get(self.h, ctypes.byref(st)), "lv_profile_get")
         return {f: {"ms": st.ms[i], "launches": st.launches[i], "bytes": st.bytes[i]}
                 for i, f in enumerate(KERNEL_FAMILIES)}
+
+
+class Server:
+    """Request loop with continuous admission (lv_server_*)."""
+
+    def __init__(self, slots=8, hbm_budget=200 << 30, min_gain=1e-6, keep_assignments=False,
+                 device=0):
+        L = lib()
+        self.L = L
+        h = ctypes.c_void_p()
+        _chk(L.lv_server_create(device, slots, hbm_budget, ctypes.c_float(min_gain),
+                                1 if keep_assignments else 0, ctypes.byref(h)),
+             "lv_server_create")
+        self.h = h
+
+    def submit(self, rid, row, col, w):
+        row = np.ascontiguousarray(row, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _chk(self.L.lv_server_submit(self.h, rid, row.size - 1, row,
+                                     col if col.size else np.zeros(1, np.uint32),
+                                     w if w.size else np.zeros(1, np.float32)),
+             "lv_server_submit")
+
+    def poll(self, max_results=1024):
+        buf = (RequestResult * max_results)()
+        cnt = ctypes.c_uint32()
+        _chk(self.L.lv_server_poll(self.h, buf, max_results, ctypes.byref(cnt)), "lv_server_poll")
+        return [buf[i] for i in range(cnt.value)]
+
+    def assignment(self, rid, n):
+        a = np.zeros(max(n, 1), np.uint32)
+        _chk(self.L.lv_server_get_assignment(self.h, rid, a, n), "lv_server_get_assignment")
+        return a[:n]
+
+    def drain(self):
+        _chk(self.L.lv_server_drain(self.h), "lv_server_drain")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lv_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

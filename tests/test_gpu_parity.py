@@ -294,3 +294,42 @@ def test_heavy_global_table_path(lv, orc):
                        timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "global-table path ok" in r.stdout
+
+
+def _server_pool(lv):
+    return [lv.gen_sbm(3000, 12, 14.0, 10.0, 31), lv.gen_sbm(20000, 40, 20.0, 10.0, 32),
+            lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 33),
+            lv.gen_rmat(14, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 34), lv.gen_fem(300, 25, 9, 35),
+            lv.gen_chung_lu(8000, 9.4, 2.5, 300, 50, 20.0, 36)]
+
+
+@pytest.mark.parametrize("slots,budget", [(4, 200 << 30), (3, None)])
+def test_server_results_match_solo(lv, slots, budget):
+    """Requests admitted concurrently (and, with a tight budget, one at a
+    time) give exactly the solo lv_run result."""
+    pool = _server_pool(lv)
+    solo = []
+    for r, c, w in pool:
+        with lv.Graph(r, c, w) as g:
+            res = g.run(1e-6)
+            solo.append((np.float32(res.q_final).view(np.uint32), res.passes, res.n_final,
+                         g.assignment()))
+    if budget is None:  # room for the largest graph only
+        budget = max(lv.lib().lv_graph_footprint(r.size - 1, c.size) for r, c, _ in pool)
+    print(f"[case] server slots={slots} budget={budget}", flush=True)
+    order = [i % len(pool) for i in range(3 * len(pool))]
+    with lv.Server(slots=slots, hbm_budget=budget, keep_assignments=True) as s:
+        for rid, gi in enumerate(order):
+            s.submit(rid, *pool[gi])
+        s.drain()
+        out = s.poll(4096)
+        assert len(out) == len(order)
+        for res in out:
+            gi = order[res.id]
+            assert res.status == 0
+            q, passes, nf, assign = solo[gi]
+            assert np.float32(res.q_final).view(np.uint32) == q
+            assert res.passes == passes and res.n_final == nf
+            assert res.t_submit_ms <= res.t_start_ms <= res.t_end_ms
+            got = s.assignment(res.id, pool[gi][0].size - 1)
+            assert np.array_equal(got, assign)
