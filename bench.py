@@ -4,7 +4,7 @@ A step is one full Louvain run (all passes, modularity optimisation and
 aggregation) over a device-resident synthetic graph; the graph is restored
 from its resident copy between steps, outside the timed region.
 
-    python bench.py [--config c2|c1|c3] [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--config c3|c1|c2|c4|c5] [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line.  value = sum over passes of CSR entries / run time.
 roofline: the dominant kernel family's algorithmic bytes / its launch time
@@ -95,7 +95,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

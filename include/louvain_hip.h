@@ -84,7 +84,8 @@ enum {
     LV_K_MOVE_GLOBAL = 6, /* bucket 7, global hash */
     LV_K_APPLY = 7,
     LV_K_AGGREGATE = 8,
-    LV_K_COUNT = 9
+    LV_K_RELABEL = 9,     /* locality order of a pass graph */
+    LV_K_COUNT = 10
 };
 
 typedef struct lv_kernel_stats {
@@ -110,6 +111,13 @@ int lv_graph_destroy(lv_graph *g);
  * assignment, so the same resident input can be processed again.  No
  * counterpart in the reference, which runs one graph per process. */
 int lv_graph_reset(lv_graph *g);
+/* Locality order of each pass graph on the device: vertices grouped by degree
+ * class so the state of high-degree vertices, which most entries point at,
+ * stays cache-resident.  mode -1: automatic (pass graphs of at least 2^20
+ * vertices), 0: never, 1: every pass.  Results are identical in every mode;
+ * ids are compared in the input numbering.  No counterpart in the reference
+ * (a performance knob); has no effect on batches. */
+int lv_graph_set_locality(lv_graph *g, int mode);
 
 /* ---- the reference operator API ------------------------------------------ */
 /* One Louvain pass of local moves.  *improved is the reference's return value

@@ -22,7 +22,7 @@ STATUS = {0: "ok", 1: "io", 2: "format", 3: "range", 4: "arg", 5: "nomem", 6: "s
           7: "hip", 8: "no_device", 9: "overflow"}
 
 KERNEL_FAMILIES = ["classify", "r1", "sumsq", "move_small", "move_wave", "move_block",
-                   "move_global", "apply", "aggregate"]
+                   "move_global", "apply", "aggregate", "relabel"]
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -65,14 +65,15 @@ class RequestResult(ctypes.Structure):
 
 
 class KernelStats(ctypes.Structure):
-    _fields_ = [("ms", ctypes.c_double * 9), ("launches", ctypes.c_uint64 * 9),
-                ("bytes", ctypes.c_double * 9)]
+    _fields_ = [("ms", ctypes.c_double * 10), ("launches", ctypes.c_uint64 * 10),
+                ("bytes", ctypes.c_double * 10)]
 
 
 # every symbol declared in include/louvain_hip.h
 EXPORTS = [
     "lv_status_string", "lv_device_count", "lv_mtx_read", "lv_host_csr_free",
-    "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_modularity_optimization", "lv_aggregation",
+    "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_graph_set_locality",
+    "lv_modularity_optimization", "lv_aggregation",
     "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
@@ -108,6 +109,7 @@ def lib():
                                   c.POINTER(c.c_void_p)]
     L.lv_graph_destroy.argtypes = [c.c_void_p]
     L.lv_graph_reset.argtypes = [c.c_void_p]
+    L.lv_graph_set_locality.argtypes = [c.c_void_p, c.c_int]
     L.lv_modularity_optimization.argtypes = [c.c_void_p, c.c_float, c.POINTER(c.c_int),
                                              c.POINTER(c.c_float)]
     L.lv_aggregation.argtypes = [c.c_void_p]
@@ -241,6 +243,10 @@ class Graph:
 
     def reset(self):
         _chk(self.L.lv_graph_reset(self.h), "lv_graph_reset")
+
+    def set_locality(self, mode):
+        """lv_graph_set_locality: -1 automatic, 0 never, 1 every pass."""
+        _chk(self.L.lv_graph_set_locality(self.h, int(mode)), "lv_graph_set_locality")
 
     def size(self):
         n, ne, n0 = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32()

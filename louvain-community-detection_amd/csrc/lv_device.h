@@ -48,10 +48,11 @@ __host__ __device__ inline int vertex_class(uint32_t d) {
 // Aggregation community classes (by total member degree).
 //   0: no entries (empty row, no kernel)                   (comm_deg == 0)
 //   1: LDS merge, one wave per community, 256-slot table   (comm_deg <= 128)
-//   2: LDS merge, one workgroup, 4096-slot table           (comm_deg <= 2048)
-//   3: global-hash merge + workgroup radix sort            (comm_deg >  2048)
-constexpr int kNumAggClasses = 4;
-constexpr int kAggGlobalClass = 3;
+//   2-4: LDS merge, one workgroup, 1024/2048/4096-slot table
+//                                          (comm_deg <= 512 / 1024 / 2048)
+//   5: global-hash merge + workgroup radix sort            (comm_deg >  2048)
+constexpr int kNumAggClasses = 6;
+constexpr int kAggGlobalClass = 5;
 constexpr uint32_t kAggSmallMax = 128;
 constexpr uint32_t kAggMidMax = 2048;
 
@@ -74,6 +75,12 @@ struct DevView {
     const float *gm;         // all_w per graph
     const uint32_t *gstate;  // per graph; only kGraphActive graphs move
     uint32_t G;              // graphs (stride of the per-graph accumulators)
+    // Locality order (single-graph mode): when non-null, vertex and community
+    // ids on the device are positions in a degree-class order and lid[x] is
+    // the id x stands for in the input numbering.  Every comparison of ids
+    // that the algorithm makes (tie-breaks, isLowerIdx, renumbering) uses
+    // lid, so results do not depend on the order.
+    const uint32_t *lid;
 };
 constexpr uint32_t kGraphActive = 1;
 
@@ -123,6 +130,23 @@ void launch_gather_u32(hipStream_t s, const uint32_t *src, const uint32_t *idx, 
                        uint32_t count);
 
 // degree classification of the current graph into `order`
+// locality order (lv_kernels.hip / lv_heavy.hip)
+struct RelabelView {
+    const uint32_t *row, *col;  // input-numbered CSR of the pass
+    const float *w;
+    const uint32_t *row2;       // new row pointers (by position)
+    uint32_t *col2;
+    float *w2;
+    const uint32_t *lid, *pmap;
+};
+void launch_relabel_map(hipStream_t s, uint32_t *order, uint32_t n, uint32_t *lid,
+                        uint32_t *pmap, const uint32_t *row, uint32_t *deg);
+void launch_relabel_rows(hipStream_t s, const RelabelView &r, uint32_t x0, uint32_t x1,
+                         int lanes);
+void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *coff, uint32_t nh,
+                          uint32_t nchunks, uint32_t x0);
+void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_t *lid, uint32_t n,
+                           uint32_t *flags, uint32_t *rank, uint32_t *new_id, uint32_t *scan_tmp);
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n,
                            uint32_t *counts /*kNumClasses*/,
                            unsigned long long *degsum /*kNumClasses*/);

@@ -46,9 +46,24 @@ __device__ __forceinline__ float gain_of(float hw, float m, float kv, float acv,
     return t1 + t4 / t6;
 }
 
-// reduction order of the reference's arg-max (modularity.cu:169-179, 363)
-__device__ __forceinline__ bool better(float ga, uint32_t ca, float gb, uint32_t cb) {
-    return ga > gb || (ga == gb && ca < cb);
+// input-numbering id of a device id (identity without a locality order)
+__device__ __forceinline__ uint32_t lid_of(const uint32_t *lid, uint32_t c) {
+    return (lid && c != kEmpty) ? lid[c] : c;
+}
+
+// reduction order of the reference's arg-max (modularity.cu:169-179, 363);
+// ids are compared in the input numbering, fetched only on a gain tie
+__device__ __forceinline__ bool better(float ga, uint32_t ca, float gb, uint32_t cb,
+                                       const uint32_t *lid) {
+    return ga > gb || (ga == gb && lid_of(lid, ca) < lid_of(lid, cb));
+}
+
+// isLowerIdx (modularity.cu:85-93): between two singleton communities only a
+// move to the lower id is allowed
+__device__ __forceinline__ bool lower_idx_ok(const DevView &g, uint32_t szv, uint32_t c,
+                                             uint32_t cv) {
+    if (szv == 1u && g.comm_sz[c] == 1u) return lid_of(g.lid, c) < lid_of(g.lid, cv);
+    return true;
 }
 
 template <int LOG2CAP>
@@ -79,12 +94,12 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 // (modularity.cu:260-275) on one wave: lane p holds virtual lanes p and
 // p+64.  Returns the winner (gain, comm, virtual lane) in lane 0.
 __device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_t c1, float &bg,
-                                        uint32_t &bc, uint32_t &bp) {
+                                        uint32_t &bc, uint32_t &bp, const uint32_t *lid) {
     const uint32_t p = __lane_id();
     bg = g0;
     bc = c0;
     bp = p;
-    if (better(g1, c1, g0, c0)) {
+    if (better(g1, c1, g0, c0, lid)) {
         bg = g1;
         bc = c1;
         bp = p + 64;
@@ -94,7 +109,7 @@ __device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_
         float og = __shfl_down(bg, o, 64);
         uint32_t oc = __shfl_down(bc, o, 64);
         uint32_t op = __shfl_down(bp, o, 64);
-        if (better(og, oc, bg, bc)) {
+        if (better(og, oc, bg, bc, lid)) {
             bg = og;
             bc = oc;
             bp = op;

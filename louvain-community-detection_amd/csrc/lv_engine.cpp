@@ -39,6 +39,9 @@ namespace {
 
 constexpr uint64_t kFixedLimit = 1ull << 61;
 constexpr uint32_t kMaxIters = 100000;
+// pass graphs from this many vertices on get the locality order by default
+// (below it the per-vertex state fits the L2 caches anyway)
+constexpr uint32_t kLocalityMinN = 1u << 20;
 
 // device scratch counters, one allocation
 // Device counters.  The head is what the host reads back each sweep; the
@@ -101,6 +104,11 @@ struct lv_graph {
     ull *A = nullptr;
     uint32_t *init_comm = nullptr;
     uint32_t *order = nullptr;
+    // locality order of the current pass (position -> input id and inverse),
+    // scratch for the renumbering under it
+    uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
+    bool relabeled = false;  // the current pass graph is in locality order
+    int locality = -1;       // lv_graph_set_locality mode
     // classification (host copies)
     uint32_t cls_count[kNumClasses] = {};
     uint32_t cls_off[kNumClasses + 1] = {};
@@ -203,6 +211,7 @@ lv::DevView view(lv_graph *g) {
     v.gm = g->gm_d;
     v.gstate = g->gstate_d;
     v.G = g->G;
+    v.lid = g->relabeled ? g->lid : nullptr;
     return v;
 }
 
@@ -234,14 +243,22 @@ struct ProfScope {
     ProfScope(lv_graph *g_, int fam_, double bytes_) : g(g_), fam(fam_), bytes(bytes_) {
         if (g->prof_on) {
             a = ev_get(g);
-            if (a) (void)hipEventRecord(a, g->stream);
+            if (a) record(a);
         }
+    }
+    // inside a capture the record must become an external event node, or the
+    // event carries no timestamp after a replay
+    void record(hipEvent_t e) {
+        if (g->capture_recs)
+            (void)hipEventRecordWithFlags(e, g->stream, hipEventRecordExternal);
+        else
+            (void)hipEventRecord(e, g->stream);
     }
     ~ProfScope() {
         if (g->prof_on && a) {
             hipEvent_t b = ev_get(g);
             if (b) {
-                (void)hipEventRecord(b, g->stream);
+                record(b);
                 // inside a captured sweep the pair becomes event-record nodes,
                 // read back after every replay
                 (g->capture_recs ? *g->capture_recs : g->prof_pending)
@@ -428,9 +445,42 @@ void modularity_launch(lv_graph *g) {
     lv::launch_sumsq(g->stream, v, g->order, g->n, limbs_ptr(g));
 }
 
+// Locality order for the pass (single-graph mode): position x of the new
+// numbering holds input vertex order[x] of the class grouping, so the heavy
+// and block classes, whose state most entries gather, are contiguous.
+bool want_locality(const lv_graph *g) {
+    if (g->G || g->n < 2) return false;
+    static const char *env = getenv("LV_LOCALITY");
+    const int mode = env ? atoi(env) : g->locality;
+    if (mode == 0) return false;
+    if (mode == 1) return true;
+    return g->n >= kLocalityMinN;
+}
+
+void relabel(lv_graph *g) {
+    hipStream_t s = g->stream;
+    const uint32_t n = g->n;
+    ProfScope ps(g, LV_K_RELABEL, 20.0 * n + 16.0 * (double)g->ne);
+    lv::launch_relabel_map(s, g->order, n, g->lid, g->pmap, g->row, g->rtmp);
+    lv::launch_scan_u32(s, g->rtmp, 0, n, g->row2, g->scan_tmp32);
+    const lv::RelabelView r{g->row, g->col, g->w, g->row2, g->col2, g->w2, g->lid, g->pmap};
+    lv::launch_relabel_rows(s, r, g->cls_off[1], g->cls_off[5], 8);
+    lv::launch_relabel_rows(s, r, g->cls_off[5], g->cls_off[lv::kClassHeavy], 64);
+    lv::launch_relabel_heavy(s, r, g->heavy_coff, g->cls_count[lv::kClassHeavy], g->heavy_chunks,
+                             g->cls_off[lv::kClassHeavy]);
+    std::swap(g->row, g->row2);
+    std::swap(g->col, g->col2);
+    std::swap(g->w, g->w2);
+    // the assignment refers to this pass's vertices: input ids -> positions
+    lv::launch_gather_u32(s, g->pmap, g->init_comm, g->init_comm, g->n0);
+    g->relabeled = true;
+}
+
 int pass_init(lv_graph *g, float *q0) {
     int rc = classify(g);
     if (rc) return rc;
+    g->relabeled = false;
+    if (want_locality(g)) relabel(g);
     zero_counters(g);
     r1_all(g, 0, 1);
     rc = sync_counters(g);
@@ -636,7 +686,11 @@ int aggregation(lv_graph *g) {
     lv::launch_fill_u32(s, g->comm_deg, 0u, n);
     lv::launch_fill_u32(s, g->mcursor, 0u, n);
     lv::launch_agg_prep(s, v, aview(g));
-    lv::launch_scan_u32(s, g->comm_sz, 1, n, g->new_id, g->scan_tmp32);
+    if (g->relabeled)  // rank in the input numbering
+        lv::launch_relabel_new_id(s, g->comm_sz, g->lid, n, g->rtmp, g->rrank, g->new_id,
+                                  g->scan_tmp32);
+    else
+        lv::launch_scan_u32(s, g->comm_sz, 1, n, g->new_id, g->scan_tmp32);
     lv::launch_scan_u32(s, g->comm_sz, 0, n, g->mstart, g->scan_tmp32);
     lv::launch_scan_u32(s, g->comm_deg, 0, n, g->epos, g->scan_tmp32);
     lv::launch_agg_members(s, v, aview(g));
@@ -695,7 +749,8 @@ int aggregation(lv_graph *g) {
     lv::launch_scan_u32(s, g->cnt, 0, nn, g->row2, g->scan_tmp32);
     uint32_t ne2 = 0;
     LV_HIP(hipMemcpyAsync(&ne2, g->row2 + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    lv::launch_agg_emit(s, v, av, g->aorder, nn, g->row2, g->col2, g->w2);
+    // communities without entries (class 0, first in aorder) have empty rows
+    lv::launch_agg_emit(s, v, av, g->aorder + aoff[1], nn - aoff[1], g->row2, g->col2, g->w2);
     lv::launch_agg_compose(s, g->comm, g->new_id, g->init_comm, g->n0);
     rc = sync_counters(g);
     if (rc) return rc;
@@ -706,6 +761,7 @@ int aggregation(lv_graph *g) {
     g->n = nn;
     g->ne = ne2;
     g->pass_ready = false;
+    g->relabeled = false;  // the contracted graph is in the input numbering
     return LV_OK;
 }
 
@@ -923,6 +979,10 @@ void free_all(lv_graph *g) {
     dfree(g->A);
     dfree(g->init_comm);
     dfree(g->order);
+    dfree(g->lid);
+    dfree(g->pmap);
+    dfree(g->rtmp);
+    dfree(g->rrank);
     dfree(g->heavy_caps);
     dfree(g->heavy_off);
     dfree(g->heavy_nch);
@@ -981,6 +1041,26 @@ void free_all(lv_graph *g) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+
+// p[lid[x]] = old p[x]
+template <typename T>
+void permute_by(T *p, const std::vector<uint32_t> &lid) {
+    if (!p) return;
+    std::vector<T> t(p, p + lid.size());
+    for (size_t x = 0; x < lid.size(); ++x) p[lid[x]] = t[x];
+}
+
+// positions -> input ids of the current pass (host copy of lid)
+int host_lid(lv_graph *g, std::vector<uint32_t> &lid) {
+    lid.resize(g->n);
+    LV_HIP(hipMemcpyAsync(lid.data(), g->lid, (size_t)g->n * 4, hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *lv_status_string(int st) {
@@ -1052,6 +1132,7 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
     e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
     e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
+    e |= dmalloc(&g->lid, N) | dmalloc(&g->pmap, N) | dmalloc(&g->rtmp, N) | dmalloc(&g->rrank, N);
     e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
     e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
     e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
@@ -1128,6 +1209,13 @@ int lv_graph_reset(lv_graph *g) {
     g->n = g->n0;
     g->ne = g->ne0;
     g->pass_ready = false;
+    g->relabeled = false;
+    return LV_OK;
+}
+
+int lv_graph_set_locality(lv_graph *g, int mode) {
+    if (!g || mode < -1 || mode > 1) return LV_ERR_ARG;
+    g->locality = mode;
     return LV_OK;
 }
 
@@ -1241,7 +1329,12 @@ int lv_run(lv_graph *g, float min_gain, lv_result *res) {
 int lv_get_assignment(lv_graph *g, uint32_t *assignment) {
     if (!g || !assignment) return LV_ERR_ARG;
     LV_HIP(hipSetDevice(g->device));
-    LV_HIP(hipMemcpyAsync(assignment, g->init_comm, (size_t)g->n0 * sizeof(uint32_t),
+    const uint32_t *src = g->init_comm;
+    if (g->relabeled) {  // positions of the current pass -> its input ids
+        lv::launch_gather_u32(g->stream, g->lid, g->init_comm, g->rtmp, g->n0);
+        src = g->rtmp;
+    }
+    LV_HIP(hipMemcpyAsync(assignment, src, (size_t)g->n0 * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, g->stream));
     LV_HIP(hipStreamSynchronize(g->stream));
     return LV_OK;
@@ -1259,11 +1352,37 @@ int lv_get_csr(lv_graph *g, uint32_t *row, uint32_t *col, float *w) {
     if (!g) return LV_ERR_ARG;
     LV_HIP(hipSetDevice(g->device));
     hipStream_t s = g->stream;
-    if (row)
-        LV_HIP(hipMemcpyAsync(row, g->row, ((size_t)g->n + 1) * 4, hipMemcpyDeviceToHost, s));
-    if (col && g->ne)
-        LV_HIP(hipMemcpyAsync(col, g->col, (size_t)g->ne * 4, hipMemcpyDeviceToHost, s));
-    if (w && g->ne) LV_HIP(hipMemcpyAsync(w, g->w, (size_t)g->ne * 4, hipMemcpyDeviceToHost, s));
+    const size_t n = g->n, ne = g->ne;
+    if (g->relabeled) {  // reported in the input numbering, rows in input order
+        std::vector<uint32_t> lid, r(n + 1), c(ne);
+        std::vector<float> x(ne);
+        int rc = host_lid(g, lid);
+        if (rc) return rc;
+        LV_HIP(hipMemcpyAsync(r.data(), g->row, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+        if (ne) {
+            LV_HIP(hipMemcpyAsync(c.data(), g->col, ne * 4, hipMemcpyDeviceToHost, s));
+            LV_HIP(hipMemcpyAsync(x.data(), g->w, ne * 4, hipMemcpyDeviceToHost, s));
+        }
+        LV_HIP(hipStreamSynchronize(s));
+        std::vector<uint32_t> pos(n), deg(n + 1, 0);
+        for (size_t p = 0; p < n; ++p) {
+            pos[lid[p]] = (uint32_t)p;
+            deg[lid[p] + 1] = r[p + 1] - r[p];
+        }
+        for (size_t v = 0; v < n; ++v) deg[v + 1] += deg[v];
+        if (row) memcpy(row, deg.data(), (n + 1) * 4);
+        for (size_t v = 0; v < n; ++v) {
+            const uint32_t p = pos[v], b = r[p], d = r[p + 1] - b;
+            for (uint32_t j = 0; j < d; ++j) {
+                if (col) col[deg[v] + j] = lid[c[b + j]];
+                if (w) w[deg[v] + j] = x[b + j];
+            }
+        }
+        return LV_OK;
+    }
+    if (row) LV_HIP(hipMemcpyAsync(row, g->row, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (col && ne) LV_HIP(hipMemcpyAsync(col, g->col, ne * 4, hipMemcpyDeviceToHost, s));
+    if (w && ne) LV_HIP(hipMemcpyAsync(w, g->w, ne * 4, hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
     return LV_OK;
 }
@@ -1278,6 +1397,17 @@ int lv_get_state(lv_graph *g, float *k, uint32_t *comm, uint32_t *comm_sz, int64
     if (comm_sz) LV_HIP(hipMemcpyAsync(comm_sz, g->comm_sz, n * 4, hipMemcpyDeviceToHost, s));
     if (a) LV_HIP(hipMemcpyAsync(a, g->A, n * 8, hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
+    if (g->relabeled) {  // positions -> input ids (vertex and community ids alike)
+        std::vector<uint32_t> lid;
+        int rc = host_lid(g, lid);
+        if (rc) return rc;
+        if (comm)
+            for (size_t x = 0; x < n; ++x) comm[x] = lid[comm[x]];
+        permute_by(k, lid);
+        permute_by(comm, lid);
+        permute_by(comm_sz, lid);
+        permute_by(a, lid);
+    }
     return LV_OK;
 }
 

@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
         const uint32_t slot = h.dlist[x.base + i];
         const uint32_t c = *tkey(tab, slot);
         float gg = -INFINITY;
-        bool cand = true;  // isLowerIdx (modularity.cu:85-93)
-        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        // isLowerIdx (modularity.cu:85-93)
+        const bool cand = lower_idx_ok(g, szv, c, cv);
         if (cand)
             gg = gain_of(fix2f(*tval(tab, slot), g.S), v_m(g, x.v), kv, acv, fix2f(g.A[c], g.S));
         *tval(tab, slot) = (unsigned long long)__float_as_uint(gg);
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
     }
     float bg;
     uint32_t bc, bp;
-    tree128(gl[0], cl[0], gl[1], cl[1], bg, bc, bp);
+    tree128(gl[0], cl[0], gl[1], cl[1], bg, bc, bp, g.lid);
     if (p == 0) {
         const uint32_t cv = g.comm[v];
         g.new_comm[v] = decide_lastedge(g, v_m(g, v), base, d, cv, bg, bp, h.hcew[i]);
@@ -462,8 +462,8 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         const uint32_t slot = dist[q];
         const uint32_t c = key[slot];
         float gg = -INFINITY;
-        bool cand = true;  // isLowerIdx (modularity.cu:85-93)
-        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        // isLowerIdx (modularity.cu:85-93)
+        const bool cand = lower_idx_ok(g, szv, c, cv);
         if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
@@ -564,6 +564,26 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
     hipLaunchKernelGGL(k_heavy_lanes, grid, block, 0, s, g, f);
     hipLaunchKernelGGL(k_heavy_decide, dim3((h.nh + 3) / 4), block, 0, s, g, h);
     hipLaunchKernelGGL(k_heavy_clean, grid, block, 0, s, g, f);
+}
+
+// locality order: heavy rows are copied one chunk of kHeavyChunk entries per
+// workgroup (positions x0 + i, i < nh, chunk offsets coff)
+__global__ __launch_bounds__(256) void k_relabel_heavy(RelabelView r, const uint32_t *coff,
+                                                       uint32_t nh, uint32_t x0) {
+    const uint32_t i = heavy_find(coff, nh, blockIdx.x);
+    const uint32_t c = blockIdx.x - coff[i];
+    const uint32_t x = x0 + i, v = r.lid[x];
+    const uint32_t b = r.row[v], d = r.row[v + 1] - b, o = r.row2[x];
+    const uint32_t j1 = min(d, (c + 1) * kHeavyChunk);
+    for (uint32_t j = c * kHeavyChunk + threadIdx.x; j < j1; j += 256) {
+        r.col2[o + j] = r.pmap[r.col[b + j]];
+        r.w2[o + j] = r.w[b + j];
+    }
+}
+void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *coff, uint32_t nh,
+                          uint32_t nchunks, uint32_t x0) {
+    if (!nh || !nchunks) return;
+    hipLaunchKernelGGL(k_relabel_heavy, dim3(nchunks), dim3(256), 0, s, r, coff, nh, x0);
 }
 
 // ---------------------------------------------------------------------------

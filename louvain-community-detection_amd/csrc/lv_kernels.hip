@@ -128,6 +128,89 @@ void launch_gather_u32(hipStream_t s, const uint32_t *src, const uint32_t *idx, 
 }
 
 // ---------------------------------------------------------------------------
+// locality order (single-graph mode, once per pass)
+//
+// A sweep gathers comm/A/comm_sz at entry endpoints, and endpoints favour
+// high-degree vertices.  Renumbering the pass graph so vertices are grouped by
+// kernel class puts the hubs' state in one compact range that stays in L2,
+// where input-numbered hubs are scattered over the whole array.  Position x
+// stands for input id lid[x]; pmap is the inverse.  Rows keep their entry
+// order, so every pinned sum and lane assignment is unchanged.
+// ---------------------------------------------------------------------------
+// lid = order (vertices grouped by class), pmap = its inverse, order -> iota
+__global__ void k_relabel_map(uint32_t *order, uint32_t n, uint32_t *lid, uint32_t *pmap) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) {
+        const uint32_t v = order[x];
+        lid[x] = v;
+        pmap[v] = x;
+        order[x] = x;
+    }
+}
+__global__ void k_relabel_deg(const uint32_t *row, const uint32_t *lid, uint32_t n,
+                              uint32_t *deg) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) {
+        const uint32_t v = lid[x];
+        deg[x] = row[v + 1] - row[v];
+    }
+}
+void launch_relabel_map(hipStream_t s, uint32_t *order, uint32_t n, uint32_t *lid,
+                        uint32_t *pmap, const uint32_t *row, uint32_t *deg) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_relabel_map, dim3((n + 255) / 256), dim3(256), 0, s, order, n, lid, pmap);
+    hipLaunchKernelGGL(k_relabel_deg, dim3((n + 255) / 256), dim3(256), 0, s, row, lid, n, deg);
+}
+
+// copy the rows of positions [x0, x1) into the new CSR, G lanes per row,
+// column ids mapped through pmap
+template <int G>
+__global__ __launch_bounds__(256) void k_relabel_rows(RelabelView r, uint32_t x0, uint32_t x1) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t x = x0 + t / G, lane = t % G;
+    if (x >= x1) return;
+    const uint32_t v = r.lid[x];
+    const uint32_t b = r.row[v], d = r.row[v + 1] - b, o = r.row2[x];
+    for (uint32_t j = lane; j < d; j += G) {
+        r.col2[o + j] = r.pmap[r.col[b + j]];
+        r.w2[o + j] = r.w[b + j];
+    }
+}
+void launch_relabel_rows(hipStream_t s, const RelabelView &r, uint32_t x0, uint32_t x1,
+                         int lanes) {
+    if (x1 <= x0) return;
+    const uint64_t threads = (uint64_t)(x1 - x0) * (uint64_t)lanes;
+    const dim3 grid((unsigned)((threads + 255) / 256));
+    if (lanes == 8)
+        hipLaunchKernelGGL(k_relabel_rows<8>, grid, dim3(256), 0, s, r, x0, x1);
+    else
+        hipLaunchKernelGGL(k_relabel_rows<64>, grid, dim3(256), 0, s, r, x0, x1);
+}
+
+// new_id under a locality order: rank of lid[c] among the non-empty
+// communities in the input numbering (rank = scan over the input numbering)
+__global__ void k_scatter_by(const uint32_t *src, const uint32_t *idx, uint32_t n,
+                             uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[idx[i]] = src[i];
+}
+__global__ void k_relabel_newid(const uint32_t *rank, const uint32_t *lid, uint32_t n,
+                                uint32_t *new_id) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) new_id[i] = rank[lid[i]];
+    else if (i == n) new_id[n] = rank[n];
+}
+void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_t *lid, uint32_t n,
+                           uint32_t *flags, uint32_t *rank, uint32_t *new_id, uint32_t *scan_tmp) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_scatter_by, dim3((n + 255) / 256), dim3(256), 0, s, comm_sz, lid, n,
+                       flags);
+    launch_scan_u32(s, flags, 1, n, rank, scan_tmp);
+    hipLaunchKernelGGL(k_relabel_newid, dim3((n + 256) / 256), dim3(256), 0, s, rank, lid, n,
+                       new_id);
+}
+
+// ---------------------------------------------------------------------------
 // degree classification: once per pass (the reference re-partitions the
 // whole vertex set 7 times per sweep, modularity.cu:468-469)
 // ---------------------------------------------------------------------------
@@ -400,8 +483,8 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         }
         __syncthreads();
         if (has) {
-            bool cand = true;  // isLowerIdx (modularity.cu:85-93)
-            if (szv == 1u && g.comm_sz[ce] == 1u) cand = ce < cv;
+            // isLowerIdx (modularity.cu:85-93)
+            const bool cand = lower_idx_ok(g, szv, ce, cv);
             if (cand) {
                 float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[ce], g.S));
                 if (gg > best_g) {
@@ -417,7 +500,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         float og = __shfl_xor(best_g, o, T);
         uint32_t oc = __shfl_xor(best_c, o, T);
         unsigned long long ow = __shfl_xor(cew, o, T);
-        if (better(og, oc, best_g, best_c)) {
+        if (better(og, oc, best_g, best_c, g.lid)) {
             best_g = og;
             best_c = oc;
         }
@@ -506,8 +589,8 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         uint32_t slot = sdist[wv][i];
         uint32_t c = key[slot];
         float gg = -INFINITY;
-        bool cand = true;
-        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        // isLowerIdx (modularity.cu:85-93)
+        const bool cand = lower_idx_ok(g, szv, c, cv);
         if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
@@ -533,7 +616,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
     }
     float bg;
     uint32_t bc, bp;
-    tree128(g0, c0, g1, c1, bg, bc, bp);
+    tree128(g0, c0, g1, c1, bg, bc, bp, g.lid);
     if (active && p == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, cew);
 }
 
@@ -543,7 +626,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
 template <int P>
 __device__ __forceinline__ void block_argmax(float tg, uint32_t tc, uint32_t tj, float *lg,
                                              uint32_t *lc, uint32_t *lj, float &bg, uint32_t &bc,
-                                             uint32_t &bp) {
+                                             uint32_t &bp, const uint32_t *lid) {
     lg[threadIdx.x] = tg;
     lc[threadIdx.x] = tc;
     lj[threadIdx.x] = tj;
@@ -570,7 +653,7 @@ __device__ __forceinline__ void block_argmax(float tg, uint32_t tc, uint32_t tj,
             gv[h] = g;
             cvv[h] = c;
         }
-        tree128(gv[0], cvv[0], gv[1], cvv[1], bg, bc, bp);
+        tree128(gv[0], cvv[0], gv[1], cvv[1], bg, bc, bp, lid);
     }
 }
 
@@ -641,8 +724,8 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
         uint32_t slot = dist[i];
         uint32_t c = key[slot];
         float gg = -INFINITY;
-        bool cand = true;
-        if (szv == 1u && g.comm_sz[c] == 1u) cand = c < cv;
+        // isLowerIdx (modularity.cu:85-93)
+        const bool cand = lower_idx_ok(g, szv, c, cv);
         if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
@@ -661,7 +744,7 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     }
     float bg = -INFINITY;
     uint32_t bc = kEmpty, bp = kEmpty;
-    block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp);
+    block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp, g.lid);
     if (t == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, scew);
 }
 
@@ -871,9 +954,10 @@ void launch_scan_u64(hipStream_t s, const unsigned long long *in, uint32_t n,
 
 // ---------------------------------------------------------------------------
 // aggregation (aggregation.cu:387-462), canonical form:
-//   new_id(c) = number of non-empty c' < c (scan of comm_sz != 0);
-//   row of new_id(c) = distinct neighbour communities d of c's members,
-//   W_cd exact then rounded once, in ascending d (== ascending new_id(d));
+//   new_id(c) = number of non-empty c' < c in the input numbering (scan of
+//   comm_sz != 0, through lid under a locality order);
+//   row of new_id(c) = new ids of the distinct neighbour communities d of
+//   c's members, ascending, W_cd exact then rounded once;
 //   communities without entries get an empty row.
 // ---------------------------------------------------------------------------
 
@@ -908,8 +992,10 @@ void launch_agg_members(hipStream_t s, const DevView &g, AggView a) {
 __device__ __forceinline__ int agg_class(uint32_t cdeg) {
     if (cdeg == 0) return 0;
     if (cdeg <= kAggSmallMax) return 1;
-    if (cdeg <= kAggMidMax) return 2;
-    return 3;
+    if (cdeg <= 512) return 2;
+    if (cdeg <= 1024) return 3;
+    if (cdeg <= kAggMidMax) return 4;
+    return kAggGlobalClass;
 }
 
 __global__ __launch_bounds__(256) void k_agg_class_count(DevView g, AggView a, uint32_t *counts) {
@@ -994,7 +1080,8 @@ __device__ void lds_bitonic(uint32_t *k, uint32_t *sl, uint32_t n2) {
 }
 
 // LDS merge: one workgroup of P threads per community; waves walk members,
-// lanes walk a member's row.  Output: sorted (d, W) at tkey/tval[epos[c]..],
+// lanes walk a member's row.  Output: (new_id(d), W) sorted at
+// tkey/tval[epos[c]..],
 // distinct count at cnt[new_id[c]].
 template <int P, int CAP>
 __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const uint32_t *comms,
@@ -1026,7 +1113,7 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
                 if (prev == kEmpty) {
                     uint32_t idx = atomicAdd(&ndist, 1u);
-                    skeys[idx] = dd;
+                    skeys[idx] = a.new_id[dd];  // sorted by new id
                     sslot[idx] = slot;
                     break;
                 }
@@ -1095,7 +1182,7 @@ __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
                     const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
                     if (prev == kEmpty) {
                         const uint32_t idx = atomicAdd(&nd_all[wv], 1u);
-                        sk[idx] = dd;
+                        sk[idx] = a.new_id[dd];  // sorted by new id
                         ss[idx] = slot;
                         break;
                     }
@@ -1257,7 +1344,7 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
         atomicAdd(&val[slot], lval[ls]);
         if (claimed) {
             my_new[nmine] = slot;
-            my_key[nmine] = dd;
+            my_key[nmine] = a.new_id[dd];  // sorted by new id
             my_rank[nmine] = atomicAdd(&nnew, 1u);
             ++nmine;
         }
@@ -1381,6 +1468,12 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
         hipLaunchKernelGGL(k_agg_merge_small, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms,
                            count);
     else if (cls == 2)
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 1024>), dim3(count), dim3(256), 0, s, g, a, comms,
+                           count);
+    else if (cls == 3)
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 2048>), dim3(count), dim3(256), 0, s, g, a, comms,
+                           count);
+    else if (cls == 4)
         hipLaunchKernelGGL((k_agg_merge_lds<256, 4096>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
     else {
@@ -1413,7 +1506,7 @@ __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const ui
     const uint32_t o = row2[nid], nd = row2[nid + 1] - o;
     const uint32_t src = a.epos[c];
     for (uint32_t i = lane; i < nd; i += 64) {
-        col2[o + i] = a.new_id[a.tkey[src + i]];
+        col2[o + i] = a.tkey[src + i];  // already renumbered
         w2[o + i] = fix2f(a.tval[src + i], g.S);
     }
 }

@@ -54,13 +54,17 @@ def _cmp_csr(g, o, where):
     assert np.array_equal(w.view(np.uint32), ow.view(np.uint32)), f"weights differ {where}"
 
 
+# locality 0: input numbering on the device; 1: every pass graph in locality
+# order (ids compared through lid) -- results must be identical either way
+@pytest.mark.parametrize("loc", [0, 1])
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
                                   "sbm_5k", "rmat16"])
-def test_stepwise_first_pass(lv, orc, graphs, name):
+def test_stepwise_first_pass(lv, orc, graphs, name, loc):
     row, col, w = graphs[name]
-    print(f"[case] stepwise {name} n={row.size - 1} ne={col.size}", flush=True)
+    print(f"[case] stepwise {name} loc={loc} n={row.size - 1} ne={col.size}", flush=True)
     o = orc.Oracle(row.astype(np.uint64), col, w)
     with lv.Graph(_u32(row), col, w) as g:
+        g.set_locality(loc)
         s, m = g.scale()
         assert s == o.scale
         q0 = g.pass_init()
@@ -79,14 +83,18 @@ def test_stepwise_first_pass(lv, orc, graphs, name):
             assert q.view(np.uint32) == oq.view(np.uint32), (it, q, oq)
 
 
+# locality 0: input numbering on the device; 1: every pass graph in locality
+# order (ids compared through lid) -- results must be identical either way
+@pytest.mark.parametrize("loc", [0, 1])
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
                                   "sbm_5k", "rmat16"])
-def test_passes_and_aggregation(lv, orc, graphs, name):
+def test_passes_and_aggregation(lv, orc, graphs, name, loc):
     row, col, w = graphs[name]
-    print(f"[case] passes {name}", flush=True)
+    print(f"[case] passes {name} loc={loc}", flush=True)
     o = orc.Oracle(row.astype(np.uint64), col, w)
     gain = np.float32(1e-6)
     with lv.Graph(_u32(row), col, w) as g:
+        g.set_locality(loc)
         for p in range(20):
             imp, qp = g.modularity_optimization(gain)
             oimp, oqp, _ = o.modularity_optimization(gain)
@@ -101,14 +109,18 @@ def test_passes_and_aggregation(lv, orc, graphs, name):
             assert np.array_equal(g.assignment(), o.init_comm()), f"assignment pass {p}"
 
 
+# locality 0: input numbering on the device; 1: every pass graph in locality
+# order (ids compared through lid) -- results must be identical either way
+@pytest.mark.parametrize("loc", [0, 1])
 @pytest.mark.parametrize("name", ["bcsstk16", "fem", "chung_lu_20k", "rmat12", "rmat14_unit",
                                   "sbm_5k", "rmat16"])
-def test_full_run(lv, orc, graphs, name):
+def test_full_run(lv, orc, graphs, name, loc):
     row, col, w = graphs[name]
-    print(f"[case] run {name}", flush=True)
+    print(f"[case] run {name} loc={loc}", flush=True)
     o = orc.Oracle(row.astype(np.uint64), col, w)
     ores = o.run(1e-6)
     with lv.Graph(_u32(row), col, w) as g:
+        g.set_locality(loc)
         res = g.run(1e-6)
         assign = g.assignment()
         n_final = g.size()[0]
@@ -146,15 +158,18 @@ def test_edge_cases(lv, orc):
             lines.append(f"{hub} {int(rng.integers(4, 60))} {1 + int(rng.integers(0, 3))}\n")
     cases["dup_hubs"] = f"60 60 {len(lines)}\n" + "".join(lines)
     for name, text in cases.items():
-        print(f"[case] edge {name}", flush=True)
         row, col, w = orc.parse_mtx(text=text)
         o = orc.Oracle(row, col, w)
         ores = o.run(1e-6)
-        with lv.Graph(_u32(row), col, w) as g:
-            res = g.run(1e-6)
-            assign = g.assignment()
-        assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32), name
-        assert np.array_equal(assign, o.init_comm()), name
+        for loc in (0, 1):
+            print(f"[case] edge {name} loc={loc}", flush=True)
+            with lv.Graph(_u32(row), col, w) as g:
+                g.set_locality(loc)
+                res = g.run(1e-6)
+                assign = g.assignment()
+            assert np.float32(res.q_final).view(np.uint32) == \
+                np.float32(ores.q_final).view(np.uint32), (name, loc)
+            assert np.array_equal(assign, o.init_comm()), (name, loc)
 
 
 def test_copurchase_full_size(lv, orc):
@@ -163,18 +178,23 @@ def test_copurchase_full_size(lv, orc):
     print(f"[case] copurchase full n={row.size - 1} ne={col.size}", flush=True)
     o = orc.Oracle(row.astype(np.uint64), col, w)
     ores = o.run(1e-6)
-    with lv.Graph(row, col, w) as g:
-        res = g.run(1e-6)
-        assign = g.assignment()
-    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)
-    assert np.array_equal(assign, o.init_comm())
-    assert res.total_iters == ores.total_iters and res.n_final == ores.n_final
+    for loc in (0, 1):
+        print(f"[case] copurchase full loc={loc}", flush=True)
+        with lv.Graph(row, col, w) as g:
+            g.set_locality(loc)
+            res = g.run(1e-6)
+            assign = g.assignment()
+        assert np.float32(res.q_final).view(np.uint32) == \
+            np.float32(ores.q_final).view(np.uint32), loc
+        assert np.array_equal(assign, o.init_comm()), loc
+        assert res.total_iters == ores.total_iters and res.n_final == ores.n_final, loc
 
 
 def test_rmat24_properties(lv, orc):
     """R-MAT scale 24 (16.8M vertices, ~521M entries): size-independent checks
-    -- identical results on two runs from the resident copy, a valid dense
-    assignment, and the printed modularity within 1e-6 of an FP64 recompute."""
+    -- identical results on two runs from the resident copy and with the
+    locality order switched off, a valid dense assignment, and the printed
+    modularity within 1e-6 of an FP64 recompute."""
     row, col, w = lv.gen_rmat(24, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)
     n = row.size - 1
     print(f"[case] rmat24 n={n} ne={col.size}", flush=True)
@@ -184,10 +204,18 @@ def test_rmat24_properties(lv, orc):
         g.reset()
         r2 = g.run(1e-6)
         a2 = g.assignment()
+        g.reset()
+        g.set_locality(0)
+        r3 = g.run(1e-6)
+        a3 = g.assignment()
     print(f"[case] rmat24 Q={r1.q_final} passes={r1.passes} sweeps={r1.total_iters} "
           f"communities={r1.n_final} ms={r1.alg_ms:.1f}", flush=True)
+    print(f"[case] rmat24 input numbering: ms={r3.alg_ms:.1f}", flush=True)
     assert np.float32(r1.q_final).view(np.uint32) == np.float32(r2.q_final).view(np.uint32)
+    assert np.float32(r1.q_final).view(np.uint32) == np.float32(r3.q_final).view(np.uint32)
     assert np.array_equal(a1, a2)
+    assert np.array_equal(a1, a3)
+    assert (r1.passes, r1.total_iters, r1.n_final) == (r3.passes, r3.total_iters, r3.n_final)
     assert a1.max() + 1 == r1.n_final
     assert np.unique(a1).size == r1.n_final
     q64 = orc.q64(row.astype(np.uint64), col, w, a1)
