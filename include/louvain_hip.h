@@ -98,8 +98,14 @@ const char *lv_status_string(int status);
 int lv_device_count(int *count);
 
 /* ---- input ---------------------------------------------------------------- */
+/* Reads a Matrix Market file (or a binary CSR written by lv_csr_write,
+ * recognised by its magic) into a host CSR.  Multi-threaded; same result as
+ * the reference parser.  Errors: LV_ERR_IO, LV_ERR_FORMAT. */
 int lv_mtx_read(const char *path, lv_host_csr *out);
 void lv_host_csr_free(lv_host_csr *csr);
+/* Binary CSR cache of a parsed graph (no counterpart in the reference, which
+ * re-parses the text every run); lv_mtx_read loads it. */
+int lv_csr_write(const char *path, const lv_host_csr *csr);
 
 /* ---- graph lifetime -------------------------------------------------------- */
 /* Uploads the CSR (row: n+1 offsets, col/w: row[n] entries) to `device`.
@@ -118,6 +124,27 @@ int lv_graph_reset(lv_graph *g);
  * ids are compared in the input numbering.  No counterpart in the reference
  * (a performance knob); has no effect on batches. */
 int lv_graph_set_locality(lv_graph *g, int mode);
+
+/* Launch-shape parameters.  None of them changes a result: the seven semantic
+ * step sets, lane orders and sums are fixed; these only choose which kernel
+ * serves a row and how work is cut.  No counterpart in the reference. */
+typedef struct lv_tuning {
+    uint32_t heavy_min_degree;      /* rows longer than this take the split heavy path
+                                       (319..4096; shorter bucket-7 rows: one workgroup) */
+    uint32_t bucket_entries;        /* target entries per LDS bucket of a heavy row
+                                       (power of two, 64..1024) */
+    int32_t locality;               /* -1 automatic, 0 never, 1 every pass */
+    uint32_t locality_min_vertices; /* automatic mode: pass graphs from this many vertices */
+} lv_tuning;
+int lv_tuning_default(lv_tuning *t);
+int lv_graph_set_tuning(lv_graph *g, const lv_tuning *t);  /* LV_ERR_ARG if out of range */
+int lv_graph_get_tuning(lv_graph *g, lv_tuning *t);
+/* Autotuner: runs the resident input (lv_graph_reset before each run) under
+ * each candidate -- or a built-in grid over heavy_min_degree, bucket_entries
+ * and locality when candidates is NULL -- keeps the fastest (best of two runs
+ * each) on g and returns it with its time.  g is left reset. */
+int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32_t count,
+                lv_tuning *best, double *best_ms);
 
 /* ---- the reference operator API ------------------------------------------ */
 /* One Louvain pass of local moves.  *improved is the reference's return value
@@ -205,6 +232,10 @@ int lv_server_destroy(lv_server *server);
 
 /* ---- profiling ------------------------------------------------------------ */
 int lv_profile_enable(lv_graph *g, int on);
+/* Families that get event pairs while profiling is on (bit f = family f;
+ * default all).  Event records captured into a sweep's HIP graph cost a few
+ * microseconds each, so a timed run profiles only the family it reports. */
+int lv_profile_select(lv_graph *g, uint32_t family_mask);
 int lv_profile_get(lv_graph *g, lv_kernel_stats *out); /* synchronises */
 int lv_profile_reset(lv_graph *g);
 

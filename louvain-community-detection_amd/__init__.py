@@ -64,6 +64,14 @@ class RequestResult(ctypes.Structure):
                 ("t_end_ms", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("heavy_min_degree", ctypes.c_uint32), ("bucket_entries", ctypes.c_uint32),
+                ("locality", ctypes.c_int32), ("locality_min_vertices", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class KernelStats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 10), ("launches", ctypes.c_uint64 * 10),
                 ("bytes", ctypes.c_double * 10)]
@@ -71,12 +79,13 @@ class KernelStats(ctypes.Structure):
 
 # every symbol declared in include/louvain_hip.h
 EXPORTS = [
-    "lv_status_string", "lv_device_count", "lv_mtx_read", "lv_host_csr_free",
+    "lv_status_string", "lv_device_count", "lv_mtx_read", "lv_host_csr_free", "lv_csr_write",
     "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_graph_set_locality",
+    "lv_tuning_default", "lv_graph_set_tuning", "lv_graph_get_tuning", "lv_autotune",
     "lv_modularity_optimization", "lv_aggregation",
     "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
-    "lv_profile_enable", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
+    "lv_profile_enable", "lv_profile_select", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
     "lv_gen_chung_lu", "lv_gen_rmat", "lv_gen_sbm", "lv_batch_create", "lv_batch_run",
     "lv_batch_get_assignment", "lv_graph_footprint", "lv_server_create", "lv_server_submit",
     "lv_server_poll", "lv_server_get_assignment", "lv_server_drain", "lv_server_destroy",
@@ -105,11 +114,17 @@ def lib():
     L.lv_mtx_read.argtypes = [c.c_char_p, c.POINTER(HostCSR)]
     L.lv_host_csr_free.argtypes = [c.POINTER(HostCSR)]
     L.lv_host_csr_free.restype = None
+    L.lv_csr_write.argtypes = [c.c_char_p, c.POINTER(HostCSR)]
     L.lv_graph_create.argtypes = [c.c_int, c.c_uint32, _u32p, _u32p, _f32p,
                                   c.POINTER(c.c_void_p)]
     L.lv_graph_destroy.argtypes = [c.c_void_p]
     L.lv_graph_reset.argtypes = [c.c_void_p]
     L.lv_graph_set_locality.argtypes = [c.c_void_p, c.c_int]
+    L.lv_tuning_default.argtypes = [c.POINTER(Tuning)]
+    L.lv_graph_set_tuning.argtypes = [c.c_void_p, c.POINTER(Tuning)]
+    L.lv_graph_get_tuning.argtypes = [c.c_void_p, c.POINTER(Tuning)]
+    L.lv_autotune.argtypes = [c.c_void_p, c.c_float, c.POINTER(Tuning), c.c_uint32,
+                              c.POINTER(Tuning), c.POINTER(c.c_double)]
     L.lv_modularity_optimization.argtypes = [c.c_void_p, c.c_float, c.POINTER(c.c_int),
                                              c.POINTER(c.c_float)]
     L.lv_aggregation.argtypes = [c.c_void_p]
@@ -125,6 +140,7 @@ def lib():
     L.lv_sweep.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_uint64)]
     L.lv_modularity.argtypes = [c.c_void_p, c.POINTER(c.c_float)]
     L.lv_profile_enable.argtypes = [c.c_void_p, c.c_int]
+    L.lv_profile_select.argtypes = [c.c_void_p, c.c_uint32]
     L.lv_profile_get.argtypes = [c.c_void_p, c.POINTER(KernelStats)]
     L.lv_profile_reset.argtypes = [c.c_void_p]
     L.lv_gen_fem.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint64, c.POINTER(HostCSR)]
@@ -181,6 +197,19 @@ def read_mtx(path):
     return _take_csr(h)
 
 
+def write_csr(path, row, col, w, nnz=0):
+    """lv_csr_write: binary CSR cache that read_mtx (lv_mtx_read) loads."""
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    h = HostCSR()
+    h.n, h.nnz, h.ne = row.size - 1, nnz, col.size
+    h.row = row.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    h.col = col.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    h.w = w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    _chk(lib().lv_csr_write(str(path).encode(), ctypes.byref(h)), "lv_csr_write")
+
+
 def gen_fem(nodes=814, band=25, couple=9, seed=1):
     h = HostCSR()
     _chk(lib().lv_gen_fem(nodes, band, couple, seed, ctypes.byref(h)), "lv_gen_fem")
@@ -208,7 +237,34 @@ def gen_sbm(n, blocks, mean_deg, p_ratio=10.0, seed=4):
     return _take_csr(h)
 
 
-class Graph:
+class _Tunable:
+    """lv_graph_get/set_tuning and lv_autotune for graph and batch handles."""
+
+    def get_tuning(self):
+        t = Tuning()
+        _chk(self.L.lv_graph_get_tuning(self.h, ctypes.byref(t)), "lv_graph_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning(**{**self.get_tuning(), **kw})
+        _chk(self.L.lv_graph_set_tuning(self.h, ctypes.byref(t)), "lv_graph_set_tuning")
+
+    def autotune(self, min_gain=1e-6, candidates=None):
+        """lv_autotune over `candidates` (dicts) or the built-in grid; returns
+        (best tuning dict, best ms); the graph keeps the best and is reset."""
+        cands = None
+        n = 0
+        if candidates:
+            base = self.get_tuning()
+            cands = (Tuning * len(candidates))(*[Tuning(**{**base, **c}) for c in candidates])
+            n = len(candidates)
+        best, ms = Tuning(), ctypes.c_double()
+        _chk(self.L.lv_autotune(self.h, ctypes.c_float(min_gain), cands, n, ctypes.byref(best),
+                                ctypes.byref(ms)), "lv_autotune")
+        return best.as_dict(), ms.value
+
+
+class Graph(_Tunable):
     """A device-resident graph (lv_graph*) with the reference operator API."""
 
     def __init__(self, row, col, w, device=0):
@@ -247,6 +303,7 @@ class Graph:
     def set_locality(self, mode):
         """lv_graph_set_locality: -1 automatic, 0 never, 1 every pass."""
         _chk(self.L.lv_graph_set_locality(self.h, int(mode)), "lv_graph_set_locality")
+
 
     def size(self):
         n, ne, n0 = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32()
@@ -320,6 +377,12 @@ class Graph:
     def profile(self, on=True):
         _chk(self.L.lv_profile_enable(self.h, 1 if on else 0), "lv_profile_enable")
 
+    def profile_select(self, families):
+        """Profile only the named kernel families (None: all)."""
+        mask = 0xFFFFFFFF if families is None else sum(1 << KERNEL_FAMILIES.index(f)
+                                                        for f in families)
+        _chk(self.L.lv_profile_select(self.h, mask), "lv_profile_select")
+
     def profile_reset(self):
         _chk(self.L.lv_profile_reset(self.h), "lv_profile_reset")
 
@@ -352,7 +415,7 @@ def concat_graphs(graphs):
             np.concatenate(ws) if ws else np.zeros(0, np.float32))
 
 
-class Batch:
+class Batch(_Tunable):
     """Many graphs processed together (lv_batch_*); results per graph equal
     single-graph runs bit for bit."""
 

@@ -19,12 +19,12 @@ constexpr uint32_t kLanes[7] = {4u, 8u, 16u, 32u, 32u, 128u, 128u};
 // Kernel classes of the move step (performance choice, not semantic).
 //   0: deg 0 (never moves)      1-4: buckets 1-4, sub-wave groups of 4/8/16/32
 //   5: bucket 5, 32-lane groups 6: bucket 6, one wave64 per vertex
-//   7-9: bucket 7 with deg <= 512 / 1024 / 2048: one workgroup per vertex,
-//        LDS table of 1024 / 2048 / 4096 slots
-//   10: bucket 7, deg > kBlockMax: chunked kernels, global tables
-constexpr int kNumClasses = 11;
-constexpr int kClassHeavy = 10;
-constexpr uint32_t kBlockMax = 2048;
+//   7-10: bucket 7 with deg <= 512 / 1024 / 2048 / 4096: one workgroup per
+//        vertex, LDS table of 1024 / 2048 / 4096 / 8192 slots
+//   11: bucket 7, deg > kBlockMax: rows split over many workgroups
+constexpr int kNumClasses = 12;
+constexpr int kClassHeavy = 11;
+constexpr uint32_t kBlockMax = 4096;
 constexpr uint32_t kEmpty = 0xffffffffu;
 // Exact integer totals are accumulated into kSpread slots (block b adds to
 // slot b % kSpread) and summed on the host: same-address atomics from
@@ -41,7 +41,8 @@ __host__ __device__ inline int vertex_class(uint32_t d) {
     if (d <= 319) return 6;
     if (d <= 512) return 7;
     if (d <= 1024) return 8;
-    if (d <= kBlockMax) return 9;
+    if (d <= 2048) return 9;
+    if (d <= kBlockMax) return 10;
     return kClassHeavy;
 }
 
@@ -86,13 +87,13 @@ constexpr uint32_t kGraphActive = 1;
 
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
 constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
-constexpr uint32_t kBucketTarget = 1024;  // heavy-row entries per hash bucket (target)
 constexpr uint32_t kMaxBuckets = 1024;    // buckets per heavy vertex (max)
 struct HeavySplit {
-    const uint32_t *verts;               // class-8 vertices
+    const uint32_t *verts;               // heavy-class vertices
     uint32_t nh;                         // their count
     uint32_t nchunks;                    // total chunks = coff[nh]
     const uint32_t *coff;                // nh+1 chunk offsets
+    const uint32_t *cvtx;                // nchunks: heavy index owning each chunk
     const unsigned long long *slot_off;  // nh+1 table offsets in the arena
     unsigned long long *tab;             // arena of 16-byte slots {key, pad, sum}
     uint32_t *dlist;                     // ne: distinct slots at [row[v], row[v+1])
@@ -104,6 +105,7 @@ struct HeavySplit {
     const uint32_t *nbuck;               // nh: buckets per vertex (pow2)
     const uint32_t *bstart;              // nh+1: first bucket of each vertex
     uint32_t nbuckets;                   // total buckets = bstart[nh]
+    const uint32_t *bvtx;                // nbuckets: heavy index owning each bucket
     uint32_t *bcnt, *bpos, *bcur;        // nbuckets / nbuckets+1 / nbuckets
     uint32_t *pc, *pj;                   // heavy-entry space: community, entry index
     float *pw;                           //                    weight
@@ -143,14 +145,17 @@ void launch_relabel_map(hipStream_t s, uint32_t *order, uint32_t n, uint32_t *li
                         uint32_t *pmap, const uint32_t *row, uint32_t *deg);
 void launch_relabel_rows(hipStream_t s, const RelabelView &r, uint32_t x0, uint32_t x1,
                          int lanes);
-void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *coff, uint32_t nh,
-                          uint32_t nchunks, uint32_t x0);
+void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *cvtx,
+                          const uint32_t *coff, uint32_t nchunks, uint32_t x0);
+// out[b] = i for b in [off[i], off[i+1]), i < count
+void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint32_t *out);
 void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_t *lid, uint32_t n,
                            uint32_t *flags, uint32_t *rank, uint32_t *new_id, uint32_t *scan_tmp);
-void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n,
+// rows longer than heavy_min (319 <= heavy_min <= kBlockMax) go to the heavy class
+void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
                            uint32_t *counts /*kNumClasses*/,
                            unsigned long long *degsum /*kNumClasses*/);
-void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n,
+void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
                              uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
 // pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v.
@@ -162,7 +167,7 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
 void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *limbs);
 
-// classes 1-7 (class 8 goes through launch_heavy_move)
+// classes 1-10 (the heavy class goes through launch_heavy_move)
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count);
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
@@ -175,7 +180,7 @@ void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *ver
                          uint32_t count, uint32_t *nchunks);
 void launch_fill_slots(hipStream_t s, unsigned long long *tab, size_t cap);
 void launch_hp_nbuck(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
-                     uint32_t *nbuck);
+                     uint32_t target, uint32_t *nbuck);
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &h);
 void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float *stage,
                      int masked, int init_state, unsigned long long *acc);
@@ -210,8 +215,37 @@ struct AggView {
     uint32_t *coff;         // per large community: chunk offsets (count+1)
     uint32_t *dcnt;         // per large community: distinct neighbour communities
     uint32_t nchunks;       // total chunks of the large communities
+    const uint32_t *only;   // non-null: global-table merge only for flagged large communities
     uint32_t *err;
 };
+// Partitioned merge of the large communities (per large community i < count:
+// nbk/bstart/cflag; per bucket b < NB: the rest).
+struct AggPart {
+    const uint32_t *nbk;     // buckets of each large community (pow2)
+    const uint32_t *bstart;  // count+1: first bucket of each
+    const uint32_t *bowner;  // NB: large community of each bucket
+    uint32_t *bcnt, *bpos, *bcur;  // NB entry counts / NB+1 ranges / NB cursors
+    uint32_t *bnd;           // NB: distinct keys of each bucket
+    const uint32_t *boff;    // NB+1: exclusive scan of bnd
+    uint32_t *cflag;         // count: bucket overflow -> global-table merge
+    uint32_t *nflag;         // number of flagged communities
+    uint32_t *fkey;          // ne: new id of each entry, entry order
+    float *fw;               //     weight of each entry
+    uint32_t *pkey;          // ne: the same, bucket order
+    float *pw;
+    uint32_t nn;             // new vertex count (keys are < nn)
+    uint32_t NB;             // total buckets
+    uint32_t dmax;           // distinct keys a bucket may hold (<= 2048)
+};
+void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+                    uint32_t *nbk);
+// phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities
+void launch_aggp(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
+                 const uint32_t *comms, uint32_t count, int phase);
+void launch_aggp_caps(hipStream_t s, AggView a, const AggPart &p, const uint32_t *comms,
+                      uint32_t count, unsigned long long *caps);
+void launch_aggp_emit(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
+                      const uint32_t *comms, const uint32_t *row2, uint32_t *col2, float *w2);
 void launch_agg_prep(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_members(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts);
@@ -224,7 +258,8 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
 void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
                            uint32_t count);
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
-                     uint32_t count, uint32_t *row2, uint32_t *col2, float *w2);
+                     uint32_t count, const uint32_t *only, uint32_t *row2, uint32_t *col2,
+                     float *w2);
 void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
                         uint32_t *init_comm, uint32_t n0);
 

@@ -16,10 +16,12 @@
 #include <cstddef>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/louvain_hip.h"
 #include "lv_device.h"
+#include "lv_host.h"
 
 using lv::kNumAggClasses;
 using lv::kNumClasses;
@@ -39,9 +41,6 @@ namespace {
 
 constexpr uint64_t kFixedLimit = 1ull << 61;
 constexpr uint32_t kMaxIters = 100000;
-// pass graphs from this many vertices on get the locality order by default
-// (below it the per-vertex state fits the L2 caches anyway)
-constexpr uint32_t kLocalityMinN = 1u << 20;
 
 // device scratch counters, one allocation
 // Device counters.  The head is what the host reads back each sweep; the
@@ -90,6 +89,8 @@ struct lv_graph {
     hipStream_t stream = nullptr;
     uint32_t n = 0, n0 = 0;
     uint64_t ne = 0, ne0 = 0;
+    uint32_t n_cap = 0;  // buffer capacities (lv_graph_create: the input's size)
+    uint64_t e_cap = 0;
     // current and alternate CSR
     uint32_t *row = nullptr, *col = nullptr;
     float *w = nullptr;
@@ -108,7 +109,7 @@ struct lv_graph {
     // scratch for the renumbering under it
     uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
     bool relabeled = false;  // the current pass graph is in locality order
-    int locality = -1;       // lv_graph_set_locality mode
+    lv_tuning tune = {lv::kBlockMax, 512u, -1, 1u << 20};  // lv_graph_set_tuning
     // classification (host copies)
     uint32_t cls_count[kNumClasses] = {};
     uint32_t cls_off[kNumClasses + 1] = {};
@@ -126,6 +127,8 @@ struct lv_graph {
     uint32_t *hp_bcnt = nullptr, *hp_bpos = nullptr, *hp_bcur = nullptr, *hp_btmp = nullptr;
     size_t hp_bcap = 0;
     uint32_t hp_nbuckets = 0;
+    uint32_t *hp_bvtx = nullptr, *hp_cvtx = nullptr;  // bucket / chunk -> heavy index
+    size_t hp_ccap = 0;
     uint32_t *dlist = nullptr, *eslot = nullptr;
     // aggregation scratch
     uint32_t *new_id = nullptr, *comm_deg = nullptr, *epos = nullptr, *mstart = nullptr;
@@ -136,6 +139,12 @@ struct lv_graph {
     uint32_t *agg_mdeg = nullptr, *agg_moff = nullptr, *agg_cnk = nullptr, *agg_coff = nullptr;
     uint32_t *agg_dcnt = nullptr;
     uint32_t agg_nchunks = 0;
+    // partitioned merge of large communities: per community / per bucket
+    uint32_t *ap_nbk = nullptr, *ap_bstart = nullptr, *ap_cflag = nullptr, *ap_nflag = nullptr;
+    uint32_t *ap_bcnt = nullptr, *ap_bpos = nullptr, *ap_bcur = nullptr, *ap_bnd = nullptr;
+    uint32_t *ap_boff = nullptr, *ap_bowner = nullptr, *ap_btmp = nullptr;
+    size_t ap_bcap = 0;
+
     uint32_t *gkey = nullptr;
     ull *gval = nullptr;
     size_t garena = 0;
@@ -164,6 +173,7 @@ struct lv_graph {
     std::vector<BatchGraph> bg;
     // profiling
     bool prof_on = false;
+    uint32_t prof_mask = ~0u;  // families that get event pairs
     std::vector<ProfRec> *capture_recs = nullptr;  // non-null while capturing a sweep
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> ev_pool;
@@ -241,7 +251,7 @@ struct ProfScope {
     double bytes;
     hipEvent_t a = nullptr;
     ProfScope(lv_graph *g_, int fam_, double bytes_) : g(g_), fam(fam_), bytes(bytes_) {
-        if (g->prof_on) {
+        if (g->prof_on && ((g->prof_mask >> fam) & 1u)) {
             a = ev_get(g);
             if (a) record(a);
         }
@@ -303,7 +313,8 @@ int classify(lv_graph *g) {
     zero_counters(g);
     {
         ProfScope ps(g, LV_K_CLASSIFY, 8.0 * g->n);
-        lv::launch_classify_count(g->stream, g->row, g->n, g->dc->counts, g->dc->degsum);
+        lv::launch_classify_count(g->stream, g->row, g->n, g->tune.heavy_min_degree,
+                                  g->dc->counts, g->dc->degsum);
     }
     int rc = sync_counters(g);
     if (rc) return rc;
@@ -319,7 +330,8 @@ int classify(lv_graph *g) {
     LV_HIP(hipStreamSynchronize(g->stream));  // `cur` is pageable stack memory
     {
         ProfScope ps(g, LV_K_CLASSIFY, 12.0 * g->n);
-        lv::launch_classify_scatter(g->stream, g->row, g->n, g->dc->cursor, g->order);
+        lv::launch_classify_scatter(g->stream, g->row, g->n, g->tune.heavy_min_degree,
+                                    g->dc->cursor, g->order);
     }
     // heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk
     // offsets of the split kernels, per-vertex scratch
@@ -331,7 +343,7 @@ int classify(lv_graph *g) {
         lv::launch_scan_u64(g->stream, g->heavy_caps, nh, g->heavy_off, g->scan_tmp64);
         lv::launch_heavy_chunks(g->stream, g->row, hv, nh, g->heavy_nch);
         lv::launch_scan_u32(g->stream, g->heavy_nch, 0, nh, g->heavy_coff, g->scan_tmp32);
-        lv::launch_hp_nbuck(g->stream, g->row, hv, nh, g->hp_nbuck);
+        lv::launch_hp_nbuck(g->stream, g->row, hv, nh, g->tune.bucket_entries, g->hp_nbuck);
         lv::launch_scan_u32(g->stream, g->hp_nbuck, 0, nh, g->hp_bstart, g->scan_tmp32);
         ull total = 0;
         uint32_t chunks = 0, nbuckets = 0;
@@ -349,11 +361,22 @@ int classify(lv_graph *g) {
             dfree(g->hp_bpos);
             dfree(g->hp_bcur);
             dfree(g->hp_btmp);
+            dfree(g->hp_bvtx);
             if (dmalloc(&g->hp_bcnt, nbuckets) || dmalloc(&g->hp_bpos, (size_t)nbuckets + 1) ||
-                dmalloc(&g->hp_bcur, nbuckets) || dmalloc(&g->hp_btmp, lv::scan_tmp_elems(nbuckets) + 8))
+                dmalloc(&g->hp_bcur, nbuckets) ||
+                dmalloc(&g->hp_btmp, lv::scan_tmp_elems(nbuckets) + 8) ||
+                dmalloc(&g->hp_bvtx, nbuckets))
                 return LV_ERR_NOMEM;
             g->hp_bcap = nbuckets;
         }
+        if (chunks > g->hp_ccap) {
+            dfree(g->hp_cvtx);
+            if (dmalloc(&g->hp_cvtx, chunks)) return LV_ERR_NOMEM;
+            g->hp_ccap = chunks;
+        }
+        // owner of every chunk and bucket, so no kernel searches for it
+        lv::launch_owner_fill(g->stream, g->heavy_coff, nh, g->hp_cvtx);
+        lv::launch_owner_fill(g->stream, g->hp_bstart, nh, g->hp_bvtx);
         if (total > g->harena) {
             dfree(g->htab);
             size_t cap = (size_t)total;
@@ -377,6 +400,8 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.nh = nh;
     h.nchunks = g->heavy_chunks;
     h.coff = g->heavy_coff;
+    h.cvtx = g->hp_cvtx;
+    h.bvtx = g->hp_bvtx;
     h.slot_off = g->heavy_off;
     h.tab = g->htab;
     h.dlist = g->dlist;
@@ -451,10 +476,10 @@ void modularity_launch(lv_graph *g) {
 bool want_locality(const lv_graph *g) {
     if (g->G || g->n < 2) return false;
     static const char *env = getenv("LV_LOCALITY");
-    const int mode = env ? atoi(env) : g->locality;
+    const int mode = env ? atoi(env) : g->tune.locality;
     if (mode == 0) return false;
     if (mode == 1) return true;
-    return g->n >= kLocalityMinN;
+    return g->n >= g->tune.locality_min_vertices;
 }
 
 void relabel(lv_graph *g) {
@@ -466,7 +491,7 @@ void relabel(lv_graph *g) {
     const lv::RelabelView r{g->row, g->col, g->w, g->row2, g->col2, g->w2, g->lid, g->pmap};
     lv::launch_relabel_rows(s, r, g->cls_off[1], g->cls_off[5], 8);
     lv::launch_relabel_rows(s, r, g->cls_off[5], g->cls_off[lv::kClassHeavy], 64);
-    lv::launch_relabel_heavy(s, r, g->heavy_coff, g->cls_count[lv::kClassHeavy], g->heavy_chunks,
+    lv::launch_relabel_heavy(s, r, g->hp_cvtx, g->heavy_coff, g->heavy_chunks,
                              g->cls_off[lv::kClassHeavy]);
     std::swap(g->row, g->row2);
     std::swap(g->col, g->col2);
@@ -510,6 +535,7 @@ int step(lv_graph *g, int b) {
                                          LV_K_MOVE_SMALL,
                                          LV_K_MOVE_SMALL,
                                          LV_K_MOVE_WAVE,
+                                         LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
@@ -670,8 +696,111 @@ lv::AggView aview(lv_graph *g) {
     a.coff = g->agg_coff;
     a.dcnt = g->agg_dcnt;
     a.nchunks = g->agg_nchunks;
+    a.only = nullptr;
     a.err = &g->dc->err;
     return a;
+}
+
+// Merge of the large communities (total member degree > 2048): partitioned
+// by key range into LDS-sized buckets; a community with a bucket that
+// overflows its LDS table is merged again on global tables (LV_AGG_GLOBAL=1
+// sends every large community there, a test hook).
+int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint32_t nl,
+                    uint32_t nn, int passes, lv::AggPart *out) {
+    hipStream_t s = g->stream;
+    const uint32_t n = g->n;
+    lv::launch_agg_large_prep(s, v, aview(g), lc, nl);  // member degrees, chunks
+    lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
+    lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
+    lv::launch_aggp_nb(s, aview(g), lc, nl, g->ap_nbk);
+    lv::launch_scan_u32(s, g->ap_nbk, 0, nl, g->ap_bstart, g->scan_tmp32);
+    uint32_t chunks = 0, NB = 0;
+    LV_HIP(hipMemcpyAsync(&chunks, g->agg_coff + nl, 4, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipMemcpyAsync(&NB, g->ap_bstart + nl, 4, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    g->agg_nchunks = chunks;
+    if (NB > g->ap_bcap) {
+        dfree(g->ap_bcnt);
+        dfree(g->ap_bpos);
+        dfree(g->ap_bcur);
+        dfree(g->ap_bnd);
+        dfree(g->ap_boff);
+        dfree(g->ap_bowner);
+        dfree(g->ap_btmp);
+        if (dmalloc(&g->ap_bcnt, NB) || dmalloc(&g->ap_bpos, (size_t)NB + 1) ||
+            dmalloc(&g->ap_bcur, NB) || dmalloc(&g->ap_bnd, NB) ||
+            dmalloc(&g->ap_boff, (size_t)NB + 1) || dmalloc(&g->ap_bowner, NB) ||
+            dmalloc(&g->ap_btmp, lv::scan_tmp_elems(NB + 1) + 8))
+            return LV_ERR_NOMEM;
+        g->ap_bcap = NB;
+    }
+    lv::AggView av = aview(g);
+    lv::AggPart &p = *out;
+    p.nbk = g->ap_nbk;
+    p.bstart = g->ap_bstart;
+    p.bowner = g->ap_bowner;
+    p.bcnt = g->ap_bcnt;
+    p.bpos = g->ap_bpos;
+    p.bcur = g->ap_bcur;
+    p.bnd = g->ap_bnd;
+    p.boff = g->ap_boff;
+    p.cflag = g->ap_cflag;
+    p.nflag = g->ap_nflag;
+    // staging: heavy-move scratch (idle during aggregation), then the next
+    // CSR's column and weight arrays (written only after the merges)
+    p.fkey = g->dlist;
+    p.fw = reinterpret_cast<float *>(g->eslot);
+    p.pkey = g->col2;
+    p.pw = g->w2;
+    p.nn = nn;
+    p.NB = NB;
+    // LV_AGG_DMAX (test hook) lowers the bucket capacity so some communities
+    // overflow and take the global-table merge next to partitioned ones
+    static const char *dmax_env = getenv("LV_AGG_DMAX");
+    p.dmax = 2048;
+    if (dmax_env && atoi(dmax_env) > 0) p.dmax = std::min(2048, atoi(dmax_env));
+    static const bool force_global = getenv("LV_AGG_GLOBAL") != nullptr;
+    uint32_t nflag = 0;
+    if (force_global) {
+        lv::launch_fill_u32(s, g->ap_cflag, 1u, nl);
+        p.NB = 0;  // no bucket output
+        nflag = nl;
+    } else {
+        lv::launch_owner_fill(s, g->ap_bstart, nl, g->ap_bowner);
+        lv::launch_fill_u32(s, g->ap_bcnt, 0u, NB);
+        lv::launch_fill_u32(s, g->ap_cflag, 0u, nl);
+        lv::launch_fill_u32(s, g->ap_nflag, 0u, 1);
+        lv::launch_aggp(s, v, av, p, lc, nl, 0);
+        lv::launch_scan_u32(s, g->ap_bcnt, 0, NB, g->ap_bpos, g->ap_btmp);
+        LV_HIP(hipMemcpyAsync(g->ap_bcur, g->ap_bpos, (size_t)NB * 4, hipMemcpyDeviceToDevice, s));
+        lv::launch_aggp(s, v, av, p, lc, nl, 1);
+        lv::launch_aggp(s, v, av, p, lc, nl, 2);
+        lv::launch_scan_u32(s, g->ap_bnd, 0, NB, g->ap_boff, g->ap_btmp);
+        LV_HIP(hipMemcpyAsync(&nflag, g->ap_nflag, 4, hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        if (nflag) lv::launch_aggp(s, v, av, p, lc, nl, 3);
+    }
+    if (!nflag) return LV_OK;
+    // global tables for the flagged communities only
+    lv::launch_aggp_caps(s, av, p, lc, nl, g->agg_caps);
+    lv::launch_scan_u64(s, g->agg_caps, nl, g->agg_off, g->scan_tmp64);
+    lv::launch_fill_u32(s, g->agg_dcnt, 0u, nl);
+    ull total = 0;
+    LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    if (total > g->garena) {
+        dfree(g->gkey);
+        dfree(g->gval);
+        if (dmalloc(&g->gkey, (size_t)total) || dmalloc(&g->gval, (size_t)total))
+            return LV_ERR_NOMEM;
+        g->garena = (size_t)total;
+        lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
+        lv::launch_fill_u64(s, g->gval, 0ull, total);
+    }
+    av = aview(g);
+    av.only = g->ap_cflag;
+    lv::launch_agg_merge(s, v, av, lv::kAggGlobalClass, lc, nl, passes);
+    return LV_OK;
 }
 
 // aggregation (aggregation.cu:387-462) + composition of init_comm
@@ -707,50 +836,37 @@ int aggregation(lv_graph *g) {
                           hipMemcpyHostToDevice, s));
     LV_HIP(hipStreamSynchronize(s));  // `aoff` is pageable stack memory
     lv::launch_agg_classify_scatter(s, v, aview(g), g->dc->acursor, g->aorder);
-    // global-hash arena for the large communities
     lv::launch_fill_u32(s, g->cnt, 0u, nn);  // edgeless communities keep 0
-    const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
-    g->agg_nchunks = 0;
-    if (nl) {
-        const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
-        lv::launch_agg_caps(s, aview(g), lc, nl, g->agg_caps);
-        lv::launch_scan_u64(s, g->agg_caps, nl, g->agg_off, g->scan_tmp64);
-        lv::launch_agg_large_prep(s, v, aview(g), lc, nl);
-        lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
-        lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
-        lv::launch_fill_u32(s, g->agg_dcnt, 0u, nl);
-        ull total = 0;
-        uint32_t chunks = 0;
-        LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
-        LV_HIP(hipMemcpyAsync(&chunks, g->agg_coff + nl, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              s));
-        LV_HIP(hipStreamSynchronize(s));
-        g->agg_nchunks = chunks;
-        if (total > g->garena) {
-            dfree(g->gkey);
-            dfree(g->gval);
-            if (dmalloc(&g->gkey, (size_t)total) || dmalloc(&g->gval, (size_t)total))
-                return LV_ERR_NOMEM;
-            g->garena = (size_t)total;
-            lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
-            lv::launch_fill_u64(s, g->gval, 0ull, total);
-        }
-    }
-    int passes = 1;
+    int passes = 1;  // radix digits of a new id
     {
         uint32_t bits = 0;
         while (bits < 32 && (n > 1) && ((uint64_t)(n - 1) >> bits)) ++bits;
         passes = std::max(1, (int)((bits + 7) / 8));
     }
     lv::AggView av = aview(g);
-    for (int c = 0; c < kNumAggClasses; ++c)
+    for (int c = 0; c < lv::kAggGlobalClass; ++c)
         lv::launch_agg_merge(s, v, av, c, g->aorder + aoff[c], aoff[c + 1] - aoff[c], passes);
+    // large communities: partitioned merge, global tables for overflowing ones
+    const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
+    const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
+    lv::AggPart ap = {};
+    g->agg_nchunks = 0;
+    if (nl) {
+        int rc2 = aggregate_large(g, v, lc, nl, nn, passes, &ap);
+        if (rc2) return rc2;
+        av = aview(g);
+    }
     // new row pointers from distinct counts
     lv::launch_scan_u32(s, g->cnt, 0, nn, g->row2, g->scan_tmp32);
     uint32_t ne2 = 0;
     LV_HIP(hipMemcpyAsync(&ne2, g->row2 + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     // communities without entries (class 0, first in aorder) have empty rows
-    lv::launch_agg_emit(s, v, av, g->aorder + aoff[1], nn - aoff[1], g->row2, g->col2, g->w2);
+    lv::launch_agg_emit(s, v, av, g->aorder + aoff[1], aoff[lv::kAggGlobalClass] - aoff[1],
+                        nullptr, g->row2, g->col2, g->w2);
+    if (nl) {
+        lv::launch_aggp_emit(s, v, av, ap, lc, g->row2, g->col2, g->w2);
+        lv::launch_agg_emit(s, v, av, lc, nl, g->ap_cflag, g->row2, g->col2, g->w2);
+    }
     lv::launch_agg_compose(s, g->comm, g->new_id, g->init_comm, g->n0);
     rc = sync_counters(g);
     if (rc) return rc;
@@ -920,38 +1036,74 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
     return LV_OK;
 }
 
-// exact host-side validation and fixed-point scale (same rule as the oracle)
+// Run f(begin, end) over [0, n) on up to 16 host threads (inputs of
+// hundreds of millions of entries are checked before every upload).
+template <typename F>
+void host_parallel(uint64_t n, F f) {
+    unsigned T = std::thread::hardware_concurrency();
+    T = std::max(1u, std::min(T, 16u));
+    if (n < (1u << 20)) T = 1;
+    if (T == 1) {
+        f(0, n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+        th.emplace_back([=] { f(n * t / T, n * (t + 1) / T, t); });
+    for (auto &x : th) x.join();
+}
+
+// exact host-side validation and fixed-point scale (same rule as the oracle):
+// S = the smallest shift making every weight an integer; the total at that
+// scale must stay below 2^61
 int scale_bits(uint64_t ne, const float *w, int *S_out) {
+    std::vector<int> sp(16, 0);
+    std::vector<char> bad(16, 0), nz(16, 0);
+    host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
+        int S = 0;
+        for (uint64_t j = a; j < b; ++j) {
+            const float x = w[j];
+            if (!(x >= 0.0f) || std::isinf(x)) {
+                bad[t] = 1;
+                return;
+            }
+            if (x == 0.0f) continue;
+            nz[t] = 1;
+            uint32_t bits;
+            memcpy(&bits, &x, 4);
+            const int ef = (int)((bits >> 23) & 0xff);
+            uint32_t man = bits & 0x7fffffu;
+            int e;
+            if (ef == 0) {
+                e = -149;
+            } else {
+                man |= 0x800000u;
+                e = ef - 150;
+            }
+            e += __builtin_ctz(man);  // man != 0 for a non-zero weight
+            if (-e > S) S = -e;
+        }
+        sp[t] = S;
+    });
     int S = 0;
-    u128 tot = 0;
-    for (uint64_t j = 0; j < ne; ++j) {
-        float x = w[j];
-        if (!(x >= 0.0f) || std::isinf(x)) return LV_ERR_RANGE;
-        if (x == 0.0f) continue;
-        uint32_t bits;
-        memcpy(&bits, &x, 4);
-        int ef = (int)((bits >> 23) & 0xff);
-        uint32_t man = bits & 0x7fffffu;
-        int e;
-        if (ef == 0) {
-            e = -149;
-        } else {
-            man |= 0x800000u;
-            e = ef - 150;
-        }
-        while ((man & 1u) == 0) {
-            man >>= 1;
-            ++e;
-        }
-        if (-e > S) S = -e;
+    bool any = false;
+    for (int t = 0; t < 16; ++t) {
+        if (bad[t]) return LV_ERR_RANGE;
+        S = std::max(S, sp[t]);
+        any |= nz[t] != 0;
     }
     if (S > 100) return LV_ERR_RANGE;
-    for (uint64_t j = 0; j < ne; ++j) {
-        if (w[j] == 0.0f) continue;
-        if (S > 62) return LV_ERR_RANGE;
-        tot += (u128)(uint64_t)ldexpf(w[j], S);
-        if (tot >= (u128)kFixedLimit) return LV_ERR_RANGE;
-    }
+    if (any && S > 62) return LV_ERR_RANGE;
+    std::vector<u128> part(16, 0);
+    host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
+        u128 tot = 0;
+        for (uint64_t j = a; j < b; ++j)
+            if (w[j] != 0.0f) tot += (u128)(uint64_t)ldexpf(w[j], S);
+        part[t] = tot;
+    });
+    u128 tot = 0;
+    for (auto p : part) tot += p;
+    if (tot >= (u128)kFixedLimit) return LV_ERR_RANGE;
     *S_out = S;
     return LV_OK;
 }
@@ -995,6 +1147,19 @@ void free_all(lv_graph *g) {
     dfree(g->hp_bpos);
     dfree(g->hp_bcur);
     dfree(g->hp_btmp);
+    dfree(g->hp_bvtx);
+    dfree(g->hp_cvtx);
+    dfree(g->ap_nbk);
+    dfree(g->ap_bstart);
+    dfree(g->ap_cflag);
+    dfree(g->ap_nflag);
+    dfree(g->ap_bcnt);
+    dfree(g->ap_bpos);
+    dfree(g->ap_bcur);
+    dfree(g->ap_bnd);
+    dfree(g->ap_boff);
+    dfree(g->ap_bowner);
+    dfree(g->ap_btmp);
     dfree(g->htab);
     dfree(g->dlist);
     dfree(g->eslot);
@@ -1061,6 +1226,122 @@ int host_lid(lv_graph *g, std::vector<uint32_t> &lid) {
 
 }  // namespace
 
+namespace lv_host {
+
+// Input checks of lv_graph_create: CSR shape, ids in range, weights finite,
+// non-negative and within the exact fixed-point range.
+int check_input(uint32_t n, const uint32_t *row, const uint32_t *col, const float *w, int *S) {
+    if (!row) return LV_ERR_ARG;
+    if (n >= 0x80000000u) return LV_ERR_ARG;
+    if (row[0] != 0) return LV_ERR_ARG;
+    std::vector<char> bad(16, 0);
+    host_parallel(n, [&](uint64_t a, uint64_t b, unsigned t) {
+        for (uint64_t v = a; v < b; ++v)
+            if (row[v + 1] < row[v]) {
+                bad[t] = 1;
+                return;
+            }
+    });
+    for (char x : bad)
+        if (x) return LV_ERR_ARG;
+    const uint64_t ne = row[n];
+    if (ne && (!col || !w)) return LV_ERR_ARG;
+    host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
+        for (uint64_t j = a; j < b; ++j)
+            if (col[j] >= n) {
+                bad[t] = 1;
+                return;
+            }
+    });
+    for (char x : bad)
+        if (x) return LV_ERR_ARG;
+    return scale_bits(ne, w, S);
+}
+
+// A graph handle with device buffers for up to n_cap vertices and e_cap
+// entries (every buffer a pass needs; arenas grow on demand).
+int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        (void)hipGetLastError();
+        return LV_ERR_NO_DEVICE;
+    }
+    lv_graph *g = new (std::nothrow) lv_graph();
+    if (!g) return LV_ERR_NOMEM;
+    g->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        delete g;
+        return LV_ERR_HIP;
+    }
+    g->n_cap = n_cap;
+    g->e_cap = e_cap;
+    const size_t N = (size_t)n_cap + 1, E = (size_t)e_cap;
+    int e = 0;
+    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E) | dmalloc(&g->w, E);
+    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E) | dmalloc(&g->w2, E);
+    e |= dmalloc(&g->row0, N) | dmalloc(&g->col0, E) | dmalloc(&g->w0, E);
+    e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
+    e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
+    e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
+    e |= dmalloc(&g->lid, N) | dmalloc(&g->pmap, N) | dmalloc(&g->rtmp, N) | dmalloc(&g->rrank, N);
+    e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
+    e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
+    e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
+    e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
+    e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);
+    e |= dmalloc(&g->cnt, N) | dmalloc(&g->aorder, N);
+    e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |
+         dmalloc(&g->sval, E);
+    e |= dmalloc(&g->agg_caps, N) | dmalloc(&g->agg_off, N + 1);
+    e |= dmalloc(&g->agg_mdeg, N) | dmalloc(&g->agg_moff, N + 1) | dmalloc(&g->agg_cnk, N);
+    e |= dmalloc(&g->agg_coff, N + 1) | dmalloc(&g->agg_dcnt, N);
+    e |= dmalloc(&g->ap_nbk, N) | dmalloc(&g->ap_bstart, N + 1) | dmalloc(&g->ap_cflag, N);
+    e |= dmalloc(&g->ap_nflag, 1);
+    e |= dmalloc(&g->scan_tmp32, lv::scan_tmp_elems(n_cap + 1) + 8);
+    e |= dmalloc(&g->scan_tmp64, lv::scan_tmp_elems(n_cap + 1) + 8);
+    e |= dmalloc(&g->dc, 1);
+    if (!e && hipHostMalloc((void **)&g->hc, sizeof(Counters)) != hipSuccess) {
+        (void)hipGetLastError();
+        e = LV_ERR_NOMEM;
+    }
+    if (e) {
+        free_all(g);
+        delete g;
+        return LV_ERR_NOMEM;
+    }
+    *out = g;
+    return LV_OK;
+}
+
+// Upload a checked input (n <= n_cap, row[n] <= e_cap) as the graph of the
+// handle, with its pristine copy and the identity assignment.
+int graph_upload(lv_graph *g, uint32_t n, const uint32_t *row, const uint32_t *col,
+                 const float *w, int S) {
+    const uint64_t ne = row[n];
+    if (n > g->n_cap || ne > g->e_cap || g->G) return LV_ERR_NOMEM;
+    LV_HIP(hipSetDevice(g->device));
+    hipStream_t s = g->stream;
+    const size_t N = (size_t)n + 1, E = (size_t)ne;
+    LV_HIP(hipMemcpyAsync(g->row0, row, N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (E) {
+        LV_HIP(hipMemcpyAsync(g->col0, col, E * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(g->w0, w, E * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    g->n = g->n0 = n;
+    g->ne = g->ne0 = ne;
+    g->S = S;
+    g->scale = ldexpf(1.0f, S);
+    g->inv_scale = ldexpf(1.0f, -S);
+    return lv_graph_reset(g);  // working copy from the pristine one, identity init_comm
+}
+
+}  // namespace lv_host
+using lv_host::check_input;
+using lv_host::graph_new;
+using lv_host::graph_upload;
+
 extern "C" {
 
 const char *lv_status_string(int st) {
@@ -1095,94 +1376,16 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
                     const float *w, lv_graph **out) {
     if (!out || !row) return LV_ERR_ARG;
     *out = nullptr;
-    if (n >= 0x80000000u) return LV_ERR_ARG;
-    if (row[0] != 0) return LV_ERR_ARG;
-    for (uint32_t v = 0; v < n; ++v)
-        if (row[v + 1] < row[v]) return LV_ERR_ARG;
-    const uint64_t ne = row[n];
-    if (ne && (!col || !w)) return LV_ERR_ARG;
-    for (uint64_t j = 0; j < ne; ++j)
-        if (col[j] >= n) return LV_ERR_ARG;
     int S = 0;
-    int rc = scale_bits(ne, w, &S);
+    int rc = check_input(n, row, col, w, &S);
     if (rc) return rc;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
-        (void)hipGetLastError();
-        return LV_ERR_NO_DEVICE;
-    }
-    lv_graph *g = new (std::nothrow) lv_graph();
-    if (!g) return LV_ERR_NOMEM;
-    g->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess) {
-        (void)hipGetLastError();
-        delete g;
-        return LV_ERR_HIP;
-    }
-    g->n = g->n0 = n;
-    g->ne = g->ne0 = ne;
-    g->S = S;
-    g->scale = ldexpf(1.0f, S);
-    g->inv_scale = ldexpf(1.0f, -S);
-    const size_t N = (size_t)n + 1, E = (size_t)ne;
-    int e = 0;
-    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E) | dmalloc(&g->w, E);
-    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E) | dmalloc(&g->w2, E);
-    e |= dmalloc(&g->row0, N) | dmalloc(&g->col0, E) | dmalloc(&g->w0, E);
-    e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
-    e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
-    e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
-    e |= dmalloc(&g->lid, N) | dmalloc(&g->pmap, N) | dmalloc(&g->rtmp, N) | dmalloc(&g->rrank, N);
-    e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
-    e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
-    e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
-    e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
-    e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);
-    e |= dmalloc(&g->cnt, N) | dmalloc(&g->aorder, N);
-    e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |
-         dmalloc(&g->sval, E);
-    e |= dmalloc(&g->agg_caps, N) | dmalloc(&g->agg_off, N + 1);
-    e |= dmalloc(&g->agg_mdeg, N) | dmalloc(&g->agg_moff, N + 1) | dmalloc(&g->agg_cnk, N);
-    e |= dmalloc(&g->agg_coff, N + 1) | dmalloc(&g->agg_dcnt, N);
-    e |= dmalloc(&g->scan_tmp32, lv::scan_tmp_elems(n + 1) + 8);
-    e |= dmalloc(&g->scan_tmp64, lv::scan_tmp_elems(n + 1) + 8);
-    e |= dmalloc(&g->dc, 1);
-    if (!e && hipHostMalloc((void **)&g->hc, sizeof(Counters)) != hipSuccess) {
-        (void)hipGetLastError();
-        e = LV_ERR_NOMEM;
-    }
-    if (e) {
-        free_all(g);
-        delete g;
-        return LV_ERR_NOMEM;
-    }
-    hipStream_t s = g->stream;
-    if (hipMemcpyAsync(g->row, row, N * sizeof(uint32_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-        (E && hipMemcpyAsync(g->col, col, E * sizeof(uint32_t), hipMemcpyHostToDevice, s) !=
-                  hipSuccess) ||
-        (E && hipMemcpyAsync(g->w, w, E * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)) {
-        (void)hipGetLastError();
-        free_all(g);
-        delete g;
-        return LV_ERR_HIP;
-    }
-    if (hipMemcpyAsync(g->row0, g->row, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
-            hipSuccess ||
-        (E && hipMemcpyAsync(g->col0, g->col, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
-                  hipSuccess) ||
-        (E && hipMemcpyAsync(g->w0, g->w, E * sizeof(float), hipMemcpyDeviceToDevice, s) !=
-                  hipSuccess)) {
-        (void)hipGetLastError();
-        free_all(g);
-        delete g;
-        return LV_ERR_HIP;
-    }
-    lv::launch_iota_u32(s, g->init_comm, n);  // setInitComm (utils.cu:66-68)
-    if (hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipGetLastError();
-        free_all(g);
-        delete g;
-        return LV_ERR_HIP;
+    lv_graph *g = nullptr;
+    rc = graph_new(device, n, row[n], &g);
+    if (rc) return rc;
+    rc = graph_upload(g, n, row, col, w, S);
+    if (rc) {
+        lv_graph_destroy(g);
+        return rc;
     }
     *out = g;
     return LV_OK;
@@ -1215,8 +1418,80 @@ int lv_graph_reset(lv_graph *g) {
 
 int lv_graph_set_locality(lv_graph *g, int mode) {
     if (!g || mode < -1 || mode > 1) return LV_ERR_ARG;
-    g->locality = mode;
+    g->tune.locality = mode;
     return LV_OK;
+}
+
+int lv_tuning_default(lv_tuning *t) {
+    if (!t) return LV_ERR_ARG;
+    *t = lv_graph().tune;
+    return LV_OK;
+}
+
+int lv_graph_set_tuning(lv_graph *g, const lv_tuning *t) {
+    if (!g || !t) return LV_ERR_ARG;
+    const uint32_t b = t->bucket_entries;
+    // heavy rows must stay in bucket 7 (deg > 319) and every shorter bucket-7
+    // row must fit a block kernel's table
+    if (t->heavy_min_degree < lv::kCut[6] || t->heavy_min_degree > lv::kBlockMax) return LV_ERR_ARG;
+    if (b < 64 || b > 1024 || (b & (b - 1))) return LV_ERR_ARG;
+    if (t->locality < -1 || t->locality > 1) return LV_ERR_ARG;
+    g->tune = *t;
+    return LV_OK;
+}
+
+int lv_graph_get_tuning(lv_graph *g, lv_tuning *t) {
+    if (!g || !t) return LV_ERR_ARG;
+    *t = g->tune;
+    return LV_OK;
+}
+
+int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32_t count,
+                lv_tuning *best, double *best_ms) {
+    if (!g || (!candidates && count)) return LV_ERR_ARG;
+    std::vector<lv_tuning> cand;
+    if (candidates) {
+        cand.assign(candidates, candidates + count);
+    } else {  // built-in grid
+        const bool big = g->n0 >= (1u << 18) && !g->G;
+        for (uint32_t hm : {2048u, lv::kBlockMax})
+            for (uint32_t be : {256u, 512u, 1024u})
+                for (int loc : {0, 1}) {
+                    if (loc && !big) continue;
+                    cand.push_back({hm, be, loc, g->tune.locality_min_vertices});
+                }
+    }
+    if (cand.empty()) return LV_ERR_ARG;
+    const lv_tuning keep = g->tune;
+    double best_t = 0.0;
+    size_t bi = 0;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        int rc = lv_graph_set_tuning(g, &cand[i]);
+        if (rc) {
+            g->tune = keep;
+            return rc;
+        }
+        double t = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+            rc = lv_graph_reset(g);
+            lv_result r;
+            if (!rc) rc = g->G ? lv_batch_run(g, min_gain, nullptr, &r) : lv_run(g, min_gain, &r);
+            if (rc) {
+                g->tune = keep;
+                lv_graph_reset(g);
+                return rc;
+            }
+            if (rep == 0 || r.alg_ms < t) t = r.alg_ms;
+        }
+        if (i == 0 || t < best_t) {
+            best_t = t;
+            bi = i;
+        }
+    }
+    g->tune = cand[bi];
+    if (best) *best = cand[bi];
+    if (best_ms) *best_ms = best_t;
+    return lv_graph_reset(g);
 }
 
 int lv_graph_destroy(lv_graph *g) {
@@ -1533,6 +1808,12 @@ int lv_batch_get_assignment(lv_graph *g, uint32_t *local_assignment) {
 int lv_profile_enable(lv_graph *g, int on) {
     if (!g) return LV_ERR_ARG;
     g->prof_on = on != 0;
+    return LV_OK;
+}
+
+int lv_profile_select(lv_graph *g, uint32_t family_mask) {
+    if (!g) return LV_ERR_ARG;
+    g->prof_mask = family_mask;
     return LV_OK;
 }
 

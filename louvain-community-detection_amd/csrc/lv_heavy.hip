@@ -29,16 +29,6 @@
 
 namespace lv {
 
-__device__ __forceinline__ uint32_t heavy_find(const uint32_t *coff, uint32_t nh, uint32_t b) {
-    uint32_t lo = 0, hi = nh;  // largest i with coff[i] <= b
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (coff[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ uint32_t f32_ord(float x) {
     uint32_t u = __float_as_uint(x);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -67,7 +57,7 @@ struct HeavyCtx {
 
 __device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit &h) {
     HeavyCtx x;
-    x.i = heavy_find(h.coff, h.nh, blockIdx.x);
+    x.i = h.cvtx[blockIdx.x];
     x.c = blockIdx.x - h.coff[x.i];
     x.v = h.verts[x.i];
     x.base = g.row[x.v];
@@ -357,17 +347,26 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
     }
 }
 
+// The chunk's entries are first ordered by bucket in LDS, so each bucket's
+// run of entries is written to its range with consecutive addresses.
 __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
-    __shared__ uint32_t lcnt[kMaxBuckets];
-    __shared__ uint32_t lbase[kMaxBuckets];
+    static_assert(kMaxBuckets == 4 * 256, "four buckets per thread in the offset scan");
+    __shared__ uint32_t lcnt[kMaxBuckets];   // entries of the chunk per bucket
+    __shared__ uint32_t loff[kMaxBuckets];   // their offset in the bucket-ordered chunk
+    __shared__ uint32_t lbase[kMaxBuckets];  // their position in the bucket's range
+    __shared__ uint32_t sc[kHeavyChunk], sj[kHeavyChunk];
+    __shared__ float sw[kHeavyChunk];
+    __shared__ uint16_t sb[kHeavyChunk];
+    __shared__ uint32_t wsum[4];
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;
     const uint32_t nb = h.nbuck[x.i];
     const int lb = 31 - __clz(nb);
-    for (uint32_t b = threadIdx.x; b < nb; b += 256) lcnt[b] = 0;
+    for (uint32_t b = threadIdx.x; b < kMaxBuckets; b += 256) lcnt[b] = 0;
     __syncthreads();
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
+    const uint32_t cnt = j1 - j0;
     uint32_t rank[kPerThread], bk[kPerThread], cc[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
@@ -380,22 +379,50 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
         }
     }
     __syncthreads();
+    // exclusive scan of lcnt: thread t owns buckets 4t..4t+3
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t c0 = lcnt[4 * t], c1 = lcnt[4 * t + 1], c2 = lcnt[4 * t + 2],
+                   c3 = lcnt[4 * t + 3];
+    const uint32_t own = c0 + c1 + c2 + c3;
+    uint32_t inc = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wsum[wv] = inc;
     const uint32_t b0 = h.bstart[x.i];
-    for (uint32_t b = threadIdx.x; b < nb; b += 256)
+    for (uint32_t b = t; b < nb; b += 256)
         if (lcnt[b]) lbase[b] = atomicAdd(&h.bcur[b0 + b], lcnt[b]);
+    __syncthreads();
+    uint32_t base = inc - own;
+    for (uint32_t k = 0; k < wv; ++k) base += wsum[k];
+    loff[4 * t] = base;
+    loff[4 * t + 1] = base + c0;
+    loff[4 * t + 2] = base + c0 + c1;
+    loff[4 * t + 3] = base + c0 + c1 + c2;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         if (bk[q] == kEmpty) continue;
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        const uint32_t pos = lbase[bk[q]] + rank[q];
-        h.pc[pos] = cc[q];
-        h.pw[pos] = g.w[x.base + j];
-        h.pj[pos] = j;
+        const uint32_t sp = loff[bk[q]] + rank[q];
+        sc[sp] = cc[q];
+        sj[sp] = j;
+        sw[sp] = g.w[x.base + j];
+        sb[sp] = (uint16_t)bk[q];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+        const uint32_t b = sb[i];
+        const uint32_t pos = lbase[b] + (i - loff[b]);
+        h.pc[pos] = sc[i];
+        h.pw[pos] = sw[i];
+        h.pj[pos] = sj[i];
     }
 }
 
-constexpr int kBucketCap = 4096;  // LDS table slots of one bucket
+constexpr int kBucketCap = 2048;  // LDS table slots of one bucket
 
 __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     constexpr int LC = Log2<kBucketCap>::value;
@@ -404,14 +431,7 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     __shared__ uint32_t dist[kBucketCap / 2];
     __shared__ unsigned long long lanes[128];
     __shared__ uint32_t nd_s, over;
-    // bucket -> heavy vertex (largest i with bstart[i] <= bucket)
-    uint32_t lo = 0, hi = h.nh;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (h.bstart[mid] <= blockIdx.x) lo = mid;
-        else hi = mid;
-    }
-    const uint32_t i = lo;
+    const uint32_t i = h.bvtx[blockIdx.x];
     const uint32_t v = h.verts[i];
     if (!v_live(g, v)) return;
     const uint32_t p0 = h.bpos[blockIdx.x], p1 = h.bpos[blockIdx.x + 1];
@@ -485,23 +505,23 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         atomicMax(&h.hlane[(size_t)i * 128 + threadIdx.x], lanes[threadIdx.x]);
 }
 
-// per heavy vertex: bucket count pow2 >= ceil(deg / kBucketTarget)
+// per heavy vertex: bucket count pow2 >= ceil(deg / target), at most kMaxBuckets
 __global__ void k_hp_nbuck(const uint32_t *row, const uint32_t *verts, uint32_t count,
-                           uint32_t *nbuck) {
+                           uint32_t target, uint32_t *nbuck) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         const uint32_t v = verts[i];
         const uint32_t d = row[v + 1] - row[v];
         uint32_t b = 1;
-        while (b < kMaxBuckets && (uint64_t)b * kBucketTarget < d) b <<= 1;
+        while (b < kMaxBuckets && (uint64_t)b * target < d) b <<= 1;
         nbuck[i] = b;
     }
 }
 void launch_hp_nbuck(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
-                     uint32_t *nbuck) {
+                     uint32_t target, uint32_t *nbuck) {
     if (!count) return;
     hipLaunchKernelGGL(k_hp_nbuck, dim3((count + 255) / 256), dim3(256), 0, s, row, verts, count,
-                       nbuck);
+                       target, nbuck);
 }
 
 // flagged vertices restart on the global-table kernels
@@ -568,9 +588,9 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
 
 // locality order: heavy rows are copied one chunk of kHeavyChunk entries per
 // workgroup (positions x0 + i, i < nh, chunk offsets coff)
-__global__ __launch_bounds__(256) void k_relabel_heavy(RelabelView r, const uint32_t *coff,
-                                                       uint32_t nh, uint32_t x0) {
-    const uint32_t i = heavy_find(coff, nh, blockIdx.x);
+__global__ __launch_bounds__(256) void k_relabel_heavy(RelabelView r, const uint32_t *cvtx,
+                                                       const uint32_t *coff, uint32_t x0) {
+    const uint32_t i = cvtx[blockIdx.x];
     const uint32_t c = blockIdx.x - coff[i];
     const uint32_t x = x0 + i, v = r.lid[x];
     const uint32_t b = r.row[v], d = r.row[v + 1] - b, o = r.row2[x];
@@ -580,10 +600,20 @@ __global__ __launch_bounds__(256) void k_relabel_heavy(RelabelView r, const uint
         r.w2[o + j] = r.w[b + j];
     }
 }
-void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *coff, uint32_t nh,
-                          uint32_t nchunks, uint32_t x0) {
-    if (!nh || !nchunks) return;
-    hipLaunchKernelGGL(k_relabel_heavy, dim3(nchunks), dim3(256), 0, s, r, coff, nh, x0);
+void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *cvtx,
+                          const uint32_t *coff, uint32_t nchunks, uint32_t x0) {
+    if (!nchunks) return;
+    hipLaunchKernelGGL(k_relabel_heavy, dim3(nchunks), dim3(256), 0, s, r, cvtx, coff, x0);
+}
+
+__global__ void k_owner_fill(const uint32_t *off, uint32_t count, uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        for (uint32_t b = off[i]; b < off[i + 1]; ++b) out[b] = i;
+}
+void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint32_t *out) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_owner_fill, dim3((count + 255) / 256), dim3(256), 0, s, off, count, out);
 }
 
 // ---------------------------------------------------------------------------

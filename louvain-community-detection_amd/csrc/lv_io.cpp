@@ -192,6 +192,50 @@ void read_weight(const char *s, const char *e, float *w) {
     *w = x;
 }
 
+// One parsed edge line.  has_w: the line carried a weight token (a token
+// that does not parse still counts: it sets the weight to 0).
+struct EdgeLine {
+    uint32_t a, b;
+    float w;
+    bool has_w;
+};
+
+// Parse body lines [s, e) whose first line is body line `first` (0-based);
+// lines at index >= nnz are ignored like the reference ignores them.
+// Returns LV_OK or LV_ERR_FORMAT; sets *zero if an id 0 was seen.
+int parse_lines(const char *s, const char *e, uint64_t first, uint64_t nnz, EdgeLine *out,
+                bool *zero) {
+    Cursor cur{s, e};
+    const char *ls, *le;
+    for (uint64_t i = first; i < nnz && cur.line(&ls, &le); ++i) {
+        uint64_t x, y;
+        const char *t = read_u32(ls, le, &x);
+        if (t) t = read_u32(t, le, &y);
+        if (!t || (t < le && !is_blank(*t))) return LV_ERR_FORMAT;
+        EdgeLine &L = out[i];
+        L.a = (uint32_t)x;
+        L.b = (uint32_t)y;
+        L.has_w = false;
+        float w = 0.0f;
+        const char *u = t;
+        while (u < le && is_blank(*u)) ++u;
+        if (u < le) {
+            read_weight(u, le, &w);
+            L.has_w = true;
+        }
+        L.w = w;
+        if (x == 0 || y == 0) *zero = true;  // isZeroStartingIndex (utils.cu:143-165)
+    }
+    return LV_OK;
+}
+
+// .mtx text -> symmetric CSR with the reference semantics (utils.cu:132-229):
+// '%' lines before the header skipped, the column count is V, ids 1-based
+// unless an id 0 appears, |weight|, a line without a weight keeps the
+// previous one (1.0 at the start), rows in file order, a self-loop once.
+// The body is split at line boundaries over worker threads; the weight
+// carried across a split is resolved afterwards, and each worker then
+// scatters the entries of its own row range in file order.
 int parse_text(const char *text, size_t len, lv_host_csr *out) {
     memset(out, 0, sizeof(*out));
     Cursor cur{text, text + len};
@@ -209,51 +253,137 @@ int parse_text(const char *text, size_t len, lv_host_csr *out) {
     (void)rows;  // the reference keeps the column count (utils.cu:140)
     const uint32_t n = (uint32_t)cols;
     if (n >= 0x80000000u) return LV_ERR_ARG;
-    std::vector<uint32_t> a(nnz), b(nnz);
-    std::vector<float> wv(nnz);
-    float wcur = 1.0f;  // utils.cu:187
+    const char *body = cur.p, *end = text + len;
+    // split the body into T pieces at line starts; count each piece's lines
+    unsigned T = n_workers();
+    if ((size_t)(end - body) < (1u << 20)) T = 1;
+    std::vector<const char *> cut(T + 1);
+    cut[0] = body;
+    cut[T] = end;
+    for (unsigned t = 1; t < T; ++t) {  // first line start at or after the even split
+        const char *q = body + (size_t)(end - body) * t / T;
+        if (q <= cut[t - 1]) {
+            cut[t] = cut[t - 1];
+            continue;
+        }
+        const char *nl = (const char *)memchr(q - 1, '\n', (size_t)(end - (q - 1)));
+        cut[t] = nl ? nl + 1 : end;
+    }
+    std::vector<uint64_t> lines(T + 1, 0);
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                uint64_t c = 0;
+                for (const char *q = cut[t]; q < cut[t + 1];) {
+                    const char *nl = (const char *)memchr(q, '\n', (size_t)(cut[t + 1] - q));
+                    ++c;
+                    if (!nl) break;
+                    q = nl + 1;
+                }
+                lines[t + 1] = c;
+            });
+        for (auto &x : th) x.join();
+    }
+    for (unsigned t = 0; t < T; ++t) lines[t + 1] += lines[t];
+    if (lines[T] < nnz) return LV_ERR_FORMAT;
+    std::vector<EdgeLine> el(nnz);
+    std::vector<int> prc(T, LV_OK);
+    std::vector<char> pzero(T, 0);
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                bool z = false;
+                prc[t] = parse_lines(cut[t], cut[t + 1], lines[t], nnz, el.data(), &z);
+                pzero[t] = z;
+            });
+        for (auto &x : th) x.join();
+    }
     bool zero_based = false;
+    for (unsigned t = 0; t < T; ++t) {
+        if (prc[t]) return prc[t];
+        zero_based |= pzero[t] != 0;
+    }
+    // carried weight (`ss >> weight` leaves it unchanged, utils.cu:187-193)
+    float wcur = 1.0f;
     for (uint64_t i = 0; i < nnz; ++i) {
-        uint64_t x, y;
-        if (!cur.line(&ls, &le)) return LV_ERR_FORMAT;
-        const char *t = read_u32(ls, le, &x);
-        if (t) t = read_u32(t, le, &y);
-        if (!t || (t < le && !is_blank(*t))) return LV_ERR_FORMAT;
-        read_weight(t, le, &wcur);
-        if (x == 0 || y == 0) zero_based = true;  // isZeroStartingIndex (utils.cu:143-165)
-        a[i] = (uint32_t)x;
-        b[i] = (uint32_t)y;
-        wv[i] = std::fabs(wcur);  // utils.cu:193
+        if (el[i].has_w) wcur = el[i].w;
+        el[i].w = std::fabs(wcur);
+        if (!zero_based) {
+            el[i].a--;
+            el[i].b--;
+        }
+        if (el[i].a >= n || el[i].b >= n) return LV_ERR_FORMAT;
     }
     std::vector<uint64_t> deg((size_t)n + 1, 0);
     for (uint64_t i = 0; i < nnz; ++i) {
-        if (!zero_based) {
-            a[i]--;
-            b[i]--;
-        }
-        if (a[i] >= n || b[i] >= n) return LV_ERR_FORMAT;
-        deg[a[i] + 1]++;
-        if (a[i] != b[i]) deg[b[i] + 1]++;
+        deg[el[i].a + 1]++;
+        if (el[i].a != el[i].b) deg[el[i].b + 1]++;
     }
     for (uint32_t v = 0; v < n; ++v) deg[v + 1] += deg[v];
     const uint64_t ne = deg[n];
     int rc = alloc_csr(out, n, ne);
     if (rc) return rc;
     for (uint32_t v = 0; v <= n; ++v) out->row[v] = (uint32_t)deg[v];
-    // rows in file order; self-loop once (utils.cu:200-224)
-    for (uint64_t i = 0; i < nnz; ++i) {
-        uint32_t x = a[i], y = b[i];
-        if (x != y) {
-            out->col[deg[x]] = y;
-            out->w[deg[x]++] = wv[i];
-            out->col[deg[y]] = x;
-            out->w[deg[y]++] = wv[i];
-        } else {
-            out->col[deg[x]] = x;
-            out->w[deg[x]++] = wv[i];
-        }
-    }
+    // rows in file order; self-loop once (utils.cu:200-224); worker t fills
+    // rows [n*t/T, n*(t+1)/T) scanning every line in order
+    const unsigned W = (nnz < (1u << 16)) ? 1u : n_workers();
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < W; ++t)
+        th.emplace_back([&, t] {
+            const uint32_t r0 = (uint32_t)((uint64_t)n * t / W);
+            const uint32_t r1 = (uint32_t)((uint64_t)n * (t + 1) / W);
+            for (uint64_t i = 0; i < nnz; ++i) {
+                const uint32_t x = el[i].a, y = el[i].b;
+                const float w = el[i].w;
+                if (x >= r0 && x < r1) {
+                    out->col[deg[x]] = y;
+                    out->w[deg[x]++] = w;
+                }
+                if (x != y && y >= r0 && y < r1) {
+                    out->col[deg[y]] = x;
+                    out->w[deg[y]++] = w;
+                }
+            }
+        });
+    for (auto &x : th) x.join();
     out->nnz = nnz;
+    return LV_OK;
+}
+
+// Binary CSR cache: "LVCSR001", n (u32), pad, nnz (u64), ne (u64), then
+// row[n+1] (u32), col[ne] (u32), w[ne] (f32), little-endian as in memory.
+constexpr char kCsrMagic[8] = {'L', 'V', 'C', 'S', 'R', '0', '0', '1'};
+
+int load_binary(const char *p, size_t len, lv_host_csr *out) {
+    memset(out, 0, sizeof(*out));
+    if (len < 32) return LV_ERR_FORMAT;
+    uint32_t n;
+    uint64_t nnz, ne;
+    memcpy(&n, p + 8, 4);
+    memcpy(&nnz, p + 16, 8);
+    memcpy(&ne, p + 24, 8);
+    if (n >= 0x80000000u || ne >= 0xffffffffull) return LV_ERR_FORMAT;
+    const size_t need = 32 + ((size_t)n + 1) * 4 + (size_t)ne * 8;
+    if (len != need) return LV_ERR_FORMAT;
+    int rc = alloc_csr(out, n, ne);
+    if (rc) return rc;
+    const char *q = p + 32;
+    memcpy(out->row, q, ((size_t)n + 1) * 4);
+    q += ((size_t)n + 1) * 4;
+    memcpy(out->col, q, (size_t)ne * 4);
+    q += (size_t)ne * 4;
+    memcpy(out->w, q, (size_t)ne * 4);
+    out->nnz = nnz;
+    // the same invariants a parsed file has
+    bool ok = out->row[0] == 0 && out->row[n] == ne;
+    for (uint32_t v = 0; ok && v < n; ++v) ok = out->row[v] <= out->row[v + 1];
+    for (uint64_t j = 0; ok && j < ne; ++j) ok = out->col[j] < n;
+    if (!ok) {
+        lv_host_csr_free(out);
+        return LV_ERR_FORMAT;
+    }
     return LV_OK;
 }
 
@@ -279,9 +409,28 @@ int lv_mtx_read(const char *path, lv_host_csr *out) {
     void *p = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
     close(fd);
     if (p == MAP_FAILED) return LV_ERR_IO;
-    int rc = parse_text((const char *)p, len, out);
+    const char *t = (const char *)p;
+    const int rc = (len >= 8 && memcmp(t, kCsrMagic, 8) == 0) ? load_binary(t, len, out)
+                                                               : parse_text(t, len, out);
     munmap(p, len);
     return rc;
+}
+
+int lv_csr_write(const char *path, const lv_host_csr *csr) {
+    if (!path || !csr || !csr->row || (csr->ne && (!csr->col || !csr->w))) return LV_ERR_ARG;
+    FILE *f = fopen(path, "wb");
+    if (!f) return LV_ERR_IO;
+    char head[32] = {0};
+    memcpy(head, kCsrMagic, 8);
+    memcpy(head + 8, &csr->n, 4);
+    memcpy(head + 16, &csr->nnz, 8);
+    memcpy(head + 24, &csr->ne, 8);
+    bool ok = fwrite(head, 1, 32, f) == 32;
+    ok = ok && fwrite(csr->row, 4, (size_t)csr->n + 1, f) == (size_t)csr->n + 1;
+    ok = ok && (!csr->ne || fwrite(csr->col, 4, csr->ne, f) == csr->ne);
+    ok = ok && (!csr->ne || fwrite(csr->w, 4, csr->ne, f) == csr->ne);
+    ok = (fclose(f) == 0) && ok;
+    return ok ? LV_OK : LV_ERR_IO;
 }
 
 void lv_host_csr_free(lv_host_csr *csr) {

@@ -214,8 +214,14 @@ void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_
 // degree classification: once per pass (the reference re-partitions the
 // whole vertex set 7 times per sweep, modularity.cu:468-469)
 // ---------------------------------------------------------------------------
+// class of a row; rows longer than heavy_min (>= 319, so always bucket 7)
+// take the split heavy path
+__device__ __forceinline__ int row_class(uint32_t d, uint32_t heavy_min) {
+    return d > heavy_min ? kClassHeavy : vertex_class(d);
+}
+
 __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uint32_t n,
-                                                        uint32_t *counts,
+                                                        uint32_t heavy_min, uint32_t *counts,
                                                         unsigned long long *degsum) {
     __shared__ uint32_t hc[kNumClasses];
     __shared__ unsigned long long hd[kNumClasses];
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uin
     __syncthreads();
     for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < n; v += gridDim.x * 256) {
         uint32_t d = row[v + 1] - row[v];
-        int c = vertex_class(d);
+        int c = row_class(d, heavy_min);
         atomicAdd(&hc[c], 1u);
         atomicAdd(&hd[c], (unsigned long long)d);
     }
@@ -240,7 +246,8 @@ __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uin
 // cursor[c] starts at the class offset; order within a class is not
 // semantic (every vertex of a step decides from the same snapshot)
 __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, uint32_t n,
-                                                          uint32_t *cursor, uint32_t *order) {
+                                                          uint32_t heavy_min, uint32_t *cursor,
+                                                          uint32_t *order) {
     __shared__ uint32_t hc[kNumClasses];
     __shared__ uint32_t base[kNumClasses];
     const uint32_t v = blockIdx.x * 256 + threadIdx.x;
@@ -249,7 +256,7 @@ __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, u
     int c = -1;
     uint32_t r = 0;
     if (v < n) {
-        c = vertex_class(row[v + 1] - row[v]);
+        c = row_class(row[v + 1] - row[v], heavy_min);
         r = atomicAdd(&hc[c], 1u);
     }
     __syncthreads();
@@ -259,17 +266,17 @@ __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, u
     if (c >= 0) order[base[c] + r] = v;
 }
 
-void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t *counts,
-                           unsigned long long *degsum) {
+void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
+                           uint32_t *counts, unsigned long long *degsum) {
     if (!n) return;
     hipLaunchKernelGGL(k_classify_count, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, row, n,
-                       counts, degsum);
+                       heavy_min, counts, degsum);
 }
-void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t *cursor,
-                             uint32_t *order) {
+void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
+                             uint32_t *cursor, uint32_t *order) {
     if (!n) return;
-    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 255) / 256), dim3(256), 0, s, row, n, cursor,
-                       order);
+    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 255) / 256), dim3(256), 0, s, row, n,
+                       heavy_min, cursor, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,6 +795,10 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
             hipLaunchKernelGGL((k_move_block<256, 4096>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
+        case 10:
+            hipLaunchKernelGGL((k_move_block<512, 8192>), dim3(count), dim3(512), 0, s, g, verts,
+                               count);
+            break;
         default:
             break;
     }
@@ -1232,6 +1243,59 @@ __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
 // slots; k_agg_merge_global then sorts the list.
 constexpr uint32_t kAggChunk = 2048;
 
+// Chunk `blk` of the large communities `comms`: the community (ci, c), its
+// flat entry range [f0, f1), and in LDS the member-entry offsets mb[0..nm] of
+// the members i0.. covering it (members of a large community have degree
+// >= 1, so at most kAggChunk of them cover a chunk).  Block-uniform.
+struct AggChunk {
+    uint32_t ci, c, f0, f1, i0, nm;
+};
+__device__ AggChunk agg_chunk(const DevView &g, const AggView &a, const uint32_t *comms,
+                              uint32_t count, uint32_t blk, uint32_t *mb, uint32_t *sh) {
+    AggChunk x;
+    uint32_t lo = 0, hi = count;  // largest ci with coff[ci] <= blk
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.coff[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    x.ci = lo;
+    x.c = comms[x.ci];
+    const uint32_t k = blk - a.coff[x.ci];
+    const uint32_t cb = a.epos[x.c], ce = cb + a.comm_deg[x.c];
+    x.f0 = cb + k * kAggChunk;
+    x.f1 = min(ce, x.f0 + kAggChunk);
+    const uint32_t m0 = a.mstart[x.c], m1 = m0 + g.comm_sz[x.c];
+    if (threadIdx.x == 0) {
+        uint32_t l = m0, h = m1;  // last member with moff <= f0
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (a.moff[mid] <= x.f0) l = mid;
+            else h = mid;
+        }
+        sh[0] = l;
+        sh[1] = min(m1 - l, (uint32_t)kAggChunk);
+    }
+    __syncthreads();
+    x.i0 = sh[0];
+    x.nm = sh[1];
+    for (uint32_t q = threadIdx.x; q <= x.nm; q += blockDim.x) mb[q] = a.moff[x.i0 + q];
+    __syncthreads();
+    return x;
+}
+// CSR entry of flat entry f of the chunk
+__device__ __forceinline__ uint32_t agg_entry(const DevView &g, const AggView &a,
+                                              const AggChunk &x, const uint32_t *mb, uint32_t f) {
+    uint32_t l = 0, h = x.nm;  // member of flat entry f
+    while (h - l > 1) {
+        const uint32_t mid = (l + h) >> 1;
+        if (mb[mid] <= f) l = mid;
+        else h = mid;
+    }
+    const uint32_t v = a.members[x.i0 + l];
+    return g.row[v] + (f - mb[l]);
+}
+
 __global__ void k_agg_mdeg(DevView g, AggView a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < g.n) {
@@ -1251,19 +1315,7 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
     __shared__ unsigned long long lval[LCAP];
     __shared__ uint32_t ldist[kAggChunk];
     __shared__ uint32_t mb[kAggChunk + 1];
-    __shared__ uint32_t nld, nnew, newbase, i0s, nmb;
-    // chunk -> community (largest ci with coff[ci] <= blockIdx.x)
-    uint32_t lo = 0, hi = count;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.coff[mid] <= blockIdx.x) lo = mid;
-        else hi = mid;
-    }
-    const uint32_t ci = lo, c = comms[ci];
-    const uint32_t k = blockIdx.x - a.coff[ci];
-    const uint32_t cb = a.epos[c], ce = cb + a.comm_deg[c];
-    const uint32_t f0 = cb + k * kAggChunk, f1 = min(ce, f0 + kAggChunk);
-    const uint32_t m0 = a.mstart[c], m1 = m0 + g.comm_sz[c];
+    __shared__ uint32_t nld, nnew, newbase, sh[2];
     for (int i = threadIdx.x; i < LCAP; i += 256) {
         lkey[i] = kEmpty;
         lval[i] = 0ull;
@@ -1271,33 +1323,16 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
     if (threadIdx.x == 0) {
         nld = 0;
         nnew = 0;
-        uint32_t l = m0, h = m1;  // last member with moff <= f0
-        while (h - l > 1) {
-            const uint32_t mid = (l + h) >> 1;
-            if (a.moff[mid] <= f0) l = mid;
-            else h = mid;
-        }
-        i0s = l;
-        // members of a large community have degree >= 1, so at most
-        // kAggChunk of them cover the chunk
-        nmb = min(m1 - l, (uint32_t)kAggChunk);
     }
-    __syncthreads();
-    const uint32_t i0 = i0s, nm = nmb;
-    for (uint32_t q = threadIdx.x; q <= nm; q += 256) mb[q] = a.moff[i0 + q];
-    __syncthreads();
+    const AggChunk x = agg_chunk(g, a, comms, count, blockIdx.x, mb, sh);
+    const uint32_t ci = x.ci, f0 = x.f0, f1 = x.f1;
+    const uint32_t cb = a.epos[x.c];
+    if (a.only && !a.only[ci]) return;  // uniform: community not on this path
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t f = f0 + threadIdx.x + 256u * q;
         if (f < f1) {
-            uint32_t l = 0, h = nm;  // member of flat entry f
-            while (h - l > 1) {
-                const uint32_t mid = (l + h) >> 1;
-                if (mb[mid] <= f) l = mid;
-                else h = mid;
-            }
-            const uint32_t v = a.members[i0 + l];
-            const uint32_t j = g.row[v] + (f - mb[l]);
+            const uint32_t j = agg_entry(g, a, x, mb, f);
             const uint32_t dd = g.comm[g.col[j]];
             const unsigned long long wf = f2fix(g.w[j], g.scale);
             uint32_t slot = hslot<LC>(dd);
@@ -1373,6 +1408,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     __shared__ uint32_t ctot[256];
     __shared__ uint32_t wcnt[NW][256];
     const uint32_t ci = blockIdx.x;
+    if (a.only && !a.only[ci]) return;  // uniform: community not on this path
     const uint32_t c = comms[ci];
     const unsigned long long off = a.goff[ci];
     uint32_t *key = a.gkey + off;
@@ -1493,14 +1529,298 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
                            comms, count);
 }
 
+// ---------------------------------------------------------------------------
+// Large communities, partitioned (the default for comm_deg > kAggMidMax).
+// Each community's entries are split by the new id of their neighbour
+// community into key ranges (buckets of about kAggBucketTarget entries); the
+// buckets are merged and sorted in LDS, and since bucket b's keys all precede
+// bucket b+1's, the bucket outputs in order are the community's sorted row.
+//   P0 count    new id of every entry (staged in entry order), bucket
+//               histogram per chunk
+//   (scan of the bucket counts -> bucket ranges)
+//   P1 scatter  (key, weight) into the bucket ranges, runs written contiguously
+//   P2 bucket   LDS table of exact sums, distinct keys sorted, written at the
+//               bucket's place in the community's output range
+// A bucket with more distinct keys than its table holds flags its community,
+// which is then merged by the global-table kernels above instead.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kAggBucketTarget = 1024;
+constexpr uint32_t kAggMaxBuckets = 4096;
+constexpr int kAggBucketCap = 4096;  // LDS table slots of one bucket
+
+__device__ __forceinline__ uint32_t agg_bucket(uint32_t key, uint32_t nb, uint32_t nn) {
+    return (uint32_t)(((unsigned long long)key * nb) / nn);
+}
+
+__global__ void k_aggp_nb(AggView a, const uint32_t *comms, uint32_t count, uint32_t *nbk) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const uint32_t d = a.comm_deg[comms[i]];
+        uint32_t b = 1;
+        while (b < kAggMaxBuckets && (unsigned long long)b * kAggBucketTarget < d) b <<= 1;
+        nbk[i] = b;
+    }
+}
+void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+                    uint32_t *nbk) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_aggp_nb, dim3((count + 255) / 256), dim3(256), 0, s, a, comms, count,
+                       nbk);
+}
+
+// P0: new id of every entry of the chunk, staged in entry order, counted per
+// bucket
+__global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPart p,
+                                                    const uint32_t *comms, uint32_t count) {
+    __shared__ uint32_t mb[kAggChunk + 1];
+    __shared__ uint32_t hist[kAggMaxBuckets];
+    __shared__ uint32_t sh[2];
+    for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) hist[b] = 0;
+    const AggChunk x = agg_chunk(g, a, comms, count, blockIdx.x, mb, sh);
+    const uint32_t nb = p.nbk[x.ci];
+    for (uint32_t f = x.f0 + threadIdx.x; f < x.f1; f += 256) {
+        const uint32_t j = agg_entry(g, a, x, mb, f);
+        const uint32_t key = a.new_id[g.comm[g.col[j]]];
+        p.fkey[f] = key;
+        p.fw[f] = g.w[j];
+        atomicAdd(&hist[agg_bucket(key, nb, p.nn)], 1u);
+    }
+    __syncthreads();
+    const uint32_t b0 = p.bstart[x.ci];
+    for (uint32_t b = threadIdx.x; b < nb; b += 256)
+        if (hist[b]) atomicAdd(&p.bcnt[b0 + b], hist[b]);
+}
+
+// P1: the chunk's entries ordered by bucket in LDS, then each bucket's run
+// written to consecutive addresses of its range
+__global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, const uint32_t *comms,
+                                                      uint32_t count) {
+    constexpr uint32_t PER = kAggMaxBuckets / 256;
+    __shared__ uint32_t lcnt[kAggMaxBuckets], loff[kAggMaxBuckets], lbase[kAggMaxBuckets];
+    __shared__ uint32_t sk[kAggChunk];
+    __shared__ float sw[kAggChunk];
+    __shared__ uint16_t sb[kAggChunk];
+    __shared__ uint32_t wsum[4];
+    uint32_t lo = 0, hi = count;  // chunk -> large community
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.coff[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t ci = lo, c = comms[ci];
+    const uint32_t cb = a.epos[c], ce = cb + a.comm_deg[c];
+    const uint32_t f0 = cb + (blockIdx.x - a.coff[ci]) * kAggChunk;
+    const uint32_t cnt = min(ce, f0 + kAggChunk) - f0;
+    const uint32_t nb = p.nbk[ci];
+    for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) lcnt[b] = 0;
+    __syncthreads();
+    uint32_t key[kAggChunk / 256], rank[kAggChunk / 256], bk[kAggChunk / 256];
+#pragma unroll
+    for (uint32_t q = 0; q < kAggChunk / 256; ++q) {
+        const uint32_t i = threadIdx.x + 256u * q;
+        bk[q] = kEmpty;
+        if (i < cnt) {
+            key[q] = p.fkey[f0 + i];
+            bk[q] = agg_bucket(key[q], nb, p.nn);
+            rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of lcnt: thread t owns buckets PER*t .. PER*t+PER-1
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t own = 0;
+    for (uint32_t k = 0; k < PER; ++k) own += lcnt[PER * t + k];
+    uint32_t inc = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    const uint32_t b0 = p.bstart[ci];
+    for (uint32_t b = t; b < nb; b += 256)
+        if (lcnt[b]) lbase[b] = atomicAdd(&p.bcur[b0 + b], lcnt[b]);
+    __syncthreads();
+    uint32_t run = inc - own;
+    for (uint32_t k = 0; k < wv; ++k) run += wsum[k];
+    for (uint32_t k = 0; k < PER; ++k) {
+        loff[PER * t + k] = run;
+        run += lcnt[PER * t + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kAggChunk / 256; ++q) {
+        if (bk[q] == kEmpty) continue;
+        const uint32_t sp = loff[bk[q]] + rank[q];
+        sk[sp] = key[q];
+        sw[sp] = p.fw[f0 + threadIdx.x + 256u * q];
+        sb[sp] = (uint16_t)bk[q];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+        const uint32_t b = sb[i];
+        const uint32_t pos = lbase[b] + (i - loff[b]);
+        p.pkey[pos] = sk[i];
+        p.pw[pos] = sw[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPart p,
+                                                     const uint32_t *comms) {
+    constexpr int LC = Log2<kAggBucketCap>::value;
+    constexpr uint32_t DMAX = kAggBucketCap / 2;
+    __shared__ uint32_t key[kAggBucketCap];
+    __shared__ unsigned long long val[kAggBucketCap];
+    __shared__ uint32_t skeys[DMAX], sslot[DMAX];
+    __shared__ uint32_t nd_s, over;
+    const uint32_t b = blockIdx.x;
+    const uint32_t ci = p.bowner[b], c = comms[ci];
+    const uint32_t q0 = p.bpos[b], q1 = p.bpos[b + 1];
+    if (threadIdx.x == 0) p.bnd[b] = 0;
+    if (q0 == q1) return;  // uniform
+    for (uint32_t i = threadIdx.x; i < kAggBucketCap; i += 256) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        nd_s = 0;
+        over = 0;
+    }
+    __syncthreads();
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
+        const uint32_t k = p.pkey[q];
+        uint32_t slot = hslot<LC>(k);
+        int probe = 0;
+        for (; probe < kAggBucketCap; ++probe) {
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, k);
+            if (prev == kEmpty) {
+                const uint32_t idx = atomicAdd(&nd_s, 1u);
+                if (idx < p.dmax) {
+                    skeys[idx] = k;
+                    sslot[idx] = slot;
+                } else {
+                    over = 1u;  // more distinct keys than the sort holds
+                }
+                break;
+            }
+            if (prev == k) break;
+            slot = (slot + 1) & (kAggBucketCap - 1);
+        }
+        if (probe == kAggBucketCap) {
+            over = 1u;
+            continue;
+        }
+        atomicAdd(&val[slot], f2fix(p.pw[q], g.scale));
+    }
+    __syncthreads();
+    if (over) {  // uniform: the community goes to the global-table merge
+        if (threadIdx.x == 0 && atomicOr(&p.cflag[ci], 1u) == 0u) atomicAdd(p.nflag, 1u);
+        return;
+    }
+    const uint32_t nd = nd_s;
+    uint32_t n2 = 1;
+    while (n2 < nd) n2 <<= 1;
+    for (uint32_t i = nd + threadIdx.x; i < n2; i += 256) {
+        skeys[i] = kEmpty;
+        sslot[i] = 0;
+    }
+    __syncthreads();
+    lds_bitonic<256>(skeys, sslot, n2);
+    const uint32_t out = a.epos[c] + (q0 - p.bpos[p.bstart[ci]]);
+    for (uint32_t i = threadIdx.x; i < nd; i += 256) {
+        a.tkey[out + i] = skeys[i];
+        a.tval[out + i] = val[sslot[i]];
+    }
+    if (threadIdx.x == 0) {
+        p.bnd[b] = nd;
+        atomicAdd(&a.cnt[a.new_id[c]], nd);
+    }
+}
+
+// communities that overflowed restart on the global-table merge: their
+// partial distinct counts are dropped
+__global__ void k_aggp_reset_flagged(AggView a, AggPart p, const uint32_t *comms,
+                                     uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && p.cflag[i]) a.cnt[a.new_id[comms[i]]] = 0;
+}
+// table capacities for flagged communities only (others get none)
+__global__ void k_aggp_caps(AggView a, AggPart p, const uint32_t *comms, uint32_t count,
+                            unsigned long long *caps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        unsigned long long cap = 0;
+        if (p.cflag[i]) {
+            const unsigned long long d = a.comm_deg[comms[i]];
+            cap = 1;
+            while (cap < 2 * d) cap <<= 1;
+        }
+        caps[i] = cap;
+    }
+}
+
+// sorted distinct lists of the bucket outputs -> the new CSR (one workgroup
+// per bucket; buckets of flagged communities are emitted per community)
+__global__ __launch_bounds__(256) void k_aggp_emit(DevView g, AggView a, AggPart p,
+                                                   const uint32_t *comms, const uint32_t *row2,
+                                                   uint32_t *col2, float *w2) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t ci = p.bowner[b];
+    if (p.cflag[ci]) return;
+    const uint32_t nd = p.bnd[b];
+    if (!nd) return;
+    const uint32_t c = comms[ci], first = p.bstart[ci];
+    const uint32_t src = a.epos[c] + (p.bpos[b] - p.bpos[first]);
+    const uint32_t dst = row2[a.new_id[c]] + (p.boff[b] - p.boff[first]);
+    for (uint32_t i = threadIdx.x; i < nd; i += 256) {
+        col2[dst + i] = a.tkey[src + i];
+        w2[dst + i] = fix2f(a.tval[src + i], g.S);
+    }
+}
+
+void launch_aggp(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
+                 const uint32_t *comms, uint32_t count, int phase) {
+    if (!count) return;
+    switch (phase) {
+        case 0:
+            hipLaunchKernelGGL(k_aggp_count, dim3(a.nchunks), dim3(256), 0, s, g, a, p, comms,
+                               count);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_aggp_scatter, dim3(a.nchunks), dim3(256), 0, s, a, p, comms,
+                               count);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_aggp_bucket, dim3(p.NB), dim3(256), 0, s, g, a, p, comms);
+            break;
+        case 3:
+            hipLaunchKernelGGL(k_aggp_reset_flagged, dim3((count + 255) / 256), dim3(256), 0, s, a,
+                               p, comms, count);
+            break;
+        default:
+            break;
+    }
+}
+void launch_aggp_caps(hipStream_t s, AggView a, const AggPart &p, const uint32_t *comms,
+                      uint32_t count, unsigned long long *caps) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_aggp_caps, dim3((count + 255) / 256), dim3(256), 0, s, a, p, comms,
+                       count, caps);
+}
+void launch_aggp_emit(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
+                      const uint32_t *comms, const uint32_t *row2, uint32_t *col2, float *w2) {
+    if (!p.NB) return;
+    hipLaunchKernelGGL(k_aggp_emit, dim3(p.NB), dim3(256), 0, s, g, a, p, comms, row2, col2, w2);
+}
+
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
 // sorted distinct list into the new CSR with renumbered ids and FP32 weights
 __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const uint32_t *comms,
-                                                  uint32_t count, uint32_t *row2, uint32_t *col2,
-                                                  float *w2) {
+                                                  uint32_t count, const uint32_t *only,
+                                                  uint32_t *row2, uint32_t *col2, float *w2) {
     const uint32_t ci = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
-    if (ci >= count) return;
+    if (ci >= count || (only && !only[ci])) return;
     const uint32_t c = comms[ci];
     const uint32_t nid = a.new_id[c];
     const uint32_t o = row2[nid], nd = row2[nid + 1] - o;
@@ -1511,10 +1831,11 @@ __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const ui
     }
 }
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
-                     uint32_t count, uint32_t *row2, uint32_t *col2, float *w2) {
+                     uint32_t count, const uint32_t *only, uint32_t *row2, uint32_t *col2,
+                     float *w2) {
     if (!count) return;
     hipLaunchKernelGGL(k_agg_emit, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms, count,
-                       row2, col2, w2);
+                       only, row2, col2, w2);
 }
 
 // graphVertToComm + graphVertToNewComm (modularity.cu:51-56,

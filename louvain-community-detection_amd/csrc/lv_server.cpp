@@ -1,16 +1,21 @@
 // lv_server.cpp -- serving loop: continuous admission of independent graphs
 // onto one GPU under an HBM budget.
 //
-// Requests enter a FIFO.  `slots` worker threads each own one HIP stream
-// (through their lv_graph); a worker takes the first queued request whose
-// device footprint fits the remaining budget, uploads it, runs the full pass
-// loop with its state resident on the device, and publishes the result.
-// Requests that arrive while others run are admitted as soon as a slot and
-// memory free up, and their kernels overlap on the device with the running
-// ones.  Each request's result is the single-graph result (lv_run), bit for
-// bit: graphs share nothing but the device.
+// Requests enter a FIFO.  `slots` worker threads each own one device-resident
+// graph handle (buffers and HIP stream) that they reuse from request to
+// request: a worker takes the first queued request that fits its buffers, or
+// whose footprint fits the budget once the worker's own buffers are grown or
+// replaced; it uploads the request, runs the full pass loop with the state
+// resident on the device and publishes the result.  Requests that arrive
+// while others run are admitted as soon as a slot and memory free up, and
+// their kernels overlap on the device with the running ones.  Reusing the
+// buffers keeps device allocation and release (which synchronise the whole
+// device) out of the steady state.  A worker whose buffers block every queued
+// request releases them.  Each request's result is the single-graph result
+// (lv_run), bit for bit: graphs share nothing but the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -23,17 +28,30 @@
 #include <vector>
 
 #include "../../include/louvain_hip.h"
+#include "lv_host.h"
 
 namespace {
 
 struct Request {
     uint64_t id = 0;
     uint32_t n = 0;
+    uint64_t ne = 0;
+    int S = 0;  // fixed-point scale bits (from the worker's input checks)
     std::vector<uint32_t> row, col;
     std::vector<float> w;
-    uint64_t bytes = 0;
     double t_submit = 0.0;
 };
+
+// one worker's resident graph handle and the budget it holds
+struct Slot {
+    lv_graph *g = nullptr;
+    uint32_t n_cap = 0;
+    uint64_t e_cap = 0;
+    uint64_t bytes = 0;
+};
+
+// what a worker must do with its buffers to take a request
+enum class Fit { kNone, kReuse, kGrow, kReplace };
 
 }  // namespace
 
@@ -59,28 +77,86 @@ struct lv_server {
 
 namespace {
 
-// first queued request that fits the remaining budget (FIFO order otherwise)
-Request *take_fitting(lv_server *s) {
-    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
-        if (s->used + (*it)->bytes <= s->budget) {
-            Request *r = *it;
-            s->queue.erase(it);
-            return r;
-        }
+// How slot `sl` can take request r under the budget (caller holds the lock).
+Fit fit_of(const lv_server *s, const Slot &sl, const Request *r, uint32_t *n_cap,
+           uint64_t *e_cap, uint64_t *bytes) {
+    if (sl.g && r->n <= sl.n_cap && r->ne <= sl.e_cap) return Fit::kReuse;
+    const uint64_t others = s->used - sl.bytes;
+    const uint32_t gn = std::max(sl.n_cap, r->n);
+    const uint64_t ge = std::max(sl.e_cap, r->ne);
+    const uint64_t grow = lv_graph_footprint(gn, ge);
+    if (others + grow <= s->budget) {
+        *n_cap = gn;
+        *e_cap = ge;
+        *bytes = grow;
+        return Fit::kGrow;
     }
-    return nullptr;
+    const uint64_t exact = lv_graph_footprint(r->n, r->ne);
+    if (others + exact <= s->budget) {
+        *n_cap = r->n;
+        *e_cap = r->ne;
+        *bytes = exact;
+        return Fit::kReplace;
+    }
+    return Fit::kNone;
+}
+
+void publish(lv_server *s, lv_request_result &res, std::vector<uint32_t> &assign, int rc) {
+    res.status = rc;
+    res.t_end_ms = s->now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->running--;
+    s->done.push_back(res);
+    if (s->keep_assignments && rc == LV_OK) s->assignments[res.id] = std::move(assign);
 }
 
 void worker(lv_server *s) {
     (void)hipSetDevice(s->device);
+    Slot sl;
     for (;;) {
         Request *r = nullptr;
+        Fit fit = Fit::kNone;
+        uint32_t n_cap = 0;
+        uint64_t e_cap = 0, bytes = 0;
+        bool release = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || (r = take_fitting(s)) != nullptr; });
-            if (!r) return;  // stopping and nothing admissible
-            s->used += r->bytes;
-            s->running++;
+            s->cv_work.wait(lk, [&] {
+                if (s->stop) return true;
+                for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+                    fit = fit_of(s, sl, *it, &n_cap, &e_cap, &bytes);
+                    if (fit != Fit::kNone) {  // first admissible in FIFO order
+                        r = *it;
+                        s->queue.erase(it);
+                        return true;
+                    }
+                }
+                // nothing admissible: idle buffers may be what blocks the queue
+                if (!s->queue.empty() && sl.g) {
+                    release = true;
+                    return true;
+                }
+                return false;
+            });
+            if (!r && !release) break;  // stopping
+            if (r) {
+                if (fit != Fit::kReuse) {  // reserve the new size now
+                    s->used = s->used - sl.bytes + bytes;
+                    sl.bytes = bytes;
+                }
+                s->running++;
+            }
+        }
+        if (release) {
+            lv_graph_destroy(sl.g);
+            const uint64_t freed = sl.bytes;
+            sl = Slot();
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->used -= freed;
+            }
+            s->cv_work.notify_all();
+            continue;
         }
         lv_request_result res;
         memset(&res, 0, sizeof(res));
@@ -88,11 +164,28 @@ void worker(lv_server *s) {
         res.t_submit_ms = r->t_submit;
         res.t_start_ms = s->now_ms();
         std::vector<uint32_t> assign;
-        lv_graph *g = nullptr;
-        int rc = lv_graph_create(s->device, r->n, r->row.data(), r->col.data(), r->w.data(), &g);
+        int rc = LV_OK;
+        if (fit != Fit::kReuse) {
+            if (sl.g) lv_graph_destroy(sl.g);
+            sl.g = nullptr;
+            rc = lv_host::graph_new(s->device, n_cap, e_cap, &sl.g);
+            if (rc == LV_OK) {
+                sl.n_cap = n_cap;
+                sl.e_cap = e_cap;
+            } else {  // give the reservation back
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->used -= sl.bytes;
+                sl = Slot();
+            }
+        }
+        // lv_graph_create's input checks, off the submitting thread
+        if (rc == LV_OK)
+            rc = lv_host::check_input(r->n, r->row.data(), r->col.data(), r->w.data(), &r->S);
+        if (rc == LV_OK)
+            rc = lv_host::graph_upload(sl.g, r->n, r->row.data(), r->col.data(), r->w.data(), r->S);
         if (rc == LV_OK) {
             lv_result lr;
-            rc = lv_run(g, s->min_gain, &lr);
+            rc = lv_run(sl.g, s->min_gain, &lr);
             if (rc == LV_OK) {
                 res.q_final = lr.q_final;
                 res.passes = lr.passes;
@@ -101,24 +194,16 @@ void worker(lv_server *s) {
                 res.sum_edges = lr.sum_edges;
                 if (s->keep_assignments) {
                     assign.resize(r->n);
-                    rc = lv_get_assignment(g, assign.data());
+                    rc = lv_get_assignment(sl.g, assign.data());
                 }
             }
-            lv_graph_destroy(g);
         }
-        res.status = rc;
-        res.t_end_ms = s->now_ms();
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->used -= r->bytes;
-            s->running--;
-            s->done.push_back(res);
-            if (s->keep_assignments && rc == LV_OK) s->assignments[r->id] = std::move(assign);
-        }
+        publish(s, res, assign, rc);
         delete r;
-        s->cv_work.notify_all();  // freed memory may admit a queued request
+        s->cv_work.notify_all();
         s->cv_done.notify_all();
     }
+    if (sl.g) lv_graph_destroy(sl.g);
 }
 
 }  // namespace
@@ -128,7 +213,7 @@ extern "C" {
 uint64_t lv_graph_footprint(uint32_t n, uint64_t ne) {
     // every buffer lv_graph_create allocates, plus the largest hash arenas a
     // pass can request (bounded by 4x the entries)
-    return 176ull * ((uint64_t)n + 1) + 176ull * ne + (1ull << 20);
+    return 184ull * ((uint64_t)n + 1) + 184ull * ne + (1ull << 20);
 }
 
 int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, float min_gain,
@@ -154,18 +239,18 @@ int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, floa
 
 int lv_server_submit(lv_server *s, uint64_t id, uint32_t n, const uint32_t *row,
                      const uint32_t *col, const float *w) {
-    if (!s || !row) return LV_ERR_ARG;
+    if (!s || !row || n >= 0x80000000u || row[0] != 0) return LV_ERR_ARG;
     const uint64_t ne = row[n];
-    const uint64_t bytes = lv_graph_footprint(n, ne);
-    if (bytes > s->budget) return LV_ERR_NOMEM;  // can never be admitted
+    if (ne && (!col || !w)) return LV_ERR_ARG;
+    if (lv_graph_footprint(n, ne) > s->budget) return LV_ERR_NOMEM;  // never admissible
     Request *r = new (std::nothrow) Request();
     if (!r) return LV_ERR_NOMEM;
     r->id = id;
     r->n = n;
+    r->ne = ne;
     r->row.assign(row, row + (size_t)n + 1);
     r->col.assign(col, col + ne);
     r->w.assign(w, w + ne);
-    r->bytes = bytes;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->t_submit = s->now_ms();

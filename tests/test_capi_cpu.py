@@ -55,6 +55,70 @@ def test_mtx_reader_matches_oracle(lv, orc, bcsstk16_path, tmp_path):
         assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
 
 
+def _big_mtx_text(n, lines, seed, zero_line=None):
+    """A multi-megabyte .mtx body exercising every parser rule across the
+    worker splits: lines without a weight (carried), unparseable weights
+    (-> 0), negative weights (abs), self-loops, CRLF endings, blank padding,
+    and trailing lines beyond nnz."""
+    rng = np.random.default_rng(seed)
+    out = ["%%MatrixMarket matrix coordinate real general\n", "% comment\n",
+           f"{n} {n} {lines}\n"]
+    for i in range(lines):
+        a = int(rng.integers(1, n))  # ids valid both 1-based and 0-based
+        b = a if rng.random() < 0.02 else int(rng.integers(1, n))
+        if zero_line is not None and i == zero_line:
+            a = 0
+        kind = rng.random()
+        if kind < 0.25:
+            wtxt = ""  # carried from the previous line
+        elif kind < 0.27:
+            wtxt = " nan?" if rng.random() < 0.5 else " x"  # does not parse -> 0
+        else:
+            wtxt = f" {rng.normal() * 3:.7g}"
+        end = "\r\n" if rng.random() < 0.05 else "\n"
+        out.append(f"{a}\t{b}{wtxt}{end}" if rng.random() < 0.1 else f"{a} {b}{wtxt}{end}")
+    out.append("1 1 5\n")  # beyond nnz: ignored
+    out.append("garbage beyond nnz\n")
+    return "".join(out)
+
+
+@pytest.mark.parametrize("zero_line", [None, 150000])
+def test_mtx_reader_multithreaded_matches_oracle(lv, orc, tmp_path, zero_line):
+    """Bodies of several MB are split over worker threads; the carried weight
+    and the 0/1-based decision cross the splits exactly like one sequential
+    pass of the reference parser."""
+    text = _big_mtx_text(50000, 200000, 7, zero_line)
+    assert len(text) > (1 << 21)
+    p = tmp_path / "big.mtx"
+    p.write_bytes(text.encode())
+    a = lv.read_mtx(str(p))
+    b = orc.parse_mtx(text=text)
+    assert np.array_equal(a[0].astype(np.uint64), b[0])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+
+
+def test_binary_csr_roundtrip(lv, bcsstk16_path, tmp_path):
+    row, col, w = lv.read_mtx(bcsstk16_path)
+    p = tmp_path / "g.lvcsr"
+    lv.write_csr(str(p), row, col, w)
+    r2, c2, w2 = lv.read_mtx(str(p))
+    assert np.array_equal(row, r2) and np.array_equal(col, c2)
+    assert np.array_equal(w.view(np.uint32), w2.view(np.uint32))
+    raw = bytearray(p.read_bytes())
+    raw[32 + 4 * (row.size - 1) + 4 * 5] = 0xFF  # a column id far out of range
+    raw[32 + 4 * (row.size - 1) + 4 * 5 + 3] = 0x7F
+    bad = tmp_path / "bad.lvcsr"
+    bad.write_bytes(bytes(raw[:-4]))  # truncated
+    with pytest.raises(lv.LvError) as e:
+        lv.read_mtx(str(bad))
+    assert e.value.rc == 2
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(lv.LvError) as e:
+        lv.read_mtx(str(bad))
+    assert e.value.rc == 2
+
+
 def test_mtx_reader_errors(lv, tmp_path):
     with pytest.raises(lv.LvError) as e:
         lv.read_mtx(str(tmp_path / "missing.mtx"))
@@ -154,3 +218,10 @@ def test_cli_missing_file(lv, tmp_path):
     r = _cli(lv, "-f", str(tmp_path / "nope.mtx"), "-g", "0.01")
     assert r.returncode == 1
     assert r.stdout == "Error opening file\n"
+
+
+def test_tuning_defaults(lv):
+    t = lv.Tuning()
+    assert lv.lib().lv_tuning_default(ctypes.byref(t)) == 0
+    assert t.as_dict() == {"heavy_min_degree": 4096, "bucket_entries": 512, "locality": -1,
+                           "locality_min_vertices": 1 << 20}

@@ -324,6 +324,43 @@ def test_heavy_global_table_path(lv, orc):
     assert "global-table path ok" in r.stdout
 
 
+@pytest.mark.parametrize("env", [{"LV_AGG_GLOBAL": "1"}, {"LV_AGG_DMAX": "24"}])
+def test_large_community_merge_paths(lv, orc, env):
+    """Large communities are merged by key-range buckets in LDS; a community
+    whose bucket overflows goes to the global-table merge.  Every community
+    on the global tables (LV_AGG_GLOBAL), and a tiny bucket capacity so both
+    paths serve communities of the same aggregation (LV_AGG_DMAX), each give
+    the oracle's contracted graphs -- in a child process so the setting does
+    not leak."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from loader import load_package; import oracle\n"
+        "lv = load_package()\n"
+        "for r, c, w in [lv.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 13),\n"
+        "                lv.gen_chung_lu(60000, 9.4, 2.5, 420, 300, 20.0, 5)]:\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w)\n"
+        "    with lv.Graph(r, c, w) as g:\n"
+        "        for p in range(20):\n"
+        "            imp, qp = g.modularity_optimization(1e-6)\n"
+        "            oimp, oqp, _ = o.modularity_optimization(1e-6)\n"
+        "            assert imp == oimp\n"
+        "            if not imp: break\n"
+        "            g.aggregation(); o.aggregation()\n"
+        "            gr, gc, gw = g.csr(); orow, ocol, ow = o.csr()\n"
+        "            assert np.array_equal(gr.astype(np.uint64), orow) and np.array_equal(gc, ocol)\n"
+        "            assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32))\n"
+        "print('merge paths ok')\n"
+    ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
+    print(f"[case] large-community merge {env}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, **env),
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "merge paths ok" in r.stdout
+
+
 def _server_pool(lv):
     return [lv.gen_sbm(3000, 12, 14.0, 10.0, 31), lv.gen_sbm(20000, 40, 20.0, 10.0, 32),
             lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 33),
@@ -361,3 +398,47 @@ def test_server_results_match_solo(lv, slots, budget):
             assert res.t_submit_ms <= res.t_start_ms <= res.t_end_ms
             got = s.assignment(res.id, pool[gi][0].size - 1)
             assert np.array_equal(got, assign)
+
+
+@pytest.mark.parametrize("tune", [
+    dict(heavy_min_degree=319, bucket_entries=64),
+    dict(heavy_min_degree=1024, bucket_entries=1024),
+    dict(heavy_min_degree=2048, bucket_entries=256, locality=1),
+    dict(heavy_min_degree=4096, bucket_entries=512, locality=0),
+])
+def test_tuning_never_changes_results(lv, orc, graphs, tune):
+    """Launch-shape parameters (which kernel serves a bucket-7 row, heavy
+    bucket sizes, locality order) leave every result bit-identical."""
+    row, col, w = graphs["rmat16"]
+    print(f"[case] tuning {tune}", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    ores = o.run(1e-6)
+    with lv.Graph(_u32(row), col, w) as g:
+        g.set_tuning(**tune)
+        assert all(g.get_tuning()[k] == v for k, v in tune.items())
+        res = g.run(1e-6)
+        assign = g.assignment()
+    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)
+    assert res.total_iters == ores.total_iters
+    assert np.array_equal(assign, o.init_comm())
+
+
+def test_autotune(lv, orc, graphs):
+    """The autotuner times candidate launch shapes on the resident graph and
+    keeps the fastest; a run afterwards still gives the oracle's result, and
+    out-of-range tunings are refused."""
+    row, col, w = graphs["rmat16"]
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    ores = o.run(1e-6)
+    with lv.Graph(_u32(row), col, w) as g:
+        with pytest.raises(lv.LvError):
+            g.set_tuning(heavy_min_degree=100)  # would pull bucket-6 rows into step 7
+        with pytest.raises(lv.LvError):
+            g.set_tuning(bucket_entries=300)
+        best, ms = g.autotune(1e-6)
+        print(f"[case] autotune best={best} ms={ms:.2f}", flush=True)
+        assert ms > 0 and g.get_tuning() == best
+        res = g.run(1e-6)
+        assign = g.assignment()
+    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)
+    assert np.array_equal(assign, o.init_comm())
