@@ -105,6 +105,7 @@ struct HeavySplit {
     const uint32_t *nbuck;               // nh: buckets per vertex (pow2)
     const uint32_t *bstart;              // nh+1: first bucket of each vertex
     uint32_t nbuckets;                   // total buckets = bstart[nh]
+    uint32_t bucket_entries;             // target entries per bucket (tuning)
     const uint32_t *bvtx;                // nbuckets: heavy index owning each bucket
     uint32_t *bcnt, *bpos, *bcur;        // nbuckets / nbuckets+1 / nbuckets
     uint32_t *pc, *pj;                   // heavy-entry space: community, entry index
@@ -215,6 +216,7 @@ struct AggView {
     uint32_t *coff;         // per large community: chunk offsets (count+1)
     uint32_t *dcnt;         // per large community: distinct neighbour communities
     uint32_t nchunks;       // total chunks of the large communities
+    uint32_t nn;            // new vertex count (new ids are < nn)
     const uint32_t *only;   // non-null: global-table merge only for flagged large communities
     uint32_t *err;
 };
@@ -235,8 +237,9 @@ struct AggPart {
     float *pw;
     uint32_t nn;             // new vertex count (keys are < nn)
     uint32_t NB;             // total buckets
-    uint32_t dmax;           // distinct keys a bucket may hold (<= 2048)
+    uint32_t dmax;           // distinct keys a bucket may hold (<= kAggBucketDistinct)
 };
+constexpr uint32_t kAggBucketDistinct = 1024;  // per bucket; its LDS table has twice the slots
 void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
                     uint32_t *nbk);
 // phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities

@@ -139,6 +139,7 @@ struct lv_graph {
     uint32_t *agg_mdeg = nullptr, *agg_moff = nullptr, *agg_cnk = nullptr, *agg_coff = nullptr;
     uint32_t *agg_dcnt = nullptr;
     uint32_t agg_nchunks = 0;
+    uint32_t agg_nn = 0;  // new vertex count of the aggregation in progress
     // partitioned merge of large communities: per community / per bucket
     uint32_t *ap_nbk = nullptr, *ap_bstart = nullptr, *ap_cflag = nullptr, *ap_nflag = nullptr;
     uint32_t *ap_bcnt = nullptr, *ap_bpos = nullptr, *ap_bcur = nullptr, *ap_bnd = nullptr;
@@ -412,6 +413,7 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.nbuck = g->hp_nbuck;
     h.bstart = g->hp_bstart;
     h.nbuckets = g->hp_nbuckets;
+    h.bucket_entries = g->tune.bucket_entries;
     h.bcnt = g->hp_bcnt;
     h.bpos = g->hp_bpos;
     h.bcur = g->hp_bcur;
@@ -696,6 +698,7 @@ lv::AggView aview(lv_graph *g) {
     a.coff = g->agg_coff;
     a.dcnt = g->agg_dcnt;
     a.nchunks = g->agg_nchunks;
+    a.nn = g->agg_nn;
     a.only = nullptr;
     a.err = &g->dc->err;
     return a;
@@ -757,8 +760,9 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     // LV_AGG_DMAX (test hook) lowers the bucket capacity so some communities
     // overflow and take the global-table merge next to partitioned ones
     static const char *dmax_env = getenv("LV_AGG_DMAX");
-    p.dmax = 2048;
-    if (dmax_env && atoi(dmax_env) > 0) p.dmax = std::min(2048, atoi(dmax_env));
+    p.dmax = lv::kAggBucketDistinct;
+    if (dmax_env && atoi(dmax_env) > 0)
+        p.dmax = std::min((int)lv::kAggBucketDistinct, atoi(dmax_env));
     static const bool force_global = getenv("LV_AGG_GLOBAL") != nullptr;
     uint32_t nflag = 0;
     if (force_global) {
@@ -832,6 +836,7 @@ int aggregation(lv_graph *g) {
     aoff[0] = 0;
     for (int c = 0; c < kNumAggClasses; ++c) aoff[c + 1] = aoff[c] + g->hc->acounts[c];
     if (aoff[kNumAggClasses] != nn) return LV_ERR_OVERFLOW;
+    g->agg_nn = nn;
     LV_HIP(hipMemcpyAsync(g->dc->acursor, aoff, sizeof(uint32_t) * kNumAggClasses,
                           hipMemcpyHostToDevice, s));
     LV_HIP(hipStreamSynchronize(s));  // `aoff` is pageable stack memory

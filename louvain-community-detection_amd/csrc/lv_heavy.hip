@@ -422,8 +422,36 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
     }
 }
 
-constexpr int kBucketCap = 2048;  // LDS table slots of one bucket
+// CAP: LDS table slots of one bucket (twice the distinct keys it may hold).
+// A bucket of at most 4 x 256 entries keeps each thread's entries and their
+// table slots in registers between the insert and the lane phase; longer
+// buckets (skewed hashes) read their entries again.
+template <int kBucketCap>
+__device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long *val, uint32_t *dist,
+                                              uint32_t *nd_s, uint32_t *over, uint32_t c,
+                                              unsigned long long wf) {
+    constexpr int LC = Log2<kBucketCap>::value;
+    uint32_t slot = pslot(c, LC);
+    for (int probe = 0; probe < kBucketCap; ++probe) {
+        const uint32_t prev = atomicCAS(&key[slot], kEmpty, c);
+        if (prev == kEmpty) {
+            const uint32_t idx = atomicAdd(nd_s, 1u);
+            if (idx < kBucketCap / 2) dist[idx] = slot;
+            else *over = 1u;  // more distinct communities than the table allows
+            atomicAdd(&val[slot], wf);
+            return slot;
+        }
+        if (prev == c) {
+            atomicAdd(&val[slot], wf);
+            return slot;
+        }
+        slot = (slot + 1) & (kBucketCap - 1);
+    }
+    *over = 1u;
+    return kEmpty;
+}
 
+template <int kBucketCap>
 __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     constexpr int LC = Log2<kBucketCap>::value;
     __shared__ uint32_t key[kBucketCap];
@@ -446,26 +474,29 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         over = 0;
     }
     __syncthreads();
-    for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
-        const uint32_t c = h.pc[q];
-        uint32_t slot = pslot(c, LC);
-        int probe = 0;
-        for (; probe < kBucketCap; ++probe) {
-            const uint32_t prev = atomicCAS(&key[slot], kEmpty, c);
-            if (prev == kEmpty) {
-                const uint32_t idx = atomicAdd(&nd_s, 1u);
-                if (idx < kBucketCap / 2) dist[idx] = slot;
-                else over = 1u;  // more distinct communities than the table allows
-                break;
+    constexpr uint32_t R = 4;
+    const bool cached = p1 - p0 <= 256u * R;  // uniform
+    uint32_t cs[R], js[R], ss[R];
+    if (cached) {
+        float ws[R];
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u) {  // all loads first
+            const uint32_t q = p0 + threadIdx.x + 256u * u;
+            cs[u] = kEmpty;
+            if (q < p1) {
+                cs[u] = h.pc[q];
+                ws[u] = h.pw[q];
+                js[u] = h.pj[q];
             }
-            if (prev == c) break;
-            slot = (slot + 1) & (kBucketCap - 1);
         }
-        if (probe == kBucketCap) {
-            over = 1u;
-            continue;
-        }
-        atomicAdd(&val[slot], f2fix(h.pw[q], g.scale));
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u)
+            ss[u] = cs[u] == kEmpty ? kEmpty
+                                    : hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, cs[u],
+                                                            f2fix(ws[u], g.scale));
+    } else {
+        for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256)
+            hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, h.pc[q], f2fix(h.pw[q], g.scale));
     }
     __syncthreads();
     if (over) {  // uniform: redo this vertex with the global-table kernels
@@ -488,16 +519,24 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         val[slot] = (unsigned long long)__float_as_uint(gg);
     }
     __syncthreads();
-    for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
-        const uint32_t c = h.pc[q];
-        uint32_t slot = pslot(c, LC);
-        while (key[slot] != c) slot = (slot + 1) & (kBucketCap - 1);  // present
+    auto lane_max = [&](uint32_t slot, uint32_t j) {
         const float gg = __uint_as_float((uint32_t)val[slot]);
         if (gg != -INFINITY) {
-            const uint32_t j = h.pj[q];
             const unsigned long long packed =
                 ((unsigned long long)f32_ord(gg) << 32) | (unsigned long long)(0xffffffffu - j);
             atomicMax(&lanes[j & 127u], packed);
+        }
+    };
+    if (cached) {
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u)
+            if (ss[u] != kEmpty) lane_max(ss[u], js[u]);
+    } else {
+        for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
+            const uint32_t c = h.pc[q];
+            uint32_t slot = pslot(c, LC);
+            while (key[slot] != c) slot = (slot + 1) & (kBucketCap - 1);  // present
+            lane_max(slot, h.pj[q]);
         }
     }
     __syncthreads();
@@ -573,7 +612,11 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
         (void)hipMemcpyAsync(h.bcur, h.bpos, (size_t)h.nbuckets * sizeof(uint32_t),
                              hipMemcpyDeviceToDevice, s);
         hipLaunchKernelGGL(k_hp_scatter, grid, block, 0, s, g, h);
-        hipLaunchKernelGGL(k_hp_bucket, dim3(h.nbuckets), block, 0, s, g, h);
+        // buckets of up to 512 target entries: 2048-slot tables; larger: 4096
+        if (h.bucket_entries <= 512)
+            hipLaunchKernelGGL(k_hp_bucket<2048>, dim3(h.nbuckets), block, 0, s, g, h);
+        else
+            hipLaunchKernelGGL(k_hp_bucket<4096>, dim3(h.nbuckets), block, 0, s, g, h);
     }
     // overflowing vertices (if any) redo the step on the global tables
     hipLaunchKernelGGL(k_hp_reset_flagged, dim3(h.nh), dim3(128), 0, s, h);

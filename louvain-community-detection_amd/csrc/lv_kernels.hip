@@ -1090,6 +1090,67 @@ __device__ void lds_bitonic(uint32_t *k, uint32_t *sl, uint32_t n2) {
     }
 }
 
+// Ascending order of nd distinct keys (keyof(i), i < nd) lying in
+// [lo, lo + W), without a comparison sort: the keys are binned by
+// (k - lo) >> s into at most NB bins, a prefix sum gives every bin its start,
+// and a key's rank is its bin's start plus the keys of its bin below it;
+// emit(rank, i) is called once per key.  Returns false, with nothing emitted,
+// when a bin holds more than kBinMax keys (clustered keys): the caller then
+// sorts.  cur/boffs/oidx: LDS u32[NB]; P threads, block-uniform.
+constexpr uint32_t kBinMax = 32;
+template <uint32_t NB, int P, typename KeyOf, typename Emit>
+__device__ bool lds_bin_rank(uint32_t nd, uint32_t lo, uint32_t W, KeyOf keyof, uint32_t *cur,
+                             uint32_t *boffs, uint32_t *oidx, uint32_t *wsum, uint32_t *maxbin,
+                             Emit emit) {
+    static_assert(NB % P == 0 && P % 64 == 0, "bins per thread, whole waves");
+    constexpr uint32_t PER = NB / P;
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t sh = 0;
+    while (((W - 1u) >> sh) >= NB) ++sh;
+    for (uint32_t i = t; i < NB; i += P) cur[i] = 0u;
+    if (t == 0) *maxbin = 0u;
+    __syncthreads();
+    for (uint32_t i = t; i < nd; i += P) {
+        const uint32_t c = atomicAdd(&cur[(keyof(i) - lo) >> sh], 1u);
+        atomicMax(maxbin, c + 1u);
+    }
+    __syncthreads();
+    if (*maxbin > kBinMax) return false;
+    uint32_t cnt[PER], own = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        cnt[k] = cur[PER * t + k];
+        own += cnt[k];
+    }
+    uint32_t inc = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    uint32_t run = inc - own;
+    for (uint32_t k = 0; k < wv; ++k) run += wsum[k];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        boffs[PER * t + k] = run;
+        cur[PER * t + k] = run;
+        run += cnt[k];
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < nd; i += P) oidx[atomicAdd(&cur[(keyof(i) - lo) >> sh], 1u)] = i;
+    __syncthreads();
+    for (uint32_t i = t; i < nd; i += P) {
+        const uint32_t k = keyof(i), b = (k - lo) >> sh;
+        const uint32_t st = boffs[b], en = cur[b];
+        uint32_t r = 0;
+        for (uint32_t j = st; j < en; ++j) r += keyof(oidx[j]) < k ? 1u : 0u;
+        emit(st + r, i);
+    }
+    return true;
+}
+
 // LDS merge: one workgroup of P threads per community; waves walk members,
 // lanes walk a member's row.  Output: (new_id(d), W) sorted at
 // tkey/tval[epos[c]..],
@@ -1101,9 +1162,11 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     constexpr int NW = P / 64;
     __shared__ uint32_t key[CAP];
     __shared__ unsigned long long val[CAP];
+    constexpr uint32_t NBIN = CAP / 4;  // bins for the ranked order
     __shared__ uint32_t skeys[CAP / 2];
     __shared__ uint32_t sslot[CAP / 2];
-    __shared__ uint32_t ndist;
+    __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[CAP / 2];
+    __shared__ uint32_t ndist, wsum[NW], maxbin;
     const uint32_t c = comms[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < CAP; i += P) {
         key[i] = kEmpty;
@@ -1136,18 +1199,26 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     }
     __syncthreads();
     const uint32_t nd = ndist;
-    uint32_t n2 = 1;
-    while (n2 < nd) n2 <<= 1;
-    for (uint32_t i = nd + threadIdx.x; i < n2; i += P) {
-        skeys[i] = kEmpty;
-        sslot[i] = 0;
-    }
-    __syncthreads();
-    lds_bitonic<P>(skeys, sslot, n2);
     const uint32_t out = a.epos[c];
-    for (uint32_t i = threadIdx.x; i < nd; i += P) {
-        a.tkey[out + i] = skeys[i];
-        a.tval[out + i] = val[sslot[i]];
+    auto keyof = [&](uint32_t i) { return skeys[i]; };
+    auto emit = [&](uint32_t pos, uint32_t i) {
+        a.tkey[out + pos] = skeys[i];
+        a.tval[out + pos] = val[sslot[i]];
+    };
+    // new ids lie in [0, nn); oidx holds up to CAP/2 distinct keys
+    if (!lds_bin_rank<NBIN, P>(nd, 0u, a.nn, keyof, bcur, boffs, oidx, wsum, &maxbin, emit)) {
+        uint32_t n2 = 1;
+        while (n2 < nd) n2 <<= 1;
+        for (uint32_t i = nd + threadIdx.x; i < n2; i += P) {
+            skeys[i] = kEmpty;
+            sslot[i] = 0;
+        }
+        __syncthreads();
+        lds_bitonic<P>(skeys, sslot, n2);
+        for (uint32_t i = threadIdx.x; i < nd; i += P) {
+            a.tkey[out + i] = skeys[i];
+            a.tval[out + i] = val[sslot[i]];
+        }
     }
     if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
 }
@@ -1544,9 +1615,9 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // A bucket with more distinct keys than its table holds flags its community,
 // which is then merged by the global-table kernels above instead.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kAggBucketTarget = 1024;
+constexpr uint32_t kAggBucketTarget = 512;
 constexpr uint32_t kAggMaxBuckets = 4096;
-constexpr int kAggBucketCap = 4096;  // LDS table slots of one bucket
+constexpr int kAggBucketCap = 2 * kAggBucketDistinct;  // LDS table slots of one bucket
 
 __device__ __forceinline__ uint32_t agg_bucket(uint32_t key, uint32_t nb, uint32_t nn) {
     return (uint32_t)(((unsigned long long)key * nb) / nn);
@@ -1671,7 +1742,9 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
     constexpr uint32_t DMAX = kAggBucketCap / 2;
     __shared__ uint32_t key[kAggBucketCap];
     __shared__ unsigned long long val[kAggBucketCap];
-    __shared__ uint32_t skeys[DMAX], sslot[DMAX];
+    __shared__ uint32_t dslot[DMAX];
+    __shared__ uint32_t skeys[DMAX], sslot[DMAX], oslot[DMAX];  // bins, or the sort
+    __shared__ uint32_t wsum[4], maxbin;
     __shared__ uint32_t nd_s, over;
     const uint32_t b = blockIdx.x;
     const uint32_t ci = p.bowner[b], c = comms[ci];
@@ -1687,30 +1760,42 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         over = 0;
     }
     __syncthreads();
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
-        const uint32_t k = p.pkey[q];
-        uint32_t slot = hslot<LC>(k);
-        int probe = 0;
-        for (; probe < kAggBucketCap; ++probe) {
-            const uint32_t prev = atomicCAS(&key[slot], kEmpty, k);
-            if (prev == kEmpty) {
-                const uint32_t idx = atomicAdd(&nd_s, 1u);
-                if (idx < p.dmax) {
-                    skeys[idx] = k;
-                    sslot[idx] = slot;
-                } else {
-                    over = 1u;  // more distinct keys than the sort holds
-                }
-                break;
+    // entries in batches of 8 per thread, loaded before any is inserted
+    for (uint32_t base = q0; base < q1; base += 256u * 8u) {
+        uint32_t kk[8];
+        float ww[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t q = base + threadIdx.x + 256u * u;
+            kk[u] = kEmpty;
+            if (q < q1) {
+                kk[u] = p.pkey[q];
+                ww[u] = p.pw[q];
             }
-            if (prev == k) break;
-            slot = (slot + 1) & (kAggBucketCap - 1);
         }
-        if (probe == kAggBucketCap) {
-            over = 1u;
-            continue;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            if (kk[u] == kEmpty) continue;
+            const uint32_t k = kk[u];
+            uint32_t slot = hslot<LC>(k);
+            int probe = 0;
+            for (; probe < kAggBucketCap; ++probe) {
+                const uint32_t prev = atomicCAS(&key[slot], kEmpty, k);
+                if (prev == kEmpty) {
+                    const uint32_t idx = atomicAdd(&nd_s, 1u);
+                    if (idx < p.dmax) dslot[idx] = slot;
+                    else over = 1u;  // more distinct keys than a bucket holds
+                    break;
+                }
+                if (prev == k) break;
+                slot = (slot + 1) & (kAggBucketCap - 1);
+            }
+            if (probe == kAggBucketCap) {
+                over = 1u;
+                continue;
+            }
+            atomicAdd(&val[slot], f2fix(ww[u], g.scale));
         }
-        atomicAdd(&val[slot], f2fix(p.pw[q], g.scale));
     }
     __syncthreads();
     if (over) {  // uniform: the community goes to the global-table merge
@@ -1718,18 +1803,31 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         return;
     }
     const uint32_t nd = nd_s;
-    uint32_t n2 = 1;
-    while (n2 < nd) n2 <<= 1;
-    for (uint32_t i = nd + threadIdx.x; i < n2; i += 256) {
-        skeys[i] = kEmpty;
-        sslot[i] = 0;
-    }
-    __syncthreads();
-    lds_bitonic<256>(skeys, sslot, n2);
     const uint32_t out = a.epos[c] + (q0 - p.bpos[p.bstart[ci]]);
-    for (uint32_t i = threadIdx.x; i < nd; i += 256) {
-        a.tkey[out + i] = skeys[i];
-        a.tval[out + i] = val[sslot[i]];
+    // the bucket's key range: keys k with floor(k * nb / nn) == bb
+    const unsigned long long nb = p.nbk[ci], bb = b - p.bstart[ci];
+    const uint32_t lo = (uint32_t)((bb * p.nn + nb - 1) / nb);
+    const uint32_t hi = (uint32_t)(((bb + 1) * p.nn + nb - 1) / nb);
+    auto keyof = [&](uint32_t i) { return key[dslot[i]]; };
+    auto emit = [&](uint32_t pos, uint32_t i) {
+        const uint32_t slot = dslot[i];
+        a.tkey[out + pos] = key[slot];
+        a.tval[out + pos] = val[slot];
+    };
+    if (!lds_bin_rank<DMAX, 256>(nd, lo, hi - lo, keyof, skeys, sslot, oslot, wsum, &maxbin,
+                                 emit)) {
+        uint32_t n2 = 1;
+        while (n2 < nd) n2 <<= 1;
+        for (uint32_t i = threadIdx.x; i < n2; i += 256) {
+            skeys[i] = i < nd ? key[dslot[i]] : kEmpty;
+            sslot[i] = i < nd ? dslot[i] : 0u;
+        }
+        __syncthreads();
+        lds_bitonic<256>(skeys, sslot, n2);
+        for (uint32_t i = threadIdx.x; i < nd; i += 256) {
+            a.tkey[out + i] = skeys[i];
+            a.tval[out + i] = val[sslot[i]];
+        }
     }
     if (threadIdx.x == 0) {
         p.bnd[b] = nd;
