@@ -149,11 +149,9 @@ def main():
         tuning, tms = g.autotune(MIN_GAIN)
         print(f"[bench] autotune: {tuning} ({tms:.2f} ms/run, search "
               f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    # warmup runs profile every kernel family to find the dominant one; the
-    # timed runs record events for that family only (an event record inside
-    # a captured sweep costs a few microseconds)
+    # warmup runs profile every kernel family to find the dominant one
     g.profile(True)
-    fam = None
+    fam, wst = None, None
     for i in range(args.warmup):
         g.reset()
         r = g.run(MIN_GAIN)
@@ -162,7 +160,9 @@ def main():
     if args.warmup:
         wst = g.profile_get()
         fam = max(wst, key=lambda k: wst[k]["ms"])
-        g.profile_select([fam])
+    # timed runs carry no events: an event record inside a captured sweep costs
+    # a few microseconds, a fifth of a run on the co-purchase graph
+    g.profile(False)
     g.profile_reset()
     times, edges, res = [], [], None
     bytes_alg = 0.0
@@ -176,11 +176,20 @@ def main():
         print(f"[bench] step {i}: {res.alg_ms:.2f} ms ({times[-1] * 1e3:.2f} ms wall) "
               f"Q={res.q_final:.6f} passes={res.passes} sweeps={res.total_iters} "
               f"communities={res.n_final}", file=sys.stderr, flush=True)
-    stats = g.profile_get()
-    g.profile(False)
     total_s = sum(times)
     value = sum(edges) / total_s
-    if fam is None:  # no warmup: every family was profiled in the timed runs
+    # one more run of the same workload with HIP events on the dominant family
+    # only (every family without a warmup): its launch durations give the
+    # roofline
+    g.profile(True)
+    g.profile_select([fam] if fam else None)
+    g.profile_reset()
+    g.reset()
+    pres = g.run(MIN_GAIN)
+    stats = g.profile_get()
+    g.profile(False)
+    print(f"[bench] profiled run: {pres.alg_ms:.2f} ms", file=sys.stderr, flush=True)
+    if fam is None:
         fam = max(stats, key=lambda k: stats[k]["ms"])
     st = stats[fam]
     avg_ms = st["ms"] / max(st["launches"], 1)
@@ -206,8 +215,9 @@ def main():
                    "tuning": tuning, "autotuned": bool(args.tune)},
         "roofline": {"bound": "hbm", "kernel": fam, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(cfg["workload"], fam,
-                                                 st["launches"] / args.steps)},
+                     "traffic": measured_traffic(cfg["workload"], fam, st["launches"]),
+                     "launches_per_run": int(st["launches"]),
+                     "avg_launch_ms": avg_ms, "bytes_per_launch": avg_bytes},
         # the boundary takes host buffers: one upload plus one run
         "pcie_inclusive": {"create_ms": create_s * 1e3,
                            "edges_per_s": (sum(edges) / args.steps) /
@@ -215,9 +225,10 @@ def main():
         "whole_run_hbm": {"bytes_alg_per_step": bytes_alg / args.steps,
                           "achieved_gbs": bytes_alg / total_s / 1e9,
                           "frac": bytes_alg / total_s / 1e9 / HBM_PEAK_GBS},
+        # per family over the warmup runs (every family profiled)
         "kernels": {k: {"ms": round(v["ms"], 4), "launches": int(v["launches"]),
                         "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
-                    for k, v in stats.items()},
+                    for k, v in (wst if args.warmup else stats).items()},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lv, args.config, graph)

@@ -258,6 +258,8 @@ void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t c
                      unsigned long long *caps);
 void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
                       uint32_t count, int radix_passes);
+// degree of every member in member order (mdeg) -- scanned into moff
+void launch_agg_mdeg(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
                            uint32_t count);
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,

@@ -117,6 +117,43 @@ __device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_
     }
 }
 
+// One FP32 gain per distinct candidate community (complete sums), written
+// over its exact sum: val[slot] <- bits of the gain, -inf for a candidate
+// isLowerIdx rejects.  Distinct slots dist[i], i = first, first + stride, ...
+// < nd; the community-state gathers of four candidates are issued together.
+__device__ __forceinline__ void distinct_gains(const DevView &g, const uint32_t *dist, uint32_t nd,
+                                               uint32_t first, uint32_t stride,
+                                               const uint32_t *key, unsigned long long *val,
+                                               float m, float kv, float acv, uint32_t szv,
+                                               uint32_t cv) {
+    constexpr int B = 4;
+    for (uint32_t i0 = first; i0 < nd; i0 += stride * B) {
+        uint32_t sl[B], c[B], sz[B];
+        unsigned long long a[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const uint32_t i = i0 + stride * b;
+            sl[b] = i < nd ? dist[i] : kEmpty;
+            c[b] = sl[b] != kEmpty ? key[sl[b]] : 0u;
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (sl[b] == kEmpty) continue;
+            a[b] = g.A[c[b]];
+            sz[b] = szv == 1u ? g.comm_sz[c[b]] : 0u;
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (sl[b] == kEmpty) continue;
+            // isLowerIdx (modularity.cu:85-93)
+            const bool cand = !(szv == 1u && sz[b] == 1u) || lid_of(g.lid, c[b]) < lid_of(g.lid, cv);
+            float gg = -INFINITY;
+            if (cand) gg = gain_of(fix2f(val[sl[b]], g.S), m, kv, acv, fix2f(a[b], g.S));
+            val[sl[b]] = (unsigned long long)__float_as_uint(gg);
+        }
+    }
+}
+
 // last entry visited by virtual lane wl of 128 on a row of d entries
 __device__ __forceinline__ uint32_t last_entry(uint32_t wl, uint32_t d) {
     return wl + 128u * ((d - 1u - wl) / 128u);

@@ -711,9 +711,7 @@ lv::AggView aview(lv_graph *g) {
 int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint32_t nl,
                     uint32_t nn, int passes, lv::AggPart *out) {
     hipStream_t s = g->stream;
-    const uint32_t n = g->n;
-    lv::launch_agg_large_prep(s, v, aview(g), lc, nl);  // member degrees, chunks
-    lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
+    lv::launch_agg_large_prep(s, v, aview(g), lc, nl);  // chunks per community
     lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
     lv::launch_aggp_nb(s, aview(g), lc, nl, g->ap_nbk);
     lv::launch_scan_u32(s, g->ap_nbk, 0, nl, g->ap_bstart, g->scan_tmp32);
@@ -848,6 +846,9 @@ int aggregation(lv_graph *g) {
         while (bits < 32 && (n > 1) && ((uint64_t)(n - 1) >> bits)) ++bits;
         passes = std::max(1, (int)((bits + 7) / 8));
     }
+    // member entry offsets in member order (merges walk flat member entries)
+    lv::launch_agg_mdeg(s, v, aview(g));
+    lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
     lv::AggView av = aview(g);
     for (int c = 0; c < lv::kAggGlobalClass; ++c)
         lv::launch_agg_merge(s, v, av, c, g->aorder + aoff[c], aoff[c + 1] - aoff[c], passes);

@@ -328,12 +328,22 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
     unsigned long long cew = 0;
-    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
-        const uint32_t e = g.col[x.base + j];
-        const uint32_t ce = g.comm[e];
-        h.eslot[x.base + j] = ce;
-        atomicAdd(&hist[bucket_of(ce, lb)], 1u);
-        if (ce == cv && e != x.v) cew += f2fix(g.w[x.base + j], g.scale);
+    uint32_t es[kPerThread], cs[kPerThread];  // loads first, then gathers
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        es[q] = j < j1 ? g.col[x.base + j] : kEmpty;
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q)
+        if (es[q] != kEmpty) cs[q] = g.comm[es[q]];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        if (es[q] == kEmpty) continue;
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        h.eslot[x.base + j] = cs[q];
+        atomicAdd(&hist[bucket_of(cs[q], lb)], 1u);
+        if (cs[q] == cv && es[q] != x.v) cew += f2fix(g.w[x.base + j], g.scale);
     }
     cew = wave_sum_u64(cew);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
@@ -509,15 +519,7 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     const uint32_t szv = g.comm_sz[cv];
     const float kv = g.k[v];
     const float acv = fix2f(g.A[cv], g.S);
-    for (uint32_t q = threadIdx.x; q < nd; q += 256) {
-        const uint32_t slot = dist[q];
-        const uint32_t c = key[slot];
-        float gg = -INFINITY;
-        // isLowerIdx (modularity.cu:85-93)
-        const bool cand = lower_idx_ok(g, szv, c, cv);
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
-        val[slot] = (unsigned long long)__float_as_uint(gg);
-    }
+    distinct_gains(g, dist, nd, threadIdx.x, 256u, key, val, m, kv, acv, szv, cv);
     __syncthreads();
     auto lane_max = [&](uint32_t slot, uint32_t j) {
         const float gg = __uint_as_float((uint32_t)val[slot]);
@@ -671,9 +673,22 @@ __global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, fl
     const uint32_t cv = g.comm[x.v];
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
-    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
-        const float wv = g.w[x.base + j];
-        stage[x.base + j] = (g.comm[g.col[x.base + j]] == cv) ? wv : 0.0f;
+    uint32_t es[kPerThread];  // loads first, then gathers
+    float ws[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        es[q] = kEmpty;
+        if (j < j1) {
+            es[q] = g.col[x.base + j];
+            ws[q] = g.w[x.base + j];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        if (es[q] == kEmpty) continue;
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        stage[x.base + j] = (g.comm[es[q]] == cv) ? ws[q] : 0.0f;
     }
 }
 

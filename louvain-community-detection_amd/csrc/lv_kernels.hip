@@ -306,20 +306,28 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
         float p = 0.0f;
         uint32_t j = lane;
         if (G == 32) {
-            for (; j + 7 * 32 < d; j += 8 * 32) {
+            // 8 rounds per batch: loads, then gathers, then the adds in order
+            for (; j < d; j += 8 * 32) {
                 float x[8];
+                uint32_t e[8];
                 bool keep[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    x[u] = wr[j + 32 * u];
-                    keep[u] = MASKED ? (g.comm[cr[j + 32 * u]] == cv) : true;
+                    const uint32_t jj = j + 32u * u;
+                    keep[u] = jj < d;
+                    x[u] = keep[u] ? wr[jj] : 0.0f;
+                    if (MASKED) e[u] = keep[u] ? cr[jj] : 0u;
+                }
+                if (MASKED) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) keep[u] = keep[u] && g.comm[e[u]] == cv;
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (keep[u]) p += x[u];
             }
         }
-        for (; j < d; j += 32) {
+        for (; j < d; j += 32) {  // G < 32: rows of at most G entries, one round
             float x = wr[j];
             if (!MASKED || g.comm[cr[j]] == cv) p += x;
         }
@@ -470,15 +478,38 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     float best_g = -INFINITY;
     uint32_t best_c = kEmpty;
     unsigned long long cew = 0;
+    // every round's loads are issued before the first insert (latency overlap)
+    uint32_t er[ROUNDS], cr[ROUNDS];
+    float wr[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const uint32_t j = (uint32_t)(r * T + lane);
-        const bool has = active && j < d;
+        er[r] = kEmpty;
+        if (active && j < d) {
+            er[r] = g.col[base + j];
+            wr[r] = g.w[base + j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+        if (er[r] != kEmpty) cr[r] = g.comm[er[r]];
+    // the candidates' community state (read-only until the step applies)
+    unsigned long long ar[ROUNDS];
+    uint32_t zr[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (er[r] == kEmpty) continue;
+        ar[r] = g.A[cr[r]];
+        zr[r] = szv == 1u ? g.comm_sz[cr[r]] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const bool has = er[r] != kEmpty;
         uint32_t e = 0, ce = 0, slot = 0;
         if (has) {
-            e = g.col[base + j];
-            ce = g.comm[e];
-            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            e = er[r];
+            ce = cr[r];
+            unsigned long long wf = f2fix(wr[r], g.scale);
             slot = hslot<LC>(ce);
             for (int probe = 0; probe < CAP; ++probe) {
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
@@ -491,9 +522,9 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         __syncthreads();
         if (has) {
             // isLowerIdx (modularity.cu:85-93)
-            const bool cand = lower_idx_ok(g, szv, ce, cv);
+            const bool cand = !(szv == 1u && zr[r] == 1u) || lid_of(g.lid, ce) < lid_of(g.lid, cv);
             if (cand) {
-                float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[ce], g.S));
+                float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(ar[r], g.S));
                 if (gg > best_g) {
                     best_g = gg;
                     best_c = ce;
@@ -564,14 +595,28 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
     __syncthreads();
     uint32_t eslot[kWaveMaxPer];
     unsigned long long cew = 0;
+    // all loads of the lane's entries first, then their community gathers
+    uint32_t ei[kWaveMaxPer], ci[kWaveMaxPer];
+    float wi[kWaveMaxPer];
 #pragma unroll
     for (int i = 0; i < kWaveMaxPer; ++i) {
         const uint32_t j = p + 64u * i;
-        eslot[i] = kEmpty;
+        ei[i] = kEmpty;
         if (active && j < d) {
-            uint32_t e = g.col[base + j];
-            uint32_t ce = g.comm[e];
-            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            ei[i] = g.col[base + j];
+            wi[i] = g.w[base + j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kWaveMaxPer; ++i)
+        if (ei[i] != kEmpty) ci[i] = g.comm[ei[i]];
+#pragma unroll
+    for (int i = 0; i < kWaveMaxPer; ++i) {
+        eslot[i] = kEmpty;
+        if (ei[i] != kEmpty) {
+            const uint32_t e = ei[i];
+            const uint32_t ce = ci[i];
+            unsigned long long wf = f2fix(wi[i], g.scale);
             uint32_t slot = hslot<LC>(ce);
             for (int probe = 0; probe < kWaveCap; ++probe) {
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
@@ -592,15 +637,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
     __syncthreads();
     // one gain per distinct community (complete sums), stored over the sum
     const uint32_t nd = active ? sndist[wv] : 0u;
-    for (uint32_t i = p; i < nd; i += 64) {
-        uint32_t slot = sdist[wv][i];
-        uint32_t c = key[slot];
-        float gg = -INFINITY;
-        // isLowerIdx (modularity.cu:85-93)
-        const bool cand = lower_idx_ok(g, szv, c, cv);
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
-        val[slot] = (unsigned long long)__float_as_uint(gg);
-    }
+    distinct_gains(g, sdist[wv], nd, p, 64u, key, val, m, kv, acv, szv, cv);
     __syncthreads();
     float g0 = -INFINITY, g1 = -INFINITY;
     uint32_t c0 = kEmpty, c1 = kEmpty;
@@ -700,14 +737,28 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     __syncthreads();
     uint32_t eslot[MAXPER];
     unsigned long long cew = 0;
+    // all loads of the thread's entries first, then their community gathers
+    uint32_t ei[MAXPER], ci[MAXPER];
+    float wi[MAXPER];
 #pragma unroll
     for (int i = 0; i < MAXPER; ++i) {
         const uint32_t j = t + (uint32_t)P * i;
-        eslot[i] = kEmpty;
+        ei[i] = kEmpty;
         if (j < d) {
-            uint32_t e = g.col[base + j];
-            uint32_t ce = g.comm[e];
-            unsigned long long wf = f2fix(g.w[base + j], g.scale);
+            ei[i] = g.col[base + j];
+            wi[i] = g.w[base + j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i)
+        if (ei[i] != kEmpty) ci[i] = g.comm[ei[i]];
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i) {
+        eslot[i] = kEmpty;
+        if (ei[i] != kEmpty) {
+            const uint32_t e = ei[i];
+            const uint32_t ce = ci[i];
+            unsigned long long wf = f2fix(wi[i], g.scale);
             uint32_t slot = hslot<LC>(ce);
             for (int probe = 0; probe < CAP; ++probe) {
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
@@ -727,15 +778,7 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     if ((t & 63) == 0 && cew) atomicAdd(&scew, cew);
     __syncthreads();
     const uint32_t nd = ndist;
-    for (uint32_t i = t; i < nd; i += P) {
-        uint32_t slot = dist[i];
-        uint32_t c = key[slot];
-        float gg = -INFINITY;
-        // isLowerIdx (modularity.cu:85-93)
-        const bool cand = lower_idx_ok(g, szv, c, cv);
-        if (cand) gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[c], g.S));
-        val[slot] = (unsigned long long)__float_as_uint(gg);
-    }
+    distinct_gains(g, dist, nd, t, (uint32_t)P, key, val, m, kv, acv, szv, cv);
     __syncthreads();
     float tg = -INFINITY;
     uint32_t tc = kEmpty, tj = kEmpty;
@@ -1160,42 +1203,76 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
                                                      uint32_t count) {
     constexpr int LC = Log2<CAP>::value;
     constexpr int NW = P / 64;
+    constexpr uint32_t MAXE = CAP / 2;  // member entries of the class (comm_deg <= CAP/2)
+    constexpr uint32_t PER = (MAXE + P - 1) / P;
     __shared__ uint32_t key[CAP];
     __shared__ unsigned long long val[CAP];
     constexpr uint32_t NBIN = CAP / 4;  // bins for the ranked order
     __shared__ uint32_t skeys[CAP / 2];
     __shared__ uint32_t sslot[CAP / 2];
-    __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[CAP / 2];
+    __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[CAP / 2 + 1];
     __shared__ uint32_t ndist, wsum[NW], maxbin;
+    uint32_t *mb = oidx;  // member entry offsets while merging, ranks afterwards
     const uint32_t c = comms[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < CAP; i += P) {
         key[i] = kEmpty;
         val[i] = 0ull;
     }
     if (threadIdx.x == 0) ndist = 0;
+    // the community's member entries, flat: members have degree >= 1, so at
+    // most MAXE of them; offsets relative to the first
+    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c], ne = a.comm_deg[c];
+    const uint32_t f0 = a.moff[m0];
+    for (uint32_t q = threadIdx.x; q <= msz; q += P) mb[q] = a.moff[m0 + q] - f0;
     __syncthreads();
-    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t mi = wv; mi < msz; mi += NW) {
-        const uint32_t v = a.members[m0 + mi];
-        const uint32_t b = g.row[v], e = g.row[v + 1];
-        for (uint32_t j = b + lane; j < e; j += 64) {
-            uint32_t dd = g.comm[g.col[j]];
-            unsigned long long wf = f2fix(g.w[j], g.scale);
-            uint32_t slot = hslot<LC>(dd);
-            for (int probe = 0; probe < CAP; ++probe) {
-                uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
-                if (prev == kEmpty) {
-                    uint32_t idx = atomicAdd(&ndist, 1u);
-                    skeys[idx] = a.new_id[dd];  // sorted by new id
-                    sslot[idx] = slot;
-                    break;
-                }
-                if (prev == dd) break;
-                slot = (slot + 1) & (CAP - 1);
+    // every entry's loads before any insert: member, row start, column and
+    // weight, neighbour community
+    uint32_t vv[PER], jj[PER], dd[PER];
+    float ww[PER];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        const uint32_t r = threadIdx.x + (uint32_t)P * u;
+        jj[u] = kEmpty;
+        if (r < ne) {
+            uint32_t l = 0, h = msz;  // member of flat entry r
+            while (h - l > 1) {
+                const uint32_t mid = (l + h) >> 1;
+                if (mb[mid] <= r) l = mid;
+                else h = mid;
             }
-            atomicAdd(&val[slot], wf);
+            vv[u] = a.members[m0 + l];
+            jj[u] = r - mb[l];
         }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) jj[u] += g.row[vv[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) {
+            dd[u] = g.col[jj[u]];
+            ww[u] = g.w[jj[u]];
+        }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) dd[u] = g.comm[dd[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        if (jj[u] == kEmpty) continue;
+        const uint32_t d2 = dd[u];
+        uint32_t slot = hslot<LC>(d2);
+        for (int probe = 0; probe < CAP; ++probe) {
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, d2);
+            if (prev == kEmpty) {
+                const uint32_t idx = atomicAdd(&ndist, 1u);
+                skeys[idx] = a.new_id[d2];  // sorted by new id
+                sslot[idx] = slot;
+                break;
+            }
+            if (prev == d2) break;
+            slot = (slot + 1) & (CAP - 1);
+        }
+        atomicAdd(&val[slot], f2fix(ww[u], g.scale));
     }
     __syncthreads();
     const uint32_t nd = ndist;
@@ -1591,10 +1668,12 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
     }
 }
 
-void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
-                           uint32_t count) {
+void launch_agg_mdeg(hipStream_t s, const DevView &g, AggView a) {
     if (g.n)
         hipLaunchKernelGGL(k_agg_mdeg, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a);
+}
+void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
+                           uint32_t count) {
     if (count)
         hipLaunchKernelGGL(k_agg_chunk_counts, dim3((count + 255) / 256), dim3(256), 0, s, a,
                            comms, count);
@@ -1649,12 +1728,48 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
     for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) hist[b] = 0;
     const AggChunk x = agg_chunk(g, a, comms, count, blockIdx.x, mb, sh);
     const uint32_t nb = p.nbk[x.ci];
-    for (uint32_t f = x.f0 + threadIdx.x; f < x.f1; f += 256) {
-        const uint32_t j = agg_entry(g, a, x, mb, f);
-        const uint32_t key = a.new_id[g.comm[g.col[j]]];
-        p.fkey[f] = key;
-        p.fw[f] = g.w[j];
-        atomicAdd(&hist[agg_bucket(key, nb, p.nn)], 1u);
+    constexpr uint32_t PER = kAggChunk / 256;
+    // each dependent load level for all of the thread's entries at once:
+    // member, row start, column + weight, community, new id
+    uint32_t vv[PER], jj[PER], kk[PER];
+    float ww[PER];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        const uint32_t f = x.f0 + threadIdx.x + 256u * u;
+        jj[u] = kEmpty;
+        if (f < x.f1) {
+            uint32_t l = 0, h = x.nm;  // member of flat entry f
+            while (h - l > 1) {
+                const uint32_t mid = (l + h) >> 1;
+                if (mb[mid] <= f) l = mid;
+                else h = mid;
+            }
+            vv[u] = a.members[x.i0 + l];
+            jj[u] = f - mb[l];
+        }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) jj[u] += g.row[vv[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) {
+            kk[u] = g.col[jj[u]];
+            ww[u] = g.w[jj[u]];
+        }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) kk[u] = g.comm[kk[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        if (jj[u] != kEmpty) kk[u] = a.new_id[kk[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        if (jj[u] == kEmpty) continue;
+        const uint32_t f = x.f0 + threadIdx.x + 256u * u;
+        p.fkey[f] = kk[u];
+        p.fw[f] = ww[u];
+        atomicAdd(&hist[agg_bucket(kk[u], nb, p.nn)], 1u);
     }
     __syncthreads();
     const uint32_t b0 = p.bstart[x.ci];
