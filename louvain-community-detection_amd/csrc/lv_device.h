@@ -112,7 +112,8 @@ struct HeavySplit {
     float *pw;                           //                    weight
     uint32_t *hflag;                     // nh: LDS overflow, redo on global tables
     const uint32_t *only;                // non-null: kernels run for flagged vertices only
-    int force_global;                    // test hook: skip the partitioned path
+    int force_global;                    // test hook: 1 global tables only, 2 mixed
+    uint32_t dmax;                       // distinct communities a bucket may hold (test hook)
     uint32_t *scan_tmp;
     uint32_t *err;
 };
@@ -239,7 +240,7 @@ struct AggPart {
     uint32_t NB;             // total buckets
     uint32_t dmax;           // distinct keys a bucket may hold (<= kAggBucketDistinct)
 };
-constexpr uint32_t kAggBucketDistinct = 1024;  // per bucket; its LDS table has twice the slots
+constexpr uint32_t kAggBucketDistinct = 512;  // per bucket; its LDS table has twice the slots
 void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
                     uint32_t *nbk);
 // phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities

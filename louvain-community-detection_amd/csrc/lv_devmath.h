@@ -120,13 +120,13 @@ __device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_
 // One FP32 gain per distinct candidate community (complete sums), written
 // over its exact sum: val[slot] <- bits of the gain, -inf for a candidate
 // isLowerIdx rejects.  Distinct slots dist[i], i = first, first + stride, ...
-// < nd; the community-state gathers of four candidates are issued together.
+// < nd.
 __device__ __forceinline__ void distinct_gains(const DevView &g, const uint32_t *dist, uint32_t nd,
                                                uint32_t first, uint32_t stride,
                                                const uint32_t *key, unsigned long long *val,
                                                float m, float kv, float acv, uint32_t szv,
                                                uint32_t cv) {
-    constexpr int B = 4;
+    constexpr int B = 1;  // batching the gathers measured slower
     for (uint32_t i0 = first; i0 < nd; i0 += stride * B) {
         uint32_t sl[B], c[B], sz[B];
         unsigned long long a[B];

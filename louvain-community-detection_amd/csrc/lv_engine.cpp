@@ -424,8 +424,14 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.pw = reinterpret_cast<float *>(g->tval);
     h.hflag = g->hp_flag;
     h.only = nullptr;
-    static const bool force_global = getenv("LV_HEAVY_GLOBAL") != nullptr;
-    h.force_global = force_global ? 1 : 0;
+    // test hook LV_HEAVY_GLOBAL: 1 every heavy vertex on the global tables,
+    // 2 the odd ones after the partitioned path (both paths in one step)
+    static const char *hg = getenv("LV_HEAVY_GLOBAL");
+    h.force_global = hg ? (atoi(hg) == 2 ? 2 : 1) : 0;
+    // test hook LV_HEAVY_DMAX: fewer distinct communities per LDS bucket, so
+    // buckets overflow and their vertices fall back to the global tables
+    static const char *hd = getenv("LV_HEAVY_DMAX");
+    h.dmax = (hd && atoi(hd) > 0) ? (uint32_t)atoi(hd) : 0xffffffffu;
     h.scan_tmp = g->hp_btmp;
     h.err = &g->dc->err;
     return h;

@@ -439,14 +439,14 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
 template <int kBucketCap>
 __device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long *val, uint32_t *dist,
                                               uint32_t *nd_s, uint32_t *over, uint32_t c,
-                                              unsigned long long wf) {
+                                              unsigned long long wf, uint32_t dmax) {
     constexpr int LC = Log2<kBucketCap>::value;
     uint32_t slot = pslot(c, LC);
     for (int probe = 0; probe < kBucketCap; ++probe) {
         const uint32_t prev = atomicCAS(&key[slot], kEmpty, c);
         if (prev == kEmpty) {
             const uint32_t idx = atomicAdd(nd_s, 1u);
-            if (idx < kBucketCap / 2) dist[idx] = slot;
+            if (idx < dmax) dist[idx] = slot;
             else *over = 1u;  // more distinct communities than the table allows
             atomicAdd(&val[slot], wf);
             return slot;
@@ -483,6 +483,7 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         nd_s = 0;
         over = 0;
     }
+    const uint32_t dmax = min(h.dmax, (uint32_t)kBucketCap / 2);
     __syncthreads();
     constexpr uint32_t R = 4;
     const bool cached = p1 - p0 <= 256u * R;  // uniform
@@ -503,10 +504,11 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         for (uint32_t u = 0; u < R; ++u)
             ss[u] = cs[u] == kEmpty ? kEmpty
                                     : hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, cs[u],
-                                                            f2fix(ws[u], g.scale));
+                                                            f2fix(ws[u], g.scale), dmax);
     } else {
         for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256)
-            hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, h.pc[q], f2fix(h.pw[q], g.scale));
+            hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, h.pc[q], f2fix(h.pw[q], g.scale),
+                                  dmax);
     }
     __syncthreads();
     if (over) {  // uniform: redo this vertex with the global-table kernels
@@ -565,6 +567,13 @@ void launch_hp_nbuck(hipStream_t s, const uint32_t *row, const uint32_t *verts, 
                        target, nbuck);
 }
 
+// test hook: every odd heavy vertex also takes the global-table path, so
+// both paths serve one step
+__global__ void k_hp_flag_odd(HeavySplit h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < h.nh && (i & 1u)) h.hflag[i] = 1u;
+}
+
 // flagged vertices restart on the global-table kernels
 __global__ void k_hp_reset_flagged(HeavySplit h) {
     const uint32_t i = blockIdx.x;
@@ -604,7 +613,7 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
     h.only = nullptr;
     const dim3 grid(h.nchunks), block(256);
     launch_fill_u64(s, h.hcnt, 0ull, (size_t)h.nh * 130);  // hcnt | hcew | hlane
-    if (h.force_global) {  // every heavy vertex on the global tables
+    if (h.force_global == 1) {  // every heavy vertex on the global tables
         launch_fill_u32(s, h.hflag, 1u, h.nh);
     } else {
         launch_fill_u32(s, h.hflag, 0u, h.nh);
@@ -619,6 +628,8 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
             hipLaunchKernelGGL(k_hp_bucket<2048>, dim3(h.nbuckets), block, 0, s, g, h);
         else
             hipLaunchKernelGGL(k_hp_bucket<4096>, dim3(h.nbuckets), block, 0, s, g, h);
+        if (h.force_global == 2)
+            hipLaunchKernelGGL(k_hp_flag_odd, dim3((h.nh + 255) / 256), dim3(256), 0, s, h);
     }
     // overflowing vertices (if any) redo the step on the global tables
     hipLaunchKernelGGL(k_hp_reset_flagged, dim3(h.nh), dim3(128), 0, s, h);

@@ -306,28 +306,20 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
         float p = 0.0f;
         uint32_t j = lane;
         if (G == 32) {
-            // 8 rounds per batch: loads, then gathers, then the adds in order
-            for (; j < d; j += 8 * 32) {
+            for (; j + 7 * 32 < d; j += 8 * 32) {
                 float x[8];
-                uint32_t e[8];
                 bool keep[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const uint32_t jj = j + 32u * u;
-                    keep[u] = jj < d;
-                    x[u] = keep[u] ? wr[jj] : 0.0f;
-                    if (MASKED) e[u] = keep[u] ? cr[jj] : 0u;
-                }
-                if (MASKED) {
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) keep[u] = keep[u] && g.comm[e[u]] == cv;
+                    x[u] = wr[j + 32 * u];
+                    keep[u] = MASKED ? (g.comm[cr[j + 32 * u]] == cv) : true;
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (keep[u]) p += x[u];
             }
         }
-        for (; j < d; j += 32) {  // G < 32: rows of at most G entries, one round
+        for (; j < d; j += 32) {
             float x = wr[j];
             if (!MASKED || g.comm[cr[j]] == cv) p += x;
         }
@@ -493,15 +485,6 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
         if (er[r] != kEmpty) cr[r] = g.comm[er[r]];
-    // the candidates' community state (read-only until the step applies)
-    unsigned long long ar[ROUNDS];
-    uint32_t zr[ROUNDS];
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-        if (er[r] == kEmpty) continue;
-        ar[r] = g.A[cr[r]];
-        zr[r] = szv == 1u ? g.comm_sz[cr[r]] : 0u;
-    }
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const bool has = er[r] != kEmpty;
@@ -522,9 +505,9 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         __syncthreads();
         if (has) {
             // isLowerIdx (modularity.cu:85-93)
-            const bool cand = !(szv == 1u && zr[r] == 1u) || lid_of(g.lid, ce) < lid_of(g.lid, cv);
+            const bool cand = lower_idx_ok(g, szv, ce, cv);
             if (cand) {
-                float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(ar[r], g.S));
+                float gg = gain_of(fix2f(val[slot], g.S), m, kv, acv, fix2f(g.A[ce], g.S));
                 if (gg > best_g) {
                     best_g = gg;
                     best_c = ce;
@@ -839,7 +822,7 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                count);
             break;
         case 10:
-            hipLaunchKernelGGL((k_move_block<512, 8192>), dim3(count), dim3(512), 0, s, g, verts,
+            hipLaunchKernelGGL((k_move_block<1024, 8192>), dim3(count), dim3(1024), 0, s, g, verts,
                                count);
             break;
         default:
@@ -1694,7 +1677,7 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // A bucket with more distinct keys than its table holds flags its community,
 // which is then merged by the global-table kernels above instead.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kAggBucketTarget = 512;
+constexpr uint32_t kAggBucketTarget = 256;
 constexpr uint32_t kAggMaxBuckets = 4096;
 constexpr int kAggBucketCap = 2 * kAggBucketDistinct;  // LDS table slots of one bucket
 

@@ -299,10 +299,13 @@ def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
     assert all(np.array_equal(x, y) for x, y in zip(a1, a2))
 
 
-def test_heavy_global_table_path(lv, orc):
+@pytest.mark.parametrize("mode", ["1", "2", "dmax"])
+def test_heavy_global_table_path(lv, orc, mode):
     """The global-table kernels that take over heavy vertices whose hash
-    buckets overflow LDS, forced for every heavy vertex (LV_HEAVY_GLOBAL),
-    in a child process so the setting does not leak."""
+    buckets overflow LDS: forced for every heavy vertex (LV_HEAVY_GLOBAL=1),
+    for every other one after the partitioned path ran (=2), and through
+    buckets that really overflow a reduced capacity (LV_HEAVY_DMAX) so both
+    paths serve one step, in a child process so the setting does not leak."""
     import os
     import subprocess
     import sys
@@ -310,14 +313,17 @@ def test_heavy_global_table_path(lv, orc):
         "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
         "from loader import load_package; import oracle\n"
         "lv = load_package()\n"
-        "r, c, w = lv.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 13)\n"
-        "o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
-        "g = lv.Graph(r, c, w); res = g.run(1e-6)\n"
-        "assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32)\n"
-        "assert np.array_equal(g.assignment(), o.init_comm())\n"
+        "for sc, tune in [(16, {}), (18, {}), (18, {'bucket_entries': 64})]:\n"
+        "    r, c, w = lv.gen_rmat(sc, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 13)\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    g = lv.Graph(r, c, w); g.set_tuning(**tune); res = g.run(1e-6)\n"
+        "    assert np.float32(res.q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32), (sc, tune)\n"
+        "    assert np.array_equal(g.assignment(), o.init_comm()), (sc, tune)\n"
         "print('global-table path ok')\n"
     ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
-    env = dict(os.environ, LV_HEAVY_GLOBAL="1")
+    env = dict(os.environ, **({"LV_HEAVY_DMAX": "48"} if mode == "dmax" else
+                              {"LV_HEAVY_GLOBAL": mode}))
+    print(f"[case] heavy global tables mode {mode}", flush=True)
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                        timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
