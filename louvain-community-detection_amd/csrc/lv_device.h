@@ -161,9 +161,11 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
                              uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
 // pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v.
-// `acc` receives the exact fixed-point total (atomic).
+// `group` lanes per vertex (4/8/16/32); with 32 lanes, `batch` (4 or 8)
+// rounds of loads are in flight at once.  `acc` receives the exact
+// fixed-point total (atomic).
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
-               int group, int masked, int init_state, unsigned long long *acc);
+               int group, int batch, int masked, int init_state, unsigned long long *acc);
 // sum of round32(a_c)^2 over the communities in `verts`; limbs: 4 arrays of
 // kSpread (single graph) or G (batch) slots
 void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,

@@ -126,7 +126,10 @@ __device__ __forceinline__ void distinct_gains(const DevView &g, const uint32_t 
                                                const uint32_t *key, unsigned long long *val,
                                                float m, float kv, float acv, uint32_t szv,
                                                uint32_t cv) {
-    constexpr int B = 1;  // batching the gathers measured slower
+    // B distinct communities per thread per round; gathers are unconditional
+    // (empty positions gather the vertex's own community).  B = 1 measured
+    // fastest (2 and 4 were 0.3 % and 1.1 % slower end to end on R-MAT-24).
+    constexpr int B = 1;
     for (uint32_t i0 = first; i0 < nd; i0 += stride * B) {
         uint32_t sl[B], c[B], sz[B];
         unsigned long long a[B];
@@ -134,13 +137,14 @@ __device__ __forceinline__ void distinct_gains(const DevView &g, const uint32_t 
         for (int b = 0; b < B; ++b) {
             const uint32_t i = i0 + stride * b;
             sl[b] = i < nd ? dist[i] : kEmpty;
-            c[b] = sl[b] != kEmpty ? key[sl[b]] : 0u;
+            c[b] = sl[b] != kEmpty ? key[sl[b]] : cv;
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            if (sl[b] == kEmpty) continue;
             a[b] = g.A[c[b]];
-            sz[b] = szv == 1u ? g.comm_sz[c[b]] : 0u;
+            // only isLowerIdx of a singleton needs the size: otherwise every
+            // lane reads the vertex's own entry (one line per vertex)
+            sz[b] = g.comm_sz[szv == 1u ? c[b] : cv];
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {

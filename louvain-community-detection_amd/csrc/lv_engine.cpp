@@ -438,18 +438,21 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 }
 
 // groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
-// 16 for 3, 32 for 4-8 (contiguous in `order`)
+// 16 for 3, 32 for 4-10 (contiguous in `order`); rows of classes 4-5 (at
+// most 84 entries) take at most 3 rounds per lane, longer rows batch 8
 int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
     const struct {
-        int G, c0, c1;
-    } grp[4] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, lv::kClassHeavy - 1}};
+        int G, B, c0, c1;
+    } grp[5] = {{4, 1, 0, 1}, {8, 1, 2, 2}, {16, 1, 3, 3}, {32, 4, 4, 5},
+                {32, 8, 6, lv::kClassHeavy - 1}};
     for (const auto &q : grp) {
         uint32_t a = g->cls_off[q.c0], b = g->cls_off[q.c1 + 1];
         double ent = 0;
         for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
-        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, masked, init_state, acc_ptr(g));
+        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, q.B, masked, init_state,
+                      acc_ptr(g));
     }
     if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy]);

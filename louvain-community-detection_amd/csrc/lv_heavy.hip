@@ -328,22 +328,26 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
     unsigned long long cew = 0;
-    uint32_t es[kPerThread], cs[kPerThread];  // loads first, then gathers
+    // every load of the chunk first (past the chunk end: its first entry),
+    // then every gather (past the end: the vertex itself), then the LDS work
+    uint32_t es[kPerThread], cs[kPerThread];
+    float ws[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        es[q] = j < j1 ? g.col[x.base + j] : kEmpty;
+        const uint32_t e = x.base + (j < j1 ? j : j0);
+        es[q] = g.col[e];
+        ws[q] = g.w[e];
     }
 #pragma unroll
-    for (int q = 0; q < kPerThread; ++q)
-        if (es[q] != kEmpty) cs[q] = g.comm[es[q]];
+    for (int q = 0; q < kPerThread; ++q) cs[q] = g.comm[es[q]];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
-        if (es[q] == kEmpty) continue;
         const uint32_t j = j0 + threadIdx.x + 256u * q;
+        if (j >= j1) continue;
         h.eslot[x.base + j] = cs[q];
         atomicAdd(&hist[bucket_of(cs[q], lb)], 1u);
-        if (cs[q] == cv && es[q] != x.v) cew += f2fix(g.w[x.base + j], g.scale);
+        if (cs[q] == cv && es[q] != x.v) cew += f2fix(ws[q], g.scale);
     }
     cew = wave_sum_u64(cew);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cew;
@@ -378,12 +382,20 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
     const uint32_t cnt = j1 - j0;
     uint32_t rank[kPerThread], bk[kPerThread], cc[kPerThread];
+    float wq[kPerThread];
+    // all loads first (past the chunk end: its first entry), then the ranks
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        const uint32_t e = x.base + (j < j1 ? j : j0);
+        cc[q] = h.eslot[e];
+        wq[q] = g.w[e];
+    }
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
         bk[q] = kEmpty;
         if (j < j1) {
-            cc[q] = h.eslot[x.base + j];
             bk[q] = bucket_of(cc[q], lb);
             rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
         }
@@ -419,7 +431,7 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
         const uint32_t sp = loff[bk[q]] + rank[q];
         sc[sp] = cc[q];
         sj[sp] = j;
-        sw[sp] = g.w[x.base + j];
+        sw[sp] = wq[q];
         sb[sp] = (uint16_t)bk[q];
     }
     __syncthreads();
@@ -491,15 +503,16 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     if (cached) {
         float ws[R];
 #pragma unroll
-        for (uint32_t u = 0; u < R; ++u) {  // all loads first
-            const uint32_t q = p0 + threadIdx.x + 256u * u;
-            cs[u] = kEmpty;
-            if (q < p1) {
-                cs[u] = h.pc[q];
-                ws[u] = h.pw[q];
-                js[u] = h.pj[q];
-            }
+        for (uint32_t u = 0; u < R; ++u) {  // all loads first, unconditional
+            const uint32_t qu = p0 + threadIdx.x + 256u * u;
+            const uint32_t q = qu < p1 ? qu : p0;
+            cs[u] = h.pc[q];
+            ws[u] = h.pw[q];
+            js[u] = h.pj[q];
         }
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u)
+            if (p0 + threadIdx.x + 256u * u >= p1) cs[u] = kEmpty;
 #pragma unroll
         for (uint32_t u = 0; u < R; ++u)
             ss[u] = cs[u] == kEmpty ? kEmpty
@@ -650,10 +663,27 @@ __global__ __launch_bounds__(256) void k_relabel_heavy(RelabelView r, const uint
     const uint32_t c = blockIdx.x - coff[i];
     const uint32_t x = x0 + i, v = r.lid[x];
     const uint32_t b = r.row[v], d = r.row[v + 1] - b, o = r.row2[x];
-    const uint32_t j1 = min(d, (c + 1) * kHeavyChunk);
-    for (uint32_t j = c * kHeavyChunk + threadIdx.x; j < j1; j += 256) {
-        r.col2[o + j] = r.pmap[r.col[b + j]];
-        r.w2[o + j] = r.w[b + j];
+    const uint32_t j0 = c * kHeavyChunk, j1 = min(d, j0 + kHeavyChunk);
+    // all loads, then all gathers, then the stores (past the chunk end: its
+    // first entry again)
+    uint32_t cs[kPerThread];
+    float ws[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        const uint32_t e = b + (j < j1 ? j : j0);
+        cs[q] = r.col[e];
+        ws[q] = r.w[e];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) cs[q] = r.pmap[cs[q]];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        if (j < j1) {
+            r.col2[o + j] = cs[q];
+            r.w2[o + j] = ws[q];
+        }
     }
 }
 void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *cvtx,
@@ -695,16 +725,22 @@ __global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, fl
             ws[q] = g.w[x.base + j];
         }
     }
+    // unconditional gathers (past the chunk end: the vertex itself), so all
+    // of them are in flight before the first compare
+    uint32_t cs[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) cs[q] = g.comm[es[q] != kEmpty ? es[q] : x.v];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         if (es[q] == kEmpty) continue;
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        stage[x.base + j] = (g.comm[es[q]] == cv) ? ws[q] : 0.0f;
+        stage[x.base + j] = (cs[q] == cv) ? ws[q] : 0.0f;
     }
 }
 
-constexpr int kR1K = 128;             // entries per lane per chunk
-constexpr int kR1Chunk = 32 * kR1K;   // 4096 entries
+constexpr int kR1K = 192;                 // entries per lane per chunk
+constexpr int kR1Chunk = 32 * kR1K;       // 6144 entries
+constexpr int kR1Per = kR1Chunk / 192;    // loads per loader thread and chunk
 
 __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const float *stage,
                                                   int masked, int init_state,
@@ -713,14 +749,25 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
     const uint32_t v = h.verts[blockIdx.x];
     if (!v_live(g, v)) return;  // uniform over the workgroup
     const uint32_t base = g.row[v];
-    const uint32_t d = g.row[v + 1] - base;
+    const uint32_t d = g.row[v + 1] - base;  // > 0: a heavy row
     const float *src = masked ? stage + base : g.w + base;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t nch = (d + kR1Chunk - 1) / kR1Chunk;
+    // loader waves 1-3: all of a thread's loads in flight at once, then the
+    // transposed LDS stores (entry 32 i + l of the chunk -> buf[l][i])
     auto load = [&](uint32_t c, int b) {
-        for (int idx = (int)threadIdx.x - 64; idx < kR1Chunk; idx += 192) {
+        const int t = (int)threadIdx.x - 64;
+        float x[kR1Per];
+#pragma unroll
+        for (int i = 0; i < kR1Per; ++i) {
+            const uint32_t j = c * kR1Chunk + (uint32_t)(t + 192 * i);
+            x[i] = src[j < d ? j : d - 1];
+        }
+#pragma unroll
+        for (int i = 0; i < kR1Per; ++i) {
+            const int idx = t + 192 * i;
             const uint32_t j = c * kR1Chunk + (uint32_t)idx;
-            buf[b][idx & 31][idx >> 5] = j < d ? src[j] : 0.0f;
+            buf[b][idx & 31][idx >> 5] = j < d ? x[i] : 0.0f;
         }
     };
     if (wv != 0) load(0, 0);
@@ -730,8 +777,12 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
         if (wv != 0) {
             if (c + 1 < nch) load(c + 1, (c + 1) & 1);
         } else if (lane < 32) {
+            // rounds holding entries of the row; the zero padding of the last
+            // round leaves p unchanged (p is never -0)
+            const uint32_t rem = d - c * kR1Chunk;
+            const int ni = rem >= (uint32_t)kR1Chunk ? kR1K : (int)((rem + 31) / 32);
             const float *r = &buf[c & 1][lane][0];
-            for (int i = 0; i < kR1K; ++i) p += r[i];
+            for (int i = 0; i < ni; ++i) p += r[i];
         }
         __syncthreads();
     }

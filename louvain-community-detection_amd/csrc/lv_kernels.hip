@@ -163,17 +163,36 @@ void launch_relabel_map(hipStream_t s, uint32_t *order, uint32_t n, uint32_t *li
 }
 
 // copy the rows of positions [x0, x1) into the new CSR, G lanes per row,
-// column ids mapped through pmap
-template <int G>
+// column ids mapped through pmap; B rounds of loads, then of gathers, in
+// flight at once (rounds past the row end reload entry 0: rows here are
+// non-empty)
+template <int G, int B>
 __global__ __launch_bounds__(256) void k_relabel_rows(RelabelView r, uint32_t x0, uint32_t x1) {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     const uint32_t x = x0 + t / G, lane = t % G;
     if (x >= x1) return;
     const uint32_t v = r.lid[x];
     const uint32_t b = r.row[v], d = r.row[v + 1] - b, o = r.row2[x];
-    for (uint32_t j = lane; j < d; j += G) {
-        r.col2[o + j] = r.pmap[r.col[b + j]];
-        r.w2[o + j] = r.w[b + j];
+    for (uint32_t j0 = lane; j0 < d; j0 += G * B) {
+        uint32_t c[B];
+        float ww[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const uint32_t j = j0 + (uint32_t)(G * u);
+            const uint32_t q = b + (j < d ? j : 0u);
+            c[u] = r.col[q];
+            ww[u] = r.w[q];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) c[u] = r.pmap[c[u]];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const uint32_t j = j0 + (uint32_t)(G * u);
+            if (j < d) {
+                r.col2[o + j] = c[u];
+                r.w2[o + j] = ww[u];
+            }
+        }
     }
 }
 void launch_relabel_rows(hipStream_t s, const RelabelView &r, uint32_t x0, uint32_t x1,
@@ -182,9 +201,9 @@ void launch_relabel_rows(hipStream_t s, const RelabelView &r, uint32_t x0, uint3
     const uint64_t threads = (uint64_t)(x1 - x0) * (uint64_t)lanes;
     const dim3 grid((unsigned)((threads + 255) / 256));
     if (lanes == 8)
-        hipLaunchKernelGGL(k_relabel_rows<8>, grid, dim3(256), 0, s, r, x0, x1);
+        hipLaunchKernelGGL((k_relabel_rows<8, 4>), grid, dim3(256), 0, s, r, x0, x1);
     else
-        hipLaunchKernelGGL(k_relabel_rows<64>, grid, dim3(256), 0, s, r, x0, x1);
+        hipLaunchKernelGGL((k_relabel_rows<64, 8>), grid, dim3(256), 0, s, r, x0, x1);
 }
 
 // new_id under a locality order: rank of lid[c] among the non-empty
@@ -285,9 +304,12 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
 // G lanes per vertex.  For G < 32 the class guarantees deg <= G, and the
 // reference's 32-lane tree restricted to G lanes gives the same bits (the
 // missing partials are +0).  Lane l sums entries l, l+32, ... sequentially;
-// loads are batched 8 rounds ahead, adds stay in order.
+// loads are batched B rounds ahead (B = 1 for G < 32), adds stay in order.
+// Every load of a batch is issued before the first gather and every gather
+// before the first add: rounds past the row end load the row's first entry
+// (always in bounds) and are skipped by the adds.
 // ---------------------------------------------------------------------------
-template <int G, bool MASKED>
+template <int G, bool MASKED, int B>
 __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, uint32_t count,
                                             int init_state, unsigned long long *acc) {
     constexpr int GPB = 256 / G;
@@ -304,24 +326,22 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
         const float *wr = g.w + base;
         const uint32_t *cr = g.col + base;
         float p = 0.0f;
-        uint32_t j = lane;
-        if (G == 32) {
-            for (; j + 7 * 32 < d; j += 8 * 32) {
-                float x[8];
-                bool keep[8];
+        for (uint32_t j = lane; j < d; j += B * 32) {
+            float x[B];
+            uint32_t c[B];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    x[u] = wr[j + 32 * u];
-                    keep[u] = MASKED ? (g.comm[cr[j + 32 * u]] == cv) : true;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (keep[u]) p += x[u];
+            for (int u = 0; u < B; ++u) {
+                const uint32_t q = j + 32u * u < d ? j + 32u * u : 0u;
+                x[u] = wr[q];
+                if (MASKED) c[u] = cr[q];
             }
-        }
-        for (; j < d; j += 32) {
-            float x = wr[j];
-            if (!MASKED || g.comm[cr[j]] == cv) p += x;
+            if (MASKED) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) c[u] = g.comm[c[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (j + 32u * u < d && (!MASKED || c[u] == cv)) p += x[u];
         }
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) p += __shfl_down(p, o, G);
@@ -343,21 +363,21 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
 }
 
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
-               int masked, int init_state, unsigned long long *acc) {
+               int batch, int masked, int init_state, unsigned long long *acc) {
     if (!count) return;
     const unsigned gpb = 256 / group;
     dim3 grid((count + gpb - 1) / gpb), block(256);
-#define LV_R1(GG)                                                                            \
-    if (group == GG) {                                                                       \
-        if (masked)                                                                          \
-            hipLaunchKernelGGL((k_r1<GG, true>), grid, block, 0, s, g, verts, count, init_state, \
-                               acc);                                                         \
-        else                                                                                 \
-            hipLaunchKernelGGL((k_r1<GG, false>), grid, block, 0, s, g, verts, count,           \
-                               init_state, acc);                                             \
-        return;                                                                              \
+#define LV_R1(GG, BB)                                                                         \
+    if (group == GG && (GG < 32 || batch == BB)) {                                            \
+        if (masked)                                                                           \
+            hipLaunchKernelGGL((k_r1<GG, true, BB>), grid, block, 0, s, g, verts, count,       \
+                               init_state, acc);                                              \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_r1<GG, false, BB>), grid, block, 0, s, g, verts, count,      \
+                               init_state, acc);                                              \
+        return;                                                                               \
     }
-    LV_R1(4) LV_R1(8) LV_R1(16) LV_R1(32)
+    LV_R1(4, 1) LV_R1(8, 1) LV_R1(16, 1) LV_R1(32, 4) LV_R1(32, 8)
 #undef LV_R1
 }
 
@@ -452,42 +472,42 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
         key[i] = kEmpty;
         val[i] = 0ull;
     }
+    // groups past the end read the last vertex (count >= 1) and write nothing;
+    // the vertex state loads are unconditional so none of them waits on a branch
     const uint32_t vi = blockIdx.x * GPB + grp;
-    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
-    uint32_t base = 0, d = 0, cv = 0, szv = 0;
-    float kv = 0.0f, acv = 0.0f, m = 1.0f;
-    if (active) {
-        m = v_m(g, v);
-        base = g.row[v];
-        d = g.row[v + 1] - base;
-        cv = g.comm[v];
-        szv = g.comm_sz[cv];
-        kv = g.k[v];
-        acv = fix2f(g.A[cv], g.S);
-    }
-    __syncthreads();
+    const float m = v_m(g, v);
+    const uint32_t base = g.row[v], rend = g.row[v + 1];
+    const uint32_t d = active ? rend - base : 0u;
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
     float best_g = -INFINITY;
     uint32_t best_c = kEmpty;
     unsigned long long cew = 0;
-    // every round's loads are issued before the first insert (latency overlap)
+    // every round's loads are issued before the first insert, all of them
+    // unconditionally (positions past the row end, or of an inactive group,
+    // load entry `base`, which exists: this class has entries), so they are
+    // all in flight at once
     uint32_t er[ROUNDS], cr[ROUNDS];
     float wr[ROUNDS];
+    bool hs[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const uint32_t j = (uint32_t)(r * T + lane);
-        er[r] = kEmpty;
-        if (active && j < d) {
-            er[r] = g.col[base + j];
-            wr[r] = g.w[base + j];
-        }
+        hs[r] = active && j < d;
+        const uint32_t e = base + (hs[r] ? j : 0u);
+        er[r] = g.col[e];
+        wr[r] = g.w[e];
     }
 #pragma unroll
-    for (int r = 0; r < ROUNDS; ++r)
-        if (er[r] != kEmpty) cr[r] = g.comm[er[r]];
+    for (int r = 0; r < ROUNDS; ++r) cr[r] = g.comm[er[r]];
+    __syncthreads();  // table cleared (the loads above do not touch LDS)
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
-        const bool has = er[r] != kEmpty;
+        const bool has = hs[r];
         uint32_t e = 0, ce = 0, slot = 0;
         if (has) {
             e = er[r];
@@ -561,38 +581,41 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         sndist[wv] = 0;
         scew[wv] = 0;
     }
+    // waves past the end read the last vertex (count >= 1) and write nothing
     const uint32_t vi = blockIdx.x * 4 + wv;
-    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
-    uint32_t base = 0, d = 0, cv = 0, szv = 0;
-    float kv = 0.0f, acv = 0.0f, m = 1.0f;
-    if (active) {
-        m = v_m(g, v);
-        base = g.row[v];
-        d = g.row[v + 1] - base;
-        cv = g.comm[v];
-        szv = g.comm_sz[cv];
-        kv = g.k[v];
-        acv = fix2f(g.A[cv], g.S);
-    }
-    __syncthreads();
+    const float m = v_m(g, v);
+    const uint32_t base = g.row[v], rend = g.row[v + 1];
+    const uint32_t d = active ? rend - base : 0u;
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
     uint32_t eslot[kWaveMaxPer];
     unsigned long long cew = 0;
     // all loads of the lane's entries first, then their community gathers
     uint32_t ei[kWaveMaxPer], ci[kWaveMaxPer];
     float wi[kWaveMaxPer];
+    // unconditional loads (past the row end, or of an inactive wave: entry
+    // `base`, which exists), so every load and then every gather is in flight
+    // at once
 #pragma unroll
     for (int i = 0; i < kWaveMaxPer; ++i) {
         const uint32_t j = p + 64u * i;
-        ei[i] = kEmpty;
-        if (active && j < d) {
-            ei[i] = g.col[base + j];
-            wi[i] = g.w[base + j];
-        }
+        const bool has = active && j < d;
+        const uint32_t e = base + (has ? j : 0u);
+        ei[i] = g.col[e];
+        wi[i] = g.w[e];
+        ci[i] = has ? 0u : kEmpty;
     }
 #pragma unroll
-    for (int i = 0; i < kWaveMaxPer; ++i)
-        if (ei[i] != kEmpty) ci[i] = g.comm[ei[i]];
+    for (int i = 0; i < kWaveMaxPer; ++i) {
+        const uint32_t c = g.comm[ei[i]];
+        ei[i] = ci[i] == kEmpty ? kEmpty : ei[i];
+        ci[i] = c;
+    }
+    __syncthreads();  // table cleared (the loads above do not touch LDS)
 #pragma unroll
     for (int i = 0; i < kWaveMaxPer; ++i) {
         eslot[i] = kEmpty;
@@ -717,24 +740,26 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     const uint32_t szv = g.comm_sz[cv];
     const float kv = g.k[v];
     const float acv = fix2f(g.A[cv], g.S);
-    __syncthreads();
     uint32_t eslot[MAXPER];
     unsigned long long cew = 0;
     // all loads of the thread's entries first, then their community gathers
     uint32_t ei[MAXPER], ci[MAXPER];
     float wi[MAXPER];
+    // unconditional (past the row end: entry `base`), so every load and then
+    // every gather is in flight at once
 #pragma unroll
     for (int i = 0; i < MAXPER; ++i) {
         const uint32_t j = t + (uint32_t)P * i;
-        ei[i] = kEmpty;
-        if (j < d) {
-            ei[i] = g.col[base + j];
-            wi[i] = g.w[base + j];
-        }
+        const uint32_t e = base + (j < d ? j : 0u);
+        ei[i] = g.col[e];
+        wi[i] = g.w[e];
     }
 #pragma unroll
-    for (int i = 0; i < MAXPER; ++i)
-        if (ei[i] != kEmpty) ci[i] = g.comm[ei[i]];
+    for (int i = 0; i < MAXPER; ++i) {
+        ci[i] = g.comm[ei[i]];
+        if (t + (uint32_t)P * i >= d) ei[i] = kEmpty;
+    }
+    __syncthreads();  // table cleared (the loads above do not touch LDS)
 #pragma unroll
     for (int i = 0; i < MAXPER; ++i) {
         eslot[i] = kEmpty;
@@ -1209,46 +1234,44 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     for (uint32_t q = threadIdx.x; q <= msz; q += P) mb[q] = a.moff[m0 + q] - f0;
     __syncthreads();
     // every entry's loads before any insert: member, row start, column and
-    // weight, neighbour community
-    uint32_t vv[PER], jj[PER], dd[PER];
+    // weight, neighbour community, its new id.  Positions past the end repeat
+    // entry 0 (ne >= 1), so no load waits on a branch.
+    uint32_t vv[PER], jj[PER], dd[PER], nid[PER];
     float ww[PER];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
-        const uint32_t r = threadIdx.x + (uint32_t)P * u;
-        jj[u] = kEmpty;
-        if (r < ne) {
-            uint32_t l = 0, h = msz;  // member of flat entry r
-            while (h - l > 1) {
-                const uint32_t mid = (l + h) >> 1;
-                if (mb[mid] <= r) l = mid;
-                else h = mid;
-            }
-            vv[u] = a.members[m0 + l];
-            jj[u] = r - mb[l];
+        const uint32_t ru = threadIdx.x + (uint32_t)P * u;
+        const uint32_t r = ru < ne ? ru : 0u;
+        uint32_t l = 0, h = msz;  // member of flat entry r
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (mb[mid] <= r) l = mid;
+            else h = mid;
         }
+        vv[u] = a.members[m0 + l];
+        jj[u] = r - mb[l];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) jj[u] += g.row[vv[u]];
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) {
-            dd[u] = g.col[jj[u]];
-            ww[u] = g.w[jj[u]];
-        }
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) dd[u] = g.comm[dd[u]];
+    for (uint32_t u = 0; u < PER; ++u) jj[u] += g.row[vv[u]];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
-        if (jj[u] == kEmpty) continue;
+        dd[u] = g.col[jj[u]];
+        ww[u] = g.w[jj[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) dd[u] = g.comm[dd[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) nid[u] = a.new_id[dd[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+        if (threadIdx.x + (uint32_t)P * u >= ne) continue;
         const uint32_t d2 = dd[u];
         uint32_t slot = hslot<LC>(d2);
         for (int probe = 0; probe < CAP; ++probe) {
             const uint32_t prev = atomicCAS(&key[slot], kEmpty, d2);
             if (prev == kEmpty) {
                 const uint32_t idx = atomicAdd(&ndist, 1u);
-                skeys[idx] = a.new_id[d2];  // sorted by new id
+                skeys[idx] = nid[u];  // sorted by new id
                 sslot[idx] = slot;
                 break;
             }
@@ -1713,43 +1736,38 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
     const uint32_t nb = p.nbk[x.ci];
     constexpr uint32_t PER = kAggChunk / 256;
     // each dependent load level for all of the thread's entries at once:
-    // member, row start, column + weight, community, new id
+    // member, row start, column + weight, community, new id.  Positions past
+    // the chunk end repeat its first entry, so no load waits on a branch.
     uint32_t vv[PER], jj[PER], kk[PER];
     float ww[PER];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
-        const uint32_t f = x.f0 + threadIdx.x + 256u * u;
-        jj[u] = kEmpty;
-        if (f < x.f1) {
-            uint32_t l = 0, h = x.nm;  // member of flat entry f
-            while (h - l > 1) {
-                const uint32_t mid = (l + h) >> 1;
-                if (mb[mid] <= f) l = mid;
-                else h = mid;
-            }
-            vv[u] = a.members[x.i0 + l];
-            jj[u] = f - mb[l];
+        const uint32_t fu = x.f0 + threadIdx.x + 256u * u;
+        const uint32_t f = fu < x.f1 ? fu : x.f0;
+        uint32_t l = 0, h = x.nm;  // member of flat entry f
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (mb[mid] <= f) l = mid;
+            else h = mid;
         }
+        vv[u] = a.members[x.i0 + l];
+        jj[u] = f - mb[l];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) jj[u] += g.row[vv[u]];
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) {
-            kk[u] = g.col[jj[u]];
-            ww[u] = g.w[jj[u]];
-        }
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) kk[u] = g.comm[kk[u]];
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u)
-        if (jj[u] != kEmpty) kk[u] = a.new_id[kk[u]];
+    for (uint32_t u = 0; u < PER; ++u) jj[u] += g.row[vv[u]];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
-        if (jj[u] == kEmpty) continue;
+        kk[u] = g.col[jj[u]];
+        ww[u] = g.w[jj[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) kk[u] = g.comm[kk[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) kk[u] = a.new_id[kk[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
         const uint32_t f = x.f0 + threadIdx.x + 256u * u;
+        if (f >= x.f1) continue;
         p.fkey[f] = kk[u];
         p.fw[f] = ww[u];
         atomicAdd(&hist[agg_bucket(kk[u], nb, p.nn)], 1u);
@@ -1784,12 +1802,20 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
     for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) lcnt[b] = 0;
     __syncthreads();
     uint32_t key[kAggChunk / 256], rank[kAggChunk / 256], bk[kAggChunk / 256];
+    float wq[kAggChunk / 256];
+    // all loads first (past the chunk end: its first entry), then the ranks
+#pragma unroll
+    for (uint32_t q = 0; q < kAggChunk / 256; ++q) {
+        const uint32_t i = threadIdx.x + 256u * q;
+        const uint32_t f = f0 + (i < cnt ? i : 0u);
+        key[q] = p.fkey[f];
+        wq[q] = p.fw[f];
+    }
 #pragma unroll
     for (uint32_t q = 0; q < kAggChunk / 256; ++q) {
         const uint32_t i = threadIdx.x + 256u * q;
         bk[q] = kEmpty;
         if (i < cnt) {
-            key[q] = p.fkey[f0 + i];
             bk[q] = agg_bucket(key[q], nb, p.nn);
             rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
         }
@@ -1822,7 +1848,7 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
         if (bk[q] == kEmpty) continue;
         const uint32_t sp = loff[bk[q]] + rank[q];
         sk[sp] = key[q];
-        sw[sp] = p.fw[f0 + threadIdx.x + 256u * q];
+        sw[sp] = wq[q];
         sb[sp] = (uint16_t)bk[q];
     }
     __syncthreads();
