@@ -160,7 +160,8 @@ void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint3
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
                              uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
-// pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v.
+// pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v, 2 = s_v
+// while every community is a singleton (pass init: no community gathers).
 // `group` lanes per vertex (4/8/16/32); with 32 lanes, `batch` (4 or 8)
 // rounds of loads are in flight at once.  `acc` receives the exact
 // fixed-point total (atomic).

@@ -439,7 +439,9 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 
 // groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
 // 16 for 3, 32 for 4-10 (contiguous in `order`); rows of classes 4-5 (at
-// most 84 entries) take at most 3 rounds per lane, longer rows batch 8
+// most 84 entries) take at most 3 rounds per lane, longer rows batch 8.
+// masked: 0 k_v, 1 s_v, 2 s_v with singleton communities (pass init).  The
+// masked sums skip class 0: an empty row adds nothing.
 int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
     const struct {
@@ -447,9 +449,10 @@ int r1_all(lv_graph *g, int masked, int init_state) {
     } grp[5] = {{4, 1, 0, 1}, {8, 1, 2, 2}, {16, 1, 3, 3}, {32, 4, 4, 5},
                 {32, 8, 6, lv::kClassHeavy - 1}};
     for (const auto &q : grp) {
-        uint32_t a = g->cls_off[q.c0], b = g->cls_off[q.c1 + 1];
+        const int c0 = (masked && q.c0 == 0) ? 1 : q.c0;
+        uint32_t a = g->cls_off[c0], b = g->cls_off[q.c1 + 1];
         double ent = 0;
-        for (int c = q.c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
+        for (int c = c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
         ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
         lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, q.B, masked, init_state,
                       acc_ptr(g));
@@ -474,8 +477,8 @@ float q_from_counters(const lv_graph *g) {
 }
 
 // launches the S_in and sum a^2 reductions into zeroed counters
-void modularity_launch(lv_graph *g) {
-    r1_all(g, 1, 0);
+void modularity_launch(lv_graph *g, bool singletons = false) {
+    r1_all(g, singletons ? 2 : 1, 0);
     lv::DevView v = view(g);
     ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
     lv::launch_sumsq(g->stream, v, g->order, g->n, limbs_ptr(g));
@@ -527,7 +530,7 @@ int pass_init(lv_graph *g, float *q0) {
         return LV_ERR_RANGE;
     }
     zero_counters(g);
-    modularity_launch(g);
+    modularity_launch(g, true);  // every community is a singleton here
     rc = sync_counters(g);
     if (rc) return rc;
     g->pass_ready = true;
@@ -960,7 +963,7 @@ int batch_pass_init(lv_graph *g) {
         if (rc) return rc;
     }
     zero_counters(g);
-    modularity_launch(g);
+    modularity_launch(g, true);  // every active community is a singleton here
     rc = sync_counters(g);
     if (rc) return rc;
     for (uint32_t gi = 0; gi < G; ++gi) {

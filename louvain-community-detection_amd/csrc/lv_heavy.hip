@@ -328,19 +328,25 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
     unsigned long long cew = 0;
-    // every load of the chunk first (past the chunk end: its first entry),
-    // then every gather (past the end: the vertex itself), then the LDS work
+    // columns first (past the chunk end: its first entry), then every
+    // community, then the LDS work
     uint32_t es[kPerThread], cs[kPerThread];
     float ws[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        const uint32_t e = x.base + (j < j1 ? j : j0);
-        es[q] = g.col[e];
-        ws[q] = g.w[e];
+        es[q] = g.col[x.base + (j < j1 ? j : j0)];
     }
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) cs[q] = g.comm[es[q]];
+    // weights only where comm_e_w needs them (entries in v's community that
+    // are not self-loops); the other lanes issue no load
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        ws[q] = 0.0f;
+        if (j < j1 && cs[q] == cv && es[q] != x.v) ws[q] = g.w[x.base + j];
+    }
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
@@ -708,33 +714,37 @@ void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint3
 
 // masked weights of the heavy rows, staged contiguously: w if the neighbour
 // is in v's community, else +0 (adding +0 to a partial >= +0 changes nothing)
-__global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, float *stage) {
+__global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, float *stage,
+                                                     int ident) {
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
     const uint32_t cv = g.comm[x.v];
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
-    uint32_t es[kPerThread];  // loads first, then gathers
+    // columns (past the chunk end: its first entry), then communities, then
+    // the weights of the entries inside v's community only (the other lanes
+    // issue no load)
+    uint32_t cs[kPerThread];
     float ws[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        es[q] = kEmpty;
-        if (j < j1) {
-            es[q] = g.col[x.base + j];
-            ws[q] = g.w[x.base + j];
-        }
+        cs[q] = g.col[x.base + (j < j1 ? j : j0)];
     }
-    // unconditional gathers (past the chunk end: the vertex itself), so all
-    // of them are in flight before the first compare
-    uint32_t cs[kPerThread];
+    if (!ident) {  // ident: every community is a singleton, comm[u] = u
 #pragma unroll
-    for (int q = 0; q < kPerThread; ++q) cs[q] = g.comm[es[q] != kEmpty ? es[q] : x.v];
+        for (int q = 0; q < kPerThread; ++q) cs[q] = g.comm[cs[q]];
+    }
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
-        if (es[q] == kEmpty) continue;
         const uint32_t j = j0 + threadIdx.x + 256u * q;
-        stage[x.base + j] = (cs[q] == cv) ? ws[q] : 0.0f;
+        ws[q] = 0.0f;
+        if (j < j1 && cs[q] == cv) ws[q] = g.w[x.base + j];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t j = j0 + threadIdx.x + 256u * q;
+        if (j < j1) stage[x.base + j] = ws[q];
     }
 }
 
@@ -808,7 +818,8 @@ void launch_heavy_r1(hipStream_t s, const DevView &g, const HeavySplit &h, float
                      int masked, int init_state, unsigned long long *acc) {
     if (!h.nh) return;
     if (masked)
-        hipLaunchKernelGGL(k_heavy_stage, dim3(h.nchunks), dim3(256), 0, s, g, h, stage);
+        hipLaunchKernelGGL(k_heavy_stage, dim3(h.nchunks), dim3(256), 0, s, g, h, stage,
+                           masked == 2 ? 1 : 0);
     hipLaunchKernelGGL(k_heavy_r1, dim3(h.nh), dim3(256), 0, s, g, h, stage, masked, init_state,
                        acc);
 }

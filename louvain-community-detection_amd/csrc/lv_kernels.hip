@@ -307,11 +307,14 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
 // loads are batched B rounds ahead (B = 1 for G < 32), adds stay in order.
 // Every load of a batch is issued before the first gather and every gather
 // before the first add: rounds past the row end load the row's first entry
-// (always in bounds) and are skipped by the adds.
+// (always in bounds) and are skipped by the adds.  At pass init every
+// community is a singleton (comm[u] = u), so the masked sum compares columns
+// directly (`ident`) and gathers nothing.
 // ---------------------------------------------------------------------------
 template <int G, bool MASKED, int B>
 __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, uint32_t count,
-                                            int init_state, unsigned long long *acc) {
+                                            int init_state, int ident,
+                                            unsigned long long *acc) {
     constexpr int GPB = 256 / G;
     const int lane = threadIdx.x % G;
     const uint32_t vi = blockIdx.x * GPB + threadIdx.x / G;
@@ -328,20 +331,33 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
         float p = 0.0f;
         for (uint32_t j = lane; j < d; j += B * 32) {
             float x[B];
-            uint32_t c[B];
-#pragma unroll
-            for (int u = 0; u < B; ++u) {
-                const uint32_t q = j + 32u * u < d ? j + 32u * u : 0u;
-                x[u] = wr[q];
-                if (MASKED) c[u] = cr[q];
-            }
             if (MASKED) {
+                // columns, then communities, then the weights of the entries
+                // inside v's community only (the other lanes issue no load)
+                uint32_t c[B];
+                bool keep[B];
 #pragma unroll
-                for (int u = 0; u < B; ++u) c[u] = g.comm[c[u]];
+                for (int u = 0; u < B; ++u) c[u] = cr[j + 32u * u < d ? j + 32u * u : 0u];
+                if (!ident) {  // ident: every community is a singleton, comm[u] = u
+#pragma unroll
+                    for (int u = 0; u < B; ++u) c[u] = g.comm[c[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    keep[u] = j + 32u * u < d && c[u] == cv;
+                    x[u] = 0.0f;
+                    if (keep[u]) x[u] = wr[j + 32u * u];
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (keep[u]) p += x[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < B; ++u) x[u] = wr[j + 32u * u < d ? j + 32u * u : 0u];
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (j + 32u * u < d) p += x[u];
             }
-#pragma unroll
-            for (int u = 0; u < B; ++u)
-                if (j + 32u * u < d && (!MASKED || c[u] == cv)) p += x[u];
         }
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) p += __shfl_down(p, o, G);
@@ -364,6 +380,7 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
 
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
                int batch, int masked, int init_state, unsigned long long *acc) {
+    const int ident = masked == 2;
     if (!count) return;
     const unsigned gpb = 256 / group;
     dim3 grid((count + gpb - 1) / gpb), block(256);
@@ -371,10 +388,10 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
     if (group == GG && (GG < 32 || batch == BB)) {                                            \
         if (masked)                                                                           \
             hipLaunchKernelGGL((k_r1<GG, true, BB>), grid, block, 0, s, g, verts, count,       \
-                               init_state, acc);                                              \
+                               init_state, ident, acc);                                       \
         else                                                                                  \
             hipLaunchKernelGGL((k_r1<GG, false, BB>), grid, block, 0, s, g, verts, count,      \
-                               init_state, acc);                                              \
+                               init_state, 0, acc);                                           \
         return;                                                                               \
     }
     LV_R1(4, 1) LV_R1(8, 1) LV_R1(16, 1) LV_R1(32, 4) LV_R1(32, 8)
