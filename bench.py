@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true",
+                    help="skip the extra profiled run (exactly warmup + steps runs, e.g. "
+                         "under an external profiler); roofline is then null")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner on the resident graph before the timed runs")
     args = ap.parse_args()
@@ -181,20 +184,27 @@ def main():
     # one more run of the same workload with HIP events on the dominant family
     # only (every family without a warmup): its launch durations give the
     # roofline
-    g.profile(True)
-    g.profile_select([fam] if fam else None)
-    g.profile_reset()
-    g.reset()
-    pres = g.run(MIN_GAIN)
-    stats = g.profile_get()
-    g.profile(False)
-    print(f"[bench] profiled run: {pres.alg_ms:.2f} ms", file=sys.stderr, flush=True)
-    if fam is None:
-        fam = max(stats, key=lambda k: stats[k]["ms"])
-    st = stats[fam]
-    avg_ms = st["ms"] / max(st["launches"], 1)
-    avg_bytes = st["bytes"] / max(st["launches"], 1)
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline, stats = None, {}
+    if not args.no_roofline:
+        g.profile(True)
+        g.profile_select([fam] if fam else None)
+        g.profile_reset()
+        g.reset()
+        pres = g.run(MIN_GAIN)
+        stats = g.profile_get()
+        g.profile(False)
+        print(f"[bench] profiled run: {pres.alg_ms:.2f} ms", file=sys.stderr, flush=True)
+        if fam is None:
+            fam = max(stats, key=lambda k: stats[k]["ms"])
+        st = stats[fam]
+        avg_ms = st["ms"] / max(st["launches"], 1)
+        avg_bytes = st["bytes"] / max(st["launches"], 1)
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        roofline = {"bound": "hbm", "kernel": fam, "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(cfg["workload"], fam, st["launches"]),
+                    "launches_per_run": int(st["launches"]),
+                    "avg_launch_ms": avg_ms, "bytes_per_launch": avg_bytes}
     line = {
         "metric": "Louvain edges/s summed over passes (plain, 1 GPU)",
         "value": value,
@@ -213,11 +223,7 @@ def main():
                    "modularity": float(res.q_final), "passes": int(res.passes),
                    "sweeps": int(res.total_iters), "communities": int(res.n_final),
                    "tuning": tuning, "autotuned": bool(args.tune)},
-        "roofline": {"bound": "hbm", "kernel": fam, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(cfg["workload"], fam, st["launches"]),
-                     "launches_per_run": int(st["launches"]),
-                     "avg_launch_ms": avg_ms, "bytes_per_launch": avg_bytes},
+        "roofline": roofline,
         # the boundary takes host buffers: one upload plus one run
         "pcie_inclusive": {"create_ms": create_s * 1e3,
                            "edges_per_s": (sum(edges) / args.steps) /

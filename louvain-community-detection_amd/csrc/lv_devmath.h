@@ -70,6 +70,12 @@ template <int LOG2CAP>
 __device__ __forceinline__ uint32_t hslot(uint32_t key) {
     return (key * 0x9E3779B1u) >> (32 - LOG2CAP);
 }
+// home slot in a table of any size CAP (multiply-high of the same hash; for a
+// power of two it is hslot)
+template <int CAP>
+__device__ __forceinline__ uint32_t hslot_n(uint32_t key) {
+    return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * (unsigned)CAP) >> 32);
+}
 
 __device__ __forceinline__ uint32_t hslot_dyn(uint32_t key, int log2cap) {
     return (key * 0x9E3779B1u) >> (32 - log2cap);
@@ -121,7 +127,8 @@ __device__ __forceinline__ void tree128(float g0, uint32_t c0, float g1, uint32_
 // over its exact sum: val[slot] <- bits of the gain, -inf for a candidate
 // isLowerIdx rejects.  Distinct slots dist[i], i = first, first + stride, ...
 // < nd.
-__device__ __forceinline__ void distinct_gains(const DevView &g, const uint32_t *dist, uint32_t nd,
+template <typename D>
+__device__ __forceinline__ void distinct_gains(const DevView &g, const D *dist, uint32_t nd,
                                                uint32_t first, uint32_t stride,
                                                const uint32_t *key, unsigned long long *val,
                                                float m, float kv, float acv, uint32_t szv,

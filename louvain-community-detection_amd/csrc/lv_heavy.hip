@@ -311,9 +311,6 @@ __global__ void k_heavy_chunks(const uint32_t *row, const uint32_t *verts, uint3
 __device__ __forceinline__ uint32_t bucket_of(uint32_t c, int lb) {
     return lb ? ((c * 0x9E3779B1u) >> (32 - lb)) : 0u;
 }
-__device__ __forceinline__ uint32_t pslot(uint32_t c, int lcap) {
-    return (c * 0x85EBCA6Bu) >> (32 - lcap);
-}
 
 __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
     __shared__ uint32_t hist[kMaxBuckets];
@@ -450,21 +447,25 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
     }
 }
 
-// CAP: LDS table slots of one bucket (twice the distinct keys it may hold).
-// A bucket of at most 4 x 256 entries keeps each thread's entries and their
-// table slots in registers between the insert and the lane phase; longer
-// buckets (skewed hashes) read their entries again.
+// CAP: LDS table slots of one bucket (any size), DMAX: distinct keys it may
+// hold before the bucket overflows to the global tables.  A bucket of at
+// most 4 x 256 entries keeps each thread's entries and their table slots in
+// registers between the insert and the lane phase; longer buckets (skewed
+// hashes) read their entries again.
 template <int kBucketCap>
-__device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long *val, uint32_t *dist,
-                                              uint32_t *nd_s, uint32_t *over, uint32_t c,
-                                              unsigned long long wf, uint32_t dmax) {
-    constexpr int LC = Log2<kBucketCap>::value;
-    uint32_t slot = pslot(c, LC);
+__device__ __forceinline__ uint32_t pslot_n(uint32_t c) {
+    return (uint32_t)(((unsigned long long)(c * 0x85EBCA6Bu) * (unsigned)kBucketCap) >> 32);
+}
+template <int kBucketCap>
+__device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long *val,
+                                              uint16_t *dist, uint32_t *nd_s, uint32_t *over,
+                                              uint32_t c, unsigned long long wf, uint32_t dmax) {
+    uint32_t slot = pslot_n<kBucketCap>(c);
     for (int probe = 0; probe < kBucketCap; ++probe) {
         const uint32_t prev = atomicCAS(&key[slot], kEmpty, c);
         if (prev == kEmpty) {
             const uint32_t idx = atomicAdd(nd_s, 1u);
-            if (idx < dmax) dist[idx] = slot;
+            if (idx < dmax) dist[idx] = (uint16_t)slot;
             else *over = 1u;  // more distinct communities than the table allows
             atomicAdd(&val[slot], wf);
             return slot;
@@ -473,18 +474,18 @@ __device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long 
             atomicAdd(&val[slot], wf);
             return slot;
         }
-        slot = (slot + 1) & (kBucketCap - 1);
+        slot = slot + 1 == (uint32_t)kBucketCap ? 0u : slot + 1;
     }
     *over = 1u;
     return kEmpty;
 }
 
-template <int kBucketCap>
+template <int kBucketCap, int kDmax>
 __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
-    constexpr int LC = Log2<kBucketCap>::value;
+    static_assert(kBucketCap > kDmax && kBucketCap <= 65536, "table shape");
     __shared__ uint32_t key[kBucketCap];
     __shared__ unsigned long long val[kBucketCap];
-    __shared__ uint32_t dist[kBucketCap / 2];
+    __shared__ uint16_t dist[kDmax];
     __shared__ unsigned long long lanes[128];
     __shared__ uint32_t nd_s, over;
     const uint32_t i = h.bvtx[blockIdx.x];
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         nd_s = 0;
         over = 0;
     }
-    const uint32_t dmax = min(h.dmax, (uint32_t)kBucketCap / 2);
+    const uint32_t dmax = min(h.dmax, (uint32_t)kDmax);
     __syncthreads();
     constexpr uint32_t R = 4;
     const bool cached = p1 - p0 <= 256u * R;  // uniform
@@ -557,8 +558,8 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     } else {
         for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
             const uint32_t c = h.pc[q];
-            uint32_t slot = pslot(c, LC);
-            while (key[slot] != c) slot = (slot + 1) & (kBucketCap - 1);  // present
+            uint32_t slot = pslot_n<kBucketCap>(c);
+            while (key[slot] != c) slot = slot + 1 == (uint32_t)kBucketCap ? 0u : slot + 1;  // present
             lane_max(slot, h.pj[q]);
         }
     }
@@ -626,6 +627,12 @@ void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *ver
                        count, nchunks);
 }
 
+#ifndef LV_HPCAP
+#define LV_HPCAP 1536
+#endif
+#ifndef LV_HPDMAX
+#define LV_HPDMAX 768
+#endif
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
     if (!hin.nh || !hin.nchunks) return;
     HeavySplit h = hin;
@@ -644,9 +651,10 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
         hipLaunchKernelGGL(k_hp_scatter, grid, block, 0, s, g, h);
         // buckets of up to 512 target entries: 2048-slot tables; larger: 4096
         if (h.bucket_entries <= 512)
-            hipLaunchKernelGGL(k_hp_bucket<2048>, dim3(h.nbuckets), block, 0, s, g, h);
+            hipLaunchKernelGGL((k_hp_bucket<LV_HPCAP, LV_HPDMAX>), dim3(h.nbuckets), block, 0, s,
+                               g, h);
         else
-            hipLaunchKernelGGL(k_hp_bucket<4096>, dim3(h.nbuckets), block, 0, s, g, h);
+            hipLaunchKernelGGL((k_hp_bucket<4096, 2048>), dim3(h.nbuckets), block, 0, s, g, h);
         if (h.force_global == 2)
             hipLaunchKernelGGL(k_hp_flag_odd, dim3((h.nh + 255) / 256), dim3(256), 0, s, h);
     }

@@ -494,6 +494,9 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     const uint32_t vi = blockIdx.x * GPB + grp;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
+    // a workgroup with no live vertex (past the end, or only frozen graphs of
+    // a batch) skips the loads below
+    if (!__syncthreads_or(active)) return;
     const float m = v_m(g, v);
     const uint32_t base = g.row[v], rend = g.row[v + 1];
     const uint32_t d = active ? rend - base : 0u;
@@ -521,7 +524,6 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     }
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) cr[r] = g.comm[er[r]];
-    __syncthreads();  // table cleared (the loads above do not touch LDS)
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const bool has = hs[r];
@@ -577,14 +579,16 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
 // bucket 6 (84 < deg <= 319): one wave64 per vertex, LDS table per wave.
 // Physical lane p visits entries p, p+64, ...: even rounds belong to virtual
 // lane p, odd rounds to virtual lane p+64.
-constexpr int kWaveCap = 512;
-constexpr int kWaveMaxPer = 5;  // ceil(319 / 64)
+#ifndef LV_WAVECAP
+#define LV_WAVECAP 400
+#endif
+constexpr int kWaveCap = LV_WAVECAP;  // slots per wave (at most 319 distinct keys)
+constexpr int kWaveMaxPer = 5;        // ceil(319 / 64)
 __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *verts,
                                                    uint32_t count) {
-    constexpr int LC = Log2<kWaveCap>::value;
     __shared__ uint32_t skey[4 * kWaveCap];
     __shared__ unsigned long long sval[4 * kWaveCap];
-    __shared__ uint32_t sdist[4][320];
+    __shared__ uint16_t sdist[4][320];
     __shared__ uint32_t sndist[4];
     __shared__ unsigned long long scew[4];
     const int wv = threadIdx.x >> 6, p = threadIdx.x & 63;
@@ -602,6 +606,9 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
     const uint32_t vi = blockIdx.x * 4 + wv;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
+    // a workgroup with no live vertex (past the end, or only frozen graphs of
+    // a batch) skips the loads below
+    if (!__syncthreads_or(active)) return;
     const float m = v_m(g, v);
     const uint32_t base = g.row[v], rend = g.row[v + 1];
     const uint32_t d = active ? rend - base : 0u;
@@ -632,7 +639,6 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         ei[i] = ci[i] == kEmpty ? kEmpty : ei[i];
         ci[i] = c;
     }
-    __syncthreads();  // table cleared (the loads above do not touch LDS)
 #pragma unroll
     for (int i = 0; i < kWaveMaxPer; ++i) {
         eslot[i] = kEmpty;
@@ -640,16 +646,16 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
             const uint32_t e = ei[i];
             const uint32_t ce = ci[i];
             unsigned long long wf = f2fix(wi[i], g.scale);
-            uint32_t slot = hslot<LC>(ce);
+            uint32_t slot = hslot_n<kWaveCap>(ce);
             for (int probe = 0; probe < kWaveCap; ++probe) {
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
                 if (prev == kEmpty) {
                     uint32_t idx = atomicAdd(&sndist[wv], 1u);
-                    sdist[wv][idx] = slot;
+                    sdist[wv][idx] = (uint16_t)slot;
                     break;
                 }
                 if (prev == ce) break;
-                slot = (slot + 1) & (kWaveCap - 1);
+                slot = slot + 1 == (uint32_t)kWaveCap ? 0u : slot + 1;
             }
             atomicAdd(&val[slot], wf);
             if (ce == cv && e != v) cew += wf;
@@ -725,20 +731,23 @@ __device__ __forceinline__ void block_argmax(float tg, uint32_t tc, uint32_t tj,
 }
 
 // bucket 7 with deg <= kBlockMax: one workgroup of P threads per vertex,
-// LDS table of CAP slots.
-template <int P, int CAP>
-__global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *verts,
-                                                  uint32_t count) {
-    constexpr int LC = Log2<CAP>::value;
-    constexpr int MAXDEG = CAP / 2;
+// LDS table of CAP slots (any size above MAXDEG: at most MAXDEG distinct
+// keys) with linear probing.  The distinct-slot list is 16-bit and the
+// per-thread lane states of the final tree reuse the sum array once the
+// lane phase is done, so the LDS footprint is about 12 B per slot.
+template <int P, int MAXDEG, int CAP, int MINB = 1>
+__global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_t *verts,
+                                                        uint32_t count) {
+    static_assert(CAP > MAXDEG && CAP <= 65536 && 3 * P * 4 <= CAP * 8, "table shape");
     constexpr int MAXPER = (MAXDEG + P - 1) / P;
-    __shared__ uint32_t key[CAP];
     __shared__ unsigned long long val[CAP];
-    __shared__ uint32_t dist[MAXDEG];
+    __shared__ uint32_t key[CAP];
+    __shared__ uint16_t dist[MAXDEG];
     __shared__ uint32_t ndist;
     __shared__ unsigned long long scew;
-    __shared__ float lg[P];
-    __shared__ uint32_t lc[P], lj[P];
+    float *lg = reinterpret_cast<float *>(val);  // lane states, after the lane phase
+    uint32_t *lc = reinterpret_cast<uint32_t *>(val) + P;
+    uint32_t *lj = reinterpret_cast<uint32_t *>(val) + 2 * P;
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < CAP; i += P) {
         key[i] = kEmpty;
@@ -784,15 +793,15 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
             const uint32_t e = ei[i];
             const uint32_t ce = ci[i];
             unsigned long long wf = f2fix(wi[i], g.scale);
-            uint32_t slot = hslot<LC>(ce);
+            uint32_t slot = hslot_n<CAP>(ce);
             for (int probe = 0; probe < CAP; ++probe) {
                 uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
                 if (prev == kEmpty) {
-                    dist[atomicAdd(&ndist, 1u)] = slot;
+                    dist[atomicAdd(&ndist, 1u)] = (uint16_t)slot;
                     break;
                 }
                 if (prev == ce) break;
-                slot = (slot + 1) & (CAP - 1);
+                slot = slot + 1 == (uint32_t)CAP ? 0u : slot + 1;
             }
             atomicAdd(&val[slot], wf);
             if (ce == cv && e != v) cew += wf;
@@ -819,10 +828,32 @@ __global__ __launch_bounds__(P) void k_move_block(DevView g, const uint32_t *ver
     }
     float bg = -INFINITY;
     uint32_t bc = kEmpty, bp = kEmpty;
+    __syncthreads();  // the lane states overwrite the sums read above
     block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp, g.lid);
     if (t == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, scew);
 }
 
+#ifndef LV_CAP7
+#define LV_CAP7 1024
+#endif
+#ifndef LV_CAP8
+#define LV_CAP8 1280
+#endif
+#ifndef LV_MINB8
+#define LV_MINB8 8
+#endif
+#ifndef LV_CAP9
+#define LV_CAP9 3072
+#endif
+#ifndef LV_CAP10
+#define LV_CAP10 5800
+#endif
+#ifndef LV_P10
+#define LV_P10 1024
+#endif
+#ifndef LV_MINB10
+#define LV_MINB10 2
+#endif
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count) {
     if (!count) return;
@@ -852,19 +883,19 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                count);
             break;
         case 7:
-            hipLaunchKernelGGL((k_move_block<256, 1024>), dim3(count), dim3(256), 0, s, g, verts,
+            hipLaunchKernelGGL((k_move_block<256, 512, LV_CAP7>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
         case 8:
-            hipLaunchKernelGGL((k_move_block<256, 2048>), dim3(count), dim3(256), 0, s, g, verts,
+            hipLaunchKernelGGL((k_move_block<256, 1024, LV_CAP8, LV_MINB8>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
         case 9:
-            hipLaunchKernelGGL((k_move_block<256, 4096>), dim3(count), dim3(256), 0, s, g, verts,
+            hipLaunchKernelGGL((k_move_block<256, 2048, LV_CAP9>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
         case 10:
-            hipLaunchKernelGGL((k_move_block<1024, 8192>), dim3(count), dim3(1024), 0, s, g, verts,
+            hipLaunchKernelGGL((k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>), dim3(count), dim3(LV_P10), 0, s, g, verts,
                                count);
             break;
         default:
