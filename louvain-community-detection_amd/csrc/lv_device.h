@@ -243,7 +243,7 @@ struct AggPart {
     uint32_t NB;             // total buckets
     uint32_t dmax;           // distinct keys a bucket may hold (<= kAggBucketDistinct)
 };
-constexpr uint32_t kAggBucketDistinct = 512;  // per bucket; its LDS table has twice the slots
+constexpr uint32_t kAggBucketDistinct = 512;  // per bucket; its LDS table has more slots
 void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
                     uint32_t *nbk);
 // phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities

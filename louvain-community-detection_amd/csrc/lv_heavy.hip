@@ -628,10 +628,10 @@ void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *ver
 }
 
 #ifndef LV_HPCAP
-#define LV_HPCAP 1536
+#define LV_HPCAP 1280
 #endif
 #ifndef LV_HPDMAX
-#define LV_HPDMAX 768
+#define LV_HPDMAX 640
 #endif
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
     if (!hin.nh || !hin.nchunks) return;

@@ -854,6 +854,9 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
 #ifndef LV_MINB10
 #define LV_MINB10 2
 #endif
+#ifndef LV_SMALLCAP5
+#define LV_SMALLCAP5 256
+#endif
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count) {
     if (!count) return;
@@ -875,7 +878,7 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                g, verts, count);
             break;
         case 5:
-            hipLaunchKernelGGL((k_move_small<32, 3, 256>), dim3((count + 7) / 8), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_move_small<32, 3, LV_SMALLCAP5>), dim3((count + 7) / 8), dim3(256), 0, s,
                                g, verts, count);
             break;
         case 6:
@@ -1254,19 +1257,20 @@ __device__ bool lds_bin_rank(uint32_t nd, uint32_t lo, uint32_t W, KeyOf keyof, 
 // lanes walk a member's row.  Output: (new_id(d), W) sorted at
 // tkey/tval[epos[c]..],
 // distinct count at cnt[new_id[c]].
-template <int P, int CAP>
+template <int P, int MAXE, int CAP>
 __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const uint32_t *comms,
                                                      uint32_t count) {
-    constexpr int LC = Log2<CAP>::value;
+    // MAXE: member entries of the class (comm_deg <= MAXE, a power of two);
+    // CAP: table slots (any size above MAXE)
+    static_assert(CAP > MAXE && (MAXE & (MAXE - 1)) == 0, "table shape");
     constexpr int NW = P / 64;
-    constexpr uint32_t MAXE = CAP / 2;  // member entries of the class (comm_deg <= CAP/2)
     constexpr uint32_t PER = (MAXE + P - 1) / P;
     __shared__ uint32_t key[CAP];
     __shared__ unsigned long long val[CAP];
-    constexpr uint32_t NBIN = CAP / 4;  // bins for the ranked order
-    __shared__ uint32_t skeys[CAP / 2];
-    __shared__ uint32_t sslot[CAP / 2];
-    __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[CAP / 2 + 1];
+    constexpr uint32_t NBIN = MAXE / 2;  // bins for the ranked order
+    __shared__ uint32_t skeys[MAXE];
+    __shared__ uint32_t sslot[MAXE];
+    __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[MAXE + 1];
     __shared__ uint32_t ndist, wsum[NW], maxbin;
     uint32_t *mb = oidx;  // member entry offsets while merging, ranks afterwards
     const uint32_t c = comms[blockIdx.x];
@@ -1314,7 +1318,7 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     for (uint32_t u = 0; u < PER; ++u) {
         if (threadIdx.x + (uint32_t)P * u >= ne) continue;
         const uint32_t d2 = dd[u];
-        uint32_t slot = hslot<LC>(d2);
+        uint32_t slot = hslot_n<CAP>(d2);
         for (int probe = 0; probe < CAP; ++probe) {
             const uint32_t prev = atomicCAS(&key[slot], kEmpty, d2);
             if (prev == kEmpty) {
@@ -1324,7 +1328,7 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
                 break;
             }
             if (prev == d2) break;
-            slot = (slot + 1) & (CAP - 1);
+            slot = slot + 1 == (uint32_t)CAP ? 0u : slot + 1;
         }
         atomicAdd(&val[slot], f2fix(ww[u], g.scale));
     }
@@ -1336,7 +1340,7 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
         a.tkey[out + pos] = skeys[i];
         a.tval[out + pos] = val[sslot[i]];
     };
-    // new ids lie in [0, nn); oidx holds up to CAP/2 distinct keys
+    // new ids lie in [0, nn); oidx holds up to MAXE distinct keys
     if (!lds_bin_rank<NBIN, P>(nd, 0u, a.nn, keyof, bcur, boffs, oidx, wsum, &maxbin, emit)) {
         uint32_t n2 = 1;
         while (n2 < nd) n2 <<= 1;
@@ -1699,6 +1703,15 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
 }
 
+#ifndef LV_AGGCAP2
+#define LV_AGGCAP2 768
+#endif
+#ifndef LV_AGGCAP3
+#define LV_AGGCAP3 1536
+#endif
+#ifndef LV_AGGCAP4
+#define LV_AGGCAP4 3072
+#endif
 void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
                       uint32_t count, int radix_passes) {
     if (!count || cls == 0) return;  // class 0: no entries, cnt stays 0
@@ -1706,13 +1719,13 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
         hipLaunchKernelGGL(k_agg_merge_small, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms,
                            count);
     else if (cls == 2)
-        hipLaunchKernelGGL((k_agg_merge_lds<256, 1024>), dim3(count), dim3(256), 0, s, g, a, comms,
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 512, LV_AGGCAP2>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
     else if (cls == 3)
-        hipLaunchKernelGGL((k_agg_merge_lds<256, 2048>), dim3(count), dim3(256), 0, s, g, a, comms,
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 1024, LV_AGGCAP3>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
     else if (cls == 4)
-        hipLaunchKernelGGL((k_agg_merge_lds<256, 4096>), dim3(count), dim3(256), 0, s, g, a, comms,
+        hipLaunchKernelGGL((k_agg_merge_lds<256, 2048, LV_AGGCAP4>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
     else {
         hipLaunchKernelGGL(k_agg_insert_chunks, dim3(a.nchunks), dim3(256), 0, s, g, a, comms,
@@ -1750,7 +1763,10 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // ---------------------------------------------------------------------------
 constexpr uint32_t kAggBucketTarget = 256;
 constexpr uint32_t kAggMaxBuckets = 4096;
-constexpr int kAggBucketCap = 2 * kAggBucketDistinct;  // LDS table slots of one bucket
+#ifndef LV_AGGBCAP
+#define LV_AGGBCAP 1024
+#endif
+constexpr int kAggBucketCap = LV_AGGBCAP;  // LDS table slots of one bucket (> kAggBucketDistinct)
 
 __device__ __forceinline__ uint32_t agg_bucket(uint32_t key, uint32_t nb, uint32_t nn) {
     return (uint32_t)(((unsigned long long)key * nb) / nn);
@@ -1910,11 +1926,11 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
 
 __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPart p,
                                                      const uint32_t *comms) {
-    constexpr int LC = Log2<kAggBucketCap>::value;
-    constexpr uint32_t DMAX = kAggBucketCap / 2;
+    static_assert(kAggBucketCap > (int)kAggBucketDistinct, "table shape");
+    constexpr uint32_t DMAX = kAggBucketDistinct;
     __shared__ uint32_t key[kAggBucketCap];
     __shared__ unsigned long long val[kAggBucketCap];
-    __shared__ uint32_t dslot[DMAX];
+    __shared__ uint16_t dslot[DMAX];
     __shared__ uint32_t skeys[DMAX], sslot[DMAX], oslot[DMAX];  // bins, or the sort
     __shared__ uint32_t wsum[4], maxbin;
     __shared__ uint32_t nd_s, over;
@@ -1949,18 +1965,18 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         for (uint32_t u = 0; u < 8; ++u) {
             if (kk[u] == kEmpty) continue;
             const uint32_t k = kk[u];
-            uint32_t slot = hslot<LC>(k);
+            uint32_t slot = hslot_n<kAggBucketCap>(k);
             int probe = 0;
             for (; probe < kAggBucketCap; ++probe) {
                 const uint32_t prev = atomicCAS(&key[slot], kEmpty, k);
                 if (prev == kEmpty) {
                     const uint32_t idx = atomicAdd(&nd_s, 1u);
-                    if (idx < p.dmax) dslot[idx] = slot;
+                    if (idx < p.dmax) dslot[idx] = (uint16_t)slot;
                     else over = 1u;  // more distinct keys than a bucket holds
                     break;
                 }
                 if (prev == k) break;
-                slot = (slot + 1) & (kAggBucketCap - 1);
+                slot = slot + 1 == (uint32_t)kAggBucketCap ? 0u : slot + 1;
             }
             if (probe == kAggBucketCap) {
                 over = 1u;
