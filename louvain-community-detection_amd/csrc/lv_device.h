@@ -201,6 +201,7 @@ size_t scan_tmp_elems(uint32_t n);
 // aggregation
 struct AggView {
     uint32_t *new_id;       // n+1 (exclusive scan of comm_sz != 0)
+    uint32_t *vnid;         // n: new id of each vertex's community, new_id[comm[u]]
     uint32_t *comm_deg;     // n
     uint32_t *epos;         // n+1 (exclusive scan of comm_deg)
     uint32_t *mstart;       // n+1 (exclusive scan of comm_sz)
@@ -269,6 +270,9 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
                      uint32_t count, const uint32_t *only, uint32_t *row2, uint32_t *col2,
                      float *w2);
+// vnid[u] = new_id[comm[u]] for u < n: one gather per entry in the merges
+void launch_agg_vnid(hipStream_t s, const uint32_t *comm, const uint32_t *new_id, uint32_t n,
+                     uint32_t *vnid);
 void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
                         uint32_t *init_comm, uint32_t n0);
 

@@ -132,6 +132,7 @@ struct lv_graph {
     uint32_t *dlist = nullptr, *eslot = nullptr;
     // aggregation scratch
     uint32_t *new_id = nullptr, *comm_deg = nullptr, *epos = nullptr, *mstart = nullptr;
+    uint32_t *vnid = nullptr;  // new id of each vertex's community
     uint32_t *mcursor = nullptr, *members = nullptr, *cnt = nullptr, *aorder = nullptr;
     uint32_t *tkey = nullptr, *skey = nullptr;
     ull *tval = nullptr, *sval = nullptr;
@@ -691,6 +692,7 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
 lv::AggView aview(lv_graph *g) {
     lv::AggView a;
     a.new_id = g->new_id;
+    a.vnid = g->vnid;
     a.comm_deg = g->comm_deg;
     a.epos = g->epos;
     a.mstart = g->mstart;
@@ -834,6 +836,7 @@ int aggregation(lv_graph *g) {
                                   g->scan_tmp32);
     else
         lv::launch_scan_u32(s, g->comm_sz, 1, n, g->new_id, g->scan_tmp32);
+    lv::launch_agg_vnid(s, g->comm, g->new_id, n, g->vnid);
     lv::launch_scan_u32(s, g->comm_sz, 0, n, g->mstart, g->scan_tmp32);
     lv::launch_scan_u32(s, g->comm_deg, 0, n, g->epos, g->scan_tmp32);
     lv::launch_agg_members(s, v, aview(g));
@@ -1182,6 +1185,7 @@ void free_all(lv_graph *g) {
     dfree(g->dlist);
     dfree(g->eslot);
     dfree(g->new_id);
+    dfree(g->vnid);
     dfree(g->comm_deg);
     dfree(g->epos);
     dfree(g->mstart);
@@ -1308,6 +1312,7 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
     e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
     e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
+    e |= dmalloc(&g->vnid, N);
     e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);
     e |= dmalloc(&g->cnt, N) | dmalloc(&g->aorder, N);
     e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |

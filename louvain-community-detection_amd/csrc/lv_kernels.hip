@@ -1212,10 +1212,14 @@ __device__ bool lds_bin_rank(uint32_t nd, uint32_t lo, uint32_t W, KeyOf keyof, 
     for (uint32_t i = t; i < NB; i += P) cur[i] = 0u;
     if (t == 0) *maxbin = 0u;
     __syncthreads();
+    uint32_t mx = 0;  // largest bin this thread saw; one LDS max per wave
     for (uint32_t i = t; i < nd; i += P) {
         const uint32_t c = atomicAdd(&cur[(keyof(i) - lo) >> sh], 1u);
-        atomicMax(maxbin, c + 1u);
+        mx = max(mx, c + 1u);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+    if (lane == 0 && mx) atomicMax(maxbin, mx);
     __syncthreads();
     if (*maxbin > kBinMax) return false;
     uint32_t cnt[PER], own = 0;
@@ -1286,8 +1290,8 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     for (uint32_t q = threadIdx.x; q <= msz; q += P) mb[q] = a.moff[m0 + q] - f0;
     __syncthreads();
     // every entry's loads before any insert: member, row start, column and
-    // weight, neighbour community, its new id.  Positions past the end repeat
-    // entry 0 (ne >= 1), so no load waits on a branch.
+    // weight, new id of the neighbour's community.  Positions past the end
+    // repeat entry 0 (ne >= 1), so no load waits on a branch.
     uint32_t vv[PER], jj[PER], dd[PER], nid[PER];
     float ww[PER];
 #pragma unroll
@@ -1311,9 +1315,9 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
         ww[u] = g.w[jj[u]];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < PER; ++u) dd[u] = g.comm[dd[u]];
+    for (uint32_t u = 0; u < PER; ++u) dd[u] = a.vnid[dd[u]];  // keyed by new id
 #pragma unroll
-    for (uint32_t u = 0; u < PER; ++u) nid[u] = a.new_id[dd[u]];
+    for (uint32_t u = 0; u < PER; ++u) nid[u] = dd[u];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
         if (threadIdx.x + (uint32_t)P * u >= ne) continue;
@@ -1392,14 +1396,14 @@ __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
             const uint32_t v = a.members[m0 + mi];
             const uint32_t b = g.row[v], e = g.row[v + 1];
             for (uint32_t j = b + lane; j < e; j += 64) {
-                const uint32_t dd = g.comm[g.col[j]];
+                const uint32_t dd = a.vnid[g.col[j]];  // keyed by new id
                 const unsigned long long wf = f2fix(g.w[j], g.scale);
                 uint32_t slot = hslot<LC>(dd);
                 for (int probe = 0; probe < CAP; ++probe) {
                     const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
                     if (prev == kEmpty) {
                         const uint32_t idx = atomicAdd(&nd_all[wv], 1u);
-                        sk[idx] = a.new_id[dd];  // sorted by new id
+                        sk[idx] = dd;  // sorted by new id
                         ss[idx] = slot;
                         break;
                     }
@@ -1539,7 +1543,7 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
         const uint32_t f = f0 + threadIdx.x + 256u * q;
         if (f < f1) {
             const uint32_t j = agg_entry(g, a, x, mb, f);
-            const uint32_t dd = g.comm[g.col[j]];
+            const uint32_t dd = a.vnid[g.col[j]];  // keyed by new id
             const unsigned long long wf = f2fix(g.w[j], g.scale);
             uint32_t slot = hslot<LC>(dd);
             for (int probe = 0; probe < LCAP; ++probe) {
@@ -1585,7 +1589,7 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
         atomicAdd(&val[slot], lval[ls]);
         if (claimed) {
             my_new[nmine] = slot;
-            my_key[nmine] = a.new_id[dd];  // sorted by new id
+            my_key[nmine] = dd;  // sorted by new id
             my_rank[nmine] = atomicAdd(&nnew, 1u);
             ++nmine;
         }
@@ -1800,8 +1804,9 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
     const uint32_t nb = p.nbk[x.ci];
     constexpr uint32_t PER = kAggChunk / 256;
     // each dependent load level for all of the thread's entries at once:
-    // member, row start, column + weight, community, new id.  Positions past
-    // the chunk end repeat its first entry, so no load waits on a branch.
+    // member, row start, column + weight, new id of the neighbour's
+    // community.  Positions past the chunk end repeat its first entry, so no
+    // load waits on a branch.
     uint32_t vv[PER], jj[PER], kk[PER];
     float ww[PER];
 #pragma unroll
@@ -1825,9 +1830,7 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
         ww[u] = g.w[jj[u]];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < PER; ++u) kk[u] = g.comm[kk[u]];
-#pragma unroll
-    for (uint32_t u = 0; u < PER; ++u) kk[u] = a.new_id[kk[u]];
+    for (uint32_t u = 0; u < PER; ++u) kk[u] = a.vnid[kk[u]];
 #pragma unroll
     for (uint32_t u = 0; u < PER; ++u) {
         const uint32_t f = x.f0 + threadIdx.x + 256u * u;
@@ -2126,6 +2129,16 @@ void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t 
 
 // graphVertToComm + graphVertToNewComm (modularity.cu:51-56,
 // aggregation.cu:380-385) fused: init_comm <- new_id[comm[init_comm]]
+__global__ void k_agg_vnid(const uint32_t *comm, const uint32_t *new_id, uint32_t n,
+                           uint32_t *vnid) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) vnid[u] = new_id[comm[u]];
+}
+void launch_agg_vnid(hipStream_t s, const uint32_t *comm, const uint32_t *new_id, uint32_t n,
+                     uint32_t *vnid) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_agg_vnid, dim3((n + 255) / 256), dim3(256), 0, s, comm, new_id, n, vnid);
+}
 __global__ void k_agg_compose(const uint32_t *comm, const uint32_t *new_id, uint32_t *init_comm,
                               uint32_t n0) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

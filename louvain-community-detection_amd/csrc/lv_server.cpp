@@ -213,7 +213,7 @@ extern "C" {
 uint64_t lv_graph_footprint(uint32_t n, uint64_t ne) {
     // every buffer lv_graph_create allocates, plus the largest hash arenas a
     // pass can request (bounded by 4x the entries)
-    return 184ull * ((uint64_t)n + 1) + 184ull * ne + (1ull << 20);
+    return 188ull * ((uint64_t)n + 1) + 184ull * ne + (1ull << 20);
 }
 
 int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, float min_gain,
