@@ -368,18 +368,21 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
 // run of entries is written to its range with consecutive addresses.
 __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
     static_assert(kMaxBuckets == 4 * 256, "four buckets per thread in the offset scan");
-    __shared__ uint32_t lcnt[kMaxBuckets];   // entries of the chunk per bucket
-    __shared__ uint32_t loff[kMaxBuckets];   // their offset in the bucket-ordered chunk
-    __shared__ uint32_t lbase[kMaxBuckets];  // their position in the bucket's range
+    static_assert(kHeavyChunk < 65536, "16-bit chunk counts and offsets");
+    // per bucket: entries of the chunk (16-bit halves of 32-bit words, so a
+    // 32-bit LDS atomic counts two buckets), their offset in the
+    // bucket-ordered chunk, their position in the bucket's range
+    __shared__ uint32_t lcnt2[kMaxBuckets / 2];
+    __shared__ uint16_t loff[kMaxBuckets];
+    __shared__ uint32_t lbase[kMaxBuckets];
     __shared__ uint32_t sc[kHeavyChunk], sj[kHeavyChunk];
     __shared__ float sw[kHeavyChunk];
-    __shared__ uint16_t sb[kHeavyChunk];
-    __shared__ uint32_t wsum[4];
+    uint32_t *wsum = sc;  // scan partials; read before the staging below starts
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;
     const uint32_t nb = h.nbuck[x.i];
     const int lb = 31 - __clz(nb);
-    for (uint32_t b = threadIdx.x; b < kMaxBuckets; b += 256) lcnt[b] = 0;
+    for (uint32_t b = threadIdx.x; b < kMaxBuckets / 2; b += 256) lcnt2[b] = 0;
     __syncthreads();
     const uint32_t j0 = x.c * kHeavyChunk;
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
@@ -400,14 +403,15 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
         bk[q] = kEmpty;
         if (j < j1) {
             bk[q] = bucket_of(cc[q], lb);
-            rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
+            const uint32_t sh = (bk[q] & 1u) << 4;
+            rank[q] = (atomicAdd(&lcnt2[bk[q] >> 1], 1u << sh) >> sh) & 0xffffu;
         }
     }
     __syncthreads();
-    // exclusive scan of lcnt: thread t owns buckets 4t..4t+3
+    // exclusive scan of the counts: thread t owns buckets 4t..4t+3
     const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const uint32_t c0 = lcnt[4 * t], c1 = lcnt[4 * t + 1], c2 = lcnt[4 * t + 2],
-                   c3 = lcnt[4 * t + 3];
+    const uint32_t p01 = lcnt2[2 * t], p23 = lcnt2[2 * t + 1];
+    const uint32_t c0 = p01 & 0xffffu, c1 = p01 >> 16, c2 = p23 & 0xffffu, c3 = p23 >> 16;
     const uint32_t own = c0 + c1 + c2 + c3;
     uint32_t inc = own;
 #pragma unroll
@@ -417,15 +421,17 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
     }
     if (lane == 63) wsum[wv] = inc;
     const uint32_t b0 = h.bstart[x.i];
-    for (uint32_t b = t; b < nb; b += 256)
-        if (lcnt[b]) lbase[b] = atomicAdd(&h.bcur[b0 + b], lcnt[b]);
+    for (uint32_t b = t; b < nb; b += 256) {
+        const uint32_t cb = (lcnt2[b >> 1] >> ((b & 1u) << 4)) & 0xffffu;
+        if (cb) lbase[b] = atomicAdd(&h.bcur[b0 + b], cb);
+    }
     __syncthreads();
     uint32_t base = inc - own;
     for (uint32_t k = 0; k < wv; ++k) base += wsum[k];
-    loff[4 * t] = base;
-    loff[4 * t + 1] = base + c0;
-    loff[4 * t + 2] = base + c0 + c1;
-    loff[4 * t + 3] = base + c0 + c1 + c2;
+    loff[4 * t] = (uint16_t)base;
+    loff[4 * t + 1] = (uint16_t)(base + c0);
+    loff[4 * t + 2] = (uint16_t)(base + c0 + c1);
+    loff[4 * t + 3] = (uint16_t)(base + c0 + c1 + c2);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
@@ -435,13 +441,13 @@ __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
         sc[sp] = cc[q];
         sj[sp] = j;
         sw[sp] = wq[q];
-        sb[sp] = (uint16_t)bk[q];
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-        const uint32_t b = sb[i];
+        const uint32_t c = sc[i];
+        const uint32_t b = bucket_of(c, lb);
         const uint32_t pos = lbase[b] + (i - loff[b]);
-        h.pc[pos] = sc[i];
+        h.pc[pos] = c;
         h.pw[pos] = sw[i];
         h.pj[pos] = sj[i];
     }

@@ -1765,7 +1765,10 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // A bucket with more distinct keys than its table holds flags its community,
 // which is then merged by the global-table kernels above instead.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kAggBucketTarget = 256;
+#ifndef LV_AGGTARGET
+#define LV_AGGTARGET 384
+#endif
+constexpr uint32_t kAggBucketTarget = LV_AGGTARGET;
 constexpr uint32_t kAggMaxBuckets = 4096;
 #ifndef LV_AGGBCAP
 #define LV_AGGBCAP 1024
@@ -1850,11 +1853,17 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
 __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, const uint32_t *comms,
                                                       uint32_t count) {
     constexpr uint32_t PER = kAggMaxBuckets / 256;
-    __shared__ uint32_t lcnt[kAggMaxBuckets], loff[kAggMaxBuckets], lbase[kAggMaxBuckets];
+    static_assert(PER % 2 == 0 && kAggChunk < 65536, "16-bit counts, in pairs");
+    // per bucket: entries of the chunk (16-bit halves of 32-bit words, so a
+    // 32-bit LDS atomic counts two buckets), their offset in the
+    // bucket-ordered chunk, their position in the bucket's range
+    __shared__ uint32_t lcnt2[kAggMaxBuckets / 2];
+    __shared__ uint16_t loff[kAggMaxBuckets];
+    __shared__ uint32_t lbase[kAggMaxBuckets];
     __shared__ uint32_t sk[kAggChunk];
     __shared__ float sw[kAggChunk];
-    __shared__ uint16_t sb[kAggChunk];
-    __shared__ uint32_t wsum[4];
+    uint32_t *wsum = sk;  // scan partials; read before the staging below starts
+    auto count_of = [&](uint32_t b) { return (lcnt2[b >> 1] >> ((b & 1u) << 4)) & 0xffffu; };
     uint32_t lo = 0, hi = count;  // chunk -> large community
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -1866,7 +1875,7 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
     const uint32_t f0 = cb + (blockIdx.x - a.coff[ci]) * kAggChunk;
     const uint32_t cnt = min(ce, f0 + kAggChunk) - f0;
     const uint32_t nb = p.nbk[ci];
-    for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) lcnt[b] = 0;
+    for (uint32_t b = threadIdx.x; b < kAggMaxBuckets / 2; b += 256) lcnt2[b] = 0;
     __syncthreads();
     uint32_t key[kAggChunk / 256], rank[kAggChunk / 256], bk[kAggChunk / 256];
     float wq[kAggChunk / 256];
@@ -1884,14 +1893,15 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
         bk[q] = kEmpty;
         if (i < cnt) {
             bk[q] = agg_bucket(key[q], nb, p.nn);
-            rank[q] = atomicAdd(&lcnt[bk[q]], 1u);
+            const uint32_t sh = (bk[q] & 1u) << 4;
+            rank[q] = (atomicAdd(&lcnt2[bk[q] >> 1], 1u << sh) >> sh) & 0xffffu;
         }
     }
     __syncthreads();
-    // exclusive scan of lcnt: thread t owns buckets PER*t .. PER*t+PER-1
+    // exclusive scan of the counts: thread t owns buckets PER*t .. PER*t+PER-1
     const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
     uint32_t own = 0;
-    for (uint32_t k = 0; k < PER; ++k) own += lcnt[PER * t + k];
+    for (uint32_t k = 0; k < PER; ++k) own += count_of(PER * t + k);
     uint32_t inc = own;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1900,14 +1910,16 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
     }
     if (lane == 63) wsum[wv] = inc;
     const uint32_t b0 = p.bstart[ci];
-    for (uint32_t b = t; b < nb; b += 256)
-        if (lcnt[b]) lbase[b] = atomicAdd(&p.bcur[b0 + b], lcnt[b]);
+    for (uint32_t b = t; b < nb; b += 256) {
+        const uint32_t cb2 = count_of(b);
+        if (cb2) lbase[b] = atomicAdd(&p.bcur[b0 + b], cb2);
+    }
     __syncthreads();
     uint32_t run = inc - own;
     for (uint32_t k = 0; k < wv; ++k) run += wsum[k];
     for (uint32_t k = 0; k < PER; ++k) {
-        loff[PER * t + k] = run;
-        run += lcnt[PER * t + k];
+        loff[PER * t + k] = (uint16_t)run;
+        run += count_of(PER * t + k);
     }
     __syncthreads();
 #pragma unroll
@@ -1916,13 +1928,13 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
         const uint32_t sp = loff[bk[q]] + rank[q];
         sk[sp] = key[q];
         sw[sp] = wq[q];
-        sb[sp] = (uint16_t)bk[q];
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-        const uint32_t b = sb[i];
+        const uint32_t k = sk[i];
+        const uint32_t b = agg_bucket(k, nb, p.nn);
         const uint32_t pos = lbase[b] + (i - loff[b]);
-        p.pkey[pos] = sk[i];
+        p.pkey[pos] = k;
         p.pw[pos] = sw[i];
     }
 }
