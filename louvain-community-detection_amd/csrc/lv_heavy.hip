@@ -499,6 +499,12 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     if (!v_live(g, v)) return;
     const uint32_t p0 = h.bpos[blockIdx.x], p1 = h.bpos[blockIdx.x + 1];
     if (p0 == p1) return;
+    // the vertex's own state, loaded now so its latency overlaps the inserts
+    const float m = v_m(g, v);
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const unsigned long long acvx = g.A[cv];
     for (int s2 = threadIdx.x; s2 < kBucketCap; s2 += 256) {
         key[s2] = kEmpty;
         val[s2] = 0ull;
@@ -542,11 +548,7 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
         return;
     }
     const uint32_t nd = nd_s;
-    const float m = v_m(g, v);
-    const uint32_t cv = g.comm[v];
-    const uint32_t szv = g.comm_sz[cv];
-    const float kv = g.k[v];
-    const float acv = fix2f(g.A[cv], g.S);
+    const float acv = fix2f(acvx, g.S);
     distinct_gains(g, dist, nd, threadIdx.x, 256u, key, val, m, kv, acv, szv, cv);
     __syncthreads();
     auto lane_max = [&](uint32_t slot, uint32_t j) {

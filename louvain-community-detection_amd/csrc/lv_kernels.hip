@@ -1954,6 +1954,11 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
     const uint32_t q0 = p.bpos[b], q1 = p.bpos[b + 1];
     if (threadIdx.x == 0) p.bnd[b] = 0;
     if (q0 == q1) return;  // uniform
+    // where the bucket's output goes and its key range, loaded now so their
+    // latency overlaps the inserts
+    const uint32_t bs = p.bstart[ci];
+    const uint32_t out = a.epos[c] + (q0 - p.bpos[bs]);
+    const unsigned long long nb = p.nbk[ci], bb = b - bs;
     for (uint32_t i = threadIdx.x; i < kAggBucketCap; i += 256) {
         key[i] = kEmpty;
         val[i] = 0ull;
@@ -2006,9 +2011,7 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         return;
     }
     const uint32_t nd = nd_s;
-    const uint32_t out = a.epos[c] + (q0 - p.bpos[p.bstart[ci]]);
     // the bucket's key range: keys k with floor(k * nb / nn) == bb
-    const unsigned long long nb = p.nbk[ci], bb = b - p.bstart[ci];
     const uint32_t lo = (uint32_t)((bb * p.nn + nb - 1) / nb);
     const uint32_t hi = (uint32_t)(((bb + 1) * p.nn + nb - 1) / nb);
     auto keyof = [&](uint32_t i) { return key[dslot[i]]; };
