@@ -87,6 +87,11 @@ struct BatchGraph {
 struct lv_graph {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the handle: bucket step 7 runs the heavy rows' kernel
+    // chain beside the block classes (fork and join by events; captured into
+    // the sweep graph as two branches)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t n = 0, n0 = 0;
     uint64_t ne = 0, ne0 = 0;
     uint32_t n_cap = 0;  // buffer capacities (lv_graph_create: the input's size)
@@ -251,8 +256,10 @@ struct ProfScope {
     lv_graph *g;
     int fam;
     double bytes;
+    hipStream_t st;
     hipEvent_t a = nullptr;
-    ProfScope(lv_graph *g_, int fam_, double bytes_) : g(g_), fam(fam_), bytes(bytes_) {
+    ProfScope(lv_graph *g_, int fam_, double bytes_, hipStream_t st_ = nullptr)
+        : g(g_), fam(fam_), bytes(bytes_), st(st_ ? st_ : g_->stream) {
         if (g->prof_on && ((g->prof_mask >> fam) & 1u)) {
             a = ev_get(g);
             if (a) record(a);
@@ -262,9 +269,9 @@ struct ProfScope {
     // event carries no timestamp after a replay
     void record(hipEvent_t e) {
         if (g->capture_recs)
-            (void)hipEventRecordWithFlags(e, g->stream, hipEventRecordExternal);
+            (void)hipEventRecordWithFlags(e, st, hipEventRecordExternal);
         else
-            (void)hipEventRecord(e, g->stream);
+            (void)hipEventRecord(e, st);
     }
     ~ProfScope() {
         if (g->prof_on && a) {
@@ -445,6 +452,18 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 // masked sums skip class 0: an empty row adds nothing.
 int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
+    // the heavy rows' staged sums run on the second stream beside the class
+    // groups (every group adds its exact total into the counters atomically)
+    const bool fork = g->cls_count[lv::kClassHeavy] != 0 &&
+                      g->cls_off[lv::kClassHeavy] > g->cls_off[masked ? 1 : 0];
+    if (fork) {
+        (void)hipEventRecord(g->ev_fork, g->stream);
+        (void)hipStreamWaitEvent(g->stream2, g->ev_fork, 0);
+        ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy],
+                     g->stream2);
+        lv::launch_heavy_r1(g->stream2, v, heavy_view(g), (float *)g->eslot, masked, init_state,
+                            acc_ptr(g));
+    }
     const struct {
         int G, B, c0, c1;
     } grp[5] = {{4, 1, 0, 1}, {8, 1, 2, 2}, {16, 1, 3, 3}, {32, 4, 4, 5},
@@ -458,7 +477,10 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, q.B, masked, init_state,
                       acc_ptr(g));
     }
-    if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
+    if (fork) {
+        (void)hipEventRecord(g->ev_join, g->stream2);
+        (void)hipStreamWaitEvent(g->stream, g->ev_join, 0);
+    } else if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy]);
         lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
                             acc_ptr(g));
@@ -555,13 +577,29 @@ int step(lv_graph *g, int b) {
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_GLOBAL};
+    // the heavy rows' chain runs on the second stream beside the block
+    // classes of the same step (they only read the step-start snapshot and
+    // write new_comm of their own vertices)
+    bool fork = false;
+    if (c1 == lv::kClassHeavy && g->cls_count[lv::kClassHeavy]) {
+        for (int c = c0; c < lv::kClassHeavy; ++c) fork |= g->cls_count[c] != 0;
+    }
+    if (fork) {
+        (void)hipEventRecord(g->ev_fork, g->stream);
+        (void)hipStreamWaitEvent(g->stream2, g->ev_fork, 0);
+    }
     for (int c = c0; c <= c1; ++c) {
         if (!g->cls_count[c]) continue;
-        ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c]);
+        hipStream_t st = (fork && c == lv::kClassHeavy) ? g->stream2 : g->stream;
+        ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c], st);
         if (c == lv::kClassHeavy)
-            lv::launch_heavy_move(g->stream, v, heavy_view(g));
+            lv::launch_heavy_move(st, v, heavy_view(g));
         else
-            lv::launch_move_class(g->stream, v, c, g->order + g->cls_off[c], g->cls_count[c]);
+            lv::launch_move_class(st, v, c, g->order + g->cls_off[c], g->cls_count[c]);
+    }
+    if (fork) {
+        (void)hipEventRecord(g->ev_join, g->stream2);
+        (void)hipStreamWaitEvent(g->stream, g->ev_join, 0);
     }
     const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
     if (e > a) {
@@ -865,18 +903,28 @@ int aggregation(lv_graph *g) {
     lv::launch_agg_mdeg(s, v, aview(g));
     lv::launch_scan_u32(s, g->agg_mdeg, 0, n, g->agg_moff, g->scan_tmp32);
     lv::AggView av = aview(g);
-    for (int c = 0; c < lv::kAggGlobalClass; ++c)
-        lv::launch_agg_merge(s, v, av, c, g->aorder + aoff[c], aoff[c + 1] - aoff[c], passes);
-    // large communities: partitioned merge, global tables for overflowing ones
+    // large communities: partitioned merge, global tables for overflowing ones;
+    // the mid-size classes merge on the second stream meanwhile (disjoint
+    // output ranges, no shared scratch)
     const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
     const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
+    const bool fork = nl && aoff[lv::kAggGlobalClass] > aoff[1];
+    hipStream_t sm = fork ? g->stream2 : s;
+    if (fork) {
+        LV_HIP(hipEventRecord(g->ev_fork, s));
+        LV_HIP(hipStreamWaitEvent(g->stream2, g->ev_fork, 0));
+    }
+    for (int c = 0; c < lv::kAggGlobalClass; ++c)
+        lv::launch_agg_merge(sm, v, av, c, g->aorder + aoff[c], aoff[c + 1] - aoff[c], passes);
     lv::AggPart ap = {};
     g->agg_nchunks = 0;
-    if (nl) {
-        int rc2 = aggregate_large(g, v, lc, nl, nn, passes, &ap);
-        if (rc2) return rc2;
-        av = aview(g);
+    int rc2 = nl ? aggregate_large(g, v, lc, nl, nn, passes, &ap) : LV_OK;
+    if (fork) {  // join, also when the large merge failed
+        LV_HIP(hipEventRecord(g->ev_join, g->stream2));
+        LV_HIP(hipStreamWaitEvent(s, g->ev_join, 0));
     }
+    if (rc2) return rc2;
+    if (nl) av = aview(g);
     // new row pointers from distinct counts
     lv::launch_scan_u32(s, g->cnt, 0, nn, g->row2, g->scan_tmp32);
     uint32_t ne2 = 0;
@@ -1219,6 +1267,11 @@ void free_all(lv_graph *g) {
     dfree(g->gcnt);
     if (g->gcnt_h) (void)hipHostFree(g->gcnt_h);
     g->gcnt_h = nullptr;
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    g->ev_fork = g->ev_join = nullptr;
+    if (g->stream2) (void)hipStreamDestroy(g->stream2);
+    g->stream2 = nullptr;
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
 }
@@ -1294,6 +1347,14 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     g->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess) {
         (void)hipGetLastError();
+        delete g;
+        return LV_ERR_HIP;
+    }
+    if (hipStreamCreate(&g->stream2) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        free_all(g);
         delete g;
         return LV_ERR_HIP;
     }
