@@ -584,13 +584,17 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
 #endif
 constexpr int kWaveCap = LV_WAVECAP;  // slots per wave (at most 319 distinct keys)
 constexpr int kWaveMaxPer = 5;        // ceil(319 / 64)
-__global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *verts,
-                                                   uint32_t count) {
-    __shared__ uint32_t skey[4 * kWaveCap];
-    __shared__ unsigned long long sval[4 * kWaveCap];
-    __shared__ uint16_t sdist[4][320];
-    __shared__ uint32_t sndist[4];
-    __shared__ unsigned long long scew[4];
+#ifndef LV_WAVE_WPB
+#define LV_WAVE_WPB 4
+#endif
+constexpr int kWaveWpb = LV_WAVE_WPB;  // waves (vertices) per workgroup
+__global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
+                                                             uint32_t count) {
+    __shared__ uint32_t skey[kWaveWpb * kWaveCap];
+    __shared__ unsigned long long sval[kWaveWpb * kWaveCap];
+    __shared__ uint16_t sdist[kWaveWpb][320];
+    __shared__ uint32_t sndist[kWaveWpb];
+    __shared__ unsigned long long scew[kWaveWpb];
     const int wv = threadIdx.x >> 6, p = threadIdx.x & 63;
     uint32_t *key = skey + wv * kWaveCap;
     unsigned long long *val = sval + wv * kWaveCap;
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(256) void k_move_wave(DevView g, const uint32_t *ve
         scew[wv] = 0;
     }
     // waves past the end read the last vertex (count >= 1) and write nothing
-    const uint32_t vi = blockIdx.x * 4 + wv;
+    const uint32_t vi = blockIdx.x * kWaveWpb + wv;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
@@ -882,8 +886,8 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                g, verts, count);
             break;
         case 6:
-            hipLaunchKernelGGL(k_move_wave, dim3((count + 3) / 4), dim3(256), 0, s, g, verts,
-                               count);
+            hipLaunchKernelGGL(k_move_wave, dim3((count + kWaveWpb - 1) / kWaveWpb),
+                               dim3(64 * kWaveWpb), 0, s, g, verts, count);
             break;
         case 7:
             hipLaunchKernelGGL((k_move_block<256, 512, LV_CAP7>), dim3(count), dim3(256), 0, s, g, verts,
