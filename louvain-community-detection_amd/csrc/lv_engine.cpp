@@ -1362,8 +1362,10 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     g->e_cap = e_cap;
     const size_t N = (size_t)n_cap + 1, E = (size_t)e_cap;
     int e = 0;
-    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E) | dmalloc(&g->w, E);
-    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E) | dmalloc(&g->w2, E);
+    // 4 entries of padding: 16-byte loads of the masked sums may start
+    // inside the last row and end past it
+    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E + 4) | dmalloc(&g->w, E + 4);
+    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E + 4) | dmalloc(&g->w2, E + 4);
     e |= dmalloc(&g->row0, N) | dmalloc(&g->col0, E) | dmalloc(&g->w0, E);
     e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
     e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);

@@ -378,10 +378,106 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
     block_add(g, acc, slot, fixed, valid);  // exact, order-free
 }
 
+// Masked sum of rows of more than 32 entries, 32 lanes per vertex: the
+// columns are read four per lane with 16-byte loads from the 4-aligned
+// position at or below the row start (BK blocks of 128 positions per round),
+// the loading lane gathers the communities and loads the kept weights, and
+// the per-entry values are exchanged through LDS so that lane p adds entries
+// p, p+32, p+64, ... of the row in order — the same pinned sequence as
+// k_r1<32, true, B>. Positions outside the row contribute nothing (their
+// column stands in as v itself, always in bounds); non-kept entries add +0,
+// which leaves the partial unchanged. The column arrays carry 4 entries of
+// padding, so a 16-byte load that starts inside the last row stays inside
+// the allocation.
+template <int BK>
+__global__ __launch_bounds__(256) void k_r1v(DevView g, const uint32_t *verts, uint32_t count,
+                                             int ident, unsigned long long *acc) {
+    __shared__ float xs[8][BK * 128];
+    const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    float *xg = xs[grp];
+    const uint32_t vi = blockIdx.x * 8 + grp;
+    unsigned long long fixed = 0;
+    uint32_t slot = 0;
+    bool valid = false;
+    const uint32_t v = vi < count ? verts[vi] : 0u;
+    if (vi < count && v_live(g, v)) {
+        const uint32_t base = g.row[v], end = g.row[v + 1];
+        const uint32_t cv = g.comm[v];
+        const uint32_t a0 = base & ~3u, rot = (lane + (base & 3u)) & 31u;
+        float p = 0.0f;
+        for (uint32_t a = a0; a < end; a += BK * 128u) {
+            uint4 c4[BK];
+#pragma unroll
+            for (int k = 0; k < BK; ++k) {
+                const uint32_t q = a + 128u * k + 4u * lane;
+                c4[k] = *reinterpret_cast<const uint4 *>(g.col + (q < end ? q : a0));
+            }
+            uint32_t c[BK][4];
+#pragma unroll
+            for (int k = 0; k < BK; ++k) {
+                const uint32_t q = a + 128u * k + 4u * lane;
+                c[k][0] = q + 0u >= base && q + 0u < end ? c4[k].x : v;
+                c[k][1] = q + 1u >= base && q + 1u < end ? c4[k].y : v;
+                c[k][2] = q + 2u >= base && q + 2u < end ? c4[k].z : v;
+                c[k][3] = q + 3u >= base && q + 3u < end ? c4[k].w : v;
+            }
+            if (!ident) {  // ident: every community is a singleton, comm[u] = u
+#pragma unroll
+                for (int k = 0; k < BK; ++k)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) c[k][i] = g.comm[c[k][i]];
+            }
+            float x[BK][4];
+#pragma unroll
+            for (int k = 0; k < BK; ++k)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t q = a + 128u * k + 4u * lane + i;
+                    // v's own stand-in never passes: the position test fails
+                    const bool keep = q >= base && q < end && c[k][i] == cv;
+                    x[k][i] = 0.0f;
+                    if (keep) x[k][i] = g.w[q];
+                }
+#pragma unroll
+            for (int k = 0; k < BK; ++k)
+                *reinterpret_cast<float4 *>(xg + 128 * k + 4 * lane) =
+                    make_float4(x[k][0], x[k][1], x[k][2], x[k][3]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int k = 0; k < BK; ++k)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) p += xg[128 * k + 32 * m + rot];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) p += __shfl_down(p, o, 32);
+        if (lane == 0) {
+            fixed = f2fix(p, g.scale);
+            slot = acc_slot(g, v);
+            valid = true;
+        }
+    }
+    block_add(g, acc, slot, fixed, valid);  // exact, order-free
+}
+
 void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
                int batch, int masked, int init_state, unsigned long long *acc) {
     const int ident = masked == 2;
     if (!count) return;
+#ifndef LV_NO_R1V
+    if (masked && group == 32) {
+        dim3 grid((count + 7) / 8), block(256);
+        if (batch == 8)
+            hipLaunchKernelGGL(k_r1v<2>, grid, block, 0, s, g, verts, count, ident, acc);
+        else
+            hipLaunchKernelGGL(k_r1v<1>, grid, block, 0, s, g, verts, count, ident, acc);
+        return;
+    }
+#endif
     const unsigned gpb = 256 / group;
     dim3 grid((count + gpb - 1) / gpb), block(256);
 #define LV_R1(GG, BB)                                                                         \
