@@ -251,6 +251,17 @@ hipEvent_t ev_get(lv_graph *g) {
     return e;
 }
 
+// Whether independent kernel chains may run on the handle's second stream.
+// While kernel-family timing is on they do not: a family's event window
+// would otherwise also cover the other stream's kernels running beside it,
+// and its durations would not be those of its own kernels. LV_ONE_STREAM
+// keeps every chain on the one stream for profiler runs that must see the
+// same kernel durations.
+bool use_fork(const lv_graph *g) {
+    static const bool off = getenv("LV_ONE_STREAM") != nullptr;
+    return !off && !g->prof_on;
+}
+
 // RAII-free profiling bracket around a launch sequence of one family
 struct ProfScope {
     lv_graph *g;
@@ -454,7 +465,7 @@ int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
     // the heavy rows' staged sums run on the second stream beside the class
     // groups (every group adds its exact total into the counters atomically)
-    const bool fork = g->cls_count[lv::kClassHeavy] != 0 &&
+    const bool fork = use_fork(g) && g->cls_count[lv::kClassHeavy] != 0 &&
                       g->cls_off[lv::kClassHeavy] > g->cls_off[masked ? 1 : 0];
     if (fork) {
         (void)hipEventRecord(g->ev_fork, g->stream);
@@ -581,7 +592,7 @@ int step(lv_graph *g, int b) {
     // classes of the same step (they only read the step-start snapshot and
     // write new_comm of their own vertices)
     bool fork = false;
-    if (c1 == lv::kClassHeavy && g->cls_count[lv::kClassHeavy]) {
+    if (c1 == lv::kClassHeavy && g->cls_count[lv::kClassHeavy] && use_fork(g)) {
         for (int c = c0; c < lv::kClassHeavy; ++c) fork |= g->cls_count[c] != 0;
     }
     if (fork) {
@@ -908,7 +919,7 @@ int aggregation(lv_graph *g) {
     // output ranges, no shared scratch)
     const uint32_t nl = g->hc->acounts[lv::kAggGlobalClass];
     const uint32_t *lc = g->aorder + aoff[lv::kAggGlobalClass];
-    const bool fork = nl && aoff[lv::kAggGlobalClass] > aoff[1];
+    const bool fork = use_fork(g) && nl && aoff[lv::kAggGlobalClass] > aoff[1];
     hipStream_t sm = fork ? g->stream2 : s;
     if (fork) {
         LV_HIP(hipEventRecord(g->ev_fork, s));

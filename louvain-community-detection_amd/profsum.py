@@ -19,7 +19,7 @@ FAMILY_PATTERNS = [
     ("relabel", r"k_relabel_(map|deg|rows|heavy)"),
     ("aggregate", r"k_agg_|k_aggp_|k_scatter_by|k_relabel_newid"),
     ("move_global", r"k_hp_|k_heavy_(insert|gain|lanes|decide|clean)"),
-    ("r1", r"k_r1<|k_heavy_stage|k_heavy_r1"),
+    ("r1", r"k_r1<|k_r1v<|k_heavy_stage|k_heavy_r1"),
     ("move_block", r"k_move_block"),
     ("move_wave", r"k_move_wave"),
     ("move_small", r"k_move_small"),

@@ -299,15 +299,16 @@ def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
     assert all(np.array_equal(x, y) for x, y in zip(a1, a2))
 
 
-@pytest.mark.parametrize("mode", ["1", "2", "dmax", "nograph"])
+@pytest.mark.parametrize("mode", ["1", "2", "dmax", "nograph", "onestream"])
 def test_heavy_global_table_path(lv, orc, mode):
     """The global-table kernels that take over heavy vertices whose hash
     buckets overflow LDS: forced for every heavy vertex (LV_HEAVY_GLOBAL=1),
     for every other one after the partitioned path ran (=2), and through
     buckets that really overflow a reduced capacity (LV_HEAVY_DMAX) so both
-    paths serve one step; and the eager launch path (LV_NO_GRAPH, the heavy
+    paths serve one step; the eager launch path (LV_NO_GRAPH, the heavy
     chain and the block classes on two live streams instead of graph
-    branches). Each in a child process so the setting does not leak."""
+    branches); and every chain on one stream (LV_ONE_STREAM, as profiler
+    runs use). Each in a child process so the setting does not leak."""
     import os
     import subprocess
     import sys
@@ -325,6 +326,7 @@ def test_heavy_global_table_path(lv, orc, mode):
     ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
     env = dict(os.environ, **({"LV_HEAVY_DMAX": "48"} if mode == "dmax" else
                               {"LV_NO_GRAPH": "1"} if mode == "nograph" else
+                              {"LV_ONE_STREAM": "1"} if mode == "onestream" else
                               {"LV_HEAVY_GLOBAL": mode}))
     print(f"[case] heavy global tables mode {mode}", flush=True)
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
