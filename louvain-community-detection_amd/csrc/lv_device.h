@@ -111,7 +111,7 @@ struct HeavySplit {
     uint32_t *pc, *pj;                   // heavy-entry space: community, entry index
     float *pw;                           //                    weight
     uint32_t *hflag;                     // nh: LDS overflow, redo on global tables
-    const uint32_t *only;                // non-null: kernels run for flagged vertices only
+    uint32_t *flist;                     // 1 + nchunks: count, then the chunks of flagged vertices
     int force_global;                    // test hook: 1 global tables only, 2 mixed
     uint32_t dmax;                       // distinct communities a bucket may hold (test hook)
     uint32_t *scan_tmp;

@@ -133,6 +133,7 @@ struct lv_graph {
     size_t hp_bcap = 0;
     uint32_t hp_nbuckets = 0;
     uint32_t *hp_bvtx = nullptr, *hp_cvtx = nullptr;  // bucket / chunk -> heavy index
+    uint32_t *hp_flist = nullptr;  // count + chunks of flagged heavy vertices
     size_t hp_ccap = 0;
     uint32_t *dlist = nullptr, *eslot = nullptr;
     // aggregation scratch
@@ -391,7 +392,9 @@ int classify(lv_graph *g) {
         }
         if (chunks > g->hp_ccap) {
             dfree(g->hp_cvtx);
-            if (dmalloc(&g->hp_cvtx, chunks)) return LV_ERR_NOMEM;
+            dfree(g->hp_flist);
+            if (dmalloc(&g->hp_cvtx, chunks) || dmalloc(&g->hp_flist, (size_t)chunks + 1))
+                return LV_ERR_NOMEM;
             g->hp_ccap = chunks;
         }
         // owner of every chunk and bucket, so no kernel searches for it
@@ -442,7 +445,7 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.pj = g->skey;
     h.pw = reinterpret_cast<float *>(g->tval);
     h.hflag = g->hp_flag;
-    h.only = nullptr;
+    h.flist = g->hp_flist;
     // test hook LV_HEAVY_GLOBAL: 1 every heavy vertex on the global tables,
     // 2 the odd ones after the partitioned path (both paths in one step)
     static const char *hg = getenv("LV_HEAVY_GLOBAL");
@@ -1229,6 +1232,7 @@ void free_all(lv_graph *g) {
     dfree(g->hp_btmp);
     dfree(g->hp_bvtx);
     dfree(g->hp_cvtx);
+    dfree(g->hp_flist);
     dfree(g->ap_nbk);
     dfree(g->ap_bstart);
     dfree(g->ap_cflag);

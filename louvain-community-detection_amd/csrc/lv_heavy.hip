@@ -55,10 +55,13 @@ struct HeavyCtx {
     bool live;
 };
 
-__device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit &h) {
+// context of heavy chunk b (the workgroup's own chunk unless given)
+__device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit &h,
+                                              uint32_t b = 0xffffffffu) {
+    if (b == 0xffffffffu) b = blockIdx.x;
     HeavyCtx x;
-    x.i = h.cvtx[blockIdx.x];
-    x.c = blockIdx.x - h.coff[x.i];
+    x.i = h.cvtx[b];
+    x.c = b - h.coff[x.i];
     x.v = h.verts[x.i];
     x.base = g.row[x.v];
     x.d = g.row[x.v + 1] - x.base;
@@ -83,9 +86,11 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
     __shared__ uint32_t ldist[kHeavyChunk];
     __shared__ uint32_t nld, nnew, newbase;
     __shared__ unsigned long long part[4];
-    const HeavyCtx x = heavy_ctx(g, h);
-    if (!x.live) return;  // uniform over the workgroup
-    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
+    const uint32_t nfl = h.flist[0];
+    for (uint32_t fb = blockIdx.x; fb < nfl; fb += gridDim.x) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    const HeavyCtx x = heavy_ctx(g, h, h.flist[1 + fb]);
+    if (!x.live) continue;  // uniform over the workgroup
     unsigned long long *tab = h.tab + 2ull * x.off;
     for (int i = threadIdx.x; i < kLocalCap; i += 256) {
         lkey[i] = kEmpty;
@@ -172,16 +177,18 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
         const uint32_t j = j0 + threadIdx.x + 256u * q;
         if (lslot[q] != kEmpty) h.eslot[x.base + j] = lglob[lslot[q]];
     }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
-    const HeavyCtx x = heavy_ctx(g, h);
-    if (!x.live) return;  // uniform over the workgroup
-    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
+    const uint32_t nfl = h.flist[0];
+    for (uint32_t fb = blockIdx.x; fb < nfl; fb += gridDim.x) {
+    const HeavyCtx x = heavy_ctx(g, h, h.flist[1 + fb]);
+    if (!x.live) continue;  // uniform over the workgroup
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
-    if (i0 >= nd) return;
+    if (i0 >= nd) continue;
     const uint32_t i1 = min(nd, i0 + kHeavyChunk);
     const uint32_t cv = g.comm[x.v];
     const uint32_t szv = g.comm_sz[cv];
@@ -197,12 +204,17 @@ __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
             gg = gain_of(fix2f(*tval(tab, slot), g.S), v_m(g, x.v), kv, acv, fix2f(g.A[c], g.S));
         *tval(tab, slot) = (unsigned long long)__float_as_uint(gg);
     }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
-    const HeavyCtx x = heavy_ctx(g, h);
-    if (!x.live) return;  // uniform over the workgroup
-    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
+    __shared__ float sg[256];
+    __shared__ uint32_t sj[256];
+    const uint32_t nfl = h.flist[0];
+    for (uint32_t fb = blockIdx.x; fb < nfl; fb += gridDim.x) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    const HeavyCtx x = heavy_ctx(g, h, h.flist[1 + fb]);
+    if (!x.live) continue;  // uniform over the workgroup
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t j0 = x.c * kHeavyChunk;  // multiple of 128: thread t serves lane t % 128
     const uint32_t j1 = min(x.d, j0 + kHeavyChunk);
@@ -215,8 +227,6 @@ __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
             tj = j;
         }
     }
-    __shared__ float sg[256];
-    __shared__ uint32_t sj[256];
     sg[threadIdx.x] = tg;
     sj[threadIdx.x] = tj;
     __syncthreads();
@@ -232,6 +242,7 @@ __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
                 ((unsigned long long)f32_ord(tg) << 32) | (unsigned long long)(0xffffffffu - tj);
             atomicMax(&h.hlane[(size_t)x.i * 128 + threadIdx.x], packed);
         }
+    }
     }
 }
 
@@ -268,18 +279,20 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
 }
 
 __global__ __launch_bounds__(256) void k_heavy_clean(DevView g, HeavySplit h) {
-    const HeavyCtx x = heavy_ctx(g, h);
-    if (!x.live) return;  // uniform over the workgroup
-    if (h.only && !h.only[x.i]) return;  // fallback for flagged vertices only
+    const uint32_t nfl = h.flist[0];
+    for (uint32_t fb = blockIdx.x; fb < nfl; fb += gridDim.x) {
+    const HeavyCtx x = heavy_ctx(g, h, h.flist[1 + fb]);
+    if (!x.live) continue;  // uniform over the workgroup
     unsigned long long *tab = h.tab + 2ull * x.off;
     const uint32_t nd = (uint32_t)h.hcnt[x.i];
     const uint32_t i0 = x.c * kHeavyChunk;
-    if (i0 >= nd) return;
+    if (i0 >= nd) continue;
     const uint32_t i1 = min(nd, i0 + kHeavyChunk);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
         const uint32_t slot = h.dlist[x.base + i];
         tab[2ull * slot] = (unsigned long long)kEmpty;
         tab[2ull * slot + 1ull] = 0ull;
+    }
     }
 }
 
@@ -602,15 +615,22 @@ __global__ void k_hp_flag_odd(HeavySplit h) {
     if (i < h.nh && (i & 1u)) h.hflag[i] = 1u;
 }
 
-// flagged vertices restart on the global-table kernels
+// flagged vertices restart on the global-table kernels: their state is
+// cleared and their chunks are appended to the list those kernels walk
+// (flist[0] counts the listed chunks, cleared before this launch)
 __global__ void k_hp_reset_flagged(HeavySplit h) {
+    __shared__ uint32_t pos;
     const uint32_t i = blockIdx.x;
-    if (i >= h.nh || !h.hflag[i]) return;
+    if (i >= h.nh || !h.hflag[i]) return;  // uniform over the workgroup
+    const uint32_t c0 = h.coff[i], nc = h.coff[i + 1] - c0;
     if (threadIdx.x == 0) {
         h.hcnt[i] = 0ull;
         h.hcew[i] = 0ull;
+        pos = atomicAdd(h.flist, nc);
     }
     if (threadIdx.x < 128) h.hlane[(size_t)i * 128 + threadIdx.x] = 0ull;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) h.flist[1 + pos + t] = c0 + t;
 }
 
 __global__ void k_fill_slots(unsigned long long *tab, size_t cap) {
@@ -641,10 +661,11 @@ void launch_heavy_chunks(hipStream_t s, const uint32_t *row, const uint32_t *ver
 #ifndef LV_HPDMAX
 #define LV_HPDMAX 640
 #endif
+constexpr uint32_t kFallbackGrid = 1024;  // workgroups of the global-table kernels
+
 void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
     if (!hin.nh || !hin.nchunks) return;
-    HeavySplit h = hin;
-    h.only = nullptr;
+    const HeavySplit &h = hin;
     const dim3 grid(h.nchunks), block(256);
     launch_fill_u64(s, h.hcnt, 0ull, (size_t)h.nh * 130);  // hcnt | hcew | hlane
     if (h.force_global == 1) {  // every heavy vertex on the global tables
@@ -666,15 +687,17 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
         if (h.force_global == 2)
             hipLaunchKernelGGL(k_hp_flag_odd, dim3((h.nh + 255) / 256), dim3(256), 0, s, h);
     }
-    // overflowing vertices (if any) redo the step on the global tables
+    // overflowing vertices (if any) redo the step on the global tables; those
+    // kernels walk the list of their chunks with a fixed grid, so a step
+    // without overflow dispatches a few idle workgroups, not one per chunk
+    launch_fill_u32(s, h.flist, 0u, 1);
     hipLaunchKernelGGL(k_hp_reset_flagged, dim3(h.nh), dim3(128), 0, s, h);
-    HeavySplit f = h;
-    f.only = h.hflag;
-    hipLaunchKernelGGL(k_heavy_insert, grid, block, 0, s, g, f);
-    hipLaunchKernelGGL(k_heavy_gain, grid, block, 0, s, g, f);
-    hipLaunchKernelGGL(k_heavy_lanes, grid, block, 0, s, g, f);
+    const dim3 fgrid(h.nchunks < kFallbackGrid ? h.nchunks : kFallbackGrid);
+    hipLaunchKernelGGL(k_heavy_insert, fgrid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_gain, fgrid, block, 0, s, g, h);
+    hipLaunchKernelGGL(k_heavy_lanes, fgrid, block, 0, s, g, h);
     hipLaunchKernelGGL(k_heavy_decide, dim3((h.nh + 3) / 4), block, 0, s, g, h);
-    hipLaunchKernelGGL(k_heavy_clean, grid, block, 0, s, g, f);
+    hipLaunchKernelGGL(k_heavy_clean, fgrid, block, 0, s, g, h);
 }
 
 // locality order: heavy rows are copied one chunk of kHeavyChunk entries per
