@@ -222,7 +222,10 @@ struct AggView {
     uint32_t *dcnt;         // per large community: distinct neighbour communities
     uint32_t nchunks;       // total chunks of the large communities
     uint32_t nn;            // new vertex count (new ids are < nn)
-    const uint32_t *only;   // non-null: global-table merge only for flagged large communities
+    // global-table merge: the flagged large communities (clist) and their
+    // chunks (flist), each list led by its length (nfl, ncl on the host)
+    const uint32_t *flist, *clist;
+    uint32_t nfl, ncl;
     uint32_t *err;
 };
 // Partitioned merge of the large communities (per large community i < count:
@@ -261,6 +264,10 @@ void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uin
                                  uint32_t *order);
 void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
                      unsigned long long *caps);
+// lists of the flagged large communities and of their chunks (counts at
+// flist[0] and clist[0], cleared before the launch)
+void launch_agg_flag_list(hipStream_t s, const uint32_t *flag, const uint32_t *coff,
+                          uint32_t count, uint32_t *flist, uint32_t *clist);
 void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const uint32_t *comms,
                       uint32_t count, int radix_passes);
 // degree of every member in member order (mdeg) -- scanned into moff

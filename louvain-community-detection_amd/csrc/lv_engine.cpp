@@ -144,6 +144,8 @@ struct lv_graph {
     ull *tval = nullptr, *sval = nullptr;
     ull *agg_caps = nullptr, *agg_off = nullptr;
     uint32_t *agg_mdeg = nullptr, *agg_moff = nullptr, *agg_cnk = nullptr, *agg_coff = nullptr;
+    uint32_t *agg_flist = nullptr;  // flagged large communities' chunks, then the communities
+    size_t agg_flcap = 0;
     uint32_t *agg_dcnt = nullptr;
     uint32_t agg_nchunks = 0;
     uint32_t agg_nn = 0;  // new vertex count of the aggregation in progress
@@ -764,8 +766,9 @@ lv::AggView aview(lv_graph *g) {
     a.coff = g->agg_coff;
     a.dcnt = g->agg_dcnt;
     a.nchunks = g->agg_nchunks;
+    a.flist = a.clist = nullptr;
+    a.nfl = a.ncl = 0;
     a.nn = g->agg_nn;
-    a.only = nullptr;
     a.err = &g->dc->err;
     return a;
 }
@@ -865,8 +868,27 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
         lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
         lv::launch_fill_u64(s, g->gval, 0ull, total);
     }
+    // the global-table kernels take one workgroup per flagged community and
+    // per chunk of one, from lists built on the device
+    const size_t lcap = (size_t)g->agg_nchunks + nl + 2;
+    if (lcap > g->agg_flcap) {
+        dfree(g->agg_flist);
+        if (dmalloc(&g->agg_flist, lcap)) return LV_ERR_NOMEM;
+        g->agg_flcap = lcap;
+    }
+    uint32_t *flist = g->agg_flist, *clist = g->agg_flist + g->agg_nchunks + 1;
+    lv::launch_fill_u32(s, flist, 0u, 1);
+    lv::launch_fill_u32(s, clist, 0u, 1);
+    lv::launch_agg_flag_list(s, g->ap_cflag, g->agg_coff, nl, flist, clist);
+    uint32_t nfl = 0, ncl = 0;
+    LV_HIP(hipMemcpyAsync(&nfl, flist, 4, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipMemcpyAsync(&ncl, clist, 4, hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
     av = aview(g);
-    av.only = g->ap_cflag;
+    av.flist = flist;
+    av.clist = clist;
+    av.nfl = nfl;
+    av.ncl = ncl;
     lv::launch_agg_merge(s, v, av, lv::kAggGlobalClass, lc, nl, passes);
     return LV_OK;
 }
@@ -1265,6 +1287,8 @@ void free_all(lv_graph *g) {
     dfree(g->agg_mdeg);
     dfree(g->agg_moff);
     dfree(g->agg_cnk);
+    dfree(g->agg_flist);
+    g->agg_flcap = 0;
     dfree(g->agg_coff);
     dfree(g->agg_dcnt);
     dfree(g->gkey);

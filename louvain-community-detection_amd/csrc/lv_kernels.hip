@@ -1634,10 +1634,10 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
         nld = 0;
         nnew = 0;
     }
-    const AggChunk x = agg_chunk(g, a, comms, count, blockIdx.x, mb, sh);
+    // one workgroup per listed chunk (chunks of flagged communities only)
+    const AggChunk x = agg_chunk(g, a, comms, count, a.flist[1 + blockIdx.x], mb, sh);
     const uint32_t ci = x.ci, f0 = x.f0, f1 = x.f1;
     const uint32_t cb = a.epos[x.c];
-    if (a.only && !a.only[ci]) return;  // uniform: community not on this path
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t f = f0 + threadIdx.x + 256u * q;
@@ -1717,8 +1717,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     __shared__ uint32_t binbase[256];
     __shared__ uint32_t ctot[256];
     __shared__ uint32_t wcnt[NW][256];
-    const uint32_t ci = blockIdx.x;
-    if (a.only && !a.only[ci]) return;  // uniform: community not on this path
+    const uint32_t ci = a.clist[1 + blockIdx.x];  // one workgroup per flagged community
     const uint32_t c = comms[ci];
     const unsigned long long off = a.goff[ci];
     uint32_t *key = a.gkey + off;
@@ -1831,12 +1830,28 @@ void launch_agg_merge(hipStream_t s, const DevView &g, AggView a, int cls, const
     else if (cls == 4)
         hipLaunchKernelGGL((k_agg_merge_lds<256, 2048, LV_AGGCAP4>), dim3(count), dim3(256), 0, s, g, a, comms,
                            count);
-    else {
-        hipLaunchKernelGGL(k_agg_insert_chunks, dim3(a.nchunks), dim3(256), 0, s, g, a, comms,
+    else if (a.ncl) {  // the listed (flagged) communities and chunks only
+        hipLaunchKernelGGL(k_agg_insert_chunks, dim3(a.nfl), dim3(256), 0, s, g, a, comms,
                            count);
-        hipLaunchKernelGGL(k_agg_merge_global, dim3(count), dim3(kRadixThreads), 0, s, g, a, comms,
-                           count, radix_passes);
+        hipLaunchKernelGGL(k_agg_merge_global, dim3(a.ncl), dim3(kRadixThreads), 0, s, g, a,
+                           comms, count, radix_passes);
     }
+}
+
+__global__ void k_agg_flag_list(const uint32_t *flag, const uint32_t *coff, uint32_t count,
+                                uint32_t *flist, uint32_t *clist) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= count || !flag[ci]) return;
+    clist[1 + atomicAdd(clist, 1u)] = ci;
+    const uint32_t c0 = coff[ci], nc = coff[ci + 1] - c0;
+    const uint32_t pos = atomicAdd(flist, nc);
+    for (uint32_t t = 0; t < nc; ++t) flist[1 + pos + t] = c0 + t;
+}
+void launch_agg_flag_list(hipStream_t s, const uint32_t *flag, const uint32_t *coff,
+                          uint32_t count, uint32_t *flist, uint32_t *clist) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_agg_flag_list, dim3((count + 255) / 256), dim3(256), 0, s, flag, coff,
+                       count, flist, clist);
 }
 
 void launch_agg_mdeg(hipStream_t s, const DevView &g, AggView a) {
