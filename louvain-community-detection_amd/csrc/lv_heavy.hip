@@ -737,14 +737,18 @@ void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *c
     hipLaunchKernelGGL(k_relabel_heavy, dim3(nchunks), dim3(256), 0, s, r, cvtx, coff, x0);
 }
 
-__global__ void k_owner_fill(const uint32_t *off, uint32_t count, uint32_t *out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        for (uint32_t b = off[i]; b < off[i + 1]; ++b) out[b] = i;
+// one wave per owner: a vertex with a thousand buckets is 16 rounds of its
+// wave, not a thousand stores of one thread
+__global__ __launch_bounds__(256) void k_owner_fill(const uint32_t *off, uint32_t count,
+                                                    uint32_t *out) {
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= count) return;
+    const uint32_t b1 = off[i + 1];
+    for (uint32_t b = off[i] + (threadIdx.x & 63); b < b1; b += 64) out[b] = i;
 }
 void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint32_t *out) {
     if (!count) return;
-    hipLaunchKernelGGL(k_owner_fill, dim3((count + 255) / 256), dim3(256), 0, s, off, count, out);
+    hipLaunchKernelGGL(k_owner_fill, dim3((count + 3) / 4), dim3(256), 0, s, off, count, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -264,25 +264,35 @@ __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uin
 
 // cursor[c] starts at the class offset; order within a class is not
 // semantic (every vertex of a step decides from the same snapshot)
+constexpr uint32_t kScatterPer = 16;
 __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, uint32_t n,
                                                           uint32_t heavy_min, uint32_t *cursor,
                                                           uint32_t *order) {
+    // kScatterPer vertices per thread: one global cursor add per class and
+    // workgroup of 256 * kScatterPer vertices
     __shared__ uint32_t hc[kNumClasses];
     __shared__ uint32_t base[kNumClasses];
-    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
     if (threadIdx.x < kNumClasses) hc[threadIdx.x] = 0;
     __syncthreads();
-    int c = -1;
-    uint32_t r = 0;
-    if (v < n) {
-        c = row_class(row[v + 1] - row[v], heavy_min);
-        r = atomicAdd(&hc[c], 1u);
+    int c[kScatterPer];
+    uint32_t r[kScatterPer];
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q) {
+        const uint32_t v = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
+        c[q] = -1;
+        r[q] = 0;
+        if (v < n) {
+            c[q] = row_class(row[v + 1] - row[v], heavy_min);
+            r[q] = atomicAdd(&hc[c[q]], 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x < kNumClasses && hc[threadIdx.x])
         base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hc[threadIdx.x]);
     __syncthreads();
-    if (c >= 0) order[base[c] + r] = v;
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q)
+        if (c[q] >= 0) order[base[c[q]] + r[q]] = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
 }
 
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
@@ -294,7 +304,8 @@ void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint3
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
                              uint32_t *cursor, uint32_t *order) {
     if (!n) return;
-    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 255) / 256), dim3(256), 0, s, row, n,
+    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 256 * kScatterPer - 1) / (256 * kScatterPer)),
+                       dim3(256), 0, s, row, n,
                        heavy_min, cursor, order);
 }
 
@@ -1224,20 +1235,27 @@ __global__ __launch_bounds__(256) void k_agg_class_count(DevView g, AggView a, u
 __global__ __launch_bounds__(256) void k_agg_class_scatter(DevView g, AggView a, uint32_t *cursor,
                                                            uint32_t *order) {
     __shared__ uint32_t hc[kNumAggClasses], base[kNumAggClasses];
-    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     if (threadIdx.x < kNumAggClasses) hc[threadIdx.x] = 0;
     __syncthreads();
-    int k = -1;
-    uint32_t r = 0;
-    if (c < g.n && g.comm_sz[c]) {
-        k = agg_class(a.comm_deg[c]);
-        r = atomicAdd(&hc[k], 1u);
+    int k[kScatterPer];
+    uint32_t r[kScatterPer];
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q) {
+        const uint32_t c = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
+        k[q] = -1;
+        r[q] = 0;
+        if (c < g.n && g.comm_sz[c]) {
+            k[q] = agg_class(a.comm_deg[c]);
+            r[q] = atomicAdd(&hc[k[q]], 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x < kNumAggClasses && hc[threadIdx.x])
         base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hc[threadIdx.x]);
     __syncthreads();
-    if (k >= 0) order[base[k] + r] = c;
+#pragma unroll
+    for (int q = 0; q < kScatterPer; ++q)
+        if (k[q] >= 0) order[base[k[q]] + r[q]] = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
 }
 void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts) {
     if (!g.n) return;
@@ -1247,8 +1265,9 @@ void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint3
 void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uint32_t *cursor,
                                  uint32_t *order) {
     if (!g.n) return;
-    hipLaunchKernelGGL(k_agg_class_scatter, dim3((g.n + 255) / 256), dim3(256), 0, s, g, a,
-                       cursor, order);
+    hipLaunchKernelGGL(k_agg_class_scatter,
+                       dim3((g.n + 256 * kScatterPer - 1) / (256 * kScatterPer)), dim3(256), 0, s,
+                       g, a, cursor, order);
 }
 
 __global__ void k_agg_caps(AggView a, const uint32_t *comms, uint32_t count,
