@@ -157,8 +157,10 @@ void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
                            uint32_t *counts /*kNumClasses*/,
                            unsigned long long *degsum /*kNumClasses*/);
+// run: consecutive vertices per thread (a multiple of 16; longer runs keep
+// more neighbouring ids together in each class, fewer workgroups)
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
-                             uint32_t *cursor /*kNumClasses*/, uint32_t *order);
+                             uint32_t run, uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
 // pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v, 2 = s_v
 // while every community is a singleton (pass init: no community gathers).

@@ -265,34 +265,53 @@ __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uin
 // cursor[c] starts at the class offset; order within a class is not
 // semantic (every vertex of a step decides from the same snapshot)
 constexpr uint32_t kScatterPer = 16;
+// Vertices into class order. A workgroup takes 256 * run consecutive vertices
+// (run a multiple of 16): it counts them per class, reserves each class's
+// range with one global cursor add, then places them (a second pass over the
+// same rows), so every class holds runs of nearby ids — the order the
+// locality renumbering inherits.
 __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, uint32_t n,
-                                                          uint32_t heavy_min, uint32_t *cursor,
-                                                          uint32_t *order) {
-    // kScatterPer vertices per thread: one global cursor add per class and
-    // workgroup of 256 * kScatterPer vertices
+                                                          uint32_t heavy_min, uint32_t run,
+                                                          uint32_t *cursor, uint32_t *order) {
     __shared__ uint32_t hc[kNumClasses];
     __shared__ uint32_t base[kNumClasses];
     if (threadIdx.x < kNumClasses) hc[threadIdx.x] = 0;
     __syncthreads();
-    int c[kScatterPer];
-    uint32_t r[kScatterPer];
+    const uint32_t v0 = blockIdx.x * run * 256u + threadIdx.x;
+    // 16 rows' lengths per round are loaded before their class is used
+    auto lengths = [&](uint32_t q0, int (&c)[16]) {
+        uint32_t a[16], b[16];
 #pragma unroll
-    for (int q = 0; q < kScatterPer; ++q) {
-        const uint32_t v = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
-        c[q] = -1;
-        r[q] = 0;
-        if (v < n) {
-            c[q] = row_class(row[v + 1] - row[v], heavy_min);
-            r[q] = atomicAdd(&hc[c[q]], 1u);
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t v = v0 + 256u * (q0 + u), vv = v < n ? v : n - 1u;
+            a[u] = row[vv];
+            b[u] = row[vv + 1];
         }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            c[u] = v0 + 256u * (q0 + u) < n ? row_class(b[u] - a[u], heavy_min) : -1;
+    };
+    for (uint32_t q0 = 0; q0 < run; q0 += 16) {
+        int c[16];
+        lengths(q0, c);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (c[u] >= 0) atomicAdd(&hc[c[u]], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < kNumClasses && hc[threadIdx.x])
-        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hc[threadIdx.x]);
+    if (threadIdx.x < kNumClasses) {
+        const uint32_t h = hc[threadIdx.x];
+        base[threadIdx.x] = h ? atomicAdd(&cursor[threadIdx.x], h) : 0u;
+        hc[threadIdx.x] = 0;
+    }
     __syncthreads();
+    for (uint32_t q0 = 0; q0 < run; q0 += 16) {
+        int c[16];
+        lengths(q0, c);
 #pragma unroll
-    for (int q = 0; q < kScatterPer; ++q)
-        if (c[q] >= 0) order[base[c[q]] + r[q]] = (blockIdx.x * kScatterPer + q) * 256 + threadIdx.x;
+        for (int u = 0; u < 16; ++u)
+            if (c[u] >= 0) order[base[c[u]] + atomicAdd(&hc[c[u]], 1u)] = v0 + 256u * (q0 + u);
+    }
 }
 
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
@@ -302,11 +321,12 @@ void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint3
                        heavy_min, counts, degsum);
 }
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
-                             uint32_t *cursor, uint32_t *order) {
+                             uint32_t run, uint32_t *cursor, uint32_t *order) {
     if (!n) return;
-    hipLaunchKernelGGL(k_classify_scatter, dim3((n + 256 * kScatterPer - 1) / (256 * kScatterPer)),
-                       dim3(256), 0, s, row, n,
-                       heavy_min, cursor, order);
+    run = run < 16 ? 16u : run & ~15u;
+    const uint32_t per = 256u * run;
+    hipLaunchKernelGGL(k_classify_scatter, dim3((uint32_t)(((uint64_t)n + per - 1) / per)),
+                       dim3(256), 0, s, row, n, heavy_min, run, cursor, order);
 }
 
 // ---------------------------------------------------------------------------
