@@ -153,14 +153,18 @@ void launch_relabel_heavy(hipStream_t s, const RelabelView &r, const uint32_t *c
 void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint32_t *out);
 void launch_relabel_new_id(hipStream_t s, const uint32_t *comm_sz, const uint32_t *lid, uint32_t n,
                            uint32_t *flags, uint32_t *rank, uint32_t *new_id, uint32_t *scan_tmp);
-// rows longer than heavy_min (319 <= heavy_min <= kBlockMax) go to the heavy class
+// rows longer than heavy_min (319 <= heavy_min <= kBlockMax) go to the heavy
+// class; with gid non-null (batch), vertices of graphs that are not active
+// are left out of the classes
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
+                           const uint32_t *gid, const uint32_t *gstate,
                            uint32_t *counts /*kNumClasses*/,
                            unsigned long long *degsum /*kNumClasses*/);
 // run: consecutive vertices per thread (a multiple of 16; longer runs keep
 // more neighbouring ids together in each class, fewer workgroups)
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
-                             uint32_t run, uint32_t *cursor /*kNumClasses*/, uint32_t *order);
+                             const uint32_t *gid, const uint32_t *gstate, uint32_t run,
+                             uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
 // pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v, 2 = s_v
 // while every community is a singleton (pass init: no community gathers).

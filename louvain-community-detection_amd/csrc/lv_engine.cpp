@@ -17,6 +17,7 @@
 #include <cstring>
 #include <new>
 #include <thread>
+#include <memory>
 #include <vector>
 
 #include "../../include/louvain_hip.h"
@@ -336,8 +337,10 @@ int classify(lv_graph *g) {
     zero_counters(g);
     {
         ProfScope ps(g, LV_K_CLASSIFY, 8.0 * g->n);
+        // a batch lists the vertices of its active graphs only
         lv::launch_classify_count(g->stream, g->row, g->n, g->tune.heavy_min_degree,
-                                  g->dc->counts, g->dc->degsum);
+                                  g->G ? g->gid : nullptr, g->gstate_d, g->dc->counts,
+                                  g->dc->degsum);
     }
     int rc = sync_counters(g);
     if (rc) return rc;
@@ -358,8 +361,9 @@ int classify(lv_graph *g) {
         // R-MAT-24); a batch keeps runs of 16, where longer runs measured
         // slower (+2.6 % at 64 on the 4096-graph batch)
         const uint32_t run = g->G ? 16u : std::min(256u, std::max(16u, g->n / (256u * 1024u)));
-        lv::launch_classify_scatter(g->stream, g->row, g->n, g->tune.heavy_min_degree, run,
-                                    g->dc->cursor, g->order);
+        lv::launch_classify_scatter(g->stream, g->row, g->n, g->tune.heavy_min_degree,
+                                    g->G ? g->gid : nullptr, g->gstate_d, run, g->dc->cursor,
+                                    g->order);
     }
     // heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk
     // offsets of the split kernels, per-vertex scratch
@@ -525,7 +529,8 @@ void modularity_launch(lv_graph *g, bool singletons = false) {
     r1_all(g, singletons ? 2 : 1, 0);
     lv::DevView v = view(g);
     ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
-    lv::launch_sumsq(g->stream, v, g->order, g->n, limbs_ptr(g));
+    // every listed vertex (a batch lists its active graphs' vertices only)
+    lv::launch_sumsq(g->stream, v, g->order, g->cls_off[kNumClasses], limbs_ptr(g));
 }
 
 // Locality order for the pass (single-graph mode): position x of the new
@@ -1070,6 +1075,13 @@ int batch_pass_init(lv_graph *g) {
     return LV_OK;
 }
 
+// listed vertices below which a batch pass keeps its classes as they are
+// (test hook LV_COMPACT_MIN lowers it so small batches compact too)
+uint32_t compact_min() {
+    static const char *env = getenv("LV_COMPACT_MIN");
+    return env ? (uint32_t)strtoul(env, nullptr, 10) : (1u << 16);
+}
+
 // One pass for every active graph; returns in *more whether any graph
 // improved (and was contracted).
 int batch_pass(lv_graph *g, float min_gain, bool *more) {
@@ -1077,16 +1089,31 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
     const uint32_t G = g->G;
     int rc = batch_pass_init(g);
     if (rc) return rc;
-    SweepGraph sg;
-    const bool use_graph = graphs_enabled(g) && capture_sweep(g, sg);
+    auto sg = std::make_unique<SweepGraph>();
+    bool use_graph = graphs_enabled(g) && capture_sweep(g, *sg);
     for (;;) {
         bool any_active = false;
         for (uint32_t gi = 0; gi < G && !any_active; ++gi) any_active = g->gstate_h[gi] == kGsActive;
         if (!any_active) break;
+        // once the active graphs hold at most half of the listed vertices,
+        // the classes are rebuilt from the active graphs alone (and the sweep
+        // captured again): frozen graphs' vertices cost a dispatched and
+        // idle workgroup share in every kernel of every sweep otherwise
+        uint64_t live = 0;
+        for (uint32_t gi = 0; gi < G; ++gi)
+            if (g->gstate_h[gi] == kGsActive) live += g->voff[gi + 1] - g->voff[gi];
+        const uint32_t listed = g->cls_off[kNumClasses];
+        if (listed >= compact_min() && 2 * live <= listed) {
+            sg.reset();
+            rc = classify(g);
+            if (rc) return rc;
+            sg = std::make_unique<SweepGraph>();
+            use_graph = graphs_enabled(g) && capture_sweep(g, *sg);
+        }
         if (use_graph) {
-            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            LV_HIP(hipGraphLaunch(sg->exec, g->stream));
             rc = sweep_finish(g, nullptr, nullptr);
-            if (g->prof_on) sg.collect();
+            if (g->prof_on) sg->collect();
         } else {
             rc = sweep(g, nullptr, nullptr);
         }

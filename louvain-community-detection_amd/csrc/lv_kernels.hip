@@ -239,8 +239,12 @@ __device__ __forceinline__ int row_class(uint32_t d, uint32_t heavy_min) {
     return d > heavy_min ? kClassHeavy : vertex_class(d);
 }
 
+__device__ __forceinline__ bool listed(const uint32_t *gid, const uint32_t *gstate, uint32_t v) {
+    return !gid || gstate[gid[v]] == kGraphActive;
+}
 __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uint32_t n,
-                                                        uint32_t heavy_min, uint32_t *counts,
+                                                        uint32_t heavy_min, const uint32_t *gid,
+                                                        const uint32_t *gstate, uint32_t *counts,
                                                         unsigned long long *degsum) {
     __shared__ uint32_t hc[kNumClasses];
     __shared__ unsigned long long hd[kNumClasses];
@@ -250,6 +254,7 @@ __global__ __launch_bounds__(256) void k_classify_count(const uint32_t *row, uin
     }
     __syncthreads();
     for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < n; v += gridDim.x * 256) {
+        if (!listed(gid, gstate, v)) continue;
         uint32_t d = row[v + 1] - row[v];
         int c = row_class(d, heavy_min);
         atomicAdd(&hc[c], 1u);
@@ -271,7 +276,8 @@ constexpr uint32_t kScatterPer = 16;
 // same rows), so every class holds runs of nearby ids — the order the
 // locality renumbering inherits.
 __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, uint32_t n,
-                                                          uint32_t heavy_min, uint32_t run,
+                                                          uint32_t heavy_min, const uint32_t *gid,
+                                                          const uint32_t *gstate, uint32_t run,
                                                           uint32_t *cursor, uint32_t *order) {
     __shared__ uint32_t hc[kNumClasses];
     __shared__ uint32_t base[kNumClasses];
@@ -288,8 +294,10 @@ __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, u
             b[u] = row[vv + 1];
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            c[u] = v0 + 256u * (q0 + u) < n ? row_class(b[u] - a[u], heavy_min) : -1;
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t v = v0 + 256u * (q0 + u);
+            c[u] = v < n && listed(gid, gstate, v) ? row_class(b[u] - a[u], heavy_min) : -1;
+        }
     };
     for (uint32_t q0 = 0; q0 < run; q0 += 16) {
         int c[16];
@@ -315,18 +323,20 @@ __global__ __launch_bounds__(256) void k_classify_scatter(const uint32_t *row, u
 }
 
 void launch_classify_count(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
-                           uint32_t *counts, unsigned long long *degsum) {
+                           const uint32_t *gid, const uint32_t *gstate, uint32_t *counts,
+                           unsigned long long *degsum) {
     if (!n) return;
     hipLaunchKernelGGL(k_classify_count, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, row, n,
-                       heavy_min, counts, degsum);
+                       heavy_min, gid, gstate, counts, degsum);
 }
 void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uint32_t heavy_min,
-                             uint32_t run, uint32_t *cursor, uint32_t *order) {
+                             const uint32_t *gid, const uint32_t *gstate, uint32_t run,
+                             uint32_t *cursor, uint32_t *order) {
     if (!n) return;
     run = run < 16 ? 16u : run & ~15u;
     const uint32_t per = 256u * run;
     hipLaunchKernelGGL(k_classify_scatter, dim3((uint32_t)(((uint64_t)n + per - 1) / per)),
-                       dim3(256), 0, s, row, n, heavy_min, run, cursor, order);
+                       dim3(256), 0, s, row, n, heavy_min, gid, gstate, run, cursor, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -2137,8 +2147,22 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         }
 #pragma unroll
         for (uint32_t u = 0; u < 8; ++u) {
-            if (kk[u] == kEmpty) continue;
-            const uint32_t k = kk[u];
+            // the wave's entries of its first key are summed before the table
+            // sees them: a key that dominates a bucket (the community's own
+            // id, in graphs with few communities) costs one LDS atomic per
+            // wave instead of 64 on one slot (every lane takes part)
+            uint32_t k = kk[u];
+            unsigned long long wf = k == kEmpty ? 0ull : f2fix(ww[u], g.scale);
+            const unsigned long long act = __ballot(k != kEmpty);
+            if (act) {  // uniform
+                const int lead = __ffsll((long long)act) - 1;
+                const uint32_t k0 = __shfl(k, lead, 64);
+                const bool same = k != kEmpty && k == k0;
+                const unsigned long long s = wave_sum_u64(same ? wf : 0ull);
+                if ((int)(threadIdx.x & 63u) == lead) wf = s;
+                else if (same) k = kEmpty;  // folded into the leader's entry
+            }
+            if (k == kEmpty) continue;
             uint32_t slot = hslot_n<kAggBucketCap>(k);
             int probe = 0;
             for (; probe < kAggBucketCap; ++probe) {
@@ -2156,7 +2180,7 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
                 over = 1u;
                 continue;
             }
-            atomicAdd(&val[slot], f2fix(ww[u], g.scale));
+            atomicAdd(&val[slot], wf);
         }
     }
     __syncthreads();

@@ -286,6 +286,39 @@ def test_batch_matches_single_runs(lv, orc, bcsstk16_text):
         assert np.array_equal(assigns[i], o.init_comm()), i
 
 
+def test_batch_compaction(lv, orc):
+    """A batch pass rebuilds its degree classes from the active graphs once
+    these hold at most half of the listed vertices (LV_COMPACT_MIN=1 lets a
+    small batch do so, in a child process so the setting does not leak);
+    every graph still gets the result of its own run."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from loader import load_package; import oracle\n"
+        "lv = load_package(); rng = np.random.default_rng(7); graphs = []\n"
+        "for i in range(40):\n"
+        "    n = int(np.exp(rng.uniform(np.log(60), np.log(3000))))\n"
+        "    r, c, w = lv.gen_sbm(n, min(int(rng.integers(4, 33)), n), float(rng.uniform(4, 24)), 10.0, 300 + i)\n"
+        "    graphs.append((r.astype(np.uint32), c, w))\n"
+        "with lv.Batch(graphs) as b:\n"
+        "    per, tot = b.run(1e-6); assigns = b.assignments()\n"
+        "for i, (r, c, w) in enumerate(graphs):\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    assert np.float32(per[i].q_final).view(np.uint32) == np.float32(ores.q_final).view(np.uint32), i\n"
+        "    assert per[i].total_iters == ores.total_iters and per[i].passes == ores.passes, i\n"
+        "    assert np.array_equal(assigns[i], o.init_comm()), i\n"
+        "print('compaction ok')\n"
+    ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
+    env = dict(os.environ, LV_COMPACT_MIN="1")
+    print("[case] batch of 40 SBM graphs with class compaction", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "compaction ok" in r.stdout
+
+
 def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
     graphs = _mixed_batch(lv, orc, bcsstk16_text)[:10]
     with lv.Batch(graphs) as b:
