@@ -232,6 +232,9 @@ struct AggView {
     // chunks (flist), each list led by its length (nfl, ncl on the host)
     const uint32_t *flist, *clist;
     uint32_t nfl, ncl;
+    // batch: graph of each old vertex and first new id of each graph (a
+    // community's keys lie in its graph's new-id range); nullptr: one graph
+    const uint32_t *kgid, *kbase;
     uint32_t *err;
 };
 // Partitioned merge of the large communities (per large community i < count:
@@ -254,7 +257,7 @@ struct AggPart {
     uint32_t dmax;           // distinct keys a bucket may hold (<= kAggBucketDistinct)
 };
 constexpr uint32_t kAggBucketDistinct = 512;  // per bucket; its LDS table has more slots
-void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
                     uint32_t *nbk);
 // phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities
 void launch_aggp(hipStream_t s, const DevView &g, AggView a, const AggPart &p,

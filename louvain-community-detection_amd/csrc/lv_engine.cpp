@@ -755,6 +755,8 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
 
 lv::AggView aview(lv_graph *g) {
     lv::AggView a;
+    a.kgid = g->G ? g->gid : nullptr;
+    a.kbase = g->G ? g->vgather : nullptr;
     a.new_id = g->new_id;
     a.vnid = g->vnid;
     a.comm_deg = g->comm_deg;
@@ -792,7 +794,7 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     hipStream_t s = g->stream;
     lv::launch_agg_large_prep(s, v, aview(g), lc, nl);  // chunks per community
     lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
-    lv::launch_aggp_nb(s, aview(g), lc, nl, g->ap_nbk);
+    lv::launch_aggp_nb(s, aview(g), lc, nl, nn, g->ap_nbk);
     lv::launch_scan_u32(s, g->ap_nbk, 0, nl, g->ap_bstart, g->scan_tmp32);
     uint32_t chunks = 0, NB = 0;
     LV_HIP(hipMemcpyAsync(&chunks, g->agg_coff + nl, 4, hipMemcpyDeviceToHost, s));
@@ -921,6 +923,8 @@ int aggregation(lv_graph *g) {
     else
         lv::launch_scan_u32(s, g->comm_sz, 1, n, g->new_id, g->scan_tmp32);
     lv::launch_agg_vnid(s, g->comm, g->new_id, n, g->vnid);
+    // a batch: first new id of every graph (the key range of its communities)
+    if (g->G) lv::launch_gather_u32(s, g->new_id, g->voff_d, g->vgather, g->G + 1);
     lv::launch_scan_u32(s, g->comm_sz, 0, n, g->mstart, g->scan_tmp32);
     lv::launch_scan_u32(s, g->comm_deg, 0, n, g->epos, g->scan_tmp32);
     lv::launch_agg_members(s, v, aview(g));

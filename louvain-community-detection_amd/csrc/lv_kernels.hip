@@ -1942,21 +1942,45 @@ constexpr int kAggBucketCap = LV_AGGBCAP;  // LDS table slots of one bucket (> k
 __device__ __forceinline__ uint32_t agg_bucket(uint32_t key, uint32_t nb, uint32_t nn) {
     return (uint32_t)(((unsigned long long)key * nb) / nn);
 }
+// The key range [kb, kb + span) of community c's neighbours: the graph's
+// new ids in a batch (edges never leave a graph), all of [0, nn) otherwise.
+// Buckets split this range evenly.
+__device__ __forceinline__ void agg_keys(const AggView &a, uint32_t nn, uint32_t c, uint32_t &kb,
+                                         uint32_t &span) {
+    if (a.kgid) {
+        const uint32_t gi = a.kgid[c];
+        kb = a.kbase[gi];
+        span = a.kbase[gi + 1] - kb;
+    } else {
+        kb = 0;
+        span = nn;
+    }
+}
 
-__global__ void k_aggp_nb(AggView a, const uint32_t *comms, uint32_t count, uint32_t *nbk) {
+// buckets of a large community: pow2 >= its entries / kAggBucketTarget, but
+// no more than its key range needs (span / 64 keys per bucket at least, so a
+// small graph's community with many entries and few distinct neighbours does
+// not dispatch hundreds of empty buckets)
+__global__ void k_aggp_nb(AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
+                          uint32_t *nbk) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
-        const uint32_t d = a.comm_deg[comms[i]];
+        const uint32_t c = comms[i];
+        const uint32_t d = a.comm_deg[c];
+        uint32_t kb, span;
+        agg_keys(a, nn, c, kb, span);
         uint32_t b = 1;
-        while (b < kAggMaxBuckets && (unsigned long long)b * kAggBucketTarget < d) b <<= 1;
+        while (b < kAggMaxBuckets && (unsigned long long)b * kAggBucketTarget < d &&
+               (unsigned long long)b * 64u < span)
+            b <<= 1;
         nbk[i] = b;
     }
 }
-void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
+void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
                     uint32_t *nbk) {
     if (!count) return;
     hipLaunchKernelGGL(k_aggp_nb, dim3((count + 255) / 256), dim3(256), 0, s, a, comms, count,
-                       nbk);
+                       nn, nbk);
 }
 
 // P0: new id of every entry of the chunk, staged in entry order, counted per
@@ -1969,6 +1993,8 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
     for (uint32_t b = threadIdx.x; b < kAggMaxBuckets; b += 256) hist[b] = 0;
     const AggChunk x = agg_chunk(g, a, comms, count, blockIdx.x, mb, sh);
     const uint32_t nb = p.nbk[x.ci];
+    uint32_t kb, span;
+    agg_keys(a, p.nn, x.c, kb, span);
     constexpr uint32_t PER = kAggChunk / 256;
     // each dependent load level for all of the thread's entries at once:
     // member, row start, column + weight, new id of the neighbour's
@@ -2004,7 +2030,7 @@ __global__ __launch_bounds__(256) void k_aggp_count(DevView g, AggView a, AggPar
         if (f >= x.f1) continue;
         p.fkey[f] = kk[u];
         p.fw[f] = ww[u];
-        atomicAdd(&hist[agg_bucket(kk[u], nb, p.nn)], 1u);
+        atomicAdd(&hist[agg_bucket(kk[u] - kb, nb, span)], 1u);
     }
     __syncthreads();
     const uint32_t b0 = p.bstart[x.ci];
@@ -2039,6 +2065,8 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
     const uint32_t f0 = cb + (blockIdx.x - a.coff[ci]) * kAggChunk;
     const uint32_t cnt = min(ce, f0 + kAggChunk) - f0;
     const uint32_t nb = p.nbk[ci];
+    uint32_t kb, span;
+    agg_keys(a, p.nn, c, kb, span);
     for (uint32_t b = threadIdx.x; b < kAggMaxBuckets / 2; b += 256) lcnt2[b] = 0;
     __syncthreads();
     uint32_t key[kAggChunk / 256], rank[kAggChunk / 256], bk[kAggChunk / 256];
@@ -2056,7 +2084,7 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
         const uint32_t i = threadIdx.x + 256u * q;
         bk[q] = kEmpty;
         if (i < cnt) {
-            bk[q] = agg_bucket(key[q], nb, p.nn);
+            bk[q] = agg_bucket(key[q] - kb, nb, span);
             const uint32_t sh = (bk[q] & 1u) << 4;
             rank[q] = (atomicAdd(&lcnt2[bk[q] >> 1], 1u << sh) >> sh) & 0xffffu;
         }
@@ -2096,7 +2124,7 @@ __global__ __launch_bounds__(256) void k_aggp_scatter(AggView a, AggPart p, cons
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
         const uint32_t k = sk[i];
-        const uint32_t b = agg_bucket(k, nb, p.nn);
+        const uint32_t b = agg_bucket(k - kb, nb, span);
         const uint32_t pos = lbase[b] + (i - loff[b]);
         p.pkey[pos] = k;
         p.pw[pos] = sw[i];
@@ -2189,9 +2217,11 @@ __global__ __launch_bounds__(256) void k_aggp_bucket(DevView g, AggView a, AggPa
         return;
     }
     const uint32_t nd = nd_s;
-    // the bucket's key range: keys k with floor(k * nb / nn) == bb
-    const uint32_t lo = (uint32_t)((bb * p.nn + nb - 1) / nb);
-    const uint32_t hi = (uint32_t)(((bb + 1) * p.nn + nb - 1) / nb);
+    // the bucket's key range: keys k with floor((k - kb) * nb / span) == bb
+    uint32_t kb, span;
+    agg_keys(a, p.nn, c, kb, span);
+    const uint32_t lo = kb + (uint32_t)((bb * span + nb - 1) / nb);
+    const uint32_t hi = kb + (uint32_t)(((bb + 1) * span + nb - 1) / nb);
     auto keyof = [&](uint32_t i) { return key[dslot[i]]; };
     auto emit = [&](uint32_t pos, uint32_t i) {
         const uint32_t slot = dslot[i];
