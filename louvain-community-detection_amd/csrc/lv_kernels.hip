@@ -509,7 +509,6 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
                int batch, int masked, int init_state, unsigned long long *acc) {
     const int ident = masked == 2;
     if (!count) return;
-#ifndef LV_NO_R1V
     if (masked && group == 32) {
         dim3 grid((count + 7) / 8), block(256);
         if (batch == 8)
@@ -518,7 +517,6 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
             hipLaunchKernelGGL(k_r1v<1>, grid, block, 0, s, g, verts, count, ident, acc);
         return;
     }
-#endif
     const unsigned gpb = 256 / group;
     dim3 grid((count + gpb - 1) / gpb), block(256);
 #define LV_R1(GG, BB)                                                                         \
