@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
 constexpr int kWaveCap = LV_WAVECAP;  // slots per wave (at most 319 distinct keys)
 constexpr int kWaveMaxPer = 5;        // ceil(319 / 64)
 #ifndef LV_WAVE_WPB
-#define LV_WAVE_WPB 4
+#define LV_WAVE_WPB 2
 #endif
 constexpr int kWaveWpb = LV_WAVE_WPB;  // waves (vertices) per workgroup
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
