@@ -44,7 +44,8 @@ enum {
     LV_ERR_STATE = 6,      /* call out of order (e.g. lv_step before lv_pass_init) */
     LV_ERR_HIP = 7,        /* HIP runtime error */
     LV_ERR_NO_DEVICE = 8,  /* no usable GPU */
-    LV_ERR_OVERFLOW = 9    /* internal capacity exceeded (reported, never silent) */
+    LV_ERR_OVERFLOW = 9,   /* internal capacity exceeded (reported, never silent) */
+    LV_ERR_LIMIT = 10      /* graph larger than a fixed buffer allows */
 };
 
 typedef struct lv_graph lv_graph;
@@ -146,11 +147,31 @@ int lv_graph_get_tuning(lv_graph *g, lv_tuning *t);
 int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32_t count,
                 lv_tuning *best, double *best_ms);
 
+/* Stream: by default every call on a graph runs on the graph's own HIP
+ * stream.  lv_graph_set_stream makes later calls enqueue on the caller's
+ * stream instead (a hipStream_t passed as void*; NULL returns to the graph's
+ * own), so the engine's work is ordered after the caller's earlier work on
+ * that stream and before its later work, as the reference's operators are on
+ * the default stream (main.cu:43-48).  Kernel chains that run beside the main
+ * one fork from and join back into that stream by events.  The engine still
+ * synchronises the stream where the host takes a decision (the stop rule
+ * after each sweep, the sizes of each aggregation), so the stream must not be
+ * under graph capture.  Work queued on the previous stream completes before
+ * the call returns.  No counterpart in the reference. */
+int lv_graph_set_stream(lv_graph *g, void *stream);
+
 /* ---- the reference operator API ------------------------------------------ */
-/* One Louvain pass of local moves.  *improved is the reference's return value
- * (false when the first sweep gains less than min_gain); when false,
- * *q_printed is the modularity the reference prints (old_modularity). */
-int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_printed);
+/* One Louvain pass of local moves (modularityOptimization, modularity.cuh:6).
+ * *improved is the reference's return value: false when the first sweep
+ * gains less than min_gain.  *q_before is the modularity of the pass graph's
+ * singleton partition, computed before the first sweep (when the pass did
+ * not improve, the value the reference prints, modularity.cu:504); *q_after
+ * the modularity after the last sweep.  Any output may be NULL. */
+int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_before,
+                               float *q_after);
+/* Sweeps, q_before and q_after of the last lv_modularity_optimization (or of
+ * the last pass of lv_run).  Any output may be NULL. */
+int lv_pass_stats(lv_graph *g, uint32_t *sweeps, float *q_before, float *q_after);
 /* Contract communities into the next pass's graph; composes the assignment. */
 int lv_aggregation(lv_graph *g);
 /* The full pass loop: modularity optimisation then aggregation until a pass

@@ -82,7 +82,7 @@ EXPORTS = [
     "lv_status_string", "lv_device_count", "lv_mtx_read", "lv_host_csr_free", "lv_csr_write",
     "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_graph_set_locality",
     "lv_tuning_default", "lv_graph_set_tuning", "lv_graph_get_tuning", "lv_autotune",
-    "lv_modularity_optimization", "lv_aggregation",
+    "lv_graph_set_stream", "lv_modularity_optimization", "lv_pass_stats", "lv_aggregation",
     "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_select", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
@@ -125,8 +125,11 @@ def lib():
     L.lv_graph_get_tuning.argtypes = [c.c_void_p, c.POINTER(Tuning)]
     L.lv_autotune.argtypes = [c.c_void_p, c.c_float, c.POINTER(Tuning), c.c_uint32,
                               c.POINTER(Tuning), c.POINTER(c.c_double)]
+    L.lv_graph_set_stream.argtypes = [c.c_void_p, c.c_void_p]
     L.lv_modularity_optimization.argtypes = [c.c_void_p, c.c_float, c.POINTER(c.c_int),
-                                             c.POINTER(c.c_float)]
+                                             c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lv_pass_stats.argtypes = [c.c_void_p, c.POINTER(c.c_uint32), c.POINTER(c.c_float),
+                                c.POINTER(c.c_float)]
     L.lv_aggregation.argtypes = [c.c_void_p]
     L.lv_run.argtypes = [c.c_void_p, c.c_float, c.POINTER(Result)]
     L.lv_get_assignment.argtypes = [c.c_void_p, _u32p]
@@ -332,11 +335,26 @@ class Graph(_Tunable):
         return np.float32(q.value)
 
     def modularity_optimization(self, min_gain):
-        imp, q = ctypes.c_int(), ctypes.c_float()
+        """One pass of local moves: (improved, q_before, q_after, sweeps)."""
+        imp, qb, qa = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
         _chk(self.L.lv_modularity_optimization(self.h, ctypes.c_float(min_gain),
-                                               ctypes.byref(imp), ctypes.byref(q)),
+                                               ctypes.byref(imp), ctypes.byref(qb),
+                                               ctypes.byref(qa)),
              "lv_modularity_optimization")
-        return bool(imp.value), np.float32(q.value)
+        return bool(imp.value), np.float32(qb.value), np.float32(qa.value), self.pass_stats()[0]
+
+    def pass_stats(self):
+        """(sweeps, q_before, q_after) of the last pass."""
+        it, qb, qa = ctypes.c_uint32(), ctypes.c_float(), ctypes.c_float()
+        _chk(self.L.lv_pass_stats(self.h, ctypes.byref(it), ctypes.byref(qb), ctypes.byref(qa)),
+             "lv_pass_stats")
+        return it.value, np.float32(qb.value), np.float32(qa.value)
+
+    def set_stream(self, stream):
+        """Run later calls on a caller's HIP stream (an integer handle such as
+        torch.cuda.Stream().cuda_stream); 0 or None returns to the graph's own."""
+        _chk(self.L.lv_graph_set_stream(self.h, ctypes.c_void_p(stream or None)),
+             "lv_graph_set_stream")
 
     def aggregation(self):
         _chk(self.L.lv_aggregation(self.h), "lv_aggregation")

@@ -1,141 +1,177 @@
 // louvain_main.cpp -- command-line drop-in for the reference binary
 // (main.cu, utils.cu:231-293):
 //
-//   louvain -f <file.mtx> -g <min-gain> [-v]
+//   louvain -f <file.mtx> -g <min-gain> [-v] [-t]
 //
-// stdout carries exactly what the reference prints: the final modularity
-// (modularity.cu:504), then "alg_ms total_ms" (main.cu:56), then with -v the
-// community listing of printCommunities (utils.cu:270-293).  Extra per-pass
-// figures go to stderr with -t.
+// Output contract (what a caller of the reference binary sees):
+//   stdout line 1  final modularity, default ostream formatting (modularity.cu:504)
+//   stdout line 2  "alg_ms total_ms" (main.cu:56)
+//   with -v        the number of communities, then one line per non-empty
+//                  community in increasing id: its 1-based rank, then its
+//                  members as 1-based vertex ids in increasing order, each
+//                  followed by one space (utils.cu:270-293)
+//   errors         usage line on stdout and exit status 1 for a missing -f/-g
+//                  or an unknown option; "-g option invalid argument: stof" /
+//                  "-g option error out of range: stof" for a gain std::stof
+//                  rejects; "Error opening file" for an unreadable input
+//                  (utils.cu:171-174, :245-264).
+// -t (not in the reference) prints per-run figures on stderr.
 #include <getopt.h>
-#include <hip/hip_runtime.h>
 
+#include <cerrno>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
-#include <iterator>
-#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/louvain_hip.h"
 
-static void usage_exit() {
-    std::cout << "Usage: ./gpulouvain -f mtx-matrix-file -g min-gain [-v]" << std::endl;
-    exit(1);
+namespace {
+
+const char kUsage[] = "Usage: ./gpulouvain -f mtx-matrix-file -g min-gain [-v]";
+
+struct CliOptions {
+    const char *path = nullptr;
+    float min_gain = 0.0f;
+    bool have_path = false, have_gain = false;
+    bool list_communities = false;  // -v
+    bool timings = false;           // -t
+};
+
+enum class GainParse { ok, invalid, range };
+
+// std::stof's acceptance rule without exceptions: leading white space
+// skipped, the longest prefix strtof converts is used, nothing converted ->
+// invalid, ERANGE -> out of range.
+GainParse parse_gain(const char *text, float *out) {
+    char *end = nullptr;
+    errno = 0;
+    const float x = strtof(text, &end);
+    if (end == text) return GainParse::invalid;
+    if (errno == ERANGE) return GainParse::range;
+    *out = x;
+    return GainParse::ok;
 }
 
-// parseArgs (utils.cu:231-267)
-static void parse_args(int argc, char **argv, float &threshold, bool &verbose, bool &timing,
-                       char *&filename) {
-    int c;
-    bool fopt = false, gopt = false;
-    while ((c = getopt(argc, argv, "f:g:vt")) != -1) {
-        switch (c) {
-            case 'f':
-                filename = optarg;
-                fopt = true;
-                break;
-            case 'g':
-                try {
-                    threshold = std::stof(std::string(optarg));
-                } catch (const std::invalid_argument &ia) {
-                    std::cout << "-g option invalid argument: " << ia.what() << std::endl;
-                    exit(1);
-                } catch (const std::out_of_range &oor) {
-                    std::cout << "-g option error out of range: " << oor.what() << std::endl;
-                    exit(1);
-                }
-                gopt = true;
-                break;
-            case 'v':
-                verbose = true;
-                break;
-            case 't':
-                timing = true;
-                break;
-            case '?':
-                usage_exit();
-                break;
-            default:
-                std::cout << "?? arg_parser error ??" << std::endl;
+// Returns -1 when the options are complete, else the exit status after the
+// message has been printed.
+int read_options(int argc, char **argv, CliOptions *o) {
+    for (int opt; (opt = getopt(argc, argv, "f:g:vt")) != -1;) {
+        if (opt == 'f') {
+            o->path = optarg;
+            o->have_path = true;
+        } else if (opt == 'g') {
+            const GainParse r = parse_gain(optarg, &o->min_gain);
+            if (r != GainParse::ok) {
+                std::cout << (r == GainParse::invalid ? "-g option invalid argument: "
+                                                      : "-g option error out of range: ")
+                          << "stof" << std::endl;
+                return 1;
+            }
+            o->have_gain = true;
+        } else if (opt == 'v') {
+            o->list_communities = true;
+        } else if (opt == 't') {
+            o->timings = true;
+        } else {  // unknown option or missing argument
+            std::cout << kUsage << std::endl;
+            return 1;
         }
     }
-    if (!fopt || !gopt) usage_exit();
+    if (!o->have_path || !o->have_gain) {
+        std::cout << kUsage << std::endl;
+        return 1;
+    }
+    return -1;
 }
 
-static int fail(int rc) {
+// Members of every community in increasing vertex order by a counting sort,
+// written as one buffer.
+void list_communities(uint32_t n_comm, const std::vector<uint32_t> &assign) {
+    std::vector<uint64_t> start(n_comm + 1, 0);
+    for (uint32_t c : assign) start[c + 1]++;
+    for (uint32_t c = 0; c < n_comm; ++c) start[c + 1] += start[c];
+    std::vector<uint32_t> members(assign.size());
+    std::vector<uint64_t> fill(start.begin(), start.end() - 1);
+    for (size_t v = 0; v < assign.size(); ++v) members[fill[assign[v]]++] = (uint32_t)v;
+    std::string out = std::to_string(n_comm) + "\n";
+    uint64_t rank = 0;
+    char num[24];
+    for (uint32_t c = 0; c < n_comm; ++c) {
+        if (start[c] == start[c + 1]) continue;
+        out += std::to_string(++rank);
+        out += ' ';
+        for (uint64_t i = start[c]; i < start[c + 1]; ++i) {
+            const int len = snprintf(num, sizeof(num), "%u ", members[i] + 1u);
+            out.append(num, (size_t)len);
+        }
+        out += '\n';
+        if (out.size() > (1u << 20)) {
+            std::cout.write(out.data(), (std::streamsize)out.size());
+            out.clear();
+        }
+    }
+    std::cout.write(out.data(), (std::streamsize)out.size());
+    std::cout.flush();
+}
+
+int report(int rc) {
     std::cerr << "louvain: " << lv_status_string(rc) << std::endl;
     return 1;
 }
 
+}  // namespace
+
 int main(int argc, char **argv) {
-    char *filename = nullptr;
-    float threshold = 0.0f;
-    bool verbose = false, timing = false;
-    parse_args(argc, argv, threshold, verbose, timing, filename);
+    CliOptions opt;
+    const int early = read_options(argc, argv, &opt);
+    if (early >= 0) return early;
 
     lv_host_csr csr;
-    int rc = lv_mtx_read(filename, &csr);
+    int rc = lv_mtx_read(opt.path, &csr);
     if (rc == LV_ERR_IO) {
-        std::cout << "Error opening file" << std::endl;  // utils.cu:171-174
+        std::cout << "Error opening file" << std::endl;
         return 1;
     }
-    if (rc) return fail(rc);
+    if (rc) return report(rc);
 
-    // copy phase: upload + state init (main.cu:19-36)
-    auto c0 = std::chrono::steady_clock::now();
+    // the reference's "copy" time: upload and state set-up (main.cu:19-36)
+    const auto t0 = std::chrono::steady_clock::now();
     lv_graph *g = nullptr;
     rc = lv_graph_create(0, csr.n, csr.row, csr.col, csr.w, &g);
-    auto c1 = std::chrono::steady_clock::now();
-    if (rc) {
-        lv_host_csr_free(&csr);
-        return fail(rc);
-    }
-    const float copy_ms = std::chrono::duration<float, std::milli>(c1 - c0).count();
+    const auto t1 = std::chrono::steady_clock::now();
+    lv_host_csr_free(&csr);
+    if (rc) return report(rc);
+    const float upload_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
 
     lv_result res;
-    rc = lv_run(g, threshold, &res);
+    rc = lv_run(g, opt.min_gain, &res);
     if (rc) {
         lv_graph_destroy(g);
-        lv_host_csr_free(&csr);
-        return fail(rc);
+        return report(rc);
     }
-    const float alg_ms = (float)res.alg_ms;
-    std::cout << res.q_final << std::endl;                                 // modularity.cu:504
-    std::cout << alg_ms << " " << alg_ms + copy_ms << std::endl;           // main.cu:56
-    if (timing) {
+    const float run_ms = (float)res.alg_ms;
+    std::cout << res.q_final << std::endl;
+    std::cout << run_ms << " " << run_ms + upload_ms << std::endl;
+    if (opt.timings)
         std::cerr << "passes " << res.passes << " iters " << res.total_iters << " communities "
                   << res.n_final << " edges/s " << (double)res.sum_edges / (res.alg_ms * 1e-3)
                   << std::endl;
-    }
-    if (verbose) {  // printCommunities (utils.cu:270-293)
+    if (opt.list_communities) {
         uint32_t n = 0, n0 = 0;
         uint64_t ne = 0;
         lv_graph_size(g, &n, &ne, &n0);
-        std::vector<uint32_t> comm(n0);
-        rc = lv_get_assignment(g, comm.data());
+        std::vector<uint32_t> assign(n0);
+        rc = lv_get_assignment(g, assign.data());
         if (rc) {
             lv_graph_destroy(g);
-            lv_host_csr_free(&csr);
-            return fail(rc);
+            return report(rc);
         }
-        std::vector<std::vector<uint32_t>> comm_vert(n);
-        for (uint32_t i = 0; i < n0; ++i) comm_vert[comm[i]].push_back(i + 1);
-        unsigned int comm_n = 1;
-        std::cout << n << std::endl;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!comm_vert[i].empty()) {
-                std::cout << comm_n << " ";
-                std::copy(comm_vert[i].begin(), comm_vert[i].end(),
-                          std::ostream_iterator<unsigned int>(std::cout, " "));
-                std::cout << std::endl;
-                comm_n++;
-            }
-        }
+        list_communities(n, assign);
     }
     lv_graph_destroy(g);
-    lv_host_csr_free(&csr);
     return 0;
 }

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -85,9 +86,22 @@ struct BatchGraph {
 
 }  // namespace
 
+// Bump allocator over a caller-provided device region (the fixed scratch
+// array of lv_run_noalloc): every buffer of a handle is carved from it with
+// 256-byte alignment, nothing is ever freed.
+struct Carver {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct lv_graph {
     int device = 0;
+    // the stream every call on the graph runs on: the handle's own, or one
+    // the caller set (lv_graph_set_stream)
     hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    // non-null: device buffers come from a fixed region, no allocation
+    Carver *carve = nullptr;
     // second stream of the handle: bucket step 7 runs the heavy rows' kernel
     // chain beside the block classes (fork and join by events; captured into
     // the sweep graph as two branches)
@@ -169,6 +183,7 @@ struct lv_graph {
     float m = 0.0f;
     bool pass_ready = false;
     uint32_t last_iters = 0;
+    float last_q_before = 0.0f, last_q_after = 0.0f;  // of the last pass
     // batch mode (G > 0): G graphs in one segmented CSR, vertex ids global
     uint32_t G = 0;
     uint32_t *gid = nullptr;       // device: vertex -> graph
@@ -182,6 +197,7 @@ struct lv_graph {
     std::vector<float> gm_h;
     std::vector<uint32_t> gstate_h;
     std::vector<BatchGraph> bg;
+    std::vector<int> gstatus0;  // per graph: LV_ERR_RANGE if its weights do not fit
     // profiling
     bool prof_on = false;
     uint32_t prof_mask = ~0u;  // families that get event pairs
@@ -211,6 +227,27 @@ int dmalloc(T **p, size_t n) {
 template <typename T>
 void dfree(T *&p) {
     if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// A buffer of the handle: hipMalloc, or the next 256-byte aligned piece of
+// the fixed region (LV_ERR_LIMIT once it is exhausted).
+template <typename T>
+int galloc(lv_graph *g, T **p, size_t n) {
+    if (!g->carve) return dmalloc(p, n);
+    *p = nullptr;
+    if (n == 0) n = 1;
+    Carver &c = *g->carve;
+    const size_t off = (c.used + 255) & ~(size_t)255;
+    if (n > (c.cap - std::min(c.cap, off)) / sizeof(T)) return LV_ERR_LIMIT;
+    *p = reinterpret_cast<T *>(c.base + off);
+    c.used = off + n * sizeof(T);
+    return LV_OK;
+}
+
+template <typename T>
+void gfree(lv_graph *g, T *&p) {
+    if (!g->carve) dfree(p);
     p = nullptr;
 }
 
@@ -263,7 +300,7 @@ hipEvent_t ev_get(lv_graph *g) {
 // same kernel durations.
 bool use_fork(const lv_graph *g) {
     static const bool off = getenv("LV_ONE_STREAM") != nullptr;
-    return !off && !g->prof_on;
+    return !off && !g->prof_on && g->stream2 != nullptr;
 }
 
 // RAII-free profiling bracket around a launch sequence of one family
@@ -389,38 +426,40 @@ int classify(lv_graph *g) {
         g->heavy_chunks = chunks;
         g->hp_nbuckets = nbuckets;
         if (nbuckets > g->hp_bcap) {
-            dfree(g->hp_bcnt);
-            dfree(g->hp_bpos);
-            dfree(g->hp_bcur);
-            dfree(g->hp_btmp);
-            dfree(g->hp_bvtx);
-            if (dmalloc(&g->hp_bcnt, nbuckets) || dmalloc(&g->hp_bpos, (size_t)nbuckets + 1) ||
-                dmalloc(&g->hp_bcur, nbuckets) ||
-                dmalloc(&g->hp_btmp, lv::scan_tmp_elems(nbuckets) + 8) ||
-                dmalloc(&g->hp_bvtx, nbuckets))
-                return LV_ERR_NOMEM;
+            gfree(g, g->hp_bcnt);
+            gfree(g, g->hp_bpos);
+            gfree(g, g->hp_bcur);
+            gfree(g, g->hp_btmp);
+            gfree(g, g->hp_bvtx);
+            int e = galloc(g, &g->hp_bcnt, nbuckets);
+            if (!e) e = galloc(g, &g->hp_bpos, (size_t)nbuckets + 1);
+            if (!e) e = galloc(g, &g->hp_bcur, nbuckets);
+            if (!e) e = galloc(g, &g->hp_btmp, lv::scan_tmp_elems(nbuckets) + 8);
+            if (!e) e = galloc(g, &g->hp_bvtx, nbuckets);
+            if (e) return e;
             g->hp_bcap = nbuckets;
         }
         if (chunks > g->hp_ccap) {
-            dfree(g->hp_cvtx);
-            dfree(g->hp_flist);
-            if (dmalloc(&g->hp_cvtx, chunks) || dmalloc(&g->hp_flist, (size_t)chunks + 1))
-                return LV_ERR_NOMEM;
+            gfree(g, g->hp_cvtx);
+            gfree(g, g->hp_flist);
+            int e = galloc(g, &g->hp_cvtx, chunks);
+            if (!e) e = galloc(g, &g->hp_flist, (size_t)chunks + 1);
+            if (e) return e;
             g->hp_ccap = chunks;
         }
         // owner of every chunk and bucket, so no kernel searches for it
         lv::launch_owner_fill(g->stream, g->heavy_coff, nh, g->hp_cvtx);
         lv::launch_owner_fill(g->stream, g->hp_bstart, nh, g->hp_bvtx);
         if (total > g->harena) {
-            dfree(g->htab);
+            gfree(g, g->htab);
             size_t cap = (size_t)total;
-            if (dmalloc(&g->htab, 2 * cap)) return LV_ERR_NOMEM;
+            if (int e = galloc(g, &g->htab, 2 * cap)) return e;
             g->harena = cap;
             lv::launch_fill_slots(g->stream, g->htab, cap);
         }
         if ((size_t)nh * 130 > g->hscr_cap) {
-            dfree(g->hscr);
-            if (dmalloc(&g->hscr, (size_t)nh * 130)) return LV_ERR_NOMEM;
+            gfree(g, g->hscr);
+            if (int e = galloc(g, &g->hscr, (size_t)nh * 130)) return e;
             g->hscr_cap = (size_t)nh * 130;
         }
     }
@@ -688,9 +727,10 @@ struct SweepGraph {
     }
 };
 
-bool graphs_enabled(const lv_graph *) {
+// A carved handle launches eagerly: instantiating a HIP graph allocates.
+bool graphs_enabled(const lv_graph *g) {
     static const bool off = getenv("LV_NO_GRAPH") != nullptr;
-    return !off;
+    return !off && !g->carve;
 }
 
 // Captures one sweep; on failure the caller falls back to eager launches.
@@ -721,6 +761,7 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
     float act = 0.0f, old = 0.0f;
     int rc = pass_init(g, &act);
     if (rc) return rc;
+    g->last_q_before = act;
     bool first = true, ret = true;
     uint32_t it = 0;
     SweepGraph sg;
@@ -748,6 +789,7 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
         if (stop) break;
     }
     g->last_iters = it;
+    g->last_q_after = act;
     if (improved) *improved = ret ? 1 : 0;
     if (iters) *iters = it;
     return LV_OK;
@@ -802,18 +844,21 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     LV_HIP(hipStreamSynchronize(s));
     g->agg_nchunks = chunks;
     if (NB > g->ap_bcap) {
-        dfree(g->ap_bcnt);
-        dfree(g->ap_bpos);
-        dfree(g->ap_bcur);
-        dfree(g->ap_bnd);
-        dfree(g->ap_boff);
-        dfree(g->ap_bowner);
-        dfree(g->ap_btmp);
-        if (dmalloc(&g->ap_bcnt, NB) || dmalloc(&g->ap_bpos, (size_t)NB + 1) ||
-            dmalloc(&g->ap_bcur, NB) || dmalloc(&g->ap_bnd, NB) ||
-            dmalloc(&g->ap_boff, (size_t)NB + 1) || dmalloc(&g->ap_bowner, NB) ||
-            dmalloc(&g->ap_btmp, lv::scan_tmp_elems(NB + 1) + 8))
-            return LV_ERR_NOMEM;
+        gfree(g, g->ap_bcnt);
+        gfree(g, g->ap_bpos);
+        gfree(g, g->ap_bcur);
+        gfree(g, g->ap_bnd);
+        gfree(g, g->ap_boff);
+        gfree(g, g->ap_bowner);
+        gfree(g, g->ap_btmp);
+        int e = galloc(g, &g->ap_bcnt, NB);
+        if (!e) e = galloc(g, &g->ap_bpos, (size_t)NB + 1);
+        if (!e) e = galloc(g, &g->ap_bcur, NB);
+        if (!e) e = galloc(g, &g->ap_bnd, NB);
+        if (!e) e = galloc(g, &g->ap_boff, (size_t)NB + 1);
+        if (!e) e = galloc(g, &g->ap_bowner, NB);
+        if (!e) e = galloc(g, &g->ap_btmp, lv::scan_tmp_elems(NB + 1) + 8);
+        if (e) return e;
         g->ap_bcap = NB;
     }
     lv::AggView av = aview(g);
@@ -872,10 +917,11 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
     if (total > g->garena) {
-        dfree(g->gkey);
-        dfree(g->gval);
-        if (dmalloc(&g->gkey, (size_t)total) || dmalloc(&g->gval, (size_t)total))
-            return LV_ERR_NOMEM;
+        gfree(g, g->gkey);
+        gfree(g, g->gval);
+        int e = galloc(g, &g->gkey, (size_t)total);
+        if (!e) e = galloc(g, &g->gval, (size_t)total);
+        if (e) return e;
         g->garena = (size_t)total;
         lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
         lv::launch_fill_u64(s, g->gval, 0ull, total);
@@ -884,8 +930,8 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     // per chunk of one, from lists built on the device
     const size_t lcap = (size_t)g->agg_nchunks + nl + 2;
     if (lcap > g->agg_flcap) {
-        dfree(g->agg_flist);
-        if (dmalloc(&g->agg_flist, lcap)) return LV_ERR_NOMEM;
+        gfree(g, g->agg_flist);
+        if (int e = galloc(g, &g->agg_flist, lcap)) return e;
         g->agg_flcap = lcap;
     }
     uint32_t *flist = g->agg_flist, *clist = g->agg_flist + g->agg_nchunks + 1;
@@ -1194,62 +1240,88 @@ void host_parallel(uint64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
-// exact host-side validation and fixed-point scale (same rule as the oracle):
-// S = the smallest shift making every weight an integer; the total at that
-// scale must stay below 2^61
-int scale_bits(uint64_t ne, const float *w, int *S_out) {
-    std::vector<int> sp(16, 0);
-    std::vector<char> bad(16, 0), nz(16, 0);
-    host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
-        int S = 0;
-        for (uint64_t j = a; j < b; ++j) {
-            const float x = w[j];
-            if (!(x >= 0.0f) || std::isinf(x)) {
-                bad[t] = 1;
-                return;
-            }
-            if (x == 0.0f) continue;
-            nz[t] = 1;
-            uint32_t bits;
-            memcpy(&bits, &x, 4);
-            const int ef = (int)((bits >> 23) & 0xff);
-            uint32_t man = bits & 0x7fffffu;
-            int e;
-            if (ef == 0) {
-                e = -149;
-            } else {
-                man |= 0x800000u;
-                e = ef - 150;
-            }
-            e += __builtin_ctz(man);  // man != 0 for a non-zero weight
-            if (-e > S) S = -e;
+// Fixed-point scale of weights w[a, b): *S = the smallest shift making every
+// weight an integer (0 if all are zero); LV_ERR_RANGE for a negative,
+// infinite or NaN weight or a shift above 62 (the oracle's rule).
+int min_shift(const float *w, uint64_t a, uint64_t b, int *S_out) {
+    int S = 0;
+    for (uint64_t j = a; j < b; ++j) {
+        const float x = w[j];
+        if (!(x >= 0.0f) || std::isinf(x)) return LV_ERR_RANGE;
+        if (x == 0.0f) continue;
+        uint32_t bits;
+        memcpy(&bits, &x, 4);
+        const int ef = (int)((bits >> 23) & 0xff);
+        uint32_t man = bits & 0x7fffffu;
+        int e;
+        if (ef == 0) {
+            e = -149;
+        } else {
+            man |= 0x800000u;
+            e = ef - 150;
         }
-        sp[t] = S;
+        e += __builtin_ctz(man);  // man != 0 for a non-zero weight
+        if (-e > S) S = -e;
+    }
+    if (S > 62) return LV_ERR_RANGE;
+    *S_out = S;
+    return LV_OK;
+}
+
+// Whether sum(w[a, b) * 2^S) stays below 2^61 (every exact accumulator then
+// fits an int64 with headroom).  A single weight at or above the limit fails
+// before its integer conversion, which would not be representable.
+bool total_fits(const float *w, uint64_t a, uint64_t b, int S) {
+    const float lim = ldexpf(1.0f, 61);
+    u128 tot = 0;
+    for (uint64_t j = a; j < b; ++j) {
+        if (w[j] == 0.0f) continue;
+        const float x = ldexpf(w[j], S);
+        if (!(x < lim)) return false;
+        tot += (u128)(uint64_t)x;
+        if (tot >= (u128)kFixedLimit) return false;
+    }
+    return true;
+}
+
+// exact host-side validation and fixed-point scale of one graph (same rule as
+// the oracle): S = the smallest shift making every weight an integer; the
+// total at that scale must stay below 2^61
+int scale_bits(uint64_t ne, const float *w, int *S_out) {
+    std::vector<int> sp(16, 0), bad(16, 0);
+    host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
+        bad[t] = min_shift(w, a, b, &sp[t]);
     });
     int S = 0;
-    bool any = false;
     for (int t = 0; t < 16; ++t) {
         if (bad[t]) return LV_ERR_RANGE;
         S = std::max(S, sp[t]);
-        any |= nz[t] != 0;
     }
-    if (S > 100) return LV_ERR_RANGE;
-    if (any && S > 62) return LV_ERR_RANGE;
+    std::vector<char> fits(16, 1);
     std::vector<u128> part(16, 0);
     host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned t) {
+        const float lim = ldexpf(1.0f, 61);
         u128 tot = 0;
-        for (uint64_t j = a; j < b; ++j)
-            if (w[j] != 0.0f) tot += (u128)(uint64_t)ldexpf(w[j], S);
+        for (uint64_t j = a; j < b && fits[t]; ++j) {
+            if (w[j] == 0.0f) continue;
+            const float x = ldexpf(w[j], S);
+            if (!(x < lim)) fits[t] = 0;
+            else tot += (u128)(uint64_t)x;
+        }
         part[t] = tot;
     });
     u128 tot = 0;
-    for (auto p : part) tot += p;
+    for (int t = 0; t < 16; ++t) {
+        if (!fits[t]) return LV_ERR_RANGE;
+        tot += part[t];
+    }
     if (tot >= (u128)kFixedLimit) return LV_ERR_RANGE;
     *S_out = S;
     return LV_OK;
 }
 
 void free_all(lv_graph *g) {
+    if (g->carve) return;  // carved buffers belong to the fixed region
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (auto &r : g->prof_pending) {
         (void)hipEventDestroy(r.a);
@@ -1347,8 +1419,53 @@ void free_all(lv_graph *g) {
     g->ev_fork = g->ev_join = nullptr;
     if (g->stream2) (void)hipStreamDestroy(g->stream2);
     g->stream2 = nullptr;
-    if (g->stream) (void)hipStreamDestroy(g->stream);
-    g->stream = nullptr;
+    if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+    g->own_stream = g->stream = nullptr;
+}
+
+// The pass loop of main (main.cu:43-48): modularity optimisation, then
+// aggregation while the pass improved; per-run figures into *res (not the
+// time).
+int run_passes(lv_graph *g, float min_gain, lv_result *res) {
+    int rc = LV_OK;
+    double bytes = 0;
+    for (;;) {
+        const double V = g->n, E = (double)g->ne;
+        int imp = 0;
+        uint32_t it = 0;
+        float qp = 0.0f;
+        rc = modularity_optimization(g, min_gain, &imp, &qp, &it);
+        if (rc) break;
+        res->total_iters += it;
+        res->sum_edges += g->ne;
+        res->sum_edge_visits += (uint64_t)(it + 1) * g->ne;
+        bytes += 8 * E + 8 * V + it * (24 * E + 44 * V);
+        if (!imp) {
+            res->q_final = qp;
+            break;
+        }
+        rc = aggregation(g);
+        if (rc) break;
+        bytes += 12 * E + 8 * (double)g->ne + 24 * V + 4 * (double)g->n;
+        res->passes++;
+    }
+    res->bytes_alg = bytes;
+    res->n_final = g->n;
+    res->status = rc;
+    return rc;
+}
+
+// original vertex -> vertex of the current graph, into host memory
+int download_assignment(lv_graph *g, uint32_t *assignment) {
+    const uint32_t *src = g->init_comm;
+    if (g->relabeled) {  // positions of the current pass -> its input ids
+        lv::launch_gather_u32(g->stream, g->lid, g->init_comm, g->rtmp, g->n0);
+        src = g->rtmp;
+    }
+    LV_HIP(hipMemcpyAsync(assignment, src, (size_t)g->n0 * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
 }
 
 }  // namespace
@@ -1408,8 +1525,77 @@ int check_input(uint32_t n, const uint32_t *row, const uint32_t *col, const floa
     return scale_bits(ne, w, S);
 }
 
-// A graph handle with device buffers for up to n_cap vertices and e_cap
-// entries (every buffer a pass needs; arenas grow on demand).
+// Every buffer a pass needs, for up to n_cap vertices and e_cap entries.
+// The arenas that depend on the pass graph (heavy tables and buckets, merge
+// buckets and tables of large communities) grow on demand, except on a
+// carved handle, which reserves them here at their worst case so that no
+// pass ever needs more:
+//   heavy rows have more than 319 entries, so nh <= E/320;
+//   chunks   sum ceil(d/2048)           <= E/2048 + nh       < E/256
+//   buckets  sum pow2 >= d/64 (<= 2d/64) <= E/32 + nh        < E/16
+//   tables   sum pow2 >= 2d             <= 4E slots
+//   large communities have more than 2048 member entries, so nl <= E/2048;
+//   merge buckets  sum (2d/384 + 1)     <= E/192 + nl        < E/128
+//   merge tables   sum pow2 >= 2d       <= 4E
+//   chunk list     sum ceil(d/2048) + nl + 2 <= E/1024 + 2 + nl.
+int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
+    g->n_cap = n_cap;
+    g->e_cap = e_cap;
+    const size_t N = (size_t)n_cap + 1, E = (size_t)e_cap;
+    int e = LV_OK;
+    auto A = [&](auto **p, size_t count) {
+        if (!e) e = galloc(g, p, count);
+    };
+    // 4 entries of padding: 16-byte loads of the masked sums may start
+    // inside the last row and end past it
+    A(&g->row, N), A(&g->col, E + 4), A(&g->w, E + 4);
+    A(&g->row2, N), A(&g->col2, E + 4), A(&g->w2, E + 4);
+    if (pristine) A(&g->row0, N), A(&g->col0, E), A(&g->w0, E);
+    A(&g->k, N), A(&g->comm, N), A(&g->comm_sz, N);
+    A(&g->new_comm, N), A(&g->A, N), A(&g->init_comm, N);
+    A(&g->order, N), A(&g->heavy_caps, N), A(&g->heavy_off, N + 1);
+    A(&g->lid, N), A(&g->pmap, N), A(&g->rtmp, N), A(&g->rrank, N);
+    A(&g->heavy_nch, N), A(&g->heavy_coff, N + 1);
+    A(&g->hp_nbuck, N), A(&g->hp_bstart, N + 1), A(&g->hp_flag, N);
+    A(&g->dlist, E), A(&g->eslot, E);
+    A(&g->new_id, N), A(&g->comm_deg, N), A(&g->epos, N), A(&g->vnid, N);
+    A(&g->mstart, N), A(&g->mcursor, N), A(&g->members, N);
+    A(&g->cnt, N), A(&g->aorder, N);
+    A(&g->tkey, E), A(&g->skey, E), A(&g->tval, E), A(&g->sval, E);
+    A(&g->agg_caps, N), A(&g->agg_off, N + 1);
+    A(&g->agg_mdeg, N), A(&g->agg_moff, N + 1), A(&g->agg_cnk, N);
+    A(&g->agg_coff, N + 1), A(&g->agg_dcnt, N);
+    A(&g->ap_nbk, N), A(&g->ap_bstart, N + 1), A(&g->ap_cflag, N);
+    A(&g->ap_nflag, 1);
+    A(&g->scan_tmp32, lv::scan_tmp_elems(n_cap + 1) + 8);
+    A(&g->scan_tmp64, lv::scan_tmp_elems(n_cap + 1) + 8);
+    A(&g->dc, 1);
+    if (e || !g->carve) return e;
+    const size_t hb = E / 16 + 1, hc = E / 256 + 1, slots = 4 * E + 1;
+    A(&g->hp_bcnt, hb), A(&g->hp_bpos, hb + 1), A(&g->hp_bcur, hb);
+    A(&g->hp_btmp, lv::scan_tmp_elems((uint32_t)hb) + 8), A(&g->hp_bvtx, hb);
+    A(&g->hp_cvtx, hc), A(&g->hp_flist, hc + 1);
+    A(&g->htab, 2 * slots);
+    A(&g->hscr, (E / 320 + 1) * 130);
+    const size_t ab = E / 128 + 1, lcap = E / 1024 + E / 2048 + 4;
+    A(&g->ap_bcnt, ab), A(&g->ap_bpos, ab + 1), A(&g->ap_bcur, ab), A(&g->ap_bnd, ab);
+    A(&g->ap_boff, ab + 1), A(&g->ap_bowner, ab);
+    A(&g->ap_btmp, lv::scan_tmp_elems((uint32_t)ab + 1) + 8);
+    A(&g->gkey, slots), A(&g->gval, slots);
+    A(&g->agg_flist, lcap);
+    if (e) return e;
+    g->hp_bcap = hb;
+    g->hp_ccap = hc;
+    g->harena = slots;
+    g->hscr_cap = (E / 320 + 1) * 130;
+    g->ap_bcap = ab;
+    g->garena = slots;
+    g->agg_flcap = lcap;
+    return LV_OK;
+}
+
+// A graph handle with its own streams and device buffers for up to n_cap
+// vertices and e_cap entries.
 int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     *out = nullptr;
     int ndev = 0;
@@ -1420,11 +1606,12 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     lv_graph *g = new (std::nothrow) lv_graph();
     if (!g) return LV_ERR_NOMEM;
     g->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&g->own_stream) != hipSuccess) {
         (void)hipGetLastError();
         delete g;
         return LV_ERR_HIP;
     }
+    g->stream = g->own_stream;
     if (hipStreamCreate(&g->stream2) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -1433,36 +1620,7 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
         delete g;
         return LV_ERR_HIP;
     }
-    g->n_cap = n_cap;
-    g->e_cap = e_cap;
-    const size_t N = (size_t)n_cap + 1, E = (size_t)e_cap;
-    int e = 0;
-    // 4 entries of padding: 16-byte loads of the masked sums may start
-    // inside the last row and end past it
-    e |= dmalloc(&g->row, N) | dmalloc(&g->col, E + 4) | dmalloc(&g->w, E + 4);
-    e |= dmalloc(&g->row2, N) | dmalloc(&g->col2, E + 4) | dmalloc(&g->w2, E + 4);
-    e |= dmalloc(&g->row0, N) | dmalloc(&g->col0, E) | dmalloc(&g->w0, E);
-    e |= dmalloc(&g->k, N) | dmalloc(&g->comm, N) | dmalloc(&g->comm_sz, N);
-    e |= dmalloc(&g->new_comm, N) | dmalloc(&g->A, N) | dmalloc(&g->init_comm, N);
-    e |= dmalloc(&g->order, N) | dmalloc(&g->heavy_caps, N) | dmalloc(&g->heavy_off, N + 1);
-    e |= dmalloc(&g->lid, N) | dmalloc(&g->pmap, N) | dmalloc(&g->rtmp, N) | dmalloc(&g->rrank, N);
-    e |= dmalloc(&g->heavy_nch, N) | dmalloc(&g->heavy_coff, N + 1);
-    e |= dmalloc(&g->hp_nbuck, N) | dmalloc(&g->hp_bstart, N + 1) | dmalloc(&g->hp_flag, N);
-    e |= dmalloc(&g->dlist, E) | dmalloc(&g->eslot, E);
-    e |= dmalloc(&g->new_id, N) | dmalloc(&g->comm_deg, N) | dmalloc(&g->epos, N);
-    e |= dmalloc(&g->vnid, N);
-    e |= dmalloc(&g->mstart, N) | dmalloc(&g->mcursor, N) | dmalloc(&g->members, N);
-    e |= dmalloc(&g->cnt, N) | dmalloc(&g->aorder, N);
-    e |= dmalloc(&g->tkey, E) | dmalloc(&g->skey, E) | dmalloc(&g->tval, E) |
-         dmalloc(&g->sval, E);
-    e |= dmalloc(&g->agg_caps, N) | dmalloc(&g->agg_off, N + 1);
-    e |= dmalloc(&g->agg_mdeg, N) | dmalloc(&g->agg_moff, N + 1) | dmalloc(&g->agg_cnk, N);
-    e |= dmalloc(&g->agg_coff, N + 1) | dmalloc(&g->agg_dcnt, N);
-    e |= dmalloc(&g->ap_nbk, N) | dmalloc(&g->ap_bstart, N + 1) | dmalloc(&g->ap_cflag, N);
-    e |= dmalloc(&g->ap_nflag, 1);
-    e |= dmalloc(&g->scan_tmp32, lv::scan_tmp_elems(n_cap + 1) + 8);
-    e |= dmalloc(&g->scan_tmp64, lv::scan_tmp_elems(n_cap + 1) + 8);
-    e |= dmalloc(&g->dc, 1);
+    int e = graph_buffers(g, n_cap, e_cap, true);
     if (!e && hipHostMalloc((void **)&g->hc, sizeof(Counters)) != hipSuccess) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
@@ -1496,6 +1654,57 @@ int graph_upload(lv_graph *g, uint32_t n, const uint32_t *row, const uint32_t *c
     g->scale = ldexpf(1.0f, S);
     g->inv_scale = ldexpf(1.0f, -S);
     return lv_graph_reset(g);  // working copy from the pristine one, identity init_comm
+}
+
+static_assert(sizeof(Counters) <= kHostCountersBytes, "host counter buffer");
+
+int run_fixed(int device, char *base, size_t cap, void *host_counters, uint32_t n,
+              const uint32_t *row, const uint32_t *col, const float *w, float min_gain,
+              void *stream, uint32_t *assignment, lv_result *res) {
+    memset(res, 0, sizeof(*res));
+    int S = 0;
+    int rc = check_input(n, row, col, w, &S);
+    if (rc) return rc;
+    const uint64_t ne = row[n];
+    Carver cv;
+    cv.base = base;
+    cv.cap = cap;
+    lv_graph g;
+    g.device = device;
+    g.carve = &cv;
+    g.stream = static_cast<hipStream_t>(stream);
+    g.hc = static_cast<Counters *>(host_counters);
+    rc = graph_buffers(&g, n, ne, false);
+    if (rc) return rc;  // LV_ERR_LIMIT: the graph needs more than the region
+    hipStream_t s = g.stream;
+    LV_HIP(hipMemcpyAsync(g.row, row, ((size_t)n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          s));
+    if (ne) {
+        LV_HIP(hipMemcpyAsync(g.col, col, ne * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(g.w, w, ne * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    g.n = g.n0 = n;
+    g.ne = g.ne0 = ne;
+    g.S = S;
+    g.scale = ldexpf(1.0f, S);
+    g.inv_scale = ldexpf(1.0f, -S);
+    // the region keeps the previous call's contents: everything that must
+    // start empty is cleared here (the rest is written before it is read)
+    lv::launch_iota_u32(s, g.init_comm, n);
+    lv::launch_fill_slots(s, g.htab, g.harena);
+    lv::launch_fill_u32(s, g.gkey, lv::kEmpty, g.garena);
+    lv::launch_fill_u64(s, g.gval, 0ull, g.garena);
+    LV_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = run_passes(&g, min_gain, res);
+    if (!rc)
+        rc = download_assignment(&g, assignment);  // synchronizes the stream
+    else
+        (void)hipStreamSynchronize(s);
+    res->alg_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                      .count();
+    res->status = rc;
+    return rc;
 }
 
 }  // namespace lv_host
@@ -1701,15 +1910,33 @@ int lv_modularity(lv_graph *g, float *q) {
     return LV_OK;
 }
 
-int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_printed) {
+int lv_modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_before,
+                               float *q_after) {
     if (!g) return LV_ERR_ARG;
     LV_HIP(hipSetDevice(g->device));
-    float qp = 0.0f;
     int imp = 0;
-    int rc = modularity_optimization(g, min_gain, &imp, &qp, nullptr);
+    int rc = modularity_optimization(g, min_gain, &imp, nullptr, nullptr);
     if (rc) return rc;
     if (improved) *improved = imp;
-    if (q_printed && !imp) *q_printed = qp;
+    if (q_before) *q_before = g->last_q_before;
+    if (q_after) *q_after = g->last_q_after;
+    return LV_OK;
+}
+
+int lv_pass_stats(lv_graph *g, uint32_t *sweeps, float *q_before, float *q_after) {
+    if (!g) return LV_ERR_ARG;
+    if (sweeps) *sweeps = g->last_iters;
+    if (q_before) *q_before = g->last_q_before;
+    if (q_after) *q_after = g->last_q_after;
+    return LV_OK;
+}
+
+int lv_graph_set_stream(lv_graph *g, void *stream) {
+    if (!g || g->carve) return LV_ERR_ARG;
+    LV_HIP(hipSetDevice(g->device));
+    // work already queued on the previous stream completes first
+    LV_HIP(hipStreamSynchronize(g->stream));
+    g->stream = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
     return LV_OK;
 }
 
@@ -1727,28 +1954,7 @@ int lv_run(lv_graph *g, float min_gain, lv_result *res) {
     LV_HIP(hipEventCreate(&t0));
     LV_HIP(hipEventCreate(&t1));
     LV_HIP(hipEventRecord(t0, g->stream));
-    int rc = LV_OK;
-    double bytes = 0;
-    for (;;) {
-        const double V = g->n, E = (double)g->ne;
-        int imp = 0;
-        uint32_t it = 0;
-        float qp = 0.0f;
-        rc = modularity_optimization(g, min_gain, &imp, &qp, &it);
-        if (rc) break;
-        res->total_iters += it;
-        res->sum_edges += g->ne;
-        res->sum_edge_visits += (uint64_t)(it + 1) * g->ne;
-        bytes += 8 * E + 8 * V + it * (24 * E + 44 * V);
-        if (!imp) {
-            res->q_final = qp;
-            break;
-        }
-        rc = aggregation(g);
-        if (rc) break;
-        bytes += 12 * E + 8 * (double)g->ne + 24 * V + 4 * (double)g->n;
-        res->passes++;
-    }
+    const int rc = run_passes(g, min_gain, res);
     (void)hipEventRecord(t1, g->stream);
     (void)hipEventSynchronize(t1);
     float ms = 0.0f;
@@ -1756,24 +1962,13 @@ int lv_run(lv_graph *g, float min_gain, lv_result *res) {
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     res->alg_ms = ms;
-    res->bytes_alg = bytes;
-    res->n_final = g->n;
-    res->status = rc;
     return rc;
 }
 
 int lv_get_assignment(lv_graph *g, uint32_t *assignment) {
     if (!g || !assignment) return LV_ERR_ARG;
     LV_HIP(hipSetDevice(g->device));
-    const uint32_t *src = g->init_comm;
-    if (g->relabeled) {  // positions of the current pass -> its input ids
-        lv::launch_gather_u32(g->stream, g->lid, g->init_comm, g->rtmp, g->n0);
-        src = g->rtmp;
-    }
-    LV_HIP(hipMemcpyAsync(assignment, src, (size_t)g->n0 * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, g->stream));
-    LV_HIP(hipStreamSynchronize(g->stream));
-    return LV_OK;
+    return download_assignment(g, assignment);
 }
 
 int lv_graph_size(lv_graph *g, uint32_t *n, uint64_t *ne, uint32_t *n0) {
@@ -1867,14 +2062,47 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
     if (row[0] != 0) return LV_ERR_ARG;
     for (uint32_t v = 0; v < n; ++v)
         if (row[v + 1] < row[v]) return LV_ERR_ARG;
-    // every entry stays inside its graph's vertex range
-    for (uint32_t gi = 0; gi < ngraphs; ++gi)
-        for (uint64_t j = row[graph_off[gi]]; j < row[graph_off[gi + 1]]; ++j)
-            if (col[j] < graph_off[gi] || col[j] >= graph_off[gi + 1]) return LV_ERR_ARG;
-    lv_graph *g = nullptr;
-    int rc = lv_graph_create(device, n, row, col, w, &g);
-    if (rc) return rc;
+    const uint64_t ne = row[n];
+    if (ne && (!col || !w)) return LV_ERR_ARG;
     const uint32_t G = ngraphs;
+    // per graph, on the host threads: every entry stays inside its graph's
+    // vertex range, weights are valid, and the graph's own fixed-point shift
+    std::vector<int> gS(G, 0), gbad(G, 0);
+    host_parallel(G, [&](uint64_t a, uint64_t b, unsigned) {
+        for (uint64_t gi = a; gi < b; ++gi) {
+            const uint64_t e0 = row[graph_off[gi]], e1 = row[graph_off[gi + 1]];
+            for (uint64_t j = e0; j < e1; ++j)
+                if (col[j] < graph_off[gi] || col[j] >= graph_off[gi + 1]) {
+                    gbad[gi] = LV_ERR_ARG;
+                    break;
+                }
+            if (!gbad[gi]) gbad[gi] = min_shift(w, e0, e1, &gS[gi]);
+        }
+    });
+    int S = 0;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        if (gbad[gi]) return gbad[gi];
+        S = std::max(S, gS[gi]);
+    }
+    // One scale serves the batch: exact values and their FP32 roundings do
+    // not depend on it.  A graph whose total does not fit at the batch's
+    // scale (weights of very different magnitudes across graphs) is reported
+    // with LV_ERR_RANGE by lv_batch_run and left out of the run; every other
+    // graph is unaffected.
+    std::vector<int> status(G, LV_OK);
+    host_parallel(G, [&](uint64_t a, uint64_t b, unsigned) {
+        for (uint64_t gi = a; gi < b; ++gi)
+            if (!total_fits(w, row[graph_off[gi]], row[graph_off[gi + 1]], S))
+                status[gi] = LV_ERR_RANGE;
+    });
+    lv_graph *g = nullptr;
+    int rc = graph_new(device, n, ne, &g);
+    if (rc) return rc;
+    rc = graph_upload(g, n, row, col, w, S);
+    if (rc) {
+        lv_graph_destroy(g);
+        return rc;
+    }
     g->G = G;
     int e = 0;
     e |= dmalloc(&g->gid, (size_t)n + 1) | dmalloc(&g->gm_d, G) | dmalloc(&g->gstate_d, G);
@@ -1894,6 +2122,7 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
     g->gm_h.assign(G, 0.0f);
     g->gstate_h.assign(G, kGsActive);
     g->bg.assign(G, BatchGraph());
+    g->gstatus0 = std::move(status);
     if (hipMemcpy(g->voff_d, g->voff.data(), (G + 1) * sizeof(uint32_t),
                   hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
@@ -1920,8 +2149,10 @@ int lv_batch_run(lv_graph *g, float min_gain, lv_batch_graph_result *per_graph,
     LV_HIP(hipEventCreate(&t1));
     LV_HIP(hipEventRecord(t0, g->stream));
     g->bg.assign(G, BatchGraph());
-    for (uint32_t gi = 0; gi < G; ++gi)
-        g->gstate_h[gi] = g->voff[gi + 1] > g->voff[gi] ? kGsActive : kGsDone;
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        g->bg[gi].status = g->gstatus0[gi];
+        g->gstate_h[gi] = (g->voff[gi + 1] > g->voff[gi] && !g->gstatus0[gi]) ? kGsActive : kGsDone;
+    }
     int rc = push_gstate(g);
     bool more = true;
     while (!rc && more) rc = batch_pass(g, min_gain, &more);

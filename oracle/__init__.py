@@ -75,6 +75,9 @@ def lib():
                      ("or_m", ctypes.c_float)]:
         getattr(L, name).argtypes = [ctypes.c_void_p]
         getattr(L, name).restype = rt
+    L.or_last_pass_q.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_float)]
+    L.or_last_pass_q.restype = None
     L.or_get_csr.argtypes = [ctypes.c_void_p, _u64p, _u32p, _f32p]
     L.or_get_k.argtypes = [ctypes.c_void_p, _f32p]
     L.or_get_comm.argtypes = [ctypes.c_void_p, _u32p]
@@ -221,6 +224,12 @@ class Oracle:
                                                     ctypes.byref(imp), ctypes.byref(qp),
                                                     ctypes.byref(it)))
         return bool(imp.value), np.float32(qp.value), it.value
+
+    def last_pass_q(self):
+        """(Q at the last pass's init, Q after its last sweep)."""
+        qb, qa = ctypes.c_float(0), ctypes.c_float(0)
+        self.L.or_last_pass_q(self.h, ctypes.byref(qb), ctypes.byref(qa))
+        return np.float32(qb.value), np.float32(qa.value)
 
     def aggregation(self):
         self._chk(self.L.or_aggregation(self.h))

@@ -57,6 +57,8 @@ struct or_state {
     int64_t *acc;
     uint32_t *stamp;
     uint32_t stamp_gen;
+    /* modularity before the first and after the last sweep of the last pass */
+    float q_before, q_after;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -415,8 +417,16 @@ or_state *or_create(uint32_t n, const uint64_t *row, const uint32_t *col, const 
     i128 tot = 0;
     for (uint64_t j = 0; j < ne; j++) {
         if (w[j] == 0.0f) continue;
-        tot += (i128)(int64_t)ldexpf(w[j], S);
-        if (S > 62 || tot >= (i128)OR_FIXED_LIMIT) {
+        /* a single weight at or above the limit fails before its cast, which
+         * would not be representable in int64 for w*2^S >= 2^63 */
+        const float x = ldexpf(w[j], S);
+        if (S > 62 || !(x < ldexpf(1.0f, 61))) {
+            free(st);
+            *err = OR_ERR_RANGE;
+            return NULL;
+        }
+        tot += (i128)(int64_t)x;
+        if (tot >= (i128)OR_FIXED_LIMIT) {
             free(st);
             *err = OR_ERR_RANGE;
             return NULL;
@@ -722,6 +732,7 @@ int or_modularity_optimization(or_state *st, float min_gain, int *improved, floa
     float act = 0.0f, old;
     int rc = or_pass_init(st, &act);
     if (rc) return rc;
+    st->q_before = act;
     int first = 1, ret = 1;
     uint32_t it = 0;
     for (;;) {
@@ -738,6 +749,7 @@ int or_modularity_optimization(or_state *st, float min_gain, int *improved, floa
         first = 0;
         if (stop) break;
     }
+    st->q_after = act;
     if (ret) {
         /* graphVertToComm (:51-56, :515-517) */
         for (uint32_t i = 0; i < st->n0; i++) st->init_comm[i] = st->comm[st->init_comm[i]];
@@ -887,6 +899,10 @@ uint64_t or_ne(const or_state *st) { return st->row[st->n]; }
 uint32_t or_n0(const or_state *st) { return st->n0; }
 int or_scale(const or_state *st) { return st->S; }
 float or_m(const or_state *st) { return st->m; }
+void or_last_pass_q(const or_state *st, float *q_before, float *q_after) {
+    if (q_before) *q_before = st->q_before;
+    if (q_after) *q_after = st->q_after;
+}
 void or_get_csr(const or_state *st, uint64_t *row, uint32_t *col, float *w) {
     memcpy(row, st->row, ((size_t)st->n + 1) * sizeof(uint64_t));
     memcpy(col, st->col, st->row[st->n] * sizeof(uint32_t));

@@ -87,6 +87,8 @@ uint64_t or_ne(const or_state *st);
 uint32_t or_n0(const or_state *st);
 int or_scale(const or_state *st);
 float or_m(const or_state *st);
+/* modularity at the last pass's init and after its last sweep */
+void or_last_pass_q(const or_state *st, float *q_before, float *q_after);
 void or_get_csr(const or_state *st, uint64_t *row, uint32_t *col, float *w);
 void or_get_k(const or_state *st, float *k);
 void or_get_comm(const or_state *st, uint32_t *comm);

@@ -142,12 +142,37 @@ def test_graph_create_validates_before_device(lv):
     rc = L.lv_graph_create(0, 2, np.array([0, 2, 1], np.uint32), np.array([1, 0], np.uint32),
                            np.array([1, 1], np.float32), ctypes.byref(h))
     assert rc == 4
-    # non-finite and out-of-range weights
-    for bad in (np.inf, np.nan, 1e-30):
+    # non-finite and out-of-range weights: too fine for a shift of 62 bits,
+    # or single weights beyond the 2^61 limit of the exact sums (rejected
+    # before their conversion to integers)
+    for bad in (np.inf, np.nan, 1e-30, 1e30, 2.0 ** 70, 2.0 ** 61):
         rc = L.lv_graph_create(0, 2, np.array([0, 1, 2], np.uint32),
                                np.array([1, 0], np.uint32),
                                np.array([bad, bad], np.float32), ctypes.byref(h))
         assert rc == 3
+
+
+def test_batch_create_validates_before_device(lv):
+    L = lv.lib()
+    h = ctypes.c_void_p()
+    off = np.array([0, 2, 4], np.uint64)
+    row = np.array([0, 1, 2, 3, 4], np.uint32)
+    w = np.ones(4, np.float32)
+    # entries present but no column or weight array (a second handle on the
+    # library whose argument types admit NULL)
+    L2 = ctypes.CDLL(lv.LIB_PATH)
+    L2.lv_batch_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_void_p)]
+    assert L2.lv_batch_create(0, 2, off.ctypes.data, row.ctypes.data, None, None,
+                              ctypes.byref(h)) == 4
+    # an entry of graph 0 pointing into graph 1
+    col = np.array([1, 2, 3, 2], np.uint32)
+    assert L.lv_batch_create(0, 2, off, row, col, w, ctypes.byref(h)) == 4
+    # a negative weight in one graph
+    col = np.array([1, 0, 3, 2], np.uint32)
+    assert L.lv_batch_create(0, 2, off, row, col, np.array([1, 1, -1, -1], np.float32),
+                             ctypes.byref(h)) == 3
 
 
 def test_no_device_is_an_error(lv):

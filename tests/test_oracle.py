@@ -52,8 +52,33 @@ def test_bcsstk16_facts(orc, bcsstk16_text):
     assert row.size - 1 == 4884 and col.size == 290378
     assert deg.max() == 81
     assert int((col == np.repeat(np.arange(4884), deg)).sum()) == 4884  # self-loops
-    assert int((w == 0).sum()) == 2 * 759 - 0 or int((w == 0).sum()) > 0
+    # zero-weight entries counted straight from the text: an off-diagonal line
+    # of weight 0 gives two CSR entries, a diagonal one gives one
+    zeros = 0
+    for line in bcsstk16_text.splitlines()[1:]:
+        if line.startswith("%"):
+            continue
+        p = line.split()
+        if len(p) >= 3 and float(p[2]) == 0.0:
+            zeros += 1 if p[0] == p[1] else 2
+    assert zeros == 1492
+    assert int((w == 0).sum()) == zeros
     assert orc.scale_bits(w) == 29
+
+
+def test_weights_beyond_the_exact_range(orc):
+    """A weight whose fixed-point value reaches 2^61 is refused before its
+    conversion to an integer (it would not be representable in int64)."""
+    row = np.array([0, 1, 2], np.uint64)
+    col = np.array([1, 0], np.uint32)
+    for big in (1e30, 2.0 ** 70, 2.0 ** 61):
+        with pytest.raises(orc.OracleError):
+            orc.Oracle(row, col, np.array([big, big], np.float32))
+    # just below the limit: accepted, m is the weight itself
+    x = np.float32(2.0 ** 59)
+    o = orc.Oracle(row, col, np.array([x, x], np.float32))
+    o.pass_init()
+    assert o.m == x
 
 
 # ------------------------------------------------------- arithmetic pins ----
