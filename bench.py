@@ -1,15 +1,21 @@
-"""Benchmark: plain Louvain on one MI355X, edges/s summed over passes.
+"""Benchmark: Louvain on one MI355X -- edges/s summed over passes on R-MAT
+scale 24, plus graphs/s of the 4,096-graph batch against one graph at a time.
 
 A step is one full Louvain run (all passes, modularity optimisation and
 aggregation) over a device-resident synthetic graph; the graph is restored
 from its resident copy between steps, outside the timed region.
 
-    python bench.py [--config c3|c1|c2|c4|c5] [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--config headline|c1|c2|c3|c4|c5] [--steps K] [--warmup W] [--gpus 1]
 
-Prints one JSON line.  value = sum over passes of CSR entries / run time.
+The default (headline) prints one JSON line for R-MAT-24 (value = sum over
+passes of CSR entries / run time) with a "batch" object for the 4,096-graph
+SBM batch (graphs/s batched and one at a time on the same GPU, every graph's
+result checked against the oracle's committed digests).
 roofline: the dominant kernel family's algorithmic bytes / its launch time
-(hipEvents on the engine's stream, over the timed steps), against 8 TB/s.
-cpu_baseline: the CPU oracle (single core C restatement) on a bounded sample.
+(hipEvents on the engine's stream), against 8 TB/s.
+cpu_baseline: the CPU oracle on all host cores and on one core, timed on pass
+1's initialisation and first sweep of the same R-MAT-24 graph and
+extrapolated to the whole run by entry visits.
 """
 import argparse
 import json
@@ -24,52 +30,34 @@ sys.path.insert(0, os.path.join(ROOT, "louvain-community-detection_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MIN_GAIN = 1e-6
-
-CONFIGS = {
-    # FEM-band analogue of bcsstk16
-    "c1": dict(workload="fem-band-4884", gen="fem", args=(814, 25, 9, 1)),
-    # co-purchase-like Chung-Lu over a planted partition
-    "c2": dict(workload="copurchase-chung-lu-262111", gen="chung_lu",
-               args=(262111, 9.4, 2.5, 420, 2000, 20.0, 2)),
-    # R-MAT scale 24, edge factor 16, w ~ U(0.5, 1.5)
-    "c3": dict(workload="rmat-24-ef16", gen="rmat", args=(24, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)),
-    # Poisson request stream of SBM and R-MAT graphs through the serving loop
-    "c5": dict(workload="serve-poisson-200rps", gen="serve", args=(400, 200.0, 5)),
-    # 4096 stochastic-block-model graphs in one segmented CSR
-    "c4": dict(workload="batch-4096-sbm", gen="batch", args=(4096, 100, 50000, 8, 64, 8.0, 32.0,
-                                                            10.0, 4)),
-}
-# the same recipe at a size the single-core oracle finishes in ~10-30 s
-CPU_SAMPLES = {
-    "c1": None,
-    "c2": None,
-    "c3": dict(workload="rmat-18-ef16 (same recipe, 1/64 of the vertices)", gen="rmat",
-               args=(18, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 3)),
-}
+METRIC = "Louvain edges/s summed over passes + graphs/s batched vs plain, 1 GPU"
+DTYPE = "f32 gains, exact i64 fixed-point sums"
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
-def batch_specs(G, vmin, vmax, bmin, bmax, dmin, dmax, p_ratio, seed):
-    """Per-graph SBM parameters: V log-uniform, blocks and mean degree uniform."""
-    rng = np.random.default_rng(seed)
-    n = np.exp(rng.uniform(np.log(vmin), np.log(vmax), G)).astype(np.int64)
-    blocks = rng.integers(bmin, bmax + 1, G)
-    deg = rng.uniform(dmin, dmax, G)
-    return [(int(n[i]), int(min(blocks[i], n[i])), float(deg[i]), p_ratio, seed * 100003 + i)
-            for i in range(G)]
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_batch(lv, args):
-    from concurrent.futures import ThreadPoolExecutor
-    specs = batch_specs(*args)
-    with ThreadPoolExecutor(max_workers=16) as ex:  # generators release the GIL
-        return list(ex.map(lambda sp: lv.gen_sbm(*sp), specs))
+def qbits(q):
+    return int(np.float32(q).view(np.uint32))
 
 
-def make_graph(lv, cfg):
-    if cfg["gen"] == "batch":
-        return make_batch(lv, cfg["args"])
-    return getattr(lv, "gen_" + cfg["gen"])(*cfg["args"])
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
 
 
 def measured_traffic(workload, fam, launches_per_run):
@@ -90,76 +78,67 @@ def measured_traffic(workload, fam, launches_per_run):
     return (d["fetch"] + d["write"]) / launches_per_run
 
 
-def cpu_baseline(lv, cfg_name, graph):
-    import oracle
-    samp = CPU_SAMPLES[cfg_name]
-    if samp is not None:
-        row, col, w = make_graph(lv, samp)
-        desc = samp["workload"]
-    else:
-        row, col, w = graph
-        desc = "full workload"
-    o = oracle.Oracle(row.astype(np.uint64), col, w)
+def cpu_sample(o, threads, ne, res):
+    """The oracle `o` (holding the workload's graph) on `threads` host threads:
+    pass 1's initialisation and first sweep on the full graph, extrapolated to
+    the whole run by entry visits (each pass reads its entries once at init
+    and once per sweep, for the moves and again for Q); aggregation time is
+    left out, which favours the CPU."""
+    o.set_threads(threads)
     t0 = time.perf_counter()
-    r = o.run(MIN_GAIN)
+    o.pass_init()
+    o.sweep()
     dt = time.perf_counter() - t0
-    return {"value": r.sum_edges / dt, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": f"{desc}: {row.size - 1} vertices, {col.size} entries, {r.passes} passes, "
-                      f"{r.total_iters} sweeps, {dt:.2f} s single-core C oracle"}
+    t_full = dt * res.sum_edge_visits / (2 * ne)
+    return dt, t_full, res.sum_edges / t_full
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-roofline", action="store_true",
-                    help="skip the extra profiled run (exactly warmup + steps runs, e.g. "
-                         "under an external profiler); roofline is then null")
-    ap.add_argument("--tune", action="store_true",
-                    help="run the autotuner on the resident graph before the timed runs")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU is design-only "
-                                   "(graphs shard by HBM footprint, no collectives)",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    from loader import load_package
-    lv = load_package()
-    if lv.device_count() < 1:
-        raise RuntimeError("no GPU visible")
-    cfg = CONFIGS[args.config]
-    if cfg["gen"] == "batch":
-        return bench_batch(lv, cfg, args)
-    if cfg["gen"] == "serve":
-        return bench_serve(lv, cfg, args)
-    t0 = time.perf_counter()
-    graph = make_graph(lv, cfg)
+def cpu_baseline_c3(graph, res, single):
+    import oracle
+    model, nproc, avail = cpu_info()
+    threads = max(1, min(16, avail))  # the host share of one GPU
     row, col, w = graph
-    gen_s = time.perf_counter() - t0
-    print(f"[bench] {cfg['workload']}: {row.size - 1} vertices, {col.size} entries "
-          f"(generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    ne = int(col.size)
+    o = oracle.Oracle(row.astype(np.uint64), col, w)
+    dt, t_full, value = cpu_sample(o, threads, ne, res)
+    log(f"cpu baseline: {threads} threads, sample {dt:.2f} s, extrapolated run {t_full:.1f} s")
+    out = {"value": value, "unit": "edges/s", "cores": threads, "kind": "port",
+           "sample": f"pass 1 init + first sweep of the full R-MAT-24 graph ({ne} entries) on "
+                     f"{threads} threads of the C oracle: {dt:.2f} s; whole run extrapolated "
+                     f"by entry visits ({res.sum_edge_visits} / {2 * ne}) = {t_full:.1f} s, "
+                     f"aggregation excluded",
+           "cpu_model": model, "nproc": nproc, "affinity_cpus": avail}
+    if single:
+        dt1, t1, v1 = cpu_sample(o, 1, ne, res)  # pass_init restarts from singletons
+        log(f"cpu baseline: 1 thread, sample {dt1:.2f} s, extrapolated run {t1:.1f} s")
+        out["single_core"] = {"value": v1, "unit": "edges/s", "cores": 1,
+                              "sample_s": dt1, "extrapolated_run_s": t1}
+    o.close()
+    return out
 
+
+def bench_plain(lv, W, cfg, args):
+    """One graph resident in HBM, K timed runs; returns the JSON line."""
+    t0 = time.perf_counter()
+    graph = W.make_graph(lv, cfg)
+    row, col, w = graph
+    log(f"{cfg['workload']}: {row.size - 1} vertices, {col.size} entries "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     g = lv.Graph(row, col, w)  # host checks, device buffers, PCIe upload
     create_s = time.perf_counter() - t0
     tuning = g.get_tuning()
     if args.tune:  # untimed: candidate launch shapes on this graph, fastest kept
         t0 = time.perf_counter()
-        tuning, tms = g.autotune(MIN_GAIN)
-        print(f"[bench] autotune: {tuning} ({tms:.2f} ms/run, search "
-              f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        tuning, tms = g.autotune(W.MIN_GAIN)
+        log(f"autotune: {tuning} ({tms:.2f} ms/run, search {time.perf_counter() - t0:.1f} s)")
     # warmup runs profile every kernel family to find the dominant one
     g.profile(True)
     fam, wst = None, None
     for i in range(args.warmup):
         g.reset()
-        r = g.run(MIN_GAIN)
-        print(f"[bench] warmup {i}: {r.alg_ms:.2f} ms Q={r.q_final:.6f}", file=sys.stderr,
-              flush=True)
+        r = g.run(W.MIN_GAIN)
+        log(f"warmup {i}: {r.alg_ms:.2f} ms Q={r.q_final:.6f}")
     if args.warmup:
         wst = g.profile_get()
         fam = max(wst, key=lambda k: wst[k]["ms"])
@@ -172,13 +151,13 @@ def main():
     for i in range(args.steps):
         g.reset()
         t = time.perf_counter()
-        res = g.run(MIN_GAIN)
+        res = g.run(W.MIN_GAIN)
         times.append(time.perf_counter() - t)
         edges.append(res.sum_edges)
         bytes_alg += res.bytes_alg
-        print(f"[bench] step {i}: {res.alg_ms:.2f} ms ({times[-1] * 1e3:.2f} ms wall) "
-              f"Q={res.q_final:.6f} passes={res.passes} sweeps={res.total_iters} "
-              f"communities={res.n_final}", file=sys.stderr, flush=True)
+        log(f"step {i}: {res.alg_ms:.2f} ms ({times[-1] * 1e3:.2f} ms wall) Q={res.q_final:.6f} "
+            f"passes={res.passes} sweeps={res.total_iters} communities={res.n_final}")
+    assign = g.assignment()
     total_s = sum(times)
     value = sum(edges) / total_s
     # one more run of the same workload with HIP events on the dominant family
@@ -190,10 +169,10 @@ def main():
         g.profile_select([fam] if fam else None)
         g.profile_reset()
         g.reset()
-        pres = g.run(MIN_GAIN)
+        pres = g.run(W.MIN_GAIN)
         stats = g.profile_get()
         g.profile(False)
-        print(f"[bench] profiled run: {pres.alg_ms:.2f} ms", file=sys.stderr, flush=True)
+        log(f"profiled run: {pres.alg_ms:.2f} ms")
         if fam is None:
             fam = max(stats, key=lambda k: stats[k]["ms"])
         st = stats[fam]
@@ -205,8 +184,9 @@ def main():
                     "traffic": measured_traffic(cfg["workload"], fam, st["launches"]),
                     "launches_per_run": int(st["launches"]),
                     "avg_launch_ms": avg_ms, "bytes_per_launch": avg_bytes}
+    g.close()
     line = {
-        "metric": "Louvain edges/s summed over passes (plain, 1 GPU)",
+        "metric": METRIC,
         "value": value,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -216,10 +196,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 gains, exact i64 fixed-point sums",
+        "dtype": DTYPE,
         "data": "synthetic (seeded generator)",
         "config": {"workload": cfg["workload"], "vertices": int(row.size - 1),
-                   "entries": int(col.size), "min_gain": MIN_GAIN,
+                   "entries": int(col.size), "min_gain": W.MIN_GAIN,
                    "modularity": float(res.q_final), "passes": int(res.passes),
                    "sweeps": int(res.total_iters), "communities": int(res.n_final),
                    "tuning": tuning, "autotuned": bool(args.tune)},
@@ -236,147 +216,131 @@ def main():
                         "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
                     for k, v in (wst if args.warmup else stats).items()},
     }
+    digest = os.path.join(GOLDEN, f"digests_{cfg['workload']}.json")
+    if os.path.exists(digest):  # the oracle's result for this exact input
+        with open(digest) as f:
+            d = json.load(f)
+        line["identical_to_oracle_digest"] = bool(
+            qbits(res.q_final) == d["q_bits"] and res.passes == d["passes"] and
+            res.total_iters == d["sweeps"] and res.n_final == d["n_final"] and
+            res.sum_edges == d["sum_edges"] and W.sha256_arrays(assign) == d["assign"])
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(lv, args.config, graph)
-    g.close()
-    print(json.dumps(line))
+        if cfg["gen"] == "rmat" and row.size - 1 >= (1 << 22):
+            line["cpu_baseline"] = cpu_baseline_c3(graph, res, not args.no_cpu_single)
+        else:  # small graphs: the oracle's whole run, one core
+            import oracle
+            o = oracle.Oracle(row.astype(np.uint64), col, w)
+            t = time.perf_counter()
+            r = o.run(W.MIN_GAIN)
+            dt = time.perf_counter() - t
+            model, nproc, avail = cpu_info()
+            line["cpu_baseline"] = {"value": r.sum_edges / dt, "unit": "edges/s", "cores": 1,
+                                    "kind": "port", "cpu_model": model, "nproc": nproc,
+                                    "sample": f"full workload, single-core C oracle, {dt:.2f} s"}
+    return line
 
 
-def bench_batch(lv, cfg, args):
+def bench_batch(lv, W, cfg, args):
     """graphs/s of the batched engine vs one-graph-at-a-time plain runs on the
-    same GPU, with identical per-graph results."""
+    same GPU; every graph's result against the oracle's digests."""
     t0 = time.perf_counter()
-    graphs = make_graph(lv, cfg)
+    graphs = W.make_graph(lv, cfg)
     nv = sum(int(r.size) - 1 for r, _, _ in graphs)
     ne = sum(int(c.size) for _, c, _ in graphs)
-    print(f"[bench] {cfg['workload']}: {len(graphs)} graphs, {nv} vertices, {ne} entries "
-          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    log(f"{cfg['workload']}: {len(graphs)} graphs, {nv} vertices, {ne} entries "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
     b = lv.Batch(graphs)
     tuning = b.get_tuning()
     if args.tune:
         t0 = time.perf_counter()
-        tuning, tms = b.autotune(MIN_GAIN)
-        print(f"[bench] autotune: {tuning} ({tms:.1f} ms/run, search "
-              f"{time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        tuning, tms = b.autotune(W.MIN_GAIN)
+        log(f"autotune: {tuning} ({tms:.1f} ms/run, search {time.perf_counter() - t0:.1f} s)")
     for i in range(args.warmup):
         b.reset()
-        per, tot = b.run(MIN_GAIN)
-        print(f"[bench] warmup {i}: {tot.alg_ms:.1f} ms", file=sys.stderr, flush=True)
+        per, tot = b.run(W.MIN_GAIN)
+        log(f"batch warmup {i}: {tot.alg_ms:.1f} ms")
     times = []
     for i in range(args.steps):
         b.reset()
         t = time.perf_counter()
-        per, tot = b.run(MIN_GAIN)
+        per, tot = b.run(W.MIN_GAIN)
         times.append(time.perf_counter() - t)
-        print(f"[bench] step {i}: {tot.alg_ms:.1f} ms ({times[-1] * 1e3:.1f} ms wall) "
-              f"passes={tot.passes} sweeps={tot.total_iters}", file=sys.stderr, flush=True)
+        log(f"batch step {i}: {tot.alg_ms:.1f} ms ({times[-1] * 1e3:.1f} ms wall) "
+            f"passes={tot.passes} sweeps={tot.total_iters}")
+    assigns = b.assignments()
     b.close()
-    batch_gps = len(graphs) / (sum(times) / len(times))
-    # plain path, one graph at a time, on a strided sample; same results
+    ms_batch = sum(times) / len(times) * 1e3
+    batch_gps = len(graphs) / (ms_batch * 1e-3)
+    # every graph against the oracle's committed result for it
+    with open(os.path.join(GOLDEN, f"digests_{cfg['workload']}.json")) as f:
+        d = json.load(f)
+
+    def matches(i, q, passes, sweeps, n_final, sum_edges, assign):
+        return (qbits(q), passes, sweeps, n_final, sum_edges,
+                W.sha256_arrays(np.asarray(assign, np.uint32))[:16]) == \
+            (d["q_bits"][i], d["passes"][i], d["sweeps"][i], d["n_final"][i],
+             d["sum_edges"][i], d["assign"][i])
+
+    batch_ok = sum(matches(i, per[i].q_final, per[i].passes, per[i].total_iters,
+                           per[i].n_final, per[i].sum_edges, assigns[i])
+                   for i in range(len(graphs)))
+    # plain path, one graph at a time, on a strided sample
     sample = list(range(0, len(graphs), max(1, len(graphs) // 256)))
     plain_ms = 0.0
-    same = True
+    plain_ok = 0
     for i in sample:
         r, c, w = graphs[i]
         with lv.Graph(r, c, w) as g:
-            res = g.run(MIN_GAIN)
+            res = g.run(W.MIN_GAIN)
+            a = g.assignment()
         plain_ms += res.alg_ms
-        same &= (np.float32(res.q_final).view(np.uint32) ==
-                 np.float32(per[i].q_final).view(np.uint32)) and res.passes == per[i].passes
+        plain_ok += matches(i, res.q_final, res.passes, res.total_iters, res.n_final,
+                            res.sum_edges, a)
     plain_gps = len(sample) / (plain_ms * 1e-3)
-    print(f"[bench] plain: {len(sample)} graphs in {plain_ms:.1f} ms -> {plain_gps:.1f} graphs/s; "
-          f"identical results: {same}", file=sys.stderr, flush=True)
-    line = {
-        "metric": "Louvain graphs/s, batched vs plain (1 GPU)",
-        "value": batch_gps,
-        "unit": "graphs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sum(times) / len(times) * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 gains, exact i64 fixed-point sums",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": cfg["workload"], "graphs": len(graphs), "vertices": nv,
-                   "entries": ne, "min_gain": MIN_GAIN, "tuning": tuning,
-                   "autotuned": bool(args.tune)},
-        "plain": {"graphs_per_s": plain_gps, "sample_graphs": len(sample),
-                  "identical_results": bool(same)},
-        "batched_over_plain": batch_gps / plain_gps,
-        "edges_per_s": sum(p.sum_edges for p in per) / (sum(times) / len(times)),
-    }
+    log(f"batch {batch_gps:.0f} graphs/s, plain {plain_gps:.1f} graphs/s ({len(sample)} graphs); "
+        f"oracle digests matched: batch {batch_ok}/{len(graphs)}, plain {plain_ok}/{len(sample)}")
+    out = {"workload": cfg["workload"], "graphs": len(graphs), "vertices": nv, "entries": ne,
+           "graphs_per_s": batch_gps, "ms_per_batch": ms_batch, "steps": args.steps,
+           "plain_graphs_per_s": plain_gps, "plain_sample_graphs": len(sample),
+           "batched_over_plain": batch_gps / plain_gps,
+           "edges_per_s": sum(p.sum_edges for p in per) / (ms_batch * 1e-3),
+           "identical_to_oracle_digests": bool(batch_ok == len(graphs) and
+                                               plain_ok == len(sample)),
+           "graphs_checked": len(graphs) + len(sample),
+           "tuning": tuning, "autotuned": bool(args.tune)}
     if not args.no_cpu:
         import oracle
         cs = sample[:8]
         t = time.perf_counter()
         for i in cs:
             r, c, w = graphs[i]
-            oracle.Oracle(r.astype(np.uint64), c, w).run(MIN_GAIN)
+            oracle.Oracle(r.astype(np.uint64), c, w).run(W.MIN_GAIN)
         dt = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": len(cs) / dt, "unit": "graphs/s", "cores": 1,
-                                "kind": "port",
-                                "sample": f"{len(cs)} graphs of the batch, single-core C oracle, "
-                                          f"{dt:.2f} s"}
-    print(json.dumps(line))
+        out["cpu_baseline"] = {"value": len(cs) / dt, "unit": "graphs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(cs)} graphs of the batch, single-core C oracle, "
+                                         f"{dt:.2f} s"}
+    return out
 
 
-def sym_weights(row, col, lo, hi, seed):
-    """w ~ U(lo, hi) per undirected pair, equal on both directions."""
-    n = row.size - 1
-    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(row.astype(np.int64)))
-    a = np.minimum(src, col.astype(np.uint64))
-    b = np.maximum(src, col.astype(np.uint64))
-    x = (a * np.uint64(0x9E3779B97F4A7C15) + b * np.uint64(0xBF58476D1CE4E5B9) +
-         np.uint64(seed)) & np.uint64(0xFFFFFFFFFFFFFFFF)
-    x ^= x >> np.uint64(31)
-    x = (x * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
-    x ^= x >> np.uint64(29)
-    u = (x >> np.uint64(11)).astype(np.float64) / float(1 << 53)
-    return (np.float32(lo) + np.float32(hi - lo) * u.astype(np.float32)).astype(np.float32)
-
-
-def serve_pool(lv, seed):
-    """SBM graphs with 1k-200k vertices and R-MAT scale 14-21, w ~ U(0.5, 1.5)."""
-    from concurrent.futures import ThreadPoolExecutor
-    rng = np.random.default_rng(seed)
-    sbm_specs = []
-    for i in range(12):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
-        sbm_specs.append((n, int(rng.integers(8, 65)), float(rng.uniform(8, 32)), 10.0,
-                          seed * 1000 + i))
-    rmat_specs = [(sc, 16, 0.57, 0.19, 0.19, 0.5, 1.5, seed * 1000 + 100 + sc)
-                  for sc in range(14, 22)]
-    with ThreadPoolExecutor(max_workers=8) as ex:
-        sbm = list(ex.map(lambda sp: lv.gen_sbm(*sp), sbm_specs))
-        rmat = list(ex.map(lambda sp: lv.gen_rmat(*sp), rmat_specs))
-    sbm = [(r, c, sym_weights(r, c, 0.5, 1.5, 7 + i)) for i, (r, c, _) in enumerate(sbm)]
-    return sbm, rmat
-
-
-def bench_serve(lv, cfg, args):
+def bench_serve(lv, W, cfg, args):
     """Poisson request stream through the serving loop: p50/p99 latency and
-    graphs/s, with every result checked against a solo run."""
+    graphs/s, every result checked against the oracle's digests."""
     nreq, rate, seed = cfg["args"]
     t0 = time.perf_counter()
-    sbm, rmat = serve_pool(lv, seed)
+    sbm, rmat = W.serve_pool(lv, seed)
     pool = sbm + rmat
-    print(f"[bench] pool: {len(sbm)} SBM + {len(rmat)} R-MAT graphs "
-          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    solo = []
-    for r, c, w in pool:
+    log(f"pool: {len(sbm)} SBM + {len(rmat)} R-MAT graphs "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    with open(os.path.join(GOLDEN, "digests_serve-pool.json")) as f:
+        d = json.load(f)["graphs"]
+    solo_ms = []
+    for r, c, w in pool:  # one at a time, for the comparison rate
         with lv.Graph(r, c, w) as g:
-            res = g.run(MIN_GAIN)
-        solo.append((np.float32(res.q_final).view(np.uint32), res.passes, res.n_final,
-                     res.alg_ms))
-    rng = np.random.default_rng(seed + 1)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    kinds = rng.integers(0, 2, nreq)
-    picks = [int(rng.integers(0, len(sbm))) if k == 0 else len(sbm) + int(rng.integers(0, len(rmat)))
-             for k in kinds]
+            solo_ms.append(g.run(W.MIN_GAIN).alg_ms)
+    arrivals, picks = W.serve_stream(nreq, rate, seed, len(sbm), len(rmat))
     results = []
-    with lv.Server(slots=8, hbm_budget=200 << 30, min_gain=MIN_GAIN) as s:
+    with lv.Server(slots=8, hbm_budget=200 << 30, min_gain=W.MIN_GAIN) as s:
         start = time.perf_counter()
         for rid in range(nreq):
             delay = arrivals[rid] - (time.perf_counter() - start)
@@ -389,33 +353,71 @@ def bench_serve(lv, cfg, args):
     assert len(results) == nreq
     lat = np.array([r.t_end_ms - r.t_submit_ms for r in results])
     span_s = (max(r.t_end_ms for r in results) - min(r.t_submit_ms for r in results)) / 1e3
-    same = all(r.status == 0 and np.float32(r.q_final).view(np.uint32) == solo[picks[r.id]][0] and
-               r.passes == solo[picks[r.id]][1] for r in results)
-    solo_gps = nreq / (sum(solo[p][3] for p in picks) * 1e-3)
-    print(f"[bench] {nreq} requests over {span_s:.2f} s; p50 {np.percentile(lat, 50):.1f} ms "
-          f"p99 {np.percentile(lat, 99):.1f} ms; identical to solo runs: {same}",
-          file=sys.stderr, flush=True)
-    line = {
+    ok = sum(r.status == 0 and
+             (qbits(r.q_final), r.passes, r.total_iters, r.n_final, r.sum_edges) ==
+             (d[picks[r.id]]["q_bits"], d[picks[r.id]]["passes"], d[picks[r.id]]["sweeps"],
+              d[picks[r.id]]["n_final"], d[picks[r.id]]["sum_edges"]) for r in results)
+    solo_gps = nreq / (sum(solo_ms[p] for p in picks) * 1e-3)
+    log(f"{nreq} requests over {span_s:.2f} s; p50 {np.percentile(lat, 50):.1f} ms "
+        f"p99 {np.percentile(lat, 99):.1f} ms; oracle digests matched {ok}/{nreq}")
+    return {
         "metric": "Louvain served graphs/s (Poisson stream, 1 GPU)",
-        "value": nreq / span_s,
-        "unit": "graphs/s",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
-        "ms_per_step": span_s * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 gains, exact i64 fixed-point sums",
-        "data": "synthetic (seeded generator)",
+        "value": nreq / span_s, "unit": "graphs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": span_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DTYPE, "data": "synthetic (seeded generator)",
         "config": {"workload": cfg["workload"], "requests": nreq, "rate_rps": rate,
                    "pool": "12 SBM (1k-200k vertices) + R-MAT scale 14-21, w ~ U(0.5,1.5)",
                    "slots": 8, "hbm_budget_gb": 200},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "mean": float(lat.mean())},
         "one_at_a_time_graphs_per_s": solo_gps,
-        "identical_to_solo": bool(same),
+        "identical_to_oracle_digests": bool(ok == nreq),
     }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="headline",
+                    choices=["headline", "c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-cpu-single", action="store_true",
+                    help="skip the single-core CPU row of the R-MAT-24 baseline")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="headline without the 4,096-graph batch object")
+    ap.add_argument("--no-roofline", action="store_true",
+                    help="skip the extra profiled run (exactly warmup + steps runs, e.g. "
+                         "under an external profiler); roofline is then null")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on the resident graph before the timed runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU is design-only "
+                                   "(graphs shard by HBM footprint, no collectives)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from loader import load_package
+    lv = load_package()
+    from louvain_amd import workloads as W
+    if lv.device_count() < 1:
+        raise RuntimeError("no GPU visible")
+    if args.config == "c4":
+        out = bench_batch(lv, W, W.CONFIGS["c4"], args)
+        line = {"metric": METRIC, "value": out["graphs_per_s"], "unit": "graphs/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_batch"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": DTYPE, "data": "synthetic (seeded generator)",
+                "config": {"workload": out["workload"]}, "batch": out}
+    elif args.config == "c5":
+        line = bench_serve(lv, W, W.CONFIGS["c5"], args)
+    else:
+        name = "c3" if args.config == "headline" else args.config
+        line = bench_plain(lv, W, W.CONFIGS[name], args)
+        if args.config == "headline" and not args.no_batch:
+            line["batch"] = bench_batch(lv, W, W.CONFIGS["c4"], args)
     print(json.dumps(line))
 
 

@@ -59,6 +59,7 @@ def lib():
     L.or_create.restype = ctypes.c_void_p
     L.or_destroy.argtypes = [ctypes.c_void_p]
     L.or_destroy.restype = None
+    L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.or_pass_init.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
     L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     L.or_modularity.argtypes = [ctypes.c_void_p]
@@ -196,6 +197,11 @@ class Oracle:
     @property
     def m(self):
         return np.float32(self.L.or_m(self.h))
+
+    def set_threads(self, nthreads):
+        """Host threads for pass init, bucket steps and modularity (results do
+        not depend on the count)."""
+        self._chk(self.L.or_set_threads(self.h, int(nthreads)))
 
     def pass_init(self):
         q = ctypes.c_float(0)

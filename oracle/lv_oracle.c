@@ -18,6 +18,7 @@
 #include <ctype.h>
 #include <errno.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -30,6 +31,14 @@ static const uint32_t k_cuts[8] = {0, 4, 8, 16, 32, 84, 319, 0x7fffffffu};
 static const uint32_t k_lanes[7] = {4, 8, 16, 32, 32, 128, 128};
 
 #define OR_FIXED_LIMIT ((uint64_t)1 << 61)
+
+/* per-vertex community accumulators of one decider (a generation stamp
+ * marks which entries belong to the vertex being decided) */
+typedef struct {
+    int64_t *acc;
+    uint32_t *stamp;
+    uint32_t gen;
+} or_scratch;
 
 struct or_state {
     /* current graph */
@@ -59,6 +68,10 @@ struct or_state {
     uint32_t stamp_gen;
     /* modularity before the first and after the last sweep of the last pass */
     float q_before, q_after;
+    /* host threads of pass init, bucket steps and modularity (1: the plain
+     * sequential loops); per-thread accumulator scratch beyond the first */
+    int nthreads;
+    or_scratch *tscr;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -390,8 +403,12 @@ static int set_graph(or_state *st, uint32_t n, uint64_t *row, uint32_t *col, flo
     st->acc = (int64_t *)calloc(nn, sizeof(int64_t));
     st->stamp = (uint32_t *)calloc(nn, sizeof(uint32_t));
     st->stamp_gen = 0;
+    if (!st->tscr) {
+        st->tscr = (or_scratch *)calloc(1, sizeof(or_scratch));
+        st->nthreads = 1;
+    }
     if (!st->k || !st->K || !st->comm || !st->comm_sz || !st->new_comm || !st->A || !st->acc ||
-        !st->stamp)
+        !st->stamp || !st->tscr)
         return OR_ERR_NOMEM;
     return OR_OK;
 }
@@ -471,8 +488,21 @@ or_state *or_create(uint32_t n, const uint64_t *row, const uint32_t *col, const 
     return st;
 }
 
+/* thread scratch beyond the state's own (entry 0) */
+static void free_threads(or_state *st) {
+    if (st->tscr)
+        for (int t = 1; t < st->nthreads; t++) {
+            free(st->tscr[t].acc);
+            free(st->tscr[t].stamp);
+        }
+    free(st->tscr);
+    st->tscr = NULL;
+    st->nthreads = 1;
+}
+
 void or_destroy(or_state *st) {
     if (!st) return;
+    free_threads(st);
     free(st->row);
     free(st->col);
     free(st->w);
@@ -489,6 +519,59 @@ void or_destroy(or_state *st) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Host threads (CPU baseline only): vertex ranges are handed out in chunks  */
+/* from a shared counter; every per-vertex result depends only on the step   */
+/* snapshot and every total is an exact integer sum, so the results do not   */
+/* depend on the number of threads or the order of the chunks.               */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*or_range_fn)(void *ctx, int t, uint32_t a, uint32_t b);
+
+typedef struct {
+    or_range_fn fn;
+    void *ctx;
+    uint32_t n, chunk;
+    uint32_t next; /* shared */
+} or_pool;
+
+typedef struct {
+    or_pool *pool;
+    int t;
+} or_worker;
+
+static void *or_work(void *arg) {
+    or_worker *wk = (or_worker *)arg;
+    or_pool *p = wk->pool;
+    for (;;) {
+        uint32_t a = __atomic_fetch_add(&p->next, p->chunk, __ATOMIC_RELAXED);
+        if (a >= p->n) break;
+        uint32_t b = p->n - a < p->chunk ? p->n : a + p->chunk;
+        p->fn(p->ctx, wk->t, a, b);
+    }
+    return NULL;
+}
+
+#define OR_MAX_THREADS 256
+
+static void parallel_for(int T, uint32_t n, or_range_fn fn, void *ctx) {
+    if (T <= 1 || n < 8192) {
+        fn(ctx, 0, 0, n);
+        return;
+    }
+    or_pool pool = {fn, ctx, n, 1024, 0};
+    pthread_t th[OR_MAX_THREADS];
+    or_worker wk[OR_MAX_THREADS];
+    int started = 1;
+    for (int t = 0; t < T; t++) wk[t] = (or_worker){&pool, t};
+    for (int t = 1; t < T; t++) {
+        if (pthread_create(&th[t], NULL, or_work, &wk[t]) != 0) break;
+        started++;
+    }
+    or_work(&wk[0]);
+    for (int t = 1; t < started; t++) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Modularity (modularity.cu:400-436)                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -496,16 +579,37 @@ void or_destroy(or_state *st) {
  * s_v: pinned order.  S_in: exact sum of the FP32 s_v (R3).  sum a_c^2: FP32
  * squares (square<float>, modularity.cu:20-25) summed exactly at scale 2^2S
  * and rounded once (R4); the reference's thrust order is unpinned. */
-float or_modularity(or_state *st) {
+typedef struct {
+    const or_state *st;
+    int64_t sin[OR_MAX_THREADS];
+    i128 sq[OR_MAX_THREADS];
+} or_qsum;
+
+static void q_range(void *ctx, int t, uint32_t a, uint32_t b) {
+    or_qsum *q = (or_qsum *)ctx;
+    const or_state *st = q->st;
     int64_t sin_fixed = 0;
-    for (uint32_t v = 0; v < st->n; v++) {
+    i128 sq = 0;
+    for (uint32_t v = a; v < b; v++) {
         float s = r1_sum_masked(st, v);
         sin_fixed += (int64_t)(s * st->scale);
+        float av = fix2f(st, st->A[v]);
+        sq += f32_to_fixed128(av * av, 2 * st->S);
     }
+    q->sin[t] += sin_fixed;
+    q->sq[t] += sq;
+}
+
+float or_modularity(or_state *st) {
+    or_qsum q;
+    memset(&q, 0, sizeof(q));
+    q.st = st;
+    parallel_for(st->nthreads, st->n, q_range, &q);
+    int64_t sin_fixed = 0;
     i128 sq = 0;
-    for (uint32_t c = 0; c < st->n; c++) {
-        float a = fix2f(st, st->A[c]);
-        sq += f32_to_fixed128(a * a, 2 * st->S);
+    for (int t = 0; t < OR_MAX_THREADS; t++) {
+        sin_fixed += q.sin[t];
+        sq += q.sq[t];
     }
     float S_in = fix2f(st, sin_fixed);
     /* sq * 2^-2S, correctly rounded once */
@@ -518,15 +622,32 @@ float or_modularity(or_state *st) {
 /* Pass init (modularity.cu:455-463)                                         */
 /* ------------------------------------------------------------------------ */
 
-int or_pass_init(or_state *st, float *q0) {
+typedef struct {
+    or_state *st;
+    int64_t tot[OR_MAX_THREADS];
+} or_ksum;
+
+static void k_range(void *ctx, int t, uint32_t a, uint32_t b) {
+    or_ksum *q = (or_ksum *)ctx;
+    or_state *st = q->st;
     int64_t tot = 0;
-    for (uint32_t v = 0; v < st->n; v++) {
-        uint64_t b = st->row[v];
-        float k = or_r1_sum(st->w + b, (uint32_t)(st->row[v + 1] - b));
+    for (uint32_t v = a; v < b; v++) {
+        uint64_t r = st->row[v];
+        float k = or_r1_sum(st->w + r, (uint32_t)(st->row[v + 1] - r));
         st->k[v] = k;
         st->K[v] = (int64_t)(k * st->scale);
         tot += st->K[v];
     }
+    q->tot[t] += tot;
+}
+
+int or_pass_init(or_state *st, float *q0) {
+    or_ksum ks;
+    memset(&ks, 0, sizeof(ks));
+    ks.st = st;
+    parallel_for(st->nthreads, st->n, k_range, &ks);
+    int64_t tot = 0;
+    for (int t = 0; t < OR_MAX_THREADS; t++) tot += ks.tot[t];
     /* all_w = reduce(neigh_w)/2 (modularity.cu:457-458): exact sum, one
      * rounding, exact halving */
     st->m = fix2f(st, tot) / 2.0f;
@@ -569,7 +690,7 @@ static int better(float ga, uint32_t ca, float gb, uint32_t cb) {
 }
 
 /* Decide one vertex: returns its new community. */
-static uint32_t decide(or_state *st, uint32_t v, int bucket) {
+static uint32_t decide(const or_state *st, or_scratch *sc, uint32_t v, int bucket) {
     const uint32_t T = k_lanes[bucket - 1];
     const uint64_t base = st->row[v];
     const uint32_t d = (uint32_t)(st->row[v + 1] - base);
@@ -584,12 +705,14 @@ static uint32_t decide(or_state *st, uint32_t v, int bucket) {
         lane_c[l] = 0xffffffffu;
     }
     /* fresh accumulator generation for this vertex */
-    st->stamp_gen++;
-    if (st->stamp_gen == 0) {
-        memset(st->stamp, 0, (size_t)st->n * sizeof(uint32_t));
-        st->stamp_gen = 1;
+    sc->gen++;
+    if (sc->gen == 0) {
+        memset(sc->stamp, 0, (size_t)st->n * sizeof(uint32_t));
+        sc->gen = 1;
     }
-    const uint32_t gen = st->stamp_gen;
+    const uint32_t gen = sc->gen;
+    int64_t *acc = sc->acc;
+    uint32_t *stamp = sc->stamp;
     int64_t cew = 0;
     const uint32_t rounds = (d + T - 1) / T;
     /* buckets 1-5: lock-step rounds, the gain of an entry in round r sees the
@@ -602,11 +725,11 @@ static uint32_t decide(or_state *st, uint32_t v, int bucket) {
             uint32_t e = st->col[base + j];
             uint32_t ce = st->comm[e];
             int64_t wf = (int64_t)(st->w[base + j] * st->scale);
-            if (st->stamp[ce] != gen) {
-                st->stamp[ce] = gen;
-                st->acc[ce] = 0;
+            if (stamp[ce] != gen) {
+                stamp[ce] = gen;
+                acc[ce] = 0;
             }
-            st->acc[ce] += wf;
+            acc[ce] += wf;
             if (ce == cv && e != v) cew += wf; /* :143-145 */
         }
         if (!prefix) continue;
@@ -614,7 +737,7 @@ static uint32_t decide(or_state *st, uint32_t v, int bucket) {
             uint32_t e = st->col[base + j];
             uint32_t ce = st->comm[e];
             if (!is_lower_idx(st, cv, ce)) continue;
-            float g = gain_of(fix2f(st, st->acc[ce]), m, kv, acv, fix2f(st, st->A[ce]));
+            float g = gain_of(fix2f(st, acc[ce]), m, kv, acv, fix2f(st, st->A[ce]));
             uint32_t l = j % T;
             if (g > lane_g[l]) { /* strict: lane keeps its first max (:344) */
                 lane_g[l] = g;
@@ -627,7 +750,7 @@ static uint32_t decide(or_state *st, uint32_t v, int bucket) {
             uint32_t e = st->col[base + j];
             uint32_t ce = st->comm[e];
             if (!is_lower_idx(st, cv, ce)) continue;
-            float g = gain_of(fix2f(st, st->acc[ce]), m, kv, acv, fix2f(st, st->A[ce]));
+            float g = gain_of(fix2f(st, acc[ce]), m, kv, acv, fix2f(st, st->A[ce]));
             uint32_t l = j % T;
             if (g > lane_g[l]) {
                 lane_g[l] = g;
@@ -678,31 +801,83 @@ static uint32_t decide(or_state *st, uint32_t v, int bucket) {
     return cv;
 }
 
-int or_step(or_state *st, int bucket, uint64_t *moved) {
-    if (!st->pass_ready) return OR_ERR_STATE;
-    if (bucket < 1 || bucket > 7) return OR_ERR_ARG;
-    uint64_t mv = 0;
-    /* decide every vertex of the step from the snapshot ... */
-    for (uint32_t v = 0; v < st->n; v++) {
+typedef struct {
+    or_state *st;
+    int bucket;
+    uint64_t moved[OR_MAX_THREADS];
+} or_stepctx;
+
+static void decide_range(void *ctx, int t, uint32_t a, uint32_t b) {
+    or_stepctx *c = (or_stepctx *)ctx;
+    or_state *st = c->st;
+    or_scratch *sc = &st->tscr[t];  /* entry 0 views the state's own scratch */
+    for (uint32_t v = a; v < b; v++) {
         uint32_t d = (uint32_t)(st->row[v + 1] - st->row[v]);
-        if (or_bucket_of(d) != bucket) continue;
-        st->new_comm[v] = decide(st, v, bucket);
+        if (or_bucket_of(d) != c->bucket) continue;
+        st->new_comm[v] = decide(st, sc, v, c->bucket);
     }
-    /* ... then apply them together (assignCommunity :382-391) and update the
-     * community weights; exact incremental update == recompute (:393-398) */
-    for (uint32_t v = 0; v < st->n; v++) {
+}
+
+/* assignCommunity (:382-391) and the community weights; the exact
+ * incremental update equals the recompute of :393-398 */
+static void apply_range(void *ctx, int t, uint32_t a, uint32_t b) {
+    or_stepctx *c = (or_stepctx *)ctx;
+    or_state *st = c->st;
+    uint64_t mv = 0;
+    for (uint32_t v = a; v < b; v++) {
         uint32_t d = (uint32_t)(st->row[v + 1] - st->row[v]);
-        if (or_bucket_of(d) != bucket) continue;
+        if (or_bucket_of(d) != c->bucket) continue;
         uint32_t o = st->comm[v], nc = st->new_comm[v];
         if (o == nc) continue;
-        st->comm_sz[o]--;
-        st->comm_sz[nc]++;
-        st->A[o] -= st->K[v];
-        st->A[nc] += st->K[v];
+        __atomic_fetch_sub(&st->comm_sz[o], 1u, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&st->comm_sz[nc], 1u, __ATOMIC_RELAXED);
+        __atomic_fetch_sub(&st->A[o], st->K[v], __ATOMIC_RELAXED);
+        __atomic_fetch_add(&st->A[nc], st->K[v], __ATOMIC_RELAXED);
         st->comm[v] = nc;
         mv++;
     }
+    c->moved[t] += mv;
+}
+
+int or_step(or_state *st, int bucket, uint64_t *moved) {
+    if (!st->pass_ready) return OR_ERR_STATE;
+    if (bucket < 1 || bucket > 7) return OR_ERR_ARG;
+    or_stepctx c;
+    memset(&c, 0, sizeof(c));
+    c.st = st;
+    c.bucket = bucket;
+    /* decide every vertex of the step from the snapshot ... */
+    st->tscr[0] = (or_scratch){st->acc, st->stamp, st->stamp_gen};
+    parallel_for(st->nthreads, st->n, decide_range, &c);
+    st->stamp_gen = st->tscr[0].gen;
+    /* ... then apply them together */
+    parallel_for(st->nthreads, st->n, apply_range, &c);
+    uint64_t mv = 0;
+    for (int t = 0; t < OR_MAX_THREADS; t++) mv += c.moved[t];
     if (moved) *moved = mv;
+    return OR_OK;
+}
+
+int or_set_threads(or_state *st, int nthreads) {
+    if (nthreads < 1 || nthreads > OR_MAX_THREADS) return OR_ERR_ARG;
+    const size_t nn = st->n0 ? st->n0 : 1;
+    or_scratch *ts = (or_scratch *)calloc((size_t)nthreads, sizeof(or_scratch));
+    if (!ts) return OR_ERR_NOMEM;
+    for (int t = 1; t < nthreads; t++) {
+        ts[t].acc = (int64_t *)calloc(nn, sizeof(int64_t));
+        ts[t].stamp = (uint32_t *)calloc(nn, sizeof(uint32_t));
+        if (!ts[t].acc || !ts[t].stamp) {
+            for (int u = 1; u <= t; u++) {
+                free(ts[u].acc);
+                free(ts[u].stamp);
+            }
+            free(ts);
+            return OR_ERR_NOMEM;
+        }
+    }
+    free_threads(st);
+    st->tscr = ts;
+    st->nthreads = nthreads;
     return OR_OK;
 }
 

@@ -72,6 +72,12 @@ or_state *or_create(uint32_t n, const uint64_t *row, const uint32_t *col,
                     const float *w, int *err);
 void or_destroy(or_state *st);
 
+/* Host threads for pass init, bucket steps and modularity (default 1).  The
+ * results are the same for every thread count (decisions read only the step
+ * snapshot, every total is an exact integer sum); used to time the CPU
+ * baseline on all cores. */
+int or_set_threads(or_state *st, int nthreads);
+
 int or_pass_init(or_state *st, float *q0);
 int or_step(or_state *st, int bucket, uint64_t *moved);
 float or_modularity(or_state *st);

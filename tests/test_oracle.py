@@ -302,3 +302,20 @@ def test_contraction_invariants(orc, lv):
         assert r.size - 1 == ids.size
         assert abs(float(o.pass_init()) - orc.q64(row.astype(np.uint64), col, w,
                                                   o.init_comm())) < 1e-6
+
+
+def test_threads_do_not_change_results(lv, orc):
+    """The oracle on several host threads (the all-core CPU baseline) gives
+    the single-thread result: decisions read only the step snapshot and every
+    total is an exact integer sum."""
+    for r, c, w in [lv.gen_rmat(15, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 21),
+                    lv.gen_sbm(40000, 30, 12.0, 10.0, 22)]:
+        out = []
+        for threads in (1, 4):
+            o = orc.Oracle(r.astype(np.uint64), c, w)
+            o.set_threads(threads)
+            res = o.run(1e-6)
+            out.append((F32(res.q_final).view(np.uint32), res.passes, res.total_iters,
+                         res.n_final, res.sum_edges, o.init_comm().tobytes()))
+        assert out[0] == out[1]
+
