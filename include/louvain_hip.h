@@ -178,6 +178,24 @@ int lv_aggregation(lv_graph *g);
  * fails its first sweep. */
 int lv_run(lv_graph *g, float min_gain, lv_result *res);
 
+/* The whole run (lv_graph_create + lv_run + lv_get_assignment) with no device
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture of the caller's own
+ * work, tight serving loops, harnesses that forbid hipMalloc).  Every device
+ * buffer is carved from a fixed 128 MiB array that the library holds on each
+ * GPU it runs on until the process ends; graphs whose buffers do not fit it
+ * (about 600k entries and 64k vertices, depending on shape) return
+ * LV_ERR_LIMIT.  The graph is the host CSR of lv_graph_create (checked the
+ * same way); assignment receives n original-vertex -> community ids; res the
+ * figures of lv_run (alg_ms from the host clock).  Runs on `stream` (a
+ * hipStream_t; NULL: the null stream), which it synchronises before it
+ * returns.  Calls on one GPU share the array and run one at a time (they are
+ * serialised inside the library); the array keeps the last call's contents,
+ * and each call clears what it needs empty. */
+int lv_run_noalloc(int device, uint32_t n, const uint32_t *row, const uint32_t *col,
+                   const float *w, float min_gain, void *stream, uint32_t *assignment,
+                   lv_result *res);
+
 /* ---- results and state --------------------------------------------------- */
 /* original vertex -> vertex of the current graph (length n0) */
 int lv_get_assignment(lv_graph *g, uint32_t *assignment);

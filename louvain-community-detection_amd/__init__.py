@@ -19,7 +19,7 @@ CLI_PATH = os.path.join(HERE, "louvain")
 
 LV_OK = 0
 STATUS = {0: "ok", 1: "io", 2: "format", 3: "range", 4: "arg", 5: "nomem", 6: "state",
-          7: "hip", 8: "no_device", 9: "overflow"}
+          7: "hip", 8: "no_device", 9: "overflow", 10: "limit"}
 
 KERNEL_FAMILIES = ["classify", "r1", "sumsq", "move_small", "move_wave", "move_block",
                    "move_global", "apply", "aggregate", "relabel"]
@@ -83,7 +83,7 @@ EXPORTS = [
     "lv_graph_create", "lv_graph_destroy", "lv_graph_reset", "lv_graph_set_locality",
     "lv_tuning_default", "lv_graph_set_tuning", "lv_graph_get_tuning", "lv_autotune",
     "lv_graph_set_stream", "lv_modularity_optimization", "lv_pass_stats", "lv_aggregation",
-    "lv_run", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
+    "lv_run", "lv_run_noalloc", "lv_get_assignment", "lv_graph_size", "lv_get_csr", "lv_get_state",
     "lv_get_scale", "lv_pass_init", "lv_step", "lv_sweep", "lv_modularity",
     "lv_profile_enable", "lv_profile_select", "lv_profile_get", "lv_profile_reset", "lv_gen_fem",
     "lv_gen_chung_lu", "lv_gen_rmat", "lv_gen_sbm", "lv_batch_create", "lv_batch_run",
@@ -132,6 +132,8 @@ def lib():
                                 c.POINTER(c.c_float)]
     L.lv_aggregation.argtypes = [c.c_void_p]
     L.lv_run.argtypes = [c.c_void_p, c.c_float, c.POINTER(Result)]
+    L.lv_run_noalloc.argtypes = [c.c_int, c.c_uint32, _u32p, _u32p, _f32p, c.c_float,
+                                 c.c_void_p, _u32p, c.POINTER(Result)]
     L.lv_get_assignment.argtypes = [c.c_void_p, _u32p]
     L.lv_graph_size.argtypes = [c.c_void_p, c.POINTER(c.c_uint32), c.POINTER(c.c_uint64),
                                 c.POINTER(c.c_uint32)]
@@ -238,6 +240,22 @@ def gen_sbm(n, blocks, mean_deg, p_ratio=10.0, seed=4):
     h = HostCSR()
     _chk(lib().lv_gen_sbm(n, blocks, mean_deg, p_ratio, seed, ctypes.byref(h)), "lv_gen_sbm")
     return _take_csr(h)
+
+
+def run_noalloc(row, col, w, min_gain=1e-6, device=0, stream=None):
+    """lv_run_noalloc: one whole run with every device buffer carved from the
+    library's fixed scratch array.  Returns (Result, assignment)."""
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    n = row.size - 1
+    a = np.zeros(max(n, 1), np.uint32)
+    r = Result()
+    _chk(lib().lv_run_noalloc(device, n, row, col if col.size else np.zeros(1, np.uint32),
+                              w if w.size else np.zeros(1, np.float32), ctypes.c_float(min_gain),
+                              ctypes.c_void_p(stream or None), a, ctypes.byref(r)),
+         "lv_run_noalloc")
+    return r, a[:n]
 
 
 class _Tunable:

@@ -1726,6 +1726,7 @@ const char *lv_status_string(int st) {
         case LV_ERR_HIP: return "HIP runtime error";
         case LV_ERR_NO_DEVICE: return "no GPU device";
         case LV_ERR_OVERFLOW: return "internal capacity exceeded";
+        case LV_ERR_LIMIT: return "graph larger than the fixed scratch array allows";
         default: return "unknown status";
     }
 }
