@@ -131,7 +131,8 @@ int lv_graph_set_locality(lv_graph *g, int mode);
  * serves a row and how work is cut.  No counterpart in the reference. */
 typedef struct lv_tuning {
     uint32_t heavy_min_degree;      /* rows longer than this take the split heavy path
-                                       (319..4096; shorter bucket-7 rows: one workgroup) */
+                                       (319..8192, default 4096; shorter bucket-7 rows: one
+                                       workgroup with an LDS table) */
     uint32_t bucket_entries;        /* target entries per LDS bucket of a heavy row
                                        (power of two, 64..1024) */
     int32_t locality;               /* -1 automatic, 0 never, 1 every pass */

@@ -19,12 +19,13 @@ constexpr uint32_t kLanes[7] = {4u, 8u, 16u, 32u, 32u, 128u, 128u};
 // Kernel classes of the move step (performance choice, not semantic).
 //   0: deg 0 (never moves)      1-4: buckets 1-4, sub-wave groups of 4/8/16/32
 //   5: bucket 5, 32-lane groups 6: bucket 6, one wave64 per vertex
-//   7-10: bucket 7 with deg <= 512 / 1024 / 2048 / 4096: one workgroup per
-//        vertex, LDS table of 1024 / 2048 / 4096 / 8192 slots
-//   11: bucket 7, deg > kBlockMax: rows split over many workgroups
-constexpr int kNumClasses = 12;
-constexpr int kClassHeavy = 11;
-constexpr uint32_t kBlockMax = 4096;
+//   7-11: bucket 7 with deg <= 512 / 1024 / 2048 / 4096 / 8192: one workgroup
+//        per vertex, LDS table of 1024 / 1280 / 3072 / 5800 / 10240 slots
+//   12: bucket 7 rows longer than the tuning's heavy threshold (at most
+//       kBlockMax): rows split over many workgroups
+constexpr int kNumClasses = 13;
+constexpr int kClassHeavy = 12;
+constexpr uint32_t kBlockMax = 8192;
 constexpr uint32_t kEmpty = 0xffffffffu;
 // Exact integer totals are accumulated into kSpread slots (block b adds to
 // slot b % kSpread) and summed on the host: same-address atomics from
@@ -42,7 +43,8 @@ __host__ __device__ inline int vertex_class(uint32_t d) {
     if (d <= 512) return 7;
     if (d <= 1024) return 8;
     if (d <= 2048) return 9;
-    if (d <= kBlockMax) return 10;
+    if (d <= 4096) return 10;
+    if (d <= kBlockMax) return 11;
     return kClassHeavy;
 }
 
@@ -82,8 +84,44 @@ struct DevView {
     // that the algorithm makes (tie-breaks, isLowerIdx, renumbering) uses
     // lid, so results do not depend on the order.
     const uint32_t *lid;
+    // Single graph, sweeps captured several to a launch: non-null once the
+    // pass loop may have stopped on the device; when *halt is set the
+    // kernels of the remaining captured sweeps return at entry.
+    const uint32_t *halt;
 };
 constexpr uint32_t kGraphActive = 1;
+constexpr uint32_t kGraphWait = 2;  // improved in this pass, waits for the contraction
+constexpr uint32_t kGraphDone = 3;
+
+// The loop variables of modularityOptimization (modularity.cu:464-508) for
+// one graph, kept on the device so the stop rule is applied there after
+// every sweep.
+struct PassState {
+    float act, old, q_print;
+    uint32_t it;     // sweeps of the pass
+    uint32_t first;  // the next sweep is the pass's first
+    uint32_t ret;    // the pass improved (modularityOptimization's result)
+    uint32_t done;   // the pass loop has stopped
+    uint32_t err;    // a capacity overflow was flagged during a sweep
+};
+// A batch graph: its pass loop variables, its m, and its run totals.
+struct GraphState {
+    float act, old, q_final, m;
+    uint32_t pass_iters, iters, first, ret;
+    uint32_t passes, status, pad0, pad1;
+    unsigned long long ne, sum_edges, sum_visits, pad2;
+};
+// What the host reads after a batch's captured sweeps.  The first three
+// fields are recounted after every sweep (zeroed before it); halt and err
+// stay set for the rest of the batch pass.
+struct BatchSummary {
+    uint32_t active;          // graphs still sweeping
+    uint32_t waiting;         // graphs that improved and wait for the contraction
+    unsigned long long live;  // vertices of the active graphs
+    uint32_t halt;            // no graph is active: later captured sweeps return at entry
+    uint32_t err;             // a capacity overflow was flagged
+};
+constexpr int kBatchSummaryCounted = 4;  // 32-bit words recounted per sweep
 
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
 constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
@@ -117,6 +155,33 @@ struct HeavySplit {
     uint32_t *scan_tmp;
     uint32_t *err;
 };
+
+// ---- device-side pass loop (lv_kernels.hip) ---------------------------------
+// After a sweep of one graph: Q from the collapsed exact counters
+// (tot[0] = S_in, tot[1..4] = sum a^2 limbs, tot[5] = moves), then the stop
+// rule into *ps; *err_flag is the sweep's capacity-overflow flag.
+void launch_sweep_decide(hipStream_t s, const unsigned long long *tot, const uint32_t *err_flag,
+                         int S, float m, float min_gain, uint32_t max_iters, PassState *ps);
+// Batch pass init, after the k_v sums: each active graph's m (a graph
+// without weight finishes with LV_ERR_RANGE) and its entries.  acc: per-graph
+// exact k totals; voff: vertex offsets, row: the pass's row pointers.
+void launch_batch_m(hipStream_t s, uint32_t G, const unsigned long long *acc, int S,
+                    const uint32_t *voff, const uint32_t *row, uint32_t *gstate, float *gm,
+                    GraphState *gst, int range_status);
+// After the pass-init modularity: act = Q0, first sweep next.
+void launch_batch_init(hipStream_t s, uint32_t G, const unsigned long long *gcnt, int S,
+                       const uint32_t *gstate, GraphState *gst);
+// After a batch sweep: every active graph's Q and stop rule; graphs that stop
+// wait (improved) or finish.  Writes the summary (zeroed by the caller).
+void launch_batch_decide(hipStream_t s, uint32_t G, const unsigned long long *gcnt,
+                         const uint32_t *err_flag, int S, float min_gain, uint32_t max_iters,
+                         const uint32_t *voff, uint32_t *gstate, GraphState *gst,
+                         BatchSummary *sum);
+// After launch_batch_decide: halt the remaining captured sweeps when no graph
+// is active.
+void launch_batch_halt(hipStream_t s, BatchSummary *sum);
+// After a batch contraction: waiting graphs become active for the next pass.
+void launch_batch_reactivate(hipStream_t s, uint32_t G, uint32_t *gstate, GraphState *gst);
 
 // ---- launchers (lv_kernels.hip) ---------------------------------------------
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);

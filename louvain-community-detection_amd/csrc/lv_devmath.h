@@ -181,6 +181,10 @@ __device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, float m, u
     return cv;
 }
 
+// the pass loop has stopped on the device: kernels of the remaining
+// captured sweeps of a launch return at entry (uniform over the grid)
+__device__ __forceinline__ bool halted(const DevView &g) { return g.halt && *g.halt; }
+
 // ---- single-graph / batch mode --------------------------------------------
 // a vertex takes part in the current sweep (batch: its graph is active)
 __device__ __forceinline__ bool v_live(const DevView &g, uint32_t v) {

@@ -74,16 +74,6 @@ struct ProfRec {
 // graph states in batch mode
 enum : uint32_t { kGsActive = lv::kGraphActive, kGsWait = 2, kGsDone = 3 };
 
-// host-side state of one graph of a batch (the per-graph copy of the loop
-// variables of modularityOptimization and main, modularity.cu:464-508)
-struct BatchGraph {
-    float act = 0.0f, old = 0.0f, q_final = 0.0f;
-    uint32_t iters = 0, passes = 0, pass_iters = 0;
-    bool first = true, ret = true;
-    int status = 0;
-    uint64_t ne = 0, sum_edges = 0, sum_visits = 0;
-};
-
 }  // namespace
 
 // Bump allocator over a caller-provided device region (the fixed scratch
@@ -129,7 +119,7 @@ struct lv_graph {
     // scratch for the renumbering under it
     uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
     bool relabeled = false;  // the current pass graph is in locality order
-    lv_tuning tune = {lv::kBlockMax, 512u, -1, 1u << 20};  // lv_graph_set_tuning
+    lv_tuning tune = {4096u, 512u, -1, 1u << 20};  // lv_graph_set_tuning
     // classification (host copies)
     uint32_t cls_count[kNumClasses] = {};
     uint32_t cls_off[kNumClasses + 1] = {};
@@ -177,6 +167,11 @@ struct lv_graph {
     ull *scan_tmp64 = nullptr;
     Counters *dc = nullptr;  // device
     Counters *hc = nullptr;  // pinned host
+    // the pass loop's state on the device (and its pinned host copy); while
+    // halt_on is set (sweeps captured for a device-decided loop) kernels get
+    // the halt flag in their view
+    lv::PassState *pst = nullptr, *pst_h = nullptr;
+    bool halt_on = false;
     // scalars
     int S = 0;
     float scale = 1.0f, inv_scale = 1.0f;
@@ -192,11 +187,12 @@ struct lv_graph {
     uint32_t *voff_d = nullptr;    // device: G+1 current vertex offsets
     uint32_t *vgather = nullptr;   // device: G+1 scratch
     ull *gcnt = nullptr;           // device: per-graph acc[G] | limbs[4][G] | moved[G]
-    ull *gcnt_h = nullptr;         // pinned host mirror
     std::vector<uint32_t> voff, voff0;  // host: current / original vertex offsets
-    std::vector<float> gm_h;
-    std::vector<uint32_t> gstate_h;
-    std::vector<BatchGraph> bg;
+    std::vector<uint32_t> gstate_h;     // host: initial graph states of a run
+    // per graph on the device: pass loop variables and run totals; the
+    // summary the host reads after each launch of captured sweeps
+    lv::GraphState *gst_d = nullptr;
+    lv::BatchSummary *bsum_d = nullptr, *bsum_h = nullptr;
     std::vector<int> gstatus0;  // per graph: LV_ERR_RANGE if its weights do not fit
     // profiling
     bool prof_on = false;
@@ -270,6 +266,7 @@ lv::DevView view(lv_graph *g) {
     v.gstate = g->gstate_d;
     v.G = g->G;
     v.lid = g->relabeled ? g->lid : nullptr;
+    v.halt = !g->halt_on ? nullptr : g->G ? &g->bsum_d->halt : &g->pst->done;
     return v;
 }
 
@@ -360,9 +357,6 @@ int zero_counters(lv_graph *g) {
 void counters_d2h(lv_graph *g) {
     lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
     (void)hipMemcpyAsync(g->hc, g->dc, kCountersHead, hipMemcpyDeviceToHost, g->stream);
-    if (g->G)
-        (void)hipMemcpyAsync(g->gcnt_h, g->gcnt, 6ull * g->G * sizeof(ull), hipMemcpyDeviceToHost,
-                             g->stream);
 }
 
 float fix2f_host(const lv_graph *g, int64_t x) { return (float)x * g->inv_scale; }
@@ -641,6 +635,7 @@ int step(lv_graph *g, int b) {
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_BLOCK,
+                                         LV_K_MOVE_BLOCK,
                                          LV_K_MOVE_GLOBAL};
     // the heavy rows' chain runs on the second stream beside the block
     // classes of the same step (they only read the step-start snapshot and
@@ -733,15 +728,17 @@ bool graphs_enabled(const lv_graph *g) {
     return !off && !g->carve;
 }
 
-// Captures one sweep; on failure the caller falls back to eager launches.
-bool capture_sweep(lv_graph *g, SweepGraph &sg) {
+// Captures what `enqueue` launches on the graph's stream as one HIP graph;
+// on failure the caller falls back to eager launches.
+template <typename F>
+bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
     sg.g = g;
     if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
     g->capture_recs = &sg.recs;
-    sweep_enqueue(g);
+    enqueue();
     g->capture_recs = nullptr;
     if (hipStreamEndCapture(g->stream, &sg.graph) != hipSuccess) {
         (void)hipGetLastError();
@@ -756,6 +753,31 @@ bool capture_sweep(lv_graph *g, SweepGraph &sg) {
     return true;
 }
 
+// Sweeps per launch of a pass loop decided on the device (LV_SWEEPS_PER_LAUNCH,
+// default 1).  With more than one, the kernels of the sweeps captured after
+// the stop must return at entry, which costs each kernel of every sweep a
+// dependent load of the halt flag; measured on MI355X that load costs about
+// what the host round trip it saves does (a launch of a captured sweep and
+// its completion wait are about 20 us), so one sweep per launch is the
+// default.  A profiled run always takes one, so every timed launch did its
+// work.
+uint32_t sweeps_per_launch(const lv_graph *g) {
+    if (g->prof_on) return 1;
+    static const char *env = getenv("LV_SWEEPS_PER_LAUNCH");
+    if (env && atoi(env) > 0) return (uint32_t)std::min(atoi(env), 16);
+    return 1;
+}
+
+// one sweep whose end applies the stop rule on the device (no host read)
+void sweep_enqueue_decide(lv_graph *g, float min_gain) {
+    zero_counters(g);
+    for (int b = 1; b <= 7; ++b) step(g, b);
+    modularity_launch(g);
+    lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
+    lv::launch_sweep_decide(g->stream, g->dc->tot, &g->dc->err, g->S, g->m, min_gain, kMaxIters,
+                            g->pst);
+}
+
 int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_print,
                             uint32_t *iters) {
     float act = 0.0f, old = 0.0f;
@@ -764,29 +786,55 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
     g->last_q_before = act;
     bool first = true, ret = true;
     uint32_t it = 0;
+    // The loop of modularity.cu:464-508 with its stop rule evaluated on the
+    // device after every sweep: the host launches a captured group of
+    // sweeps and reads the loop state once per group.
     SweepGraph sg;
-    const bool use_graph = graphs_enabled(g) && capture_sweep(g, sg);
-    for (;;) {
-        uint64_t moved = 0;
-        old = act;
-        if (use_graph) {
+    bool use_graph = false;
+    if (graphs_enabled(g)) {
+        const uint32_t K = sweeps_per_launch(g);
+        *g->pst_h = lv::PassState{act, 0.0f, 0.0f, 0u, 1u, 1u, 0u, 0u};
+        LV_HIP(hipMemcpyAsync(g->pst, g->pst_h, sizeof(lv::PassState), hipMemcpyHostToDevice,
+                              g->stream));
+        g->halt_on = K > 1;  // kernels test the halt flag only when sweeps can follow the stop
+        use_graph = capture(g, sg, [&] {
+            for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain);
+            (void)hipMemcpyAsync(g->pst_h, g->pst, sizeof(lv::PassState), hipMemcpyDeviceToHost,
+                                 g->stream);
+        });
+        g->halt_on = false;
+    }
+    if (use_graph) {
+        for (;;) {
             LV_HIP(hipGraphLaunch(sg.exec, g->stream));
-            rc = sweep_finish(g, &act, &moved);
+            LV_HIP(hipStreamSynchronize(g->stream));
             if (g->prof_on) sg.collect();
-        } else {
+            const lv::PassState p = *g->pst_h;
+            if (p.err) return LV_ERR_OVERFLOW;
+            if (!p.done) continue;
+            act = p.act;
+            it = p.it;
+            ret = p.ret != 0u;
+            if (!ret && q_print) *q_print = p.q_print;
+            break;
+        }
+    } else {  // eager launches, the stop rule on the host after each sweep
+        for (;;) {
+            uint64_t moved = 0;
+            old = act;
             rc = sweep(g, &act, &moved);
+            if (rc) return rc;
+            ++it;
+            // the loop condition of modularity.cu:508; a sweep without moves
+            // is a fixed point and also ends the loop
+            bool stop = !(act - old >= min_gain) || moved == 0 || it >= kMaxIters;
+            if (first && (act - old < min_gain || moved == 0)) {
+                if (q_print) *q_print = old;
+                ret = false;
+            }
+            first = false;
+            if (stop) break;
         }
-        if (rc) return rc;
-        ++it;
-        // the loop condition of modularity.cu:508; a sweep without moves is a
-        // fixed point and also ends the loop
-        bool stop = !(act - old >= min_gain) || moved == 0 || it >= kMaxIters;
-        if (first && (act - old < min_gain || moved == 0)) {
-            if (q_print) *q_print = old;
-            ret = false;
-        }
-        first = false;
-        if (stop) break;
     }
     g->last_iters = it;
     g->last_q_after = act;
@@ -1059,17 +1107,6 @@ int aggregation(lv_graph *g) {
 // fixed-point scale is the largest over the graphs; exact values and their
 // FP32 roundings do not depend on the scale.
 // ---------------------------------------------------------------------------
-float q_graph(const lv_graph *g, uint32_t gi) {
-    const ull *c = g->gcnt_h;
-    const uint32_t G = g->G;
-    const float s_in = fix2f_host(g, (int64_t)c[gi]);
-    u128 sq = 0;
-    for (int l = 0; l < 4; ++l) sq += (u128)c[G + (size_t)l * G + gi] << (32 * l);
-    const float a2 = ldexpf((float)sq, -2 * g->S);
-    const float m = g->gm_h[gi];
-    return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
-}
-
 int push_gstate(lv_graph *g) {
     LV_HIP(hipMemcpyAsync(g->gstate_d, g->gstate_h.data(), g->G * sizeof(uint32_t),
                           hipMemcpyHostToDevice, g->stream));
@@ -1077,50 +1114,19 @@ int push_gstate(lv_graph *g) {
     return LV_OK;
 }
 
+// Pass init of every active graph (modularity.cu:455-463), on the device:
+// classes, k_v, each graph's m (a graph without weight finishes), Q0.
 int batch_pass_init(lv_graph *g) {
     const uint32_t G = g->G;
     int rc = classify(g);
     if (rc) return rc;
-    // entries per graph of this pass (row pointer at the graph boundaries)
-    lv::launch_gather_u32(g->stream, g->row, g->voff_d, g->vgather, G + 1);
-    std::vector<uint32_t> rb(G + 1);
-    LV_HIP(hipMemcpyAsync(rb.data(), g->vgather, (G + 1) * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, g->stream));
     zero_counters(g);
     r1_all(g, 0, 1);
-    rc = sync_counters(g);
-    if (rc) return rc;
-    bool changed = false;
-    for (uint32_t gi = 0; gi < G; ++gi) {
-        BatchGraph &b = g->bg[gi];
-        b.ne = rb[gi + 1] - rb[gi];
-        if (g->gstate_h[gi] != kGsActive) continue;
-        const float m = fix2f_host(g, (int64_t)g->gcnt_h[gi]) / 2.0f;
-        g->gm_h[gi] = m;
-        if (!(m > 0.0f)) {  // no weight: the reference would divide by zero
-            b.status = LV_ERR_RANGE;
-            g->gstate_h[gi] = kGsDone;
-            changed = true;
-        }
-    }
-    LV_HIP(hipMemcpyAsync(g->gm_d, g->gm_h.data(), G * sizeof(float), hipMemcpyHostToDevice,
-                          g->stream));
-    if (changed) {
-        rc = push_gstate(g);
-        if (rc) return rc;
-    }
+    lv::launch_batch_m(g->stream, G, g->gcnt, g->S, g->voff_d, g->row, g->gstate_d, g->gm_d,
+                       g->gst_d, LV_ERR_RANGE);
     zero_counters(g);
     modularity_launch(g, true);  // every active community is a singleton here
-    rc = sync_counters(g);
-    if (rc) return rc;
-    for (uint32_t gi = 0; gi < G; ++gi) {
-        if (g->gstate_h[gi] != kGsActive) continue;
-        BatchGraph &b = g->bg[gi];
-        b.act = q_graph(g, gi);
-        b.first = true;
-        b.ret = true;
-        b.pass_iters = 0;
-    }
+    lv::launch_batch_init(g->stream, G, g->gcnt, g->S, g->gstate_d, g->gst_d);
     g->pass_ready = true;
     return LV_OK;
 }
@@ -1133,73 +1139,67 @@ uint32_t compact_min() {
 }
 
 // One pass for every active graph; returns in *more whether any graph
-// improved (and was contracted).
+// improved (and was contracted).  After every sweep a device kernel applies
+// each active graph's stop rule (modularity.cu:503-508) and counts the
+// graphs still active; the host reads that summary once per launch of
+// captured sweeps.
 int batch_pass(lv_graph *g, float min_gain, bool *more) {
     *more = false;
     const uint32_t G = g->G;
     int rc = batch_pass_init(g);
     if (rc) return rc;
+    lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->bsum_d), 0u,
+                        sizeof(lv::BatchSummary) / 4);
+    const uint32_t K = sweeps_per_launch(g);
+    auto enqueue = [&] {
+        for (uint32_t k = 0; k < K; ++k) {
+            zero_counters(g);
+            lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->bsum_d), 0u,
+                                lv::kBatchSummaryCounted);
+            for (int b = 1; b <= 7; ++b) step(g, b);
+            modularity_launch(g);
+            lv::launch_batch_decide(g->stream, G, g->gcnt, &g->dc->err, g->S, min_gain,
+                                    kMaxIters, g->voff_d, g->gstate_d, g->gst_d, g->bsum_d);
+            lv::launch_batch_halt(g->stream, g->bsum_d);
+        }
+        (void)hipMemcpyAsync(g->bsum_h, g->bsum_d, sizeof(lv::BatchSummary),
+                             hipMemcpyDeviceToHost, g->stream);
+    };
     auto sg = std::make_unique<SweepGraph>();
-    bool use_graph = graphs_enabled(g) && capture_sweep(g, *sg);
+    auto recapture = [&] {
+        sg = std::make_unique<SweepGraph>();
+        g->halt_on = K > 1;
+        const bool ok = graphs_enabled(g) && capture(g, *sg, enqueue);
+        g->halt_on = false;
+        return ok;
+    };
+    bool use_graph = recapture();
     for (;;) {
-        bool any_active = false;
-        for (uint32_t gi = 0; gi < G && !any_active; ++gi) any_active = g->gstate_h[gi] == kGsActive;
-        if (!any_active) break;
+        if (use_graph) {
+            LV_HIP(hipGraphLaunch(sg->exec, g->stream));
+        } else {
+            g->halt_on = K > 1;
+            enqueue();
+            g->halt_on = false;
+        }
+        LV_HIP(hipStreamSynchronize(g->stream));
+        if (use_graph && g->prof_on) sg->collect();
+        const lv::BatchSummary bs = *g->bsum_h;
+        if (bs.err) return LV_ERR_OVERFLOW;
+        if (!bs.active) break;
         // once the active graphs hold at most half of the listed vertices,
-        // the classes are rebuilt from the active graphs alone (and the sweep
+        // the classes are rebuilt from the active graphs alone (and the sweeps
         // captured again): frozen graphs' vertices cost a dispatched and
         // idle workgroup share in every kernel of every sweep otherwise
-        uint64_t live = 0;
-        for (uint32_t gi = 0; gi < G; ++gi)
-            if (g->gstate_h[gi] == kGsActive) live += g->voff[gi + 1] - g->voff[gi];
         const uint32_t listed = g->cls_off[kNumClasses];
-        if (listed >= compact_min() && 2 * live <= listed) {
+        if (listed >= compact_min() && 2 * bs.live <= listed) {
             sg.reset();
             rc = classify(g);
             if (rc) return rc;
-            sg = std::make_unique<SweepGraph>();
-            use_graph = graphs_enabled(g) && capture_sweep(g, *sg);
-        }
-        if (use_graph) {
-            LV_HIP(hipGraphLaunch(sg->exec, g->stream));
-            rc = sweep_finish(g, nullptr, nullptr);
-            if (g->prof_on) sg->collect();
-        } else {
-            rc = sweep(g, nullptr, nullptr);
-        }
-        if (rc) return rc;
-        bool changed = false;
-        for (uint32_t gi = 0; gi < G; ++gi) {
-            if (g->gstate_h[gi] != kGsActive) continue;
-            BatchGraph &b = g->bg[gi];
-            const uint64_t moved = g->gcnt_h[5ull * G + gi];
-            b.old = b.act;
-            b.act = q_graph(g, gi);
-            ++b.pass_iters;
-            // modularity.cu:503-508 per graph, plus the fixed-point stop
-            const bool stop =
-                !(b.act - b.old >= min_gain) || moved == 0 || b.pass_iters >= kMaxIters;
-            if (b.first && (b.act - b.old < min_gain || moved == 0)) {
-                b.q_final = b.old;
-                b.ret = false;
-            }
-            b.first = false;
-            if (stop) {
-                b.iters += b.pass_iters;
-                b.sum_edges += b.ne;
-                b.sum_visits += (uint64_t)(b.pass_iters + 1) * b.ne;
-                g->gstate_h[gi] = b.ret ? kGsWait : kGsDone;
-                changed = true;
-            }
-        }
-        if (changed) {
-            rc = push_gstate(g);
-            if (rc) return rc;
+            use_graph = recapture();
         }
     }
-    bool any_wait = false;
-    for (uint32_t gi = 0; gi < G; ++gi) any_wait |= g->gstate_h[gi] == kGsWait;
-    if (!any_wait) return LV_OK;
+    if (!g->bsum_h->waiting) return LV_OK;
     // contract the graphs that improved; every other graph stays as it is
     lv::launch_batch_identity(g->stream, g->gid, g->gstate_d, kGsWait, g->comm, g->comm_sz, g->n);
     rc = aggregation(g);
@@ -1212,13 +1212,7 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
     LV_HIP(hipMemcpyAsync(g->voff_d, g->voff.data(), (G + 1) * sizeof(uint32_t),
                           hipMemcpyHostToDevice, g->stream));
     lv::launch_batch_gid(g->stream, g->voff_d, G, g->gid, g->n);
-    for (uint32_t gi = 0; gi < G; ++gi)
-        if (g->gstate_h[gi] == kGsWait) {
-            g->gstate_h[gi] = kGsActive;
-            g->bg[gi].passes++;
-        }
-    rc = push_gstate(g);
-    if (rc) return rc;
+    lv::launch_batch_reactivate(g->stream, G, g->gstate_d, g->gst_d);
     *more = true;
     return LV_OK;
 }
@@ -1404,16 +1398,21 @@ void free_all(lv_graph *g) {
     dfree(g->scan_tmp32);
     dfree(g->scan_tmp64);
     dfree(g->dc);
+    dfree(g->pst);
     if (g->hc) (void)hipHostFree(g->hc);
     g->hc = nullptr;
+    if (g->pst_h) (void)hipHostFree(g->pst_h);
+    g->pst_h = nullptr;
     dfree(g->gid);
     dfree(g->gm_d);
     dfree(g->gstate_d);
     dfree(g->voff_d);
     dfree(g->vgather);
     dfree(g->gcnt);
-    if (g->gcnt_h) (void)hipHostFree(g->gcnt_h);
-    g->gcnt_h = nullptr;
+    dfree(g->gst_d);
+    dfree(g->bsum_d);
+    if (g->bsum_h) (void)hipHostFree(g->bsum_h);
+    g->bsum_h = nullptr;
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
     g->ev_fork = g->ev_join = nullptr;
@@ -1570,6 +1569,7 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
     A(&g->scan_tmp32, lv::scan_tmp_elems(n_cap + 1) + 8);
     A(&g->scan_tmp64, lv::scan_tmp_elems(n_cap + 1) + 8);
     A(&g->dc, 1);
+    A(&g->pst, 1);
     if (e || !g->carve) return e;
     const size_t hb = E / 16 + 1, hc = E / 256 + 1, slots = 4 * E + 1;
     A(&g->hp_bcnt, hb), A(&g->hp_bpos, hb + 1), A(&g->hp_bcur, hb);
@@ -1621,6 +1621,10 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
         return LV_ERR_HIP;
     }
     int e = graph_buffers(g, n_cap, e_cap, true);
+    if (!e && hipHostMalloc((void **)&g->pst_h, sizeof(lv::PassState)) != hipSuccess) {
+        (void)hipGetLastError();
+        e = LV_ERR_NOMEM;
+    }
     if (!e && hipHostMalloc((void **)&g->hc, sizeof(Counters)) != hipSuccess) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
@@ -1825,7 +1829,7 @@ int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32
         cand.assign(candidates, candidates + count);
     } else {  // built-in grid
         const bool big = g->n0 >= (1u << 18) && !g->G;
-        for (uint32_t hm : {2048u, lv::kBlockMax})
+        for (uint32_t hm : {2048u, 4096u, lv::kBlockMax})
             for (uint32_t be : {256u, 512u, 1024u})
                 for (int loc : {0, 1}) {
                     if (loc && !big) continue;
@@ -2108,8 +2112,8 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
     int e = 0;
     e |= dmalloc(&g->gid, (size_t)n + 1) | dmalloc(&g->gm_d, G) | dmalloc(&g->gstate_d, G);
     e |= dmalloc(&g->voff_d, (size_t)G + 1) | dmalloc(&g->vgather, (size_t)G + 1);
-    e |= dmalloc(&g->gcnt, 6ull * G);
-    if (!e && hipHostMalloc((void **)&g->gcnt_h, 6ull * G * sizeof(ull)) != hipSuccess) {
+    e |= dmalloc(&g->gcnt, 6ull * G) | dmalloc(&g->gst_d, G) | dmalloc(&g->bsum_d, 1);
+    if (!e && hipHostMalloc((void **)&g->bsum_h, sizeof(lv::BatchSummary)) != hipSuccess) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
     }
@@ -2120,9 +2124,7 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
     g->voff0.resize(G + 1);
     for (uint32_t gi = 0; gi <= G; ++gi) g->voff0[gi] = (uint32_t)graph_off[gi];
     g->voff = g->voff0;
-    g->gm_h.assign(G, 0.0f);
     g->gstate_h.assign(G, kGsActive);
-    g->bg.assign(G, BatchGraph());
     g->gstatus0 = std::move(status);
     if (hipMemcpy(g->voff_d, g->voff.data(), (G + 1) * sizeof(uint32_t),
                   hipMemcpyHostToDevice) != hipSuccess) {
@@ -2149,11 +2151,11 @@ int lv_batch_run(lv_graph *g, float min_gain, lv_batch_graph_result *per_graph,
     LV_HIP(hipEventCreate(&t0));
     LV_HIP(hipEventCreate(&t1));
     LV_HIP(hipEventRecord(t0, g->stream));
-    g->bg.assign(G, BatchGraph());
-    for (uint32_t gi = 0; gi < G; ++gi) {
-        g->bg[gi].status = g->gstatus0[gi];
+    // graphs without vertices or with weights out of range do not run
+    for (uint32_t gi = 0; gi < G; ++gi)
         g->gstate_h[gi] = (g->voff[gi + 1] > g->voff[gi] && !g->gstatus0[gi]) ? kGsActive : kGsDone;
-    }
+    lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->gst_d), 0u,
+                        (size_t)G * sizeof(lv::GraphState) / 4);
     int rc = push_gstate(g);
     bool more = true;
     while (!rc && more) rc = batch_pass(g, min_gain, &more);
@@ -2163,6 +2165,12 @@ int lv_batch_run(lv_graph *g, float min_gain, lv_batch_graph_result *per_graph,
     (void)hipEventElapsedTime(&ms, t0, t1);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    std::vector<lv::GraphState> st(G);
+    if (hipMemcpy(st.data(), g->gst_d, G * sizeof(lv::GraphState), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        if (!rc) rc = LV_ERR_HIP;
+    }
     if (total) {
         memset(total, 0, sizeof(*total));
         total->alg_ms = ms;
@@ -2170,14 +2178,14 @@ int lv_batch_run(lv_graph *g, float min_gain, lv_batch_graph_result *per_graph,
         total->n_final = g->n;
     }
     for (uint32_t gi = 0; gi < G; ++gi) {
-        const BatchGraph &b = g->bg[gi];
+        const lv::GraphState &b = st[gi];
         if (per_graph) {
             per_graph[gi].q_final = b.q_final;
             per_graph[gi].passes = b.passes;
             per_graph[gi].n_final = g->voff[gi + 1] - g->voff[gi];
             per_graph[gi].total_iters = b.iters;
             per_graph[gi].sum_edges = b.sum_edges;
-            per_graph[gi].status = b.status;
+            per_graph[gi].status = g->gstatus0[gi] ? g->gstatus0[gi] : (int)b.status;
         }
         if (total) {
             total->passes += b.passes;

@@ -248,6 +248,7 @@ __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
 
 // one wave per heavy vertex
 __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= h.nh) return;
     const uint32_t p = threadIdx.x & 63;
@@ -326,6 +327,7 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t c, int lb) {
 }
 
 __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     __shared__ uint32_t hist[kMaxBuckets];
     __shared__ unsigned long long part[4];
     const HeavyCtx x = heavy_ctx(g, h);
@@ -380,6 +382,7 @@ __global__ __launch_bounds__(256) void k_hp_count(DevView g, HeavySplit h) {
 // The chunk's entries are first ordered by bucket in LDS, so each bucket's
 // run of entries is written to its range with consecutive addresses.
 __global__ __launch_bounds__(256) void k_hp_scatter(DevView g, HeavySplit h) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     static_assert(kMaxBuckets == 4 * 256, "four buckets per thread in the offset scan");
     static_assert(kHeavyChunk < 65536, "16-bit chunk counts and offsets");
     // per bucket: entries of the chunk (16-bit halves of 32-bit words, so a
@@ -501,6 +504,7 @@ __device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long 
 
 template <int kBucketCap, int kDmax>
 __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     static_assert(kBucketCap > kDmax && kBucketCap <= 65536, "table shape");
     __shared__ uint32_t key[kBucketCap];
     __shared__ unsigned long long val[kBucketCap];
@@ -759,6 +763,7 @@ void launch_owner_fill(hipStream_t s, const uint32_t *off, uint32_t count, uint3
 // is in v's community, else +0 (adding +0 to a partial >= +0 changes nothing)
 __global__ __launch_bounds__(256) void k_heavy_stage(DevView g, HeavySplit h, float *stage,
                                                      int ident) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const HeavyCtx x = heavy_ctx(g, h);
     if (!x.live) return;  // uniform over the workgroup
     const uint32_t cv = g.comm[x.v];
@@ -798,6 +803,7 @@ constexpr int kR1Per = kR1Chunk / 192;    // loads per loader thread and chunk
 __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const float *stage,
                                                   int masked, int init_state,
                                                   unsigned long long *acc) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     __shared__ float buf[2][32][kR1K + 1];
     const uint32_t v = h.verts[blockIdx.x];
     if (!v_live(g, v)) return;  // uniform over the workgroup

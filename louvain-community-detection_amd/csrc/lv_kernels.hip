@@ -356,6 +356,7 @@ template <int G, bool MASKED, int B>
 __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, uint32_t count,
                                             int init_state, int ident,
                                             unsigned long long *acc) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     constexpr int GPB = 256 / G;
     const int lane = threadIdx.x % G;
     const uint32_t vi = blockIdx.x * GPB + threadIdx.x / G;
@@ -433,6 +434,7 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
 template <int BK>
 __global__ __launch_bounds__(256) void k_r1v(DevView g, const uint32_t *verts, uint32_t count,
                                              int ident, unsigned long long *acc) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     __shared__ float xs[8][BK * 128];
     const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
     float *xg = xs[grp];
@@ -539,6 +541,7 @@ void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sumsq(DevView g, const uint32_t *verts, uint32_t count,
                                                unsigned long long *limbs) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const int twoS = 2 * g.S;
     const uint32_t stride = acc_stride(g);
     for (uint32_t base_i = blockIdx.x * 256; base_i < count; base_i += gridDim.x * 256) {
@@ -613,6 +616,7 @@ void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32
 template <int T, int ROUNDS, int CAP>
 __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *verts,
                                                     uint32_t count) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     constexpr int GPB = 256 / T;
     constexpr int LC = Log2<CAP>::value;
     __shared__ uint32_t skey[GPB * CAP];
@@ -725,6 +729,7 @@ constexpr int kWaveMaxPer = 5;        // ceil(319 / 64)
 constexpr int kWaveWpb = LV_WAVE_WPB;  // waves (vertices) per workgroup
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
                                                              uint32_t count) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     __shared__ uint32_t skey[kWaveWpb * kWaveCap];
     __shared__ unsigned long long sval[kWaveWpb * kWaveCap];
     __shared__ uint16_t sdist[kWaveWpb][320];
@@ -877,6 +882,7 @@ __device__ __forceinline__ void block_argmax(float tg, uint32_t tc, uint32_t tj,
 template <int P, int MAXDEG, int CAP, int MINB = 1>
 __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_t *verts,
                                                         uint32_t count) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     static_assert(CAP > MAXDEG && CAP <= 65536 && 3 * P * 4 <= CAP * 8, "table shape");
     constexpr int MAXPER = (MAXDEG + P - 1) / P;
     __shared__ unsigned long long val[CAP];
@@ -993,6 +999,9 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
 #ifndef LV_MINB10
 #define LV_MINB10 2
 #endif
+#ifndef LV_CAP11
+#define LV_CAP11 10240
+#endif
 #ifndef LV_SMALLCAP5
 #define LV_SMALLCAP5 256
 #endif
@@ -1040,6 +1049,10 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
             hipLaunchKernelGGL((k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>), dim3(count), dim3(LV_P10), 0, s, g, verts,
                                count);
             break;
+        case 11:
+            hipLaunchKernelGGL((k_move_block<1024, 8192, LV_CAP11, 1>), dim3(count), dim3(1024), 0,
+                               s, g, verts, count);
+            break;
         default:
             break;
     }
@@ -1071,6 +1084,7 @@ void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts, uint32_t count,
                                                unsigned long long *moved) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long mv = 0;
     const uint32_t v = i < count ? verts[i] : 0u;
@@ -2370,6 +2384,181 @@ void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new
     if (!n0) return;
     hipLaunchKernelGGL(k_agg_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, comm, new_id,
                        init_comm, n0);
+}
+
+// ---------------------------------------------------------------------------
+// The pass loop's decisions on the device (modularity.cu:501-508 per graph):
+// the FP32 modularity from the exact counters, in the reference's
+// expression (modularity.cu:428-436), then the stop rule.
+// ---------------------------------------------------------------------------
+
+// Correctly rounded (nearest-even) u128 -> f32.
+__device__ __forceinline__ float u128_to_f32_rn(unsigned __int128 x) {
+    const unsigned long long hi = (unsigned long long)(x >> 64);
+    if (!hi) return u64_to_f32_rn((unsigned long long)x);
+    const int lz = __clzll(hi);  // the top bit moves to bit 127
+    const unsigned __int128 y = x << lz;
+    const unsigned long long h = (unsigned long long)(y >> 64), l = (unsigned long long)y;
+    unsigned long long mant = h >> 40;  // 24 bits
+    const unsigned long long rest = h & ((1ull << 40) - 1ull), half = 1ull << 39;
+    if (rest > half || (rest == half && (l != 0ull || (mant & 1ull)))) mant++;
+    return ldexpf((float)(uint32_t)mant, (127 - lz) - 23);
+}
+
+// Q = S_in/(2m) - sum a^2/((4m) m): S_in and sum a^2 (at scale 2^2S, four
+// 32-bit limbs each summed in a u64) exact, each rounded once
+__device__ __forceinline__ float q_exact(unsigned long long sin, const unsigned long long *limb,
+                                         size_t stride, int S, float m) {
+    unsigned __int128 sq = 0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) sq += (unsigned __int128)limb[l * stride] << (32 * l);
+    const float s_in = fix2f(sin, S);
+    const float a2 = ldexpf(u128_to_f32_rn(sq), -2 * S);
+    return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
+}
+
+// one sweep's end: old/act, the first sweep's verdict, the stop rule (a
+// sweep without moves is a fixed point and also ends the loop)
+__device__ __forceinline__ bool stop_rule(float q, unsigned long long moved, float min_gain,
+                                          uint32_t max_iters, float &act, float &old,
+                                          uint32_t &it, uint32_t &first, uint32_t &ret,
+                                          float &q_print) {
+    old = act;
+    act = q;
+    ++it;
+    const bool stop = !(act - old >= min_gain) || moved == 0ull || it >= max_iters;
+    if (first && (act - old < min_gain || moved == 0ull)) {
+        q_print = old;
+        ret = 0u;
+    }
+    first = 0u;
+    return stop;
+}
+
+__global__ void k_sweep_decide(const unsigned long long *tot, const uint32_t *err_flag, int S,
+                               float m, float min_gain, uint32_t max_iters, PassState *ps) {
+    if (ps->done) return;
+    if (*err_flag) {
+        ps->err = 1u;
+        ps->done = 1u;
+        return;
+    }
+    PassState p = *ps;
+    const float q = q_exact(tot[0], tot + 1, 1, S, m);
+    if (stop_rule(q, tot[5], min_gain, max_iters, p.act, p.old, p.it, p.first, p.ret, p.q_print))
+        p.done = 1u;
+    *ps = p;
+}
+void launch_sweep_decide(hipStream_t s, const unsigned long long *tot, const uint32_t *err_flag,
+                         int S, float m, float min_gain, uint32_t max_iters, PassState *ps) {
+    hipLaunchKernelGGL(k_sweep_decide, dim3(1), dim3(1), 0, s, tot, err_flag, S, m, min_gain,
+                       max_iters, ps);
+}
+
+__global__ __launch_bounds__(256) void k_batch_m(uint32_t G, const unsigned long long *acc, int S,
+                                                 const uint32_t *voff, const uint32_t *row,
+                                                 uint32_t *gstate, float *gm, GraphState *gst,
+                                                 int range_status) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= G || gstate[gi] != kGraphActive) return;
+    GraphState &b = gst[gi];
+    b.ne = row[voff[gi + 1]] - row[voff[gi]];
+    // all_w = reduce(neigh_w) / 2 (modularity.cu:457-458): exact sum, one
+    // rounding, exact halving
+    const float m = fix2f(acc[gi], S) / 2.0f;
+    gm[gi] = m;
+    b.m = m;
+    if (!(m > 0.0f)) {  // no weight: the reference would divide by zero
+        b.status = (uint32_t)range_status;
+        gstate[gi] = kGraphDone;
+    }
+}
+void launch_batch_m(hipStream_t s, uint32_t G, const unsigned long long *acc, int S,
+                    const uint32_t *voff, const uint32_t *row, uint32_t *gstate, float *gm,
+                    GraphState *gst, int range_status) {
+    hipLaunchKernelGGL(k_batch_m, dim3((G + 255) / 256), dim3(256), 0, s, G, acc, S, voff, row,
+                       gstate, gm, gst, range_status);
+}
+
+__global__ __launch_bounds__(256) void k_batch_init(uint32_t G, const unsigned long long *gcnt,
+                                                    int S, const uint32_t *gstate,
+                                                    GraphState *gst) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi >= G || gstate[gi] != kGraphActive) return;
+    GraphState &b = gst[gi];
+    b.act = q_exact(gcnt[gi], gcnt + G + gi, G, S, b.m);
+    b.first = 1u;
+    b.ret = 1u;
+    b.pass_iters = 0u;
+}
+void launch_batch_init(hipStream_t s, uint32_t G, const unsigned long long *gcnt, int S,
+                       const uint32_t *gstate, GraphState *gst) {
+    hipLaunchKernelGGL(k_batch_init, dim3((G + 255) / 256), dim3(256), 0, s, G, gcnt, S, gstate,
+                       gst);
+}
+
+__global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, const unsigned long long *gcnt,
+                                                      const uint32_t *err_flag, int S,
+                                                      float min_gain, uint32_t max_iters,
+                                                      const uint32_t *voff, uint32_t *gstate,
+                                                      GraphState *gst, BatchSummary *sum) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    uint32_t state = 0;
+    if (gi < G) {
+        state = gstate[gi];
+        if (state == kGraphActive) {
+            GraphState b = gst[gi];
+            const float q = q_exact(gcnt[gi], gcnt + G + gi, G, S, b.m);
+            float qp = b.q_final;
+            if (stop_rule(q, gcnt[5ull * G + gi], min_gain, max_iters, b.act, b.old,
+                          b.pass_iters, b.first, b.ret, qp)) {
+                b.iters += b.pass_iters;
+                b.sum_edges += b.ne;
+                b.sum_visits += (unsigned long long)(b.pass_iters + 1u) * b.ne;
+                state = b.ret ? kGraphWait : kGraphDone;
+                gstate[gi] = state;
+            }
+            b.q_final = qp;
+            gst[gi] = b;
+        }
+    }
+    const unsigned long long live =
+        state == kGraphActive ? (unsigned long long)(voff[gi + 1] - voff[gi]) : 0ull;
+    const unsigned long long a = wave_sum_u64(state == kGraphActive ? 1ull : 0ull);
+    const unsigned long long w = wave_sum_u64(state == kGraphWait ? 1ull : 0ull);
+    const unsigned long long l = wave_sum_u64(live);
+    if ((threadIdx.x & 63) == 0) {
+        if (a) atomicAdd(&sum->active, (uint32_t)a);
+        if (w) atomicAdd(&sum->waiting, (uint32_t)w);
+        if (l) atomicAdd(&sum->live, l);
+    }
+    if (gi == 0 && *err_flag) sum->err = 1u;
+}
+__global__ void k_batch_halt(BatchSummary *sum) {
+    if (sum->active == 0u) sum->halt = 1u;
+}
+void launch_batch_halt(hipStream_t s, BatchSummary *sum) {
+    hipLaunchKernelGGL(k_batch_halt, dim3(1), dim3(1), 0, s, sum);
+}
+void launch_batch_decide(hipStream_t s, uint32_t G, const unsigned long long *gcnt,
+                         const uint32_t *err_flag, int S, float min_gain, uint32_t max_iters,
+                         const uint32_t *voff, uint32_t *gstate, GraphState *gst,
+                         BatchSummary *sum) {
+    hipLaunchKernelGGL(k_batch_decide, dim3((G + 255) / 256), dim3(256), 0, s, G, gcnt, err_flag,
+                       S, min_gain, max_iters, voff, gstate, gst, sum);
+}
+
+__global__ __launch_bounds__(256) void k_batch_reactivate(uint32_t G, uint32_t *gstate,
+                                                          GraphState *gst) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    if (gi < G && gstate[gi] == kGraphWait) {
+        gstate[gi] = kGraphActive;
+        gst[gi].passes++;
+    }
+}
+void launch_batch_reactivate(hipStream_t s, uint32_t G, uint32_t *gstate, GraphState *gst) {
+    hipLaunchKernelGGL(k_batch_reactivate, dim3((G + 255) / 256), dim3(256), 0, s, G, gstate,
+                       gst);
 }
 
 }  // namespace lv

@@ -423,6 +423,7 @@ def test_server_results_match_oracle(lv, orc, slots, budget):
     dict(heavy_min_degree=1024, bucket_entries=1024),
     dict(heavy_min_degree=2048, bucket_entries=256, locality=1),
     dict(heavy_min_degree=4096, bucket_entries=512, locality=0),
+    dict(heavy_min_degree=8192, bucket_entries=512, locality=1),
 ])
 def test_tuning_never_changes_results(lv, orc, graphs, tune):
     """Launch-shape parameters (which kernel serves a bucket-7 row, heavy
