@@ -128,7 +128,7 @@ def bench_plain(lv, W, cfg, args):
     g = lv.Graph(row, col, w)  # host checks, device buffers, PCIe upload
     create_s = time.perf_counter() - t0
     tuning = g.get_tuning()
-    if args.tune:  # untimed: candidate launch shapes on this graph, fastest kept
+    if not args.no_tune:  # untimed: candidate launch shapes on this graph, fastest kept
         t0 = time.perf_counter()
         tuning, tms = g.autotune(W.MIN_GAIN)
         log(f"autotune: {tuning} ({tms:.2f} ms/run, search {time.perf_counter() - t0:.1f} s)")
@@ -202,7 +202,7 @@ def bench_plain(lv, W, cfg, args):
                    "entries": int(col.size), "min_gain": W.MIN_GAIN,
                    "modularity": float(res.q_final), "passes": int(res.passes),
                    "sweeps": int(res.total_iters), "communities": int(res.n_final),
-                   "tuning": tuning, "autotuned": bool(args.tune)},
+                   "tuning": tuning, "autotuned": not args.no_tune},
         "roofline": roofline,
         # the boundary takes host buffers: one upload plus one run
         "pcie_inclusive": {"create_ms": create_s * 1e3,
@@ -251,7 +251,7 @@ def bench_batch(lv, W, cfg, args):
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     b = lv.Batch(graphs)
     tuning = b.get_tuning()
-    if args.tune:
+    if not args.no_tune:
         t0 = time.perf_counter()
         tuning, tms = b.autotune(W.MIN_GAIN)
         log(f"autotune: {tuning} ({tms:.1f} ms/run, search {time.perf_counter() - t0:.1f} s)")
@@ -307,7 +307,7 @@ def bench_batch(lv, W, cfg, args):
            "identical_to_oracle_digests": bool(batch_ok == len(graphs) and
                                                plain_ok == len(sample)),
            "graphs_checked": len(graphs) + len(sample),
-           "tuning": tuning, "autotuned": bool(args.tune)}
+           "tuning": tuning, "autotuned": not args.no_tune}
     if not args.no_cpu:
         import oracle
         cs = sample[:8]
@@ -390,8 +390,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true",
                     help="skip the extra profiled run (exactly warmup + steps runs, e.g. "
                          "under an external profiler); roofline is then null")
-    ap.add_argument("--tune", action="store_true",
-                    help="run the autotuner on the resident graph before the timed runs")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (by default it picks the launch shapes on the "
+                         "resident graph or batch before the timed runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU is design-only "

@@ -137,14 +137,26 @@ typedef struct lv_tuning {
                                        (power of two, 64..1024) */
     int32_t locality;               /* -1 automatic, 0 never, 1 every pass */
     uint32_t locality_min_vertices; /* automatic mode: pass graphs from this many vertices */
+    uint32_t wave_vertices;         /* bucket-6 rows (85..319 entries): vertices, one wave
+                                       each, per workgroup: 1, 2 (default) or 4 */
+    uint32_t block_threads;         /* bucket-7 rows of 2049..8192 entries: threads of the
+                                       vertex's workgroup, 512 or 1024 (default) */
+    uint32_t class8_slots;          /* bucket-7 rows of 513..1024 entries: LDS table slots,
+                                       1280 (default) or 2048 */
+    uint32_t agg_bucket_entries;    /* contraction of large communities: target entries per
+                                       key-range bucket, 256..512 (default 384) */
+    uint32_t sweeps_per_launch;     /* sweeps captured to one launch of a pass loop decided on
+                                       the device: 1 (default), 2 or 4 */
 } lv_tuning;
 int lv_tuning_default(lv_tuning *t);
 int lv_graph_set_tuning(lv_graph *g, const lv_tuning *t);  /* LV_ERR_ARG if out of range */
 int lv_graph_get_tuning(lv_graph *g, lv_tuning *t);
 /* Autotuner: runs the resident input (lv_graph_reset before each run) under
- * each candidate -- or a built-in grid over heavy_min_degree, bucket_entries
- * and locality when candidates is NULL -- keeps the fastest (best of two runs
- * each) on g and returns it with its time.  g is left reset. */
+ * each candidate and keeps the fastest (best of two runs each) on g,
+ * returning it with its time.  With candidates NULL it searches every field
+ * above one at a time (each over its listed values, the others held at the
+ * best found so far; locality only for graphs of at least 2^18 vertices).
+ * g is left reset. */
 int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32_t count,
                 lv_tuning *best, double *best_ms);
 

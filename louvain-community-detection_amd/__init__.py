@@ -66,7 +66,10 @@ class RequestResult(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("heavy_min_degree", ctypes.c_uint32), ("bucket_entries", ctypes.c_uint32),
-                ("locality", ctypes.c_int32), ("locality_min_vertices", ctypes.c_uint32)]
+                ("locality", ctypes.c_int32), ("locality_min_vertices", ctypes.c_uint32),
+                ("wave_vertices", ctypes.c_uint32), ("block_threads", ctypes.c_uint32),
+                ("class8_slots", ctypes.c_uint32), ("agg_bucket_entries", ctypes.c_uint32),
+                ("sweeps_per_launch", ctypes.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

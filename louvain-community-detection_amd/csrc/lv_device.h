@@ -244,8 +244,15 @@ void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32
                   unsigned long long *limbs);
 
 // classes 1-10 (the heavy class goes through launch_heavy_move)
+// Launch shapes the tuning selects among compiled variants of the move
+// kernels (no result depends on them).
+struct MoveShape {
+    uint32_t wave_vertices;  // bucket-6 kernel: vertices (one wave each) per workgroup: 1, 2, 4
+    uint32_t block_threads;  // rows of 2049..8192 entries: 512 or 1024 threads per vertex
+    uint32_t class8_slots;   // rows of 513..1024 entries: LDS table of 1280 or 2048 slots
+};
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
-                       uint32_t count);
+                       uint32_t count, const MoveShape &shape);
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved);
 void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts,
@@ -322,7 +329,9 @@ struct AggPart {
     uint32_t dmax;           // distinct keys a bucket may hold (<= kAggBucketDistinct)
 };
 constexpr uint32_t kAggBucketDistinct = 512;  // per bucket; its LDS table has more slots
+// target: entries per key-range bucket of a large community (256..512)
 void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
+                    uint32_t target,
                     uint32_t *nbk);
 // phase 0 count, 1 scatter, 2 bucket merge, 3 drop counts of flagged communities
 void launch_aggp(hipStream_t s, const DevView &g, AggView a, const AggPart &p,

@@ -119,7 +119,7 @@ struct lv_graph {
     // scratch for the renumbering under it
     uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
     bool relabeled = false;  // the current pass graph is in locality order
-    lv_tuning tune = {4096u, 512u, -1, 1u << 20};  // lv_graph_set_tuning
+    lv_tuning tune = {4096u, 512u, -1, 1u << 20, 2u, 1024u, 1280u, 384u, 1u};  // lv_graph_set_tuning
     // classification (host copies)
     uint32_t cls_count[kNumClasses] = {};
     uint32_t cls_off[kNumClasses + 1] = {};
@@ -655,7 +655,9 @@ int step(lv_graph *g, int b) {
         if (c == lv::kClassHeavy)
             lv::launch_heavy_move(st, v, heavy_view(g));
         else
-            lv::launch_move_class(st, v, c, g->order + g->cls_off[c], g->cls_count[c]);
+            lv::launch_move_class(st, v, c, g->order + g->cls_off[c], g->cls_count[c],
+                                  lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
+                                                g->tune.class8_slots});
     }
     if (fork) {
         (void)hipEventRecord(g->ev_join, g->stream2);
@@ -753,8 +755,8 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
     return true;
 }
 
-// Sweeps per launch of a pass loop decided on the device (LV_SWEEPS_PER_LAUNCH,
-// default 1).  With more than one, the kernels of the sweeps captured after
+// Sweeps per launch of a pass loop decided on the device (the tuning's
+// sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).  With more than one, the kernels of the sweeps captured after
 // the stop must return at entry, which costs each kernel of every sweep a
 // dependent load of the halt flag; measured on MI355X that load costs about
 // what the host round trip it saves does (a launch of a captured sweep and
@@ -765,7 +767,7 @@ uint32_t sweeps_per_launch(const lv_graph *g) {
     if (g->prof_on) return 1;
     static const char *env = getenv("LV_SWEEPS_PER_LAUNCH");
     if (env && atoi(env) > 0) return (uint32_t)std::min(atoi(env), 16);
-    return 1;
+    return g->tune.sweeps_per_launch;
 }
 
 // one sweep whose end applies the stop rule on the device (no host read)
@@ -884,7 +886,7 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     hipStream_t s = g->stream;
     lv::launch_agg_large_prep(s, v, aview(g), lc, nl);  // chunks per community
     lv::launch_scan_u32(s, g->agg_cnk, 0, nl, g->agg_coff, g->scan_tmp32);
-    lv::launch_aggp_nb(s, aview(g), lc, nl, nn, g->ap_nbk);
+    lv::launch_aggp_nb(s, aview(g), lc, nl, nn, g->tune.agg_bucket_entries, g->ap_nbk);
     lv::launch_scan_u32(s, g->ap_nbk, 0, nl, g->ap_bstart, g->scan_tmp32);
     uint32_t chunks = 0, NB = 0;
     LV_HIP(hipMemcpyAsync(&chunks, g->agg_coff + nl, 4, hipMemcpyDeviceToHost, s));
@@ -1534,7 +1536,7 @@ int check_input(uint32_t n, const uint32_t *row, const uint32_t *col, const floa
 //   buckets  sum pow2 >= d/64 (<= 2d/64) <= E/32 + nh        < E/16
 //   tables   sum pow2 >= 2d             <= 4E slots
 //   large communities have more than 2048 member entries, so nl <= E/2048;
-//   merge buckets  sum (2d/384 + 1)     <= E/192 + nl        < E/128
+//   merge buckets  sum (2d/256 + 1)     <= E/128 + nl  (the smallest target)
 //   merge tables   sum pow2 >= 2d       <= 4E
 //   chunk list     sum ceil(d/2048) + nl + 2 <= E/1024 + 2 + nl.
 int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
@@ -1577,7 +1579,7 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
     A(&g->hp_cvtx, hc), A(&g->hp_flist, hc + 1);
     A(&g->htab, 2 * slots);
     A(&g->hscr, (E / 320 + 1) * 130);
-    const size_t ab = E / 128 + 1, lcap = E / 1024 + E / 2048 + 4;
+    const size_t ab = E / 128 + E / 2048 + 2, lcap = E / 1024 + E / 2048 + 4;
     A(&g->ap_bcnt, ab), A(&g->ap_bpos, ab + 1), A(&g->ap_bcur, ab), A(&g->ap_bnd, ab);
     A(&g->ap_boff, ab + 1), A(&g->ap_bowner, ab);
     A(&g->ap_btmp, lv::scan_tmp_elems((uint32_t)ab + 1) + 8);
@@ -1811,6 +1813,12 @@ int lv_graph_set_tuning(lv_graph *g, const lv_tuning *t) {
     if (t->heavy_min_degree < lv::kCut[6] || t->heavy_min_degree > lv::kBlockMax) return LV_ERR_ARG;
     if (b < 64 || b > 1024 || (b & (b - 1))) return LV_ERR_ARG;
     if (t->locality < -1 || t->locality > 1) return LV_ERR_ARG;
+    if (t->wave_vertices != 1 && t->wave_vertices != 2 && t->wave_vertices != 4) return LV_ERR_ARG;
+    if (t->block_threads != 512 && t->block_threads != 1024) return LV_ERR_ARG;
+    if (t->class8_slots != 1280 && t->class8_slots != 2048) return LV_ERR_ARG;
+    if (t->agg_bucket_entries < 256 || t->agg_bucket_entries > 512) return LV_ERR_ARG;
+    if (t->sweeps_per_launch != 1 && t->sweeps_per_launch != 2 && t->sweeps_per_launch != 4)
+        return LV_ERR_ARG;
     g->tune = *t;
     return LV_OK;
 }
@@ -1824,48 +1832,87 @@ int lv_graph_get_tuning(lv_graph *g, lv_tuning *t) {
 int lv_autotune(lv_graph *g, float min_gain, const lv_tuning *candidates, uint32_t count,
                 lv_tuning *best, double *best_ms) {
     if (!g || (!candidates && count)) return LV_ERR_ARG;
-    std::vector<lv_tuning> cand;
-    if (candidates) {
-        cand.assign(candidates, candidates + count);
-    } else {  // built-in grid
-        const bool big = g->n0 >= (1u << 18) && !g->G;
-        for (uint32_t hm : {2048u, 4096u, lv::kBlockMax})
-            for (uint32_t be : {256u, 512u, 1024u})
-                for (int loc : {0, 1}) {
-                    if (loc && !big) continue;
-                    cand.push_back({hm, be, loc, g->tune.locality_min_vertices});
-                }
-    }
-    if (cand.empty()) return LV_ERR_ARG;
     const lv_tuning keep = g->tune;
-    double best_t = 0.0;
-    size_t bi = 0;
-    for (size_t i = 0; i < cand.size(); ++i) {
-        int rc = lv_graph_set_tuning(g, &cand[i]);
-        if (rc) {
-            g->tune = keep;
-            return rc;
-        }
-        double t = 0.0;
+    // best of two runs of the resident input under t
+    auto time_of = [&](const lv_tuning &t, double *ms) -> int {
+        int rc = lv_graph_set_tuning(g, &t);
+        if (rc) return rc;
         for (int rep = 0; rep < 2; ++rep) {
             rc = lv_graph_reset(g);
             lv_result r;
             if (!rc) rc = g->G ? lv_batch_run(g, min_gain, nullptr, &r) : lv_run(g, min_gain, &r);
+            if (rc) return rc;
+            if (rep == 0 || r.alg_ms < *ms) *ms = r.alg_ms;
+        }
+        return LV_OK;
+    };
+    lv_tuning bt = keep;
+    double bms = 0.0;
+    if (candidates) {
+        if (!count) return LV_ERR_ARG;
+        for (uint32_t i = 0; i < count; ++i) {
+            double ms = 0.0;
+            const int rc = time_of(candidates[i], &ms);
             if (rc) {
                 g->tune = keep;
                 lv_graph_reset(g);
                 return rc;
             }
-            if (rep == 0 || r.alg_ms < t) t = r.alg_ms;
+            if (i == 0 || ms < bms) {
+                bms = ms;
+                bt = candidates[i];
+            }
         }
-        if (i == 0 || t < best_t) {
-            best_t = t;
-            bi = i;
+    } else {
+        // one field at a time over its values, the others at the best so far
+        int rc = time_of(bt, &bms);
+        const bool big = g->n0 >= (1u << 18) && !g->G;
+        struct Axis {
+            uint32_t lv_tuning::*u;
+            std::vector<uint32_t> values;
+        };
+        const Axis axes[] = {
+            {&lv_tuning::heavy_min_degree, {2048u, 4096u, lv::kBlockMax}},
+            {&lv_tuning::bucket_entries, {256u, 512u, 1024u}},
+            {&lv_tuning::wave_vertices, {1u, 2u, 4u}},
+            {&lv_tuning::block_threads, {512u, 1024u}},
+            {&lv_tuning::class8_slots, {1280u, 2048u}},
+            {&lv_tuning::agg_bucket_entries, {256u, 384u, 512u}},
+            {&lv_tuning::sweeps_per_launch, {1u, 2u, 4u}},
+        };
+        for (const Axis &ax : axes) {
+            for (uint32_t val : ax.values) {
+                if (rc || bt.*ax.u == val) continue;
+                lv_tuning t = bt;
+                t.*ax.u = val;
+                double ms = 0.0;
+                rc = time_of(t, &ms);
+                if (!rc && ms < bms) {
+                    bms = ms;
+                    bt = t;
+                }
+            }
+        }
+        for (int loc : {0, 1}) {
+            if (rc || !big || bt.locality == loc) continue;
+            lv_tuning t = bt;
+            t.locality = loc;
+            double ms = 0.0;
+            rc = time_of(t, &ms);
+            if (!rc && ms < bms) {
+                bms = ms;
+                bt = t;
+            }
+        }
+        if (rc) {
+            g->tune = keep;
+            lv_graph_reset(g);
+            return rc;
         }
     }
-    g->tune = cand[bi];
-    if (best) *best = cand[bi];
-    if (best_ms) *best_ms = best_t;
+    g->tune = bt;
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
     return lv_graph_reset(g);
 }
 

@@ -723,10 +723,8 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
 #endif
 constexpr int kWaveCap = LV_WAVECAP;  // slots per wave (at most 319 distinct keys)
 constexpr int kWaveMaxPer = 5;        // ceil(319 / 64)
-#ifndef LV_WAVE_WPB
-#define LV_WAVE_WPB 2
-#endif
-constexpr int kWaveWpb = LV_WAVE_WPB;  // waves (vertices) per workgroup
+// kWaveWpb waves (vertices) per workgroup: 1, 2 or 4 (a tuning choice)
+template <int kWaveWpb>
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
                                                              uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
@@ -1006,7 +1004,7 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
 #define LV_SMALLCAP5 256
 #endif
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
-                       uint32_t count) {
+                       uint32_t count, const MoveShape &shape) {
     if (!count) return;
     switch (cls) {
         case 1:
@@ -1030,28 +1028,46 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                                g, verts, count);
             break;
         case 6:
-            hipLaunchKernelGGL(k_move_wave, dim3((count + kWaveWpb - 1) / kWaveWpb),
-                               dim3(64 * kWaveWpb), 0, s, g, verts, count);
+            if (shape.wave_vertices == 1)
+                hipLaunchKernelGGL(k_move_wave<1>, dim3(count), dim3(64), 0, s, g, verts, count);
+            else if (shape.wave_vertices == 4)
+                hipLaunchKernelGGL(k_move_wave<4>, dim3((count + 3) / 4), dim3(256), 0, s, g,
+                                   verts, count);
+            else
+                hipLaunchKernelGGL(k_move_wave<2>, dim3((count + 1) / 2), dim3(128), 0, s, g,
+                                   verts, count);
             break;
         case 7:
             hipLaunchKernelGGL((k_move_block<256, 512, LV_CAP7>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
         case 8:
-            hipLaunchKernelGGL((k_move_block<256, 1024, LV_CAP8, LV_MINB8>), dim3(count), dim3(256), 0, s, g, verts,
-                               count);
+            if (shape.class8_slots == 2048)
+                hipLaunchKernelGGL((k_move_block<256, 1024, 2048, 4>), dim3(count), dim3(256), 0, s,
+                                   g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_block<256, 1024, LV_CAP8, LV_MINB8>), dim3(count),
+                                   dim3(256), 0, s, g, verts, count);
             break;
         case 9:
             hipLaunchKernelGGL((k_move_block<256, 2048, LV_CAP9>), dim3(count), dim3(256), 0, s, g, verts,
                                count);
             break;
         case 10:
-            hipLaunchKernelGGL((k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>), dim3(count), dim3(LV_P10), 0, s, g, verts,
-                               count);
+            if (shape.block_threads == 512)
+                hipLaunchKernelGGL((k_move_block<512, 4096, LV_CAP10, LV_MINB10>), dim3(count),
+                                   dim3(512), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>), dim3(count),
+                                   dim3(LV_P10), 0, s, g, verts, count);
             break;
         case 11:
-            hipLaunchKernelGGL((k_move_block<1024, 8192, LV_CAP11, 1>), dim3(count), dim3(1024), 0,
-                               s, g, verts, count);
+            if (shape.block_threads == 512)
+                hipLaunchKernelGGL((k_move_block<512, 8192, LV_CAP11, 1>), dim3(count), dim3(512),
+                                   0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_block<1024, 8192, LV_CAP11, 1>), dim3(count),
+                                   dim3(1024), 0, s, g, verts, count);
             break;
         default:
             break;
@@ -1929,7 +1945,7 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // ---------------------------------------------------------------------------
 // Large communities, partitioned (the default for comm_deg > kAggMidMax).
 // Each community's entries are split by the new id of their neighbour
-// community into key ranges (buckets of about kAggBucketTarget entries); the
+// community into key ranges (buckets of about the tuning's target entries); the
 // buckets are merged and sorted in LDS, and since bucket b's keys all precede
 // bucket b+1's, the bucket outputs in order are the community's sorted row.
 //   P0 count    new id of every entry (staged in entry order), bucket
@@ -1941,10 +1957,6 @@ void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uin
 // A bucket with more distinct keys than its table holds flags its community,
 // which is then merged by the global-table kernels above instead.
 // ---------------------------------------------------------------------------
-#ifndef LV_AGGTARGET
-#define LV_AGGTARGET 384
-#endif
-constexpr uint32_t kAggBucketTarget = LV_AGGTARGET;
 constexpr uint32_t kAggMaxBuckets = 4096;
 #ifndef LV_AGGBCAP
 #define LV_AGGBCAP 1024
@@ -1969,12 +1981,12 @@ __device__ __forceinline__ void agg_keys(const AggView &a, uint32_t nn, uint32_t
     }
 }
 
-// buckets of a large community: pow2 >= its entries / kAggBucketTarget, but
-// no more than its key range needs (span / 64 keys per bucket at least, so a
-// small graph's community with many entries and few distinct neighbours does
-// not dispatch hundreds of empty buckets)
+// buckets of a large community: pow2 >= its entries / target, but no more
+// than its key range needs (span / 64 keys per bucket at least, so a small
+// graph's community with many entries and few distinct neighbours does not
+// dispatch hundreds of empty buckets)
 __global__ void k_aggp_nb(AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
-                          uint32_t *nbk) {
+                          uint32_t target, uint32_t *nbk) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         const uint32_t c = comms[i];
@@ -1982,17 +1994,17 @@ __global__ void k_aggp_nb(AggView a, const uint32_t *comms, uint32_t count, uint
         uint32_t kb, span;
         agg_keys(a, nn, c, kb, span);
         uint32_t b = 1;
-        while (b < kAggMaxBuckets && (unsigned long long)b * kAggBucketTarget < d &&
+        while (b < kAggMaxBuckets && (unsigned long long)b * target < d &&
                (unsigned long long)b * 64u < span)
             b <<= 1;
         nbk[i] = b;
     }
 }
 void launch_aggp_nb(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count, uint32_t nn,
-                    uint32_t *nbk) {
+                    uint32_t target, uint32_t *nbk) {
     if (!count) return;
     hipLaunchKernelGGL(k_aggp_nb, dim3((count + 255) / 256), dim3(256), 0, s, a, comms, count,
-                       nn, nbk);
+                       nn, target, nbk);
 }
 
 // P0: new id of every entry of the chunk, staged in entry order, counted per

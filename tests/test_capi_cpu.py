@@ -249,4 +249,6 @@ def test_tuning_defaults(lv):
     t = lv.Tuning()
     assert lv.lib().lv_tuning_default(ctypes.byref(t)) == 0
     assert t.as_dict() == {"heavy_min_degree": 4096, "bucket_entries": 512, "locality": -1,
-                           "locality_min_vertices": 1 << 20}
+                           "locality_min_vertices": 1 << 20, "wave_vertices": 2,
+                           "block_threads": 1024, "class8_slots": 1280,
+                           "agg_bucket_entries": 384, "sweeps_per_launch": 1}

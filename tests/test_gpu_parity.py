@@ -424,6 +424,10 @@ def test_server_results_match_oracle(lv, orc, slots, budget):
     dict(heavy_min_degree=2048, bucket_entries=256, locality=1),
     dict(heavy_min_degree=4096, bucket_entries=512, locality=0),
     dict(heavy_min_degree=8192, bucket_entries=512, locality=1),
+    dict(wave_vertices=1, block_threads=512, class8_slots=2048, agg_bucket_entries=256),
+    dict(wave_vertices=4, heavy_min_degree=8192, block_threads=512, agg_bucket_entries=512,
+         sweeps_per_launch=2),
+    dict(sweeps_per_launch=4, locality=1),
 ])
 def test_tuning_never_changes_results(lv, orc, graphs, tune):
     """Launch-shape parameters (which kernel serves a bucket-7 row, heavy
@@ -506,6 +510,31 @@ def test_caller_stream(lv, orc, graphs):
         assert np.float32(r.q_final).view(np.uint32) == np.float32(r2.q_final).view(np.uint32)
     finally:
         hip.hipStreamDestroy(st)
+
+
+@pytest.mark.parametrize("k", [2, 4])
+def test_batch_sweeps_per_launch(lv, orc, bcsstk16_text, k):
+    """Several sweeps captured to one launch: each graph's stop rule is
+    applied on the device after every sweep, and the sweeps captured after
+    the last graph stopped return at entry; every graph still gets the
+    oracle's result."""
+    graphs = _mixed_batch(lv, orc, bcsstk16_text)
+    print(f"[case] batch of {len(graphs)} graphs, {k} sweeps per launch", flush=True)
+    with lv.Batch(graphs) as b:
+        b.set_tuning(sweeps_per_launch=k)
+        per, _ = b.run(1e-6)
+        assigns = b.assignments()
+    for i, (r, c, w) in enumerate(graphs):
+        o = orc.Oracle(r.astype(np.uint64), c, w)
+        try:
+            ores = o.run(1e-6)
+        except orc.OracleError:
+            assert per[i].status != 0, i
+            continue
+        assert np.float32(per[i].q_final).view(np.uint32) == \
+            np.float32(ores.q_final).view(np.uint32), i
+        assert (per[i].passes, per[i].total_iters) == (ores.passes, ores.total_iters), i
+        assert np.array_equal(assigns[i], o.init_comm()), i
 
 
 def test_batch_reports_graph_out_of_range(lv, orc):
