@@ -114,9 +114,10 @@ int lv_csr_write(const char *path, const lv_host_csr *csr);
 int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t *col,
                     const float *w, lv_graph **out);
 int lv_graph_destroy(lv_graph *g);
-/* Restore the graph as uploaded (device-to-device copy) and the identity
- * assignment, so the same resident input can be processed again.  No
- * counterpart in the reference, which runs one graph per process. */
+/* Restore the graph as uploaded (from the handle's pinned host copy of the
+ * input) and the identity assignment, so the same input can be processed
+ * again.  No counterpart in the reference, which runs one graph per
+ * process. */
 int lv_graph_reset(lv_graph *g);
 /* Locality order of each pass graph on the device: vertices grouped by degree
  * class so the state of high-degree vertices, which most entries point at,
@@ -197,7 +198,7 @@ int lv_run(lv_graph *g, float min_gain, lv_result *res);
  * work, tight serving loops, harnesses that forbid hipMalloc).  Every device
  * buffer is carved from a fixed 128 MiB array that the library holds on each
  * GPU it runs on until the process ends; graphs whose buffers do not fit it
- * (about 600k entries and 64k vertices, depending on shape) return
+ * (about 1.4M entries with 64k vertices, depending on shape) return
  * LV_ERR_LIMIT.  The graph is the host CSR of lv_graph_create (checked the
  * same way); assignment receives n original-vertex -> community ids; res the
  * figures of lv_run (alg_ms from the host clock).  Runs on `stream` (a
@@ -268,7 +269,9 @@ typedef struct lv_request_result {
     double t_end_ms;
 } lv_request_result;
 
-/* device bytes a graph of n vertices / ne entries may occupy while running */
+/* device bytes a graph of n vertices / ne entries may occupy while running
+ * (every buffer of its handle, the on-demand arenas at their worst case:
+ * 84.5 B per entry, 176 B per vertex, 4 MiB) */
 uint64_t lv_graph_footprint(uint32_t n, uint64_t ne);
 int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, float min_gain,
                      int keep_assignments, lv_server **out);

@@ -290,8 +290,7 @@ struct AggView {
     unsigned long long *tval;   // ne
     uint32_t *skey;         // ne (radix scratch)
     unsigned long long *sval;   // ne
-    uint32_t *gkey;         // global-hash arena
-    unsigned long long *gval;
+    unsigned long long *gtab;   // global-table arena (16-byte slots, shared with heavy rows)
     const unsigned long long *goff;  // per large community: table offsets (count+1)
     uint32_t *mdeg;         // n: degree of members[i]
     uint32_t *moff;         // n+1: exclusive scan of mdeg
@@ -345,8 +344,6 @@ void launch_agg_members(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts);
 void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uint32_t *cursor,
                                  uint32_t *order);
-void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
-                     unsigned long long *caps);
 // lists of the flagged large communities and of their chunks (counts at
 // flist[0] and clist[0], cleared before the launch)
 void launch_agg_flag_list(hipStream_t s, const uint32_t *flag, const uint32_t *coff,

@@ -77,8 +77,24 @@ __device__ __forceinline__ uint32_t hslot_n(uint32_t key) {
     return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * (unsigned)CAP) >> 32);
 }
 
-__device__ __forceinline__ uint32_t hslot_dyn(uint32_t key, int log2cap) {
-    return (key * 0x9E3779B1u) >> (32 - log2cap);
+// Global hash tables (heavy rows in a sweep, large communities in the
+// contraction; never live at the same time) share one arena of 16-byte slots
+// {u32 key, u32 zero, u64 sum}, so a probe and its sum share a cache line.
+// A table of d entries has exactly 2d slots (load factor <= 1/2 however many
+// keys are distinct); the home slot is hslot_n's multiply-high with the
+// capacity known at run time, and probing wraps at the capacity.  An empty
+// slot is {kEmpty, 0}: every user returns its slots to that state.
+__device__ __forceinline__ uint32_t *slot_key(unsigned long long *tab, uint32_t s) {
+    return reinterpret_cast<uint32_t *>(tab + 2ull * s);
+}
+__device__ __forceinline__ unsigned long long *slot_sum(unsigned long long *tab, uint32_t s) {
+    return tab + 2ull * s + 1ull;
+}
+__device__ __forceinline__ uint32_t hslot_cap(uint32_t key, uint32_t cap) {
+    return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * cap) >> 32);
+}
+__device__ __forceinline__ uint32_t next_slot(uint32_t s, uint32_t cap) {
+    return s + 1u == cap ? 0u : s + 1u;
 }
 
 template <int N>

@@ -82,6 +82,7 @@ enum : uint32_t { kGsActive = lv::kGraphActive, kGsWait = 2, kGsDone = 3 };
 struct Carver {
     char *base = nullptr;
     size_t cap = 0, used = 0;
+    bool count_only = false;  // only add up the sizes (no region behind it)
 };
 
 struct lv_graph {
@@ -106,7 +107,8 @@ struct lv_graph {
     float *w = nullptr;
     uint32_t *row2 = nullptr, *col2 = nullptr;
     float *w2 = nullptr;
-    // pristine copy of the uploaded graph (lv_graph_reset)
+    // pristine copy of the uploaded graph in pinned host memory
+    // (lv_graph_reset uploads it again); null on server and carved handles
     uint32_t *row0 = nullptr, *col0 = nullptr;
     float *w0 = nullptr;
     // pass state
@@ -128,8 +130,10 @@ struct lv_graph {
     ull *heavy_caps = nullptr, *heavy_off = nullptr;
     uint32_t *heavy_nch = nullptr, *heavy_coff = nullptr;
     uint32_t heavy_chunks = 0;
-    ull *htab = nullptr;  // heavy arena: 2 words per slot
-    size_t harena = 0;
+    // global-table arena, 16-byte slots: heavy rows' tables in sweeps, large
+    // communities' tables in the contraction (never live at the same time)
+    ull *gtab = nullptr;
+    size_t gtab_slots = 0;
     ull *hscr = nullptr;
     size_t hscr_cap = 0;
     // partitioned heavy path
@@ -160,9 +164,6 @@ struct lv_graph {
     uint32_t *ap_boff = nullptr, *ap_bowner = nullptr, *ap_btmp = nullptr;
     size_t ap_bcap = 0;
 
-    uint32_t *gkey = nullptr;
-    ull *gval = nullptr;
-    size_t garena = 0;
     uint32_t *scan_tmp32 = nullptr;
     ull *scan_tmp64 = nullptr;
     Counters *dc = nullptr;  // device
@@ -236,7 +237,7 @@ int galloc(lv_graph *g, T **p, size_t n) {
     Carver &c = *g->carve;
     const size_t off = (c.used + 255) & ~(size_t)255;
     if (n > (c.cap - std::min(c.cap, off)) / sizeof(T)) return LV_ERR_LIMIT;
-    *p = reinterpret_cast<T *>(c.base + off);
+    *p = c.count_only ? nullptr : reinterpret_cast<T *>(c.base + off);
     c.used = off + n * sizeof(T);
     return LV_OK;
 }
@@ -245,6 +246,19 @@ template <typename T>
 void gfree(lv_graph *g, T *&p) {
     if (!g->carve) dfree(p);
     p = nullptr;
+}
+
+// The global-table arena holds at least `slots` empty slots.  Its users leave
+// every slot they touch empty again, so it is filled only when it grows (on
+// the handle's stream, before the kernels that use it).
+int ensure_gtab(lv_graph *g, size_t slots) {
+    if (slots <= g->gtab_slots) return LV_OK;
+    gfree(g, g->gtab);
+    g->gtab_slots = 0;
+    if (int e = galloc(g, &g->gtab, 2 * slots)) return e;
+    g->gtab_slots = slots;
+    lv::launch_fill_slots(g->stream, g->gtab, slots);
+    return LV_OK;
 }
 
 lv::DevView view(lv_graph *g) {
@@ -444,13 +458,7 @@ int classify(lv_graph *g) {
         // owner of every chunk and bucket, so no kernel searches for it
         lv::launch_owner_fill(g->stream, g->heavy_coff, nh, g->hp_cvtx);
         lv::launch_owner_fill(g->stream, g->hp_bstart, nh, g->hp_bvtx);
-        if (total > g->harena) {
-            gfree(g, g->htab);
-            size_t cap = (size_t)total;
-            if (int e = galloc(g, &g->htab, 2 * cap)) return e;
-            g->harena = cap;
-            lv::launch_fill_slots(g->stream, g->htab, cap);
-        }
+        if (int e = ensure_gtab(g, (size_t)total)) return e;
         if ((size_t)nh * 130 > g->hscr_cap) {
             gfree(g, g->hscr);
             if (int e = galloc(g, &g->hscr, (size_t)nh * 130)) return e;
@@ -470,7 +478,7 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     h.cvtx = g->hp_cvtx;
     h.bvtx = g->hp_bvtx;
     h.slot_off = g->heavy_off;
-    h.tab = g->htab;
+    h.tab = g->gtab;
     h.dlist = g->dlist;
     h.eslot = g->eslot;
     h.hcnt = g->hscr;
@@ -861,8 +869,7 @@ lv::AggView aview(lv_graph *g) {
     a.tval = g->tval;
     a.skey = g->skey;
     a.sval = g->sval;
-    a.gkey = g->gkey;
-    a.gval = g->gval;
+    a.gtab = g->gtab;
     a.goff = g->agg_off;
     a.mdeg = g->agg_mdeg;
     a.moff = g->agg_moff;
@@ -966,16 +973,7 @@ int aggregate_large(lv_graph *g, const lv::DevView &v, const uint32_t *lc, uint3
     ull total = 0;
     LV_HIP(hipMemcpyAsync(&total, g->agg_off + nl, sizeof(ull), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
-    if (total > g->garena) {
-        gfree(g, g->gkey);
-        gfree(g, g->gval);
-        int e = galloc(g, &g->gkey, (size_t)total);
-        if (!e) e = galloc(g, &g->gval, (size_t)total);
-        if (e) return e;
-        g->garena = (size_t)total;
-        lv::launch_fill_u32(s, g->gkey, lv::kEmpty, total);
-        lv::launch_fill_u64(s, g->gval, 0ull, total);
-    }
+    if (int e = ensure_gtab(g, (size_t)total)) return e;
     // the global-table kernels take one workgroup per flagged community and
     // per chunk of one, from lists built on the device
     const size_t lcap = (size_t)g->agg_nchunks + nl + 2;
@@ -1330,9 +1328,10 @@ void free_all(lv_graph *g) {
     dfree(g->row2);
     dfree(g->col2);
     dfree(g->w2);
-    dfree(g->row0);
-    dfree(g->col0);
-    dfree(g->w0);
+    for (void *p : {(void *)g->row0, (void *)g->col0, (void *)g->w0})
+        if (p) (void)hipHostFree(p);
+    g->row0 = g->col0 = nullptr;
+    g->w0 = nullptr;
     dfree(g->k);
     dfree(g->comm);
     dfree(g->comm_sz);
@@ -1370,7 +1369,7 @@ void free_all(lv_graph *g) {
     dfree(g->ap_boff);
     dfree(g->ap_bowner);
     dfree(g->ap_btmp);
-    dfree(g->htab);
+    dfree(g->gtab);
     dfree(g->dlist);
     dfree(g->eslot);
     dfree(g->new_id);
@@ -1395,8 +1394,6 @@ void free_all(lv_graph *g) {
     g->agg_flcap = 0;
     dfree(g->agg_coff);
     dfree(g->agg_dcnt);
-    dfree(g->gkey);
-    dfree(g->gval);
     dfree(g->scan_tmp32);
     dfree(g->scan_tmp64);
     dfree(g->dc);
@@ -1534,12 +1531,13 @@ int check_input(uint32_t n, const uint32_t *row, const uint32_t *col, const floa
 //   heavy rows have more than 319 entries, so nh <= E/320;
 //   chunks   sum ceil(d/2048)           <= E/2048 + nh       < E/256
 //   buckets  sum pow2 >= d/64 (<= 2d/64) <= E/32 + nh        < E/16
-//   tables   sum pow2 >= 2d             <= 4E slots
+//   tables   sum 2d                     <= 2E slots
 //   large communities have more than 2048 member entries, so nl <= E/2048;
 //   merge buckets  sum (2d/256 + 1)     <= E/128 + nl  (the smallest target)
-//   merge tables   sum pow2 >= 2d       <= 4E
+//   merge tables   sum 2d               <= 2E slots (the heavy rows' arena)
 //   chunk list     sum ceil(d/2048) + nl + 2 <= E/1024 + 2 + nl.
-int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
+// The same walk over a counting region gives lv_graph_footprint.
+int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap) {
     g->n_cap = n_cap;
     g->e_cap = e_cap;
     const size_t N = (size_t)n_cap + 1, E = (size_t)e_cap;
@@ -1551,7 +1549,6 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
     // inside the last row and end past it
     A(&g->row, N), A(&g->col, E + 4), A(&g->w, E + 4);
     A(&g->row2, N), A(&g->col2, E + 4), A(&g->w2, E + 4);
-    if (pristine) A(&g->row0, N), A(&g->col0, E), A(&g->w0, E);
     A(&g->k, N), A(&g->comm, N), A(&g->comm_sz, N);
     A(&g->new_comm, N), A(&g->A, N), A(&g->init_comm, N);
     A(&g->order, N), A(&g->heavy_caps, N), A(&g->heavy_off, N + 1);
@@ -1573,32 +1570,43 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap, bool pristine) {
     A(&g->dc, 1);
     A(&g->pst, 1);
     if (e || !g->carve) return e;
-    const size_t hb = E / 16 + 1, hc = E / 256 + 1, slots = 4 * E + 1;
+    const size_t hb = E / 16 + 1, hc = E / 256 + 1, slots = 2 * E + 1;
     A(&g->hp_bcnt, hb), A(&g->hp_bpos, hb + 1), A(&g->hp_bcur, hb);
     A(&g->hp_btmp, lv::scan_tmp_elems((uint32_t)hb) + 8), A(&g->hp_bvtx, hb);
     A(&g->hp_cvtx, hc), A(&g->hp_flist, hc + 1);
-    A(&g->htab, 2 * slots);
+    A(&g->gtab, 2 * slots);
     A(&g->hscr, (E / 320 + 1) * 130);
     const size_t ab = E / 128 + E / 2048 + 2, lcap = E / 1024 + E / 2048 + 4;
     A(&g->ap_bcnt, ab), A(&g->ap_bpos, ab + 1), A(&g->ap_bcur, ab), A(&g->ap_bnd, ab);
     A(&g->ap_boff, ab + 1), A(&g->ap_bowner, ab);
     A(&g->ap_btmp, lv::scan_tmp_elems((uint32_t)ab + 1) + 8);
-    A(&g->gkey, slots), A(&g->gval, slots);
     A(&g->agg_flist, lcap);
     if (e) return e;
     g->hp_bcap = hb;
     g->hp_ccap = hc;
-    g->harena = slots;
+    g->gtab_slots = slots;
     g->hscr_cap = (E / 320 + 1) * 130;
     g->ap_bcap = ab;
-    g->garena = slots;
     g->agg_flcap = lcap;
     return LV_OK;
 }
 
+// Device bytes of a handle for up to n vertices and ne entries with every
+// on-demand arena at its worst case: graph_buffers over a counting region.
+uint64_t footprint(uint32_t n, uint64_t ne) {
+    Carver cv;
+    cv.cap = SIZE_MAX;
+    cv.count_only = true;
+    lv_graph g;
+    g.carve = &cv;
+    (void)graph_buffers(&g, n, ne);
+    return cv.used;
+}
+
 // A graph handle with its own streams and device buffers for up to n_cap
-// vertices and e_cap entries.
-int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
+// vertices and e_cap entries; with `pristine`, also pinned host memory for a
+// copy of the input that lv_graph_reset uploads again.
+int graph_new(int device, uint32_t n_cap, uint64_t e_cap, bool pristine, lv_graph **out) {
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
@@ -1622,7 +1630,16 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
         delete g;
         return LV_ERR_HIP;
     }
-    int e = graph_buffers(g, n_cap, e_cap, true);
+    int e = graph_buffers(g, n_cap, e_cap);
+    if (!e && pristine &&
+        (hipHostMalloc((void **)&g->row0, ((size_t)n_cap + 1) * sizeof(uint32_t)) != hipSuccess ||
+         hipHostMalloc((void **)&g->col0, std::max<size_t>(e_cap, 1) * sizeof(uint32_t)) !=
+             hipSuccess ||
+         hipHostMalloc((void **)&g->w0, std::max<size_t>(e_cap, 1) * sizeof(float)) !=
+             hipSuccess)) {
+        (void)hipGetLastError();
+        e = LV_ERR_NOMEM;
+    }
     if (!e && hipHostMalloc((void **)&g->pst_h, sizeof(lv::PassState)) != hipSuccess) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
@@ -1640,26 +1657,54 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out) {
     return LV_OK;
 }
 
+// The input graph (n0 vertices, ne0 entries, host arrays) as the working CSR,
+// with the identity assignment and, for a batch, its first vertex offsets.
+int upload_working(lv_graph *g, const uint32_t *row, const uint32_t *col, const float *w) {
+    hipStream_t s = g->stream;
+    const size_t N = (size_t)g->n0 + 1, E = (size_t)g->ne0;
+    LV_HIP(hipMemcpyAsync(g->row, row, N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (E) {
+        LV_HIP(hipMemcpyAsync(g->col, col, E * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(g->w, w, E * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    lv::launch_iota_u32(s, g->init_comm, g->n0);
+    if (g->G) {
+        g->voff = g->voff0;
+        LV_HIP(hipMemcpyAsync(g->voff_d, g->voff.data(), (g->G + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        lv::launch_batch_gid(s, g->voff_d, g->G, g->gid, g->n0);
+    }
+    LV_HIP(hipStreamSynchronize(s));
+    g->n = g->n0;
+    g->ne = g->ne0;
+    g->pass_ready = false;
+    g->relabeled = false;
+    return LV_OK;
+}
+
 // Upload a checked input (n <= n_cap, row[n] <= e_cap) as the graph of the
-// handle, with its pristine copy and the identity assignment.
+// handle, with the identity assignment: through the pristine host copy when
+// the handle keeps one, straight from the caller's arrays otherwise.
 int graph_upload(lv_graph *g, uint32_t n, const uint32_t *row, const uint32_t *col,
                  const float *w, int S) {
     const uint64_t ne = row[n];
     if (n > g->n_cap || ne > g->e_cap || g->G) return LV_ERR_NOMEM;
     LV_HIP(hipSetDevice(g->device));
-    hipStream_t s = g->stream;
-    const size_t N = (size_t)n + 1, E = (size_t)ne;
-    LV_HIP(hipMemcpyAsync(g->row0, row, N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (E) {
-        LV_HIP(hipMemcpyAsync(g->col0, col, E * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-        LV_HIP(hipMemcpyAsync(g->w0, w, E * sizeof(float), hipMemcpyHostToDevice, s));
-    }
     g->n = g->n0 = n;
     g->ne = g->ne0 = ne;
     g->S = S;
     g->scale = ldexpf(1.0f, S);
     g->inv_scale = ldexpf(1.0f, -S);
-    return lv_graph_reset(g);  // working copy from the pristine one, identity init_comm
+    if (g->row0) {
+        memcpy(g->row0, row, ((size_t)n + 1) * sizeof(uint32_t));
+        if (ne)
+            host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned) {
+                memcpy(g->col0 + a, col + a, (b - a) * sizeof(uint32_t));
+                memcpy(g->w0 + a, w + a, (b - a) * sizeof(float));
+            });
+        return lv_graph_reset(g);
+    }
+    return upload_working(g, row, col, w);
 }
 
 static_assert(sizeof(Counters) <= kHostCountersBytes, "host counter buffer");
@@ -1680,7 +1725,7 @@ int run_fixed(int device, char *base, size_t cap, void *host_counters, uint32_t 
     g.carve = &cv;
     g.stream = static_cast<hipStream_t>(stream);
     g.hc = static_cast<Counters *>(host_counters);
-    rc = graph_buffers(&g, n, ne, false);
+    rc = graph_buffers(&g, n, ne);
     if (rc) return rc;  // LV_ERR_LIMIT: the graph needs more than the region
     hipStream_t s = g.stream;
     LV_HIP(hipMemcpyAsync(g.row, row, ((size_t)n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
@@ -1697,9 +1742,7 @@ int run_fixed(int device, char *base, size_t cap, void *host_counters, uint32_t 
     // the region keeps the previous call's contents: everything that must
     // start empty is cleared here (the rest is written before it is read)
     lv::launch_iota_u32(s, g.init_comm, n);
-    lv::launch_fill_slots(s, g.htab, g.harena);
-    lv::launch_fill_u32(s, g.gkey, lv::kEmpty, g.garena);
-    lv::launch_fill_u64(s, g.gval, 0ull, g.garena);
+    lv::launch_fill_slots(s, g.gtab, g.gtab_slots);
     LV_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
     rc = run_passes(&g, min_gain, res);
@@ -1717,6 +1760,7 @@ int run_fixed(int device, char *base, size_t cap, void *host_counters, uint32_t 
 using lv_host::check_input;
 using lv_host::graph_new;
 using lv_host::graph_upload;
+using lv_host::upload_working;
 
 extern "C" {
 
@@ -1757,7 +1801,7 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
     int rc = check_input(n, row, col, w, &S);
     if (rc) return rc;
     lv_graph *g = nullptr;
-    rc = graph_new(device, n, row[n], &g);
+    rc = graph_new(device, n, row[n], true, &g);
     if (rc) return rc;
     rc = graph_upload(g, n, row, col, w, S);
     if (rc) {
@@ -1770,27 +1814,9 @@ int lv_graph_create(int device, uint32_t n, const uint32_t *row, const uint32_t 
 
 int lv_graph_reset(lv_graph *g) {
     if (!g) return LV_ERR_ARG;
+    if (!g->row0) return LV_ERR_STATE;  // a handle without a pristine copy
     LV_HIP(hipSetDevice(g->device));
-    hipStream_t s = g->stream;
-    const size_t N = (size_t)g->n0 + 1, E = (size_t)g->ne0;
-    LV_HIP(hipMemcpyAsync(g->row, g->row0, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    if (E) {
-        LV_HIP(hipMemcpyAsync(g->col, g->col0, E * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        LV_HIP(hipMemcpyAsync(g->w, g->w0, E * sizeof(float), hipMemcpyDeviceToDevice, s));
-    }
-    lv::launch_iota_u32(s, g->init_comm, g->n0);
-    if (g->G) {
-        g->voff = g->voff0;
-        LV_HIP(hipMemcpyAsync(g->voff_d, g->voff.data(), (g->G + 1) * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
-        lv::launch_batch_gid(s, g->voff_d, g->G, g->gid, g->n0);
-    }
-    LV_HIP(hipStreamSynchronize(s));
-    g->n = g->n0;
-    g->ne = g->ne0;
-    g->pass_ready = false;
-    g->relabeled = false;
-    return LV_OK;
+    return upload_working(g, g->row0, g->col0, g->w0);
 }
 
 int lv_graph_set_locality(lv_graph *g, int mode) {
@@ -2148,7 +2174,7 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
                 status[gi] = LV_ERR_RANGE;
     });
     lv_graph *g = nullptr;
-    int rc = graph_new(device, n, ne, &g);
+    int rc = graph_new(device, n, ne, true, &g);
     if (rc) return rc;
     rc = graph_upload(g, n, row, col, w, S);
     if (rc) {

@@ -38,20 +38,10 @@ __device__ __forceinline__ float f32_unord(uint32_t o) {
     return __uint_as_float(u);
 }
 
-// Table slots are 16 bytes {u32 key, u32 pad, u64 sum} so a probe and its sum
-// share one cache line.
-__device__ __forceinline__ uint32_t *tkey(unsigned long long *tab, uint32_t s) {
-    return reinterpret_cast<uint32_t *>(tab + 2ull * s);
-}
-__device__ __forceinline__ unsigned long long *tval(unsigned long long *tab, uint32_t s) {
-    return tab + 2ull * s + 1ull;
-}
-
 struct HeavyCtx {
     uint32_t i, v, base, d, c;
     unsigned long long off;
     uint32_t cap;
-    int lcap;
     bool live;
 };
 
@@ -67,7 +57,6 @@ __device__ __forceinline__ HeavyCtx heavy_ctx(const DevView &g, const HeavySplit
     x.d = g.row[x.v + 1] - x.base;
     x.off = h.slot_off[x.i];
     x.cap = (uint32_t)(h.slot_off[x.i + 1] - x.off);
-    x.lcap = 31 - __clz(x.cap);
     x.live = v_live(g, x.v);
     return x;
 }
@@ -140,23 +129,23 @@ __global__ __launch_bounds__(256) void k_heavy_insert(DevView g, HeavySplit h) {
     for (uint32_t i = threadIdx.x; i < nd; i += 256) {
         const uint32_t ls = ldist[i];
         const uint32_t ce = lkey[ls];
-        uint32_t slot = hslot_dyn(ce, x.lcap);
+        uint32_t slot = hslot_cap(ce, x.cap);
         uint32_t probe = 0;
         bool claimed = false;
         for (; probe < x.cap; ++probe) {
-            const uint32_t prev = atomicCAS(tkey(tab, slot), kEmpty, ce);
+            const uint32_t prev = atomicCAS(slot_key(tab, slot), kEmpty, ce);
             if (prev == kEmpty) {
                 claimed = true;
                 break;
             }
             if (prev == ce) break;
-            slot = (slot + 1) & (x.cap - 1);
+            slot = next_slot(slot, x.cap);
         }
-        if (probe == x.cap) {  // cannot happen: cap >= 2*deg
+        if (probe == x.cap) {  // cannot happen: cap = 2*deg
             atomicOr(h.err, 1u);
             slot = 0;
         }
-        atomicAdd(tval(tab, slot), lval[ls]);
+        atomicAdd(slot_sum(tab, slot), lval[ls]);
         lglob[ls] = slot;
         if (claimed) {
             my_new[nmine] = slot;
@@ -196,13 +185,13 @@ __global__ __launch_bounds__(256) void k_heavy_gain(DevView g, HeavySplit h) {
     const float acv = fix2f(g.A[cv], g.S);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256) {
         const uint32_t slot = h.dlist[x.base + i];
-        const uint32_t c = *tkey(tab, slot);
+        const uint32_t c = *slot_key(tab, slot);
         float gg = -INFINITY;
         // isLowerIdx (modularity.cu:85-93)
         const bool cand = lower_idx_ok(g, szv, c, cv);
         if (cand)
-            gg = gain_of(fix2f(*tval(tab, slot), g.S), v_m(g, x.v), kv, acv, fix2f(g.A[c], g.S));
-        *tval(tab, slot) = (unsigned long long)__float_as_uint(gg);
+            gg = gain_of(fix2f(*slot_sum(tab, slot), g.S), v_m(g, x.v), kv, acv, fix2f(g.A[c], g.S));
+        *slot_sum(tab, slot) = (unsigned long long)__float_as_uint(gg);
     }
     }
 }
@@ -221,7 +210,7 @@ __global__ __launch_bounds__(256) void k_heavy_lanes(DevView g, HeavySplit h) {
     float tg = -INFINITY;
     uint32_t tj = 0xffffffffu;
     for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
-        const float gg = __uint_as_float((uint32_t)*tval(tab, h.eslot[x.base + j]));
+        const float gg = __uint_as_float((uint32_t)*slot_sum(tab, h.eslot[x.base + j]));
         if (gg > tg) {  // entries visited in increasing order: first maximum
             tg = gg;
             tj = j;

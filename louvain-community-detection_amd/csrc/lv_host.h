@@ -11,8 +11,12 @@ namespace lv_host {
 
 // lv_graph_create's input checks; *S = fixed-point scale bits of the weights
 int check_input(uint32_t n, const uint32_t *row, const uint32_t *col, const float *w, int *S);
-// a handle with device buffers for up to n_cap vertices and e_cap entries
-int graph_new(int device, uint32_t n_cap, uint64_t e_cap, lv_graph **out);
+// device bytes of a handle for up to n vertices and ne entries, every arena
+// that grows on demand counted at its worst case
+uint64_t footprint(uint32_t n, uint64_t ne);
+// a handle with device buffers for up to n_cap vertices and e_cap entries;
+// with `pristine`, a pinned host copy of the input for lv_graph_reset
+int graph_new(int device, uint32_t n_cap, uint64_t e_cap, bool pristine, lv_graph **out);
 // upload a checked input that fits the handle's capacity (LV_ERR_NOMEM if not)
 int graph_upload(lv_graph *g, uint32_t n, const uint32_t *row, const uint32_t *col,
                  const float *w, int S);

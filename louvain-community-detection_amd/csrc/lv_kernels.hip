@@ -1074,16 +1074,13 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
     }
 }
 
-// heavy-class table capacities: pow2 >= 2*deg
+// heavy-class table capacities: 2*deg slots
 __global__ void k_heavy_caps(const uint32_t *row, const uint32_t *verts, uint32_t count,
                              unsigned long long *caps) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         uint32_t v = verts[i];
-        unsigned long long d = row[v + 1] - row[v];
-        unsigned long long c = 1;
-        while (c < 2 * d) c <<= 1;
-        caps[i] = c;
+        caps[i] = 2ull * (row[v + 1] - row[v]);
     }
 }
 void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts, uint32_t count,
@@ -1326,23 +1323,6 @@ void launch_agg_classify_scatter(hipStream_t s, const DevView &g, AggView a, uin
     hipLaunchKernelGGL(k_agg_class_scatter,
                        dim3((g.n + 256 * kScatterPer - 1) / (256 * kScatterPer)), dim3(256), 0, s,
                        g, a, cursor, order);
-}
-
-__global__ void k_agg_caps(AggView a, const uint32_t *comms, uint32_t count,
-                           unsigned long long *caps) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) {
-        unsigned long long d = a.comm_deg[comms[i]];
-        unsigned long long c = 1;
-        while (c < 2 * d) c <<= 1;
-        caps[i] = c;
-    }
-}
-void launch_agg_caps(hipStream_t s, AggView a, const uint32_t *comms, uint32_t count,
-                     unsigned long long *caps) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_agg_caps, dim3((count + 255) / 256), dim3(256), 0, s, a, comms, count,
-                       caps);
 }
 
 // Bitonic sort of n2 (pow2) (key, slot) pairs in LDS, ascending by key.
@@ -1738,32 +1718,30 @@ __global__ __launch_bounds__(256) void k_agg_insert_chunks(DevView g, AggView a,
     __syncthreads();
     const unsigned long long off = a.goff[ci];
     const uint32_t cap = (uint32_t)(a.goff[ci + 1] - off);
-    const int lcap = 31 - __clz(cap);
-    uint32_t *key = a.gkey + off;
-    unsigned long long *val = a.gval + off;
+    unsigned long long *tab = a.gtab + 2ull * off;
     const uint32_t nd = nld;
     uint32_t my_new[PER], my_rank[PER], my_key[PER];
     int nmine = 0;
     for (uint32_t i = threadIdx.x; i < nd; i += 256) {
         const uint32_t ls = ldist[i];
         const uint32_t dd = lkey[ls];
-        uint32_t slot = hslot_dyn(dd, lcap);
+        uint32_t slot = hslot_cap(dd, cap);
         uint32_t probe = 0;
         bool claimed = false;
         for (; probe < cap; ++probe) {
-            const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
+            const uint32_t prev = atomicCAS(slot_key(tab, slot), kEmpty, dd);
             if (prev == kEmpty) {
                 claimed = true;
                 break;
             }
             if (prev == dd) break;
-            slot = (slot + 1) & (cap - 1);
+            slot = next_slot(slot, cap);
         }
-        if (probe == cap) {  // cannot happen: cap >= 2*comm_deg
+        if (probe == cap) {  // cannot happen: cap = 2*comm_deg
             atomicOr(a.err, 1u);
             continue;
         }
-        atomicAdd(&val[slot], lval[ls]);
+        atomicAdd(slot_sum(tab, slot), lval[ls]);
         if (claimed) {
             my_new[nmine] = slot;
             my_key[nmine] = dd;  // sorted by new id
@@ -1796,9 +1774,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     __shared__ uint32_t wcnt[NW][256];
     const uint32_t ci = a.clist[1 + blockIdx.x];  // one workgroup per flagged community
     const uint32_t c = comms[ci];
-    const unsigned long long off = a.goff[ci];
-    uint32_t *key = a.gkey + off;
-    unsigned long long *val = a.gval + off;
+    unsigned long long *tab = a.gtab + 2ull * a.goff[ci];
     const uint32_t out = a.epos[c];
     uint32_t *ok = a.tkey + out;
     unsigned long long *ov = a.tval + out;
@@ -1809,9 +1785,9 @@ __global__ __launch_bounds__(kRadixThreads) void k_agg_merge_global(DevView g, A
     // gather sums and clean the arena
     for (uint32_t i = threadIdx.x; i < nd; i += P) {
         uint32_t slot = (uint32_t)ov[i];
-        ov[i] = val[slot];
-        val[slot] = 0ull;
-        key[slot] = kEmpty;
+        ov[i] = *slot_sum(tab, slot);
+        tab[2ull * slot] = (unsigned long long)kEmpty;
+        *slot_sum(tab, slot) = 0ull;
     }
     __syncthreads();
     uint32_t *kin = ok, *kout = a.skey + out;
@@ -2285,13 +2261,8 @@ __global__ void k_aggp_caps(AggView a, AggPart p, const uint32_t *comms, uint32_
                             unsigned long long *caps) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
-        unsigned long long cap = 0;
-        if (p.cflag[i]) {
-            const unsigned long long d = a.comm_deg[comms[i]];
-            cap = 1;
-            while (cap < 2 * d) cap <<= 1;
-        }
-        caps[i] = cap;
+        // 2*deg slots for a flagged community (large: deg > 2048), none else
+        caps[i] = p.cflag[i] ? 2ull * a.comm_deg[comms[i]] : 0ull;
     }
 }
 

@@ -168,7 +168,8 @@ void worker(lv_server *s) {
         if (fit != Fit::kReuse) {
             if (sl.g) lv_graph_destroy(sl.g);
             sl.g = nullptr;
-            rc = lv_host::graph_new(s->device, n_cap, e_cap, &sl.g);
+            // no pristine copy: every request is uploaded once and run once
+            rc = lv_host::graph_new(s->device, n_cap, e_cap, false, &sl.g);
             if (rc == LV_OK) {
                 sl.n_cap = n_cap;
                 sl.e_cap = e_cap;
@@ -211,9 +212,9 @@ void worker(lv_server *s) {
 extern "C" {
 
 uint64_t lv_graph_footprint(uint32_t n, uint64_t ne) {
-    // every buffer lv_graph_create allocates, plus the largest hash arenas a
-    // pass can request (bounded by 4x the entries)
-    return 188ull * ((uint64_t)n + 1) + 184ull * ne + (1ull << 20);
+    // every device buffer of a handle, the arenas that grow on demand at their
+    // worst case, plus allocation granularity and the captured sweep graphs
+    return lv_host::footprint(n, ne) + (4ull << 20);
 }
 
 int lv_server_create(int device, uint32_t slots, uint64_t hbm_budget_bytes, float min_gain,

@@ -184,6 +184,23 @@ def test_no_device_is_an_error(lv):
     assert e.value.rc == 8
 
 
+def test_footprint_bound(lv):
+    """lv_graph_footprint: every device buffer of a handle with the on-demand
+    arenas at their worst case.  Per CSR entry: two working CSRs (8 B each),
+    the heavy-row lists (8), the contraction's sort keys and sums (24), the
+    shared global tables at 2 slots of 16 B (32), bucket and lane scratch; the
+    pristine input stays on the host."""
+    f = lv.lib().lv_graph_footprint
+    n, ne = 1 << 20, 1 << 24
+    per_entry = (f(n, 2 * ne) - f(n, ne)) / ne
+    per_vertex = (f(2 * n, ne) - f(n, ne)) / n
+    assert 84 <= per_entry < 85, per_entry
+    assert per_vertex < 180, per_vertex
+    # the R-MAT scale-24 handle (16.8M vertices, 520.7M entries): under 48 GB
+    assert f(1 << 24, 520_749_004) < 48e9
+    assert f(100, 0) < 8 << 20  # fixed part: small arrays and slack
+
+
 def test_generators_deterministic(lv):
     a = lv.gen_fem(814, 25, 9, 1)
     b = lv.gen_fem(814, 25, 9, 1)
