@@ -78,6 +78,118 @@ def measured_traffic(workload, fam, launches_per_run):
     return (d["fetch"] + d["write"]) / launches_per_run
 
 
+# Summaries of rocprofv3 result databases (the text committed under
+# profiles/).  Kernel name -> the engine's kernel family (lv_profile_get);
+# scans, fills and copies shared by several phases stay unattributed.
+FAMILY_PATTERNS = [
+    ("relabel", r"k_relabel_(map|deg|rows|heavy)"),
+    ("aggregate", r"k_agg_|k_aggp_|k_scatter_by|k_relabel_newid"),
+    ("move_global", r"k_hp_|k_heavy_(insert|gain|lanes|decide|clean)"),
+    ("r1", r"k_r1<|k_r1v<|k_heavy_stage|k_heavy_r1"),
+    ("move_block", r"k_move_block"),
+    ("move_wave", r"k_move_wave"),
+    ("move_small", r"k_move_small"),
+    ("apply", r"k_apply"),
+    ("sumsq", r"k_sumsq"),
+    ("classify", r"k_classify_|k_heavy_caps|k_heavy_chunks|k_owner_fill"),
+]
+
+
+def family_of(name):
+    import re
+    for fam, pat in FAMILY_PATTERNS:
+        if re.search(pat, name):
+            return fam
+    return None
+
+
+def _kernel_short(name):
+    import re
+    return re.sub(r"\(.*", "", name)
+
+
+def summarize_kernels(db):
+    """Markdown: calls, total / average / max duration per kernel of a
+    `rocprofv3 --kernel-trace` database, then per family."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    ni = cols.index("name") if "name" in cols else cols.index("kernel_name")
+    si, ei = cols.index("start"), cols.index("end")
+    agg = {}
+    for r in con.execute("select * from kernels"):
+        a = agg.setdefault(_kernel_short(r[ni]), [0, 0.0, 0.0])
+        d = (r[ei] - r[si]) / 1e3
+        a[0] += 1
+        a[1] += d
+        a[2] = max(a[2], d)
+    tot = sum(a[1] for a in agg.values())
+    print("| kernel | calls | total us | avg us | max us | % |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for nm, a in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        if 100 * a[1] / tot < 0.05:
+            continue
+        print(f"| `{nm[:90]}` | {a[0]} | {a[1]:.1f} | {a[1] / a[0]:.2f} | {a[2]:.1f} | "
+              f"{100 * a[1] / tot:.2f} |")
+    print(f"\nkernel time {tot / 1e3:.2f} ms over {sum(a[0] for a in agg.values())} dispatches "
+          "(kernels under 0.05 % not listed)\n")
+    fam = {}
+    for nm, a in agg.items():
+        d = fam.setdefault(family_of(nm) or "(shared scans/fills/copies)", [0, 0.0])
+        d[0] += a[0]
+        d[1] += a[1]
+    print("| family | dispatches | total ms | avg us | % |")
+    print("|---|---:|---:|---:|---:|")
+    for f, d in sorted(fam.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {f} | {d[0]} | {d[1] / 1e3:.3f} | {d[1] / d[0]:.2f} | {100 * d[1] / tot:.2f} |")
+
+
+def _counter_sums(db, counter):
+    import sqlite3
+    con = sqlite3.connect(db)
+    tot, disp = {}, {}
+    for name, cn, val, did in con.execute(
+            "select name, counter_name, counter_value, dispatch_id from pmc_events"):
+        if cn != counter:
+            continue
+        nm = _kernel_short(name)
+        tot[nm] = tot.get(nm, 0.0) + float(val)
+        disp.setdefault(nm, set()).add(did)
+    return tot, {k: len(v) for k, v in disp.items()}
+
+
+def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, copy_bytes):
+    """Markdown + JSON: HBM bytes per kernel family from two counter runs of
+    the same command (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each
+    scaled by what the counter reads for a plain copy kernel that reads and
+    writes copy_bytes (its largest dispatch)."""
+    fetch, calls = _counter_sums(fetch_db, "FETCH_SIZE")
+    write, _ = _counter_sums(write_db, "WRITE_SIZE")
+    kf = max(_counter_sums(cfetch_db, "FETCH_SIZE")[0].values()) * 1024.0
+    kw = max(_counter_sums(cwrite_db, "WRITE_SIZE")[0].values()) * 1024.0
+    sf, sw = copy_bytes / kf, copy_bytes / kw
+    fam = {}
+    for nm in set(fetch) | set(write):
+        f = family_of(nm)
+        if f is None:
+            continue
+        d = fam.setdefault(f, {"fetch": 0.0, "write": 0.0, "dispatches": 0})
+        d["fetch"] += fetch.get(nm, 0.0) * 1024.0 * sf
+        d["write"] += write.get(nm, 0.0) * 1024.0 * sw
+        d["dispatches"] += calls.get(nm, 0)
+    print(f"copy calibration: FETCH x{sf:.4f}, WRITE x{sw:.4f} ({copy_bytes / 1e9:.3f} GB copy "
+          f"read {kf / 1e9:.3f} GB, wrote {kw / 1e9:.3f} GB)\n")
+    print("| family | dispatches | fetch GB | write GB |")
+    print("|---|---:|---:|---:|")
+    for f, d in sorted(fam.items(), key=lambda kv: -(kv[1]["fetch"] + kv[1]["write"])):
+        print(f"| {f} | {d['dispatches']} | {d['fetch'] / 1e9:.3f} | {d['write'] / 1e9:.3f} |")
+    print("\n```json")
+    print(json.dumps({"copy_scale": {"fetch": sf, "write": sw},
+                      "per_run_bytes": {f: {"fetch": d["fetch"], "write": d["write"]}
+                                        for f, d in sorted(fam.items())}}, indent=1))
+    print("```")
+
+
 def cpu_sample(o, threads, ne, res):
     """The oracle `o` (holding the workload's graph) on `threads` host threads:
     pass 1's initialisation and first sweep on the full graph, extrapolated to
@@ -127,6 +239,8 @@ def bench_plain(lv, W, cfg, args):
     t0 = time.perf_counter()
     g = lv.Graph(row, col, w)  # host checks, device buffers, PCIe upload
     create_s = time.perf_counter() - t0
+    if args.tuning:
+        g.set_tuning(**args.tuning)
     tuning = g.get_tuning()
     if not args.no_tune:  # untimed: candidate launch shapes on this graph, fastest kept
         t0 = time.perf_counter()
@@ -250,6 +364,8 @@ def bench_batch(lv, W, cfg, args):
     log(f"{cfg['workload']}: {len(graphs)} graphs, {nv} vertices, {ne} entries "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     b = lv.Batch(graphs)
+    if args.tuning:
+        b.set_tuning(**args.tuning)
     tuning = b.get_tuning()
     if not args.no_tune:
         t0 = time.perf_counter()
@@ -375,6 +491,17 @@ def bench_serve(lv, W, cfg, args):
     }
 
 
+def _tuning_arg(text):
+    """'heavy_min_degree=8192,bucket_entries=512' -> {field: int}"""
+    out = {}
+    for item in text.split(","):
+        k, _, v = item.partition("=")
+        if not k or not v:
+            raise argparse.ArgumentTypeError(f"expected FIELD=VALUE, got {item!r}")
+        out[k.strip()] = int(v)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,7 +520,26 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (by default it picks the launch shapes on the "
                          "resident graph or batch before the timed runs)")
+    ap.add_argument("--tuning", type=_tuning_arg, metavar="FIELD=V,...",
+                    help="launch shapes to run with (fields of lv_tuning, e.g. the 'tuning' "
+                         "object of an earlier bench line) instead of the autotuner's search")
+    ap.add_argument("--summarize-kernels", metavar="DB",
+                    help="print the per-kernel and per-family summary of a rocprofv3 "
+                         "--kernel-trace result database and exit (no GPU)")
+    ap.add_argument("--summarize-pmc", nargs=5,
+                    metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "BYTES"),
+                    help="print the per-family HBM traffic of two rocprofv3 --pmc runs "
+                         "(FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a copy "
+                         "kernel of BYTES read and written, and exit (no GPU)")
     args = ap.parse_args()
+    if args.tuning:
+        args.no_tune = True
+    if args.summarize_kernels:
+        summarize_kernels(args.summarize_kernels)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc[:4], float(args.summarize_pmc[4]))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU is design-only "
                                    "(graphs shard by HBM footprint, no collectives)",

@@ -764,7 +764,8 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
 }
 
 // Sweeps per launch of a pass loop decided on the device (the tuning's
-// sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).  With more than one, the kernels of the sweeps captured after
+// sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).  With
+// more than one, the kernels of the sweeps captured after
 // the stop must return at entry, which costs each kernel of every sweep a
 // dependent load of the halt flag; measured on MI355X that load costs about
 // what the host round trip it saves does (a launch of a captured sweep and
