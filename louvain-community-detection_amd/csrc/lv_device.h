@@ -157,11 +157,19 @@ struct HeavySplit {
 };
 
 // ---- device-side pass loop (lv_kernels.hip) ---------------------------------
-// After a sweep of one graph: Q from the collapsed exact counters
-// (tot[0] = S_in, tot[1..4] = sum a^2 limbs, tot[5] = moves), then the stop
-// rule into *ps; *err_flag is the sweep's capacity-overflow flag.
-void launch_sweep_decide(hipStream_t s, const unsigned long long *tot, const uint32_t *err_flag,
-                         int S, float m, float min_gain, uint32_t max_iters, PassState *ps);
+// After a sweep of one graph: the spread counters (6 groups of kSpread
+// slots from `spread`) collapsed into tot (tot[0] = S_in, tot[1..4] = sum a^2
+// limbs, tot[5] = moves), Q from them and the stop rule into *ps; *err is
+// the sweep's capacity-overflow flag.
+struct SweepTail {
+    const unsigned long long *spread;
+    unsigned long long *tot;
+    const uint32_t *err;
+    float m, min_gain;
+    uint32_t max_iters;
+    PassState *ps;
+};
+void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S);
 // Batch pass init, after the k_v sums: each active graph's m (a graph
 // without weight finishes with LV_ERR_RANGE) and its entries.  acc: per-graph
 // exact k totals; voff: vertex offsets, row: the pass's row pointers.
@@ -231,13 +239,21 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
                              const uint32_t *gid, const uint32_t *gstate, uint32_t run,
                              uint32_t *cursor /*kNumClasses*/, uint32_t *order);
 
-// pinned-order sums: mode 0 = k_v (+ optional state init), 1 = s_v, 2 = s_v
-// while every community is a singleton (pass init: no community gathers).
-// `group` lanes per vertex (4/8/16/32); with 32 lanes, `batch` (4 or 8)
-// rounds of loads are in flight at once.  `acc` receives the exact
-// fixed-point total (atomic).
-void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
-               int group, int batch, int masked, int init_state, unsigned long long *acc);
+// Pinned-order sums: mode (masked) 0 = k_v (+ optional state init),
+// 1 = s_v, 2 = s_v while every community is a singleton (pass init: no
+// community gathers); `acc` receives the exact fixed-point total (atomic).
+// The five class groups of the pinned-order sums (4, 8, 16 lanes per vertex,
+// 32 lanes for rows of at most 84 entries, 32 lanes for longer rows) in one
+// launch, longest rows first: group i covers count[i] vertices of verts[i]
+// with workgroups [bend[i+1], bend[i]) (bend[5] = 0; an empty group repeats
+// the bound; bend[0] is the grid size).
+struct R1Groups {
+    const uint32_t *verts[5];
+    uint32_t count[5];
+    uint32_t bend[5];
+};
+void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
+                      int init_state, unsigned long long *acc);
 // sum of round32(a_c)^2 over the communities in `verts`; limbs: 4 arrays of
 // kSpread (single graph) or G (batch) slots
 void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,

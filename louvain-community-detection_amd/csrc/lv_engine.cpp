@@ -512,8 +512,9 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 }
 
 // groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
-// 16 for 3, 32 for 4-10 (contiguous in `order`); rows of classes 4-5 (at
-// most 84 entries) take at most 3 rounds per lane, longer rows batch 8.
+// 16 for 3, 32 for 4-11 (contiguous in `order`); rows of classes 4-5 (at
+// most 84 entries) take at most 3 rounds per lane, longer rows batch 8.  All
+// five groups go out in one launch (k_r1_groups).
 // masked: 0 k_v, 1 s_v, 2 s_v with singleton communities (pass init).  The
 // masked sums skip class 0: an empty row adds nothing.
 int r1_all(lv_graph *g, int masked, int init_state) {
@@ -531,17 +532,27 @@ int r1_all(lv_graph *g, int masked, int init_state) {
                             acc_ptr(g));
     }
     const struct {
-        int G, B, c0, c1;
-    } grp[5] = {{4, 1, 0, 1}, {8, 1, 2, 2}, {16, 1, 3, 3}, {32, 4, 4, 5},
-                {32, 8, 6, lv::kClassHeavy - 1}};
-    for (const auto &q : grp) {
+        int G, c0, c1;
+    } grp[5] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 5}, {32, 6, lv::kClassHeavy - 1}};
+    lv::R1Groups L{};
+    uint32_t nb = 0;
+    double bytes = 0;
+    for (int i = 4; i >= 0; --i) {  // the grid lists the longest rows first
+        const auto &q = grp[i];
         const int c0 = (masked && q.c0 == 0) ? 1 : q.c0;
         uint32_t a = g->cls_off[c0], b = g->cls_off[q.c1 + 1];
         double ent = 0;
         for (int c = c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
-        ProfScope ps(g, LV_K_R1, (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a));
-        lv::launch_r1(g->stream, v, g->order + a, b - a, q.G, q.B, masked, init_state,
-                      acc_ptr(g));
+        const uint32_t vpb = q.G < 32 ? 256u / q.G : 8u;  // vertices per workgroup
+        L.verts[i] = g->order + a;
+        L.count[i] = b - a;
+        nb += (b - a + vpb - 1) / vpb;
+        L.bend[i] = nb;
+        bytes += (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a);
+    }
+    {
+        ProfScope ps(g, LV_K_R1, bytes);
+        lv::launch_r1_groups(g->stream, v, L, masked, init_state, acc_ptr(g));
     }
     if (fork) {
         (void)hipEventRecord(g->ev_join, g->stream2);
@@ -784,9 +795,15 @@ void sweep_enqueue_decide(lv_graph *g, float min_gain) {
     zero_counters(g);
     for (int b = 1; b <= 7; ++b) step(g, b);
     modularity_launch(g);
-    lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
-    lv::launch_sweep_decide(g->stream, g->dc->tot, &g->dc->err, g->S, g->m, min_gain, kMaxIters,
-                            g->pst);
+    lv::SweepTail t;
+    t.spread = g->dc->acc;
+    t.tot = g->dc->tot;
+    t.err = &g->dc->err;
+    t.m = g->m;
+    t.min_gain = min_gain;
+    t.max_iters = kMaxIters;
+    t.ps = g->pst;
+    lv::launch_collapse_decide(g->stream, t, g->S);
 }
 
 int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_print,

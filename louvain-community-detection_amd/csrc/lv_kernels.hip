@@ -42,8 +42,9 @@ static inline unsigned grid_for(size_t n, unsigned threads, unsigned cap = 65536
     return (unsigned)g;
 }
 // sums `groups` arrays of kSpread slots into out[0..groups)
-__global__ __launch_bounds__(256) void k_collapse(const unsigned long long *spread, int groups,
-                                                  unsigned long long *out) {
+// out[gi] = sum of the kSpread slots of group gi (one workgroup of 256)
+__device__ __forceinline__ void collapse_body(const unsigned long long *spread, int groups,
+                                              unsigned long long *out) {
     for (int gi = 0; gi < groups; ++gi) {
         unsigned long long x = 0;
         for (uint32_t i = threadIdx.x; i < kSpread; i += 256) x += spread[gi * kSpread + i];
@@ -54,6 +55,10 @@ __global__ __launch_bounds__(256) void k_collapse(const unsigned long long *spre
         if (threadIdx.x == 0) out[gi] = part[0] + part[1] + part[2] + part[3];
         __syncthreads();
     }
+}
+__global__ __launch_bounds__(256) void k_collapse(const unsigned long long *spread, int groups,
+                                                  unsigned long long *out) {
+    collapse_body(spread, groups, out);
 }
 void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
                      unsigned long long *out) {
@@ -352,14 +357,15 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
 // community is a singleton (comm[u] = u), so the masked sum compares columns
 // directly (`ident`) and gathers nothing.
 // ---------------------------------------------------------------------------
+// One workgroup of 256 threads: vertices [bx * 256/G, (bx + 1) * 256/G) of
+// verts.
 template <int G, bool MASKED, int B>
-__global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, uint32_t count,
-                                            int init_state, int ident,
-                                            unsigned long long *acc) {
-    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+__device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts, uint32_t count,
+                                         int init_state, int ident, unsigned long long *acc,
+                                         uint32_t bx) {
     constexpr int GPB = 256 / G;
     const int lane = threadIdx.x % G;
-    const uint32_t vi = blockIdx.x * GPB + threadIdx.x / G;
+    const uint32_t vi = bx * GPB + threadIdx.x / G;
     unsigned long long fixed = 0;
     uint32_t slot = 0;
     bool valid = false;
@@ -426,19 +432,20 @@ __global__ __launch_bounds__(256) void k_r1(DevView g, const uint32_t *verts, ui
 // the loading lane gathers the communities and loads the kept weights, and
 // the per-entry values are exchanged through LDS so that lane p adds entries
 // p, p+32, p+64, ... of the row in order — the same pinned sequence as
-// k_r1<32, true, B>. Positions outside the row contribute nothing (their
+// r1_group<32, true, B>. Positions outside the row contribute nothing (their
 // column stands in as v itself, always in bounds); non-kept entries add +0,
 // which leaves the partial unchanged. The column arrays carry 4 entries of
 // padding, so a 16-byte load that starts inside the last row stays inside
 // the allocation.
+// One workgroup of 256 threads: vertices [8 bx, 8 bx + 8) of verts.
 template <int BK>
-__global__ __launch_bounds__(256) void k_r1v(DevView g, const uint32_t *verts, uint32_t count,
-                                             int ident, unsigned long long *acc) {
-    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+__device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *verts,
+                                           uint32_t count, int ident, unsigned long long *acc,
+                                           uint32_t bx) {
     __shared__ float xs[8][BK * 128];
     const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
     float *xg = xs[grp];
-    const uint32_t vi = blockIdx.x * 8 + grp;
+    const uint32_t vi = bx * 8 + grp;
     unsigned long long fixed = 0;
     uint32_t slot = 0;
     bool valid = false;
@@ -507,32 +514,51 @@ __global__ __launch_bounds__(256) void k_r1v(DevView g, const uint32_t *verts, u
     block_add(g, acc, slot, fixed, valid);  // exact, order-free
 }
 
-void launch_r1(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count, int group,
-               int batch, int masked, int init_state, unsigned long long *acc) {
-    const int ident = masked == 2;
-    if (!count) return;
-    if (masked && group == 32) {
-        dim3 grid((count + 7) / 8), block(256);
-        if (batch == 8)
-            hipLaunchKernelGGL(k_r1v<2>, grid, block, 0, s, g, verts, count, ident, acc);
+// Every class group's pinned sums in one launch, longest rows first:
+// workgroups [0, bend[4]) take group 4 (rows of more than 84 entries, 32
+// lanes), [bend[4], bend[3]) group 3 (at most 84 entries, 32 lanes), then
+// group 2 (16 lanes), 1 (8 lanes) and 0 (4 lanes), so the longest
+// workgroups are dispatched first and the short ones fill in behind them.
+// The groups read the same snapshot and add into order-free exact counters,
+// so running them side by side in one grid changes no result; on small
+// graphs it replaces five dependent launches of a few microseconds each by
+// one.
+template <bool MASKED>
+__global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int init_state,
+                                                   int ident, unsigned long long *acc) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    const uint32_t b = blockIdx.x;
+    if (b < L.bend[4]) {
+        if (MASKED)
+            r1_group32<2>(g, L.verts[4], L.count[4], ident, acc, b);
         else
-            hipLaunchKernelGGL(k_r1v<1>, grid, block, 0, s, g, verts, count, ident, acc);
-        return;
+            r1_group<32, false, 8>(g, L.verts[4], L.count[4], init_state, 0, acc, b);
+    } else if (b < L.bend[3]) {
+        if (MASKED)
+            r1_group32<1>(g, L.verts[3], L.count[3], ident, acc, b - L.bend[4]);
+        else
+            r1_group<32, false, 4>(g, L.verts[3], L.count[3], init_state, 0, acc,
+                                   b - L.bend[4]);
+    } else if (b < L.bend[2]) {
+        r1_group<16, MASKED, 1>(g, L.verts[2], L.count[2], init_state, ident, acc,
+                                b - L.bend[3]);
+    } else if (b < L.bend[1]) {
+        r1_group<8, MASKED, 1>(g, L.verts[1], L.count[1], init_state, ident, acc, b - L.bend[2]);
+    } else {
+        r1_group<4, MASKED, 1>(g, L.verts[0], L.count[0], init_state, ident, acc, b - L.bend[1]);
     }
-    const unsigned gpb = 256 / group;
-    dim3 grid((count + gpb - 1) / gpb), block(256);
-#define LV_R1(GG, BB)                                                                         \
-    if (group == GG && (GG < 32 || batch == BB)) {                                            \
-        if (masked)                                                                           \
-            hipLaunchKernelGGL((k_r1<GG, true, BB>), grid, block, 0, s, g, verts, count,       \
-                               init_state, ident, acc);                                       \
-        else                                                                                  \
-            hipLaunchKernelGGL((k_r1<GG, false, BB>), grid, block, 0, s, g, verts, count,      \
-                               init_state, 0, acc);                                           \
-        return;                                                                               \
-    }
-    LV_R1(4, 1) LV_R1(8, 1) LV_R1(16, 1) LV_R1(32, 4) LV_R1(32, 8)
-#undef LV_R1
+}
+
+void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
+                      int init_state, unsigned long long *acc) {
+    const uint32_t nb = L.bend[0];
+    if (!nb) return;
+    if (masked)
+        hipLaunchKernelGGL(k_r1_groups<true>, dim3(nb), dim3(256), 0, s, g, L, init_state,
+                           masked == 2, acc);
+    else
+        hipLaunchKernelGGL(k_r1_groups<false>, dim3(nb), dim3(256), 0, s, g, L, init_state, 0,
+                           acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -2418,8 +2444,11 @@ __device__ __forceinline__ bool stop_rule(float q, unsigned long long moved, flo
     return stop;
 }
 
-__global__ void k_sweep_decide(const unsigned long long *tot, const uint32_t *err_flag, int S,
-                               float m, float min_gain, uint32_t max_iters, PassState *ps) {
+// The stop rule of the pass loop (modularity.cu:503-508) after a sweep,
+// from the collapsed exact counters tot = {S_in, sum a^2 limbs[4], moved}.
+__device__ __forceinline__ void sweep_decide(const unsigned long long *tot,
+                                             const uint32_t *err_flag, int S, float m,
+                                             float min_gain, uint32_t max_iters, PassState *ps) {
     if (ps->done) return;
     if (*err_flag) {
         ps->err = 1u;
@@ -2432,10 +2461,15 @@ __global__ void k_sweep_decide(const unsigned long long *tot, const uint32_t *er
         p.done = 1u;
     *ps = p;
 }
-void launch_sweep_decide(hipStream_t s, const unsigned long long *tot, const uint32_t *err_flag,
-                         int S, float m, float min_gain, uint32_t max_iters, PassState *ps) {
-    hipLaunchKernelGGL(k_sweep_decide, dim3(1), dim3(1), 0, s, tot, err_flag, S, m, min_gain,
-                       max_iters, ps);
+
+// After a sweep's sums: the spread counters (6 groups) collapsed into tot,
+// then the stop rule, in one workgroup (one launch instead of two).
+__global__ __launch_bounds__(256) void k_collapse_decide(SweepTail t, int S) {
+    collapse_body(t.spread, 6, t.tot);
+    if (threadIdx.x == 0) sweep_decide(t.tot, t.err, S, t.m, t.min_gain, t.max_iters, t.ps);
+}
+void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S) {
+    hipLaunchKernelGGL(k_collapse_decide, dim3(1), dim3(256), 0, s, t, S);
 }
 
 __global__ __launch_bounds__(256) void k_batch_m(uint32_t G, const unsigned long long *acc, int S,
