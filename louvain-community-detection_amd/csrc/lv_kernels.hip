@@ -1545,87 +1545,104 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     if (threadIdx.x == 0) a.cnt[a.new_id[c]] = nd;
 }
 
-// Small communities (comm_deg <= 128): one wave per community, four per
-// workgroup; every wave sorts a fixed 128-element array so the block-wide
-// barriers of the bitonic sort stay uniform.
+// Small communities (at most kAggSmallMax member entries): one wave each,
+// four per workgroup, with wave-private LDS tables, so the waves never wait
+// for each other.  The member entries are walked flat (two per lane, every
+// load before any insert), distinct neighbour communities (by new id) are
+// merged in an open-addressing table with exact sums, and each distinct key
+// lands at its rank among the community's keys (the count of smaller ones),
+// which is the ascending order without a sort.
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
                                                          const uint32_t *comms, uint32_t count) {
-    constexpr int CAP = 256, LC = 8, N = 128;
+    constexpr int CAP = 256, LC = 8, N = (int)kAggSmallMax;
+    static_assert(N == 128, "two flat entries per lane of a wave");
     __shared__ uint32_t skey_all[4][CAP];
     __shared__ unsigned long long sval_all[4][CAP];
     __shared__ uint32_t sk_all[4][N];
     __shared__ uint32_t ss_all[4][N];
+    __shared__ uint32_t mb_all[4][N + 1];
     __shared__ uint32_t nd_all[4];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t ci = blockIdx.x * 4 + wv;
+    if (ci >= count) return;  // wave-uniform; no workgroup barrier below
     uint32_t *key = skey_all[wv];
     unsigned long long *val = sval_all[wv];
-    uint32_t *sk = sk_all[wv], *ss = ss_all[wv];
+    uint32_t *sk = sk_all[wv], *ss = ss_all[wv], *mb = mb_all[wv];
     for (int i = lane; i < CAP; i += 64) {
         key[i] = kEmpty;
         val[i] = 0ull;
     }
-    for (int i = lane; i < N; i += 64) {
-        sk[i] = kEmpty;
-        ss[i] = 0;
-    }
     if (lane == 0) nd_all[wv] = 0;
-    const uint32_t ci = blockIdx.x * 4 + wv;
-    const bool active = ci < count;
-    const uint32_t c = active ? comms[ci] : 0u;
-    __syncthreads();
-    if (active) {
-        const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c];
-        for (uint32_t mi = 0; mi < msz; ++mi) {
-            const uint32_t v = a.members[m0 + mi];
-            const uint32_t b = g.row[v], e = g.row[v + 1];
-            for (uint32_t j = b + lane; j < e; j += 64) {
-                const uint32_t dd = a.vnid[g.col[j]];  // keyed by new id
-                const unsigned long long wf = f2fix(g.w[j], g.scale);
-                uint32_t slot = hslot<LC>(dd);
-                for (int probe = 0; probe < CAP; ++probe) {
-                    const uint32_t prev = atomicCAS(&key[slot], kEmpty, dd);
-                    if (prev == kEmpty) {
-                        const uint32_t idx = atomicAdd(&nd_all[wv], 1u);
-                        sk[idx] = dd;  // sorted by new id
-                        ss[idx] = slot;
-                        break;
-                    }
-                    if (prev == dd) break;
-                    slot = (slot + 1) & (CAP - 1);
-                }
-                atomicAdd(&val[slot], wf);
+    const uint32_t c = comms[ci];
+    // the community's member entries, flat (members have degree >= 1, so at
+    // most N of them): offsets of the members' rows relative to the first
+    const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c], ne = a.comm_deg[c];
+    const uint32_t f0 = a.moff[m0];
+    for (uint32_t q = lane; q <= msz; q += 64) mb[q] = a.moff[m0 + q] - f0;
+    wave_sync_lds();
+    // flat entries lane and lane + 64: every load of both before any insert
+    // (member, row start, column and weight, new id of the neighbour's
+    // community); positions past the end repeat entry 0 (ne >= 1)
+    uint32_t vv[2], jj[2], dd[2];
+    float ww[2];
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) {
+        const uint32_t ru = lane + 64u * u;
+        const uint32_t r = ru < ne ? ru : 0u;
+        uint32_t l = 0, h = msz;  // member of flat entry r
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (mb[mid] <= r) l = mid;
+            else h = mid;
+        }
+        vv[u] = a.members[m0 + l];
+        jj[u] = r - mb[l];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) jj[u] += g.row[vv[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) {
+        dd[u] = g.col[jj[u]];
+        ww[u] = g.w[jj[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) dd[u] = a.vnid[dd[u]];  // keyed by new id
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) {
+        if (lane + 64u * u >= ne) continue;
+        const uint32_t d2 = dd[u];
+        uint32_t slot = hslot<LC>(d2);
+        for (int probe = 0; probe < CAP; ++probe) {
+            const uint32_t prev = atomicCAS(&key[slot], kEmpty, d2);
+            if (prev == kEmpty) {
+                const uint32_t idx = atomicAdd(&nd_all[wv], 1u);
+                sk[idx] = d2;
+                ss[idx] = slot;
+                break;
             }
+            if (prev == d2) break;
+            slot = (slot + 1) & (CAP - 1);
         }
+        atomicAdd(&val[slot], f2fix(ww[u], g.scale));
     }
-    __syncthreads();
-    for (uint32_t kk = 2; kk <= (uint32_t)N; kk <<= 1) {
-        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = lane; i < (uint32_t)N; i += 64) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & kk) == 0;
-                    const uint32_t x = sk[i], y = sk[ixj];
-                    if ((x > y) == up) {
-                        sk[i] = y;
-                        sk[ixj] = x;
-                        const uint32_t t = ss[i];
-                        ss[i] = ss[ixj];
-                        ss[ixj] = t;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    wave_sync_lds();
+    const uint32_t nd = nd_all[wv];
+    const uint32_t out = a.epos[c];
+    // rank of each distinct key among the nd keys (all distinct): its place
+    // in ascending new-id order
+    for (uint32_t i = lane; i < nd; i += 64) {
+        const uint32_t k = sk[i];
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < nd; ++t) rank += sk[t] < k ? 1u : 0u;
+        a.tkey[out + rank] = k;
+        a.tval[out + rank] = val[ss[i]];
     }
-    if (active) {
-        const uint32_t nd = nd_all[wv];
-        const uint32_t out = a.epos[c];
-        for (uint32_t i = lane; i < nd; i += 64) {
-            a.tkey[out + i] = sk[i];
-            a.tval[out + i] = val[ss[i]];
-        }
-        if (lane == 0) a.cnt[a.new_id[c]] = nd;
-    }
+    if (lane == 0) a.cnt[a.new_id[c]] = nd;
 }
 
 // Large communities (comm_deg > 2048): their member entries form the flat
