@@ -1458,14 +1458,18 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
     __shared__ uint32_t skeys[MAXE];
     __shared__ uint32_t sslot[MAXE];
     __shared__ uint32_t bcur[NBIN], boffs[NBIN], oidx[MAXE + 1];
-    __shared__ uint32_t ndist, wsum[NW], maxbin;
+    __shared__ uint32_t ndist, wsum[NW], maxbin, kmin, kmax;
     uint32_t *mb = oidx;  // member entry offsets while merging, ranks afterwards
     const uint32_t c = comms[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < CAP; i += P) {
         key[i] = kEmpty;
         val[i] = 0ull;
     }
-    if (threadIdx.x == 0) ndist = 0;
+    if (threadIdx.x == 0) {
+        ndist = 0;
+        kmin = 0xffffffffu;
+        kmax = 0u;
+    }
     // the community's member entries, flat: members have degree >= 1, so at
     // most MAXE of them; offsets relative to the first
     const uint32_t m0 = a.mstart[c], msz = g.comm_sz[c], ne = a.comm_deg[c];
@@ -1527,8 +1531,28 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
         a.tkey[out + pos] = skeys[i];
         a.tval[out + pos] = val[sslot[i]];
     };
-    // new ids lie in [0, nn); oidx holds up to MAXE distinct keys
-    if (!lds_bin_rank<NBIN, P>(nd, 0u, a.nn, keyof, bcur, boffs, oidx, wsum, &maxbin, emit)) {
+    // the bins span the community's own key range (in a batch the keys of
+    // one graph's communities are a narrow slice of [0, nn))
+    {
+        uint32_t lo = 0xffffffffu, hi = 0u;
+        for (uint32_t i = threadIdx.x; i < nd; i += P) {
+            lo = min(lo, skeys[i]);
+            hi = max(hi, skeys[i]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, (uint32_t)__shfl_xor(lo, o, 64));
+            hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0 && lo <= hi) {
+            atomicMin(&kmin, lo);
+            atomicMax(&kmax, hi);
+        }
+        __syncthreads();
+    }
+    // oidx holds up to MAXE distinct keys (nd >= 1: the community has entries)
+    if (!lds_bin_rank<NBIN, P>(nd, kmin, kmax - kmin + 1u, keyof, bcur, boffs, oidx, wsum,
+                               &maxbin, emit)) {
         uint32_t n2 = 1;
         while (n2 < nd) n2 <<= 1;
         for (uint32_t i = nd + threadIdx.x; i < n2; i += P) {
