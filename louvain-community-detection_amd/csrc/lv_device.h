@@ -242,15 +242,18 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
 // Pinned-order sums: mode (masked) 0 = k_v (+ optional state init),
 // 1 = s_v, 2 = s_v while every community is a singleton (pass init: no
 // community gathers); `acc` receives the exact fixed-point total (atomic).
-// The five class groups of the pinned-order sums (4, 8, 16 lanes per vertex,
-// 32 lanes for rows of at most 84 entries, 32 lanes for longer rows) in one
-// launch, longest rows first: group i covers count[i] vertices of verts[i]
-// with workgroups [bend[i+1], bend[i]) (bend[5] = 0; an empty group repeats
-// the bound; bend[0] is the grid size).
+// The six class groups of the pinned-order sums (rows of at most 4, 8, 16,
+// 32 and 84 entries, and longer rows) in one launch, longest rows first:
+// group i covers count[i] vertices of verts[i] with workgroups
+// [bend[i+1], bend[i]) (bend[6] = 0; an empty group repeats the bound;
+// bend[0] is the grid size).  Vertices per workgroup: 256, 128, 64, 32, 32,
+// 8 (kR1GroupVertices).
+constexpr int kR1Groups = 6;
+constexpr uint32_t kR1GroupVertices[kR1Groups] = {256, 128, 64, 32, 32, 8};
 struct R1Groups {
-    const uint32_t *verts[5];
-    uint32_t count[5];
-    uint32_t bend[5];
+    const uint32_t *verts[kR1Groups];
+    uint32_t count[kR1Groups];
+    uint32_t bend[kR1Groups];
 };
 void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
                       int init_state, unsigned long long *acc);

@@ -511,10 +511,10 @@ lv::HeavySplit heavy_view(lv_graph *g) {
     return h;
 }
 
-// groups of lanes for the pinned-order sums: G=4 for classes 0-1, 8 for 2,
-// 16 for 3, 32 for 4-11 (contiguous in `order`); rows of classes 4-5 (at
-// most 84 entries) take at most 3 rounds per lane, longer rows batch 8.  All
-// five groups go out in one launch (k_r1_groups).
+// groups of the pinned-order sums: classes 0-1, 2, 3, 4, 5 (rows of at most
+// 84 entries, fewer lanes per vertex than the reference's 32 with the same
+// bits) and 6-11 (32 lanes, 8 rounds of loads in flight), all in one launch
+// (k_r1_groups).
 // masked: 0 k_v, 1 s_v, 2 s_v with singleton communities (pass init).  The
 // masked sums skip class 0: an empty row adds nothing.
 int r1_all(lv_graph *g, int masked, int init_state) {
@@ -531,19 +531,18 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         lv::launch_heavy_r1(g->stream2, v, heavy_view(g), (float *)g->eslot, masked, init_state,
                             acc_ptr(g));
     }
-    const struct {
-        int G, c0, c1;
-    } grp[5] = {{4, 0, 1}, {8, 2, 2}, {16, 3, 3}, {32, 4, 5}, {32, 6, lv::kClassHeavy - 1}};
+    // classes of each group (contiguous in `order`)
+    const int grp[lv::kR1Groups][2] = {{0, 1}, {2, 2}, {3, 3}, {4, 4}, {5, 5},
+                                       {6, lv::kClassHeavy - 1}};
     lv::R1Groups L{};
     uint32_t nb = 0;
     double bytes = 0;
-    for (int i = 4; i >= 0; --i) {  // the grid lists the longest rows first
-        const auto &q = grp[i];
-        const int c0 = (masked && q.c0 == 0) ? 1 : q.c0;
-        uint32_t a = g->cls_off[c0], b = g->cls_off[q.c1 + 1];
+    for (int i = lv::kR1Groups - 1; i >= 0; --i) {  // the grid lists the longest rows first
+        const int c0 = (masked && grp[i][0] == 0) ? 1 : grp[i][0];
+        const uint32_t a = g->cls_off[c0], b = g->cls_off[grp[i][1] + 1];
         double ent = 0;
-        for (int c = c0; c <= q.c1; ++c) ent += (double)g->cls_deg[c];
-        const uint32_t vpb = q.G < 32 ? 256u / q.G : 8u;  // vertices per workgroup
+        for (int c = c0; c <= grp[i][1]; ++c) ent += (double)g->cls_deg[c];
+        const uint32_t vpb = lv::kR1GroupVertices[i];
         L.verts[i] = g->order + a;
         L.count[i] = b - a;
         nb += (b - a + vpb - 1) / vpb;

@@ -426,6 +426,93 @@ __device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts
     block_add(g, acc, slot, fixed, valid);  // exact, order-free
 }
 
+// Pinned-order sums of rows of at most L entries (R = 1) or at most 32 R
+// entries (L = 32), with Q lanes per vertex: lane q holds virtual lanes q,
+// q + Q, ..., q + L - Q of the reference's 32-lane order.  Virtual lane l
+// adds positions l, l + 32, ... of the row sequentially; tree levels
+// L/2 .. Q pair two virtual lanes of one lane, the levels below Q are
+// shuffles among the Q lanes, and the levels at or above L add +0 (the row
+// has no entry there) and are skipped.  The additions and their association
+// are those of the 32-lane tree, so are the bits, with 32/Q times as many
+// vertices per wave in flight: these sums are bound by the latency of their
+// dependent loads (row, column, community, weight), not by their bytes.
+template <int L, int Q, int R, bool MASKED>
+__device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *verts, uint32_t count,
+                                          int init_state, int ident, unsigned long long *acc,
+                                          uint32_t bx) {
+    static_assert(L % Q == 0 && L <= 32 && (L & (L - 1)) == 0 && (R == 1 || L == 32),
+                  "lane shape");
+    constexpr int V = L / Q, VPB = 256 / Q;
+    const int q = threadIdx.x % Q;
+    const uint32_t vi = bx * VPB + threadIdx.x / Q;
+    unsigned long long fixed = 0;
+    uint32_t slot = 0;
+    bool valid = false;
+    const uint32_t v = vi < count ? verts[vi] : 0u;
+    if (vi < count && v_live(g, v)) {
+        const uint32_t base = g.row[v];
+        const uint32_t d = g.row[v + 1] - base;
+        const uint32_t cv = MASKED ? g.comm[v] : 0u;
+        // every load before the first add: columns (past the row end the
+        // row's first entry stands in), communities, then the weights of the
+        // kept entries only
+        bool keep[R][V];
+        uint32_t c[R][V];
+        float x[R][V];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const uint32_t pos = (uint32_t)(q + Q * i + 32 * r);
+                keep[r][i] = pos < d;
+                c[r][i] = MASKED ? g.col[base + (keep[r][i] ? pos : 0u)] : 0u;
+            }
+        if (MASKED && !ident) {  // ident: every community is a singleton, comm[u] = u
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int i = 0; i < V; ++i) c[r][i] = g.comm[c[r][i]];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                if (MASKED) keep[r][i] = keep[r][i] && c[r][i] == cv;
+                x[r][i] = 0.0f;
+                if (keep[r][i]) x[r][i] = g.w[base + (uint32_t)(q + Q * i + 32 * r)];
+            }
+        float part[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            part[i] = 0.0f;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (keep[r][i]) part[i] += x[r][i];
+        }
+#pragma unroll
+        for (int o = L / 2; o >= Q; o >>= 1)
+#pragma unroll
+            for (int i = 0; i < o / Q; ++i) part[i] += part[i + o / Q];
+        float p = part[0];
+#pragma unroll
+        for (int o = Q / 2; o > 0; o >>= 1) p += __shfl_down(p, o, Q);
+        if (q == 0) {
+            fixed = f2fix(p, g.scale);
+            slot = acc_slot(g, v);
+            valid = true;
+            if (!MASKED) {
+                g.k[v] = p;
+                if (init_state) {
+                    g.comm[v] = v;
+                    g.comm_sz[v] = 1u;
+                    g.A[v] = fixed;
+                }
+            }
+        }
+    }
+    block_add(g, acc, slot, fixed, valid);  // exact, order-free
+}
+
 // Masked sum of rows of more than 32 entries, 32 lanes per vertex: the
 // columns are read four per lane with 16-byte loads from the 4-aligned
 // position at or below the row start (BK blocks of 128 positions per round),
@@ -515,37 +602,39 @@ __device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *ver
 }
 
 // Every class group's pinned sums in one launch, longest rows first:
-// workgroups [0, bend[4]) take group 4 (rows of more than 84 entries, 32
-// lanes), [bend[4], bend[3]) group 3 (at most 84 entries, 32 lanes), then
-// group 2 (16 lanes), 1 (8 lanes) and 0 (4 lanes), so the longest
-// workgroups are dispatched first and the short ones fill in behind them.
-// The groups read the same snapshot and add into order-free exact counters,
-// so running them side by side in one grid changes no result; on small
-// graphs it replaces five dependent launches of a few microseconds each by
-// one.
+// workgroups [0, bend[5]) take group 5 (rows of more than 84 entries, 32
+// lanes), [bend[5], bend[4]) group 4 (33-84 entries), then groups 3 (17-32),
+// 2 (9-16), 1 (5-8) and 0 (at most 4 entries), so the longest workgroups
+// are dispatched first and the short ones fill in behind them.  Groups 0-4
+// take 1, 2, 4, 8 and 8 lanes per vertex (r1_narrow).  The groups read the
+// same snapshot and add into order-free exact counters, so running them
+// side by side in one grid changes no result; on small graphs it replaces
+// several dependent launches of a few microseconds each by one.
 template <bool MASKED>
 __global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int init_state,
                                                    int ident, unsigned long long *acc) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const uint32_t b = blockIdx.x;
-    if (b < L.bend[4]) {
+    if (b < L.bend[5]) {
         if (MASKED)
-            r1_group32<2>(g, L.verts[4], L.count[4], ident, acc, b);
+            r1_group32<2>(g, L.verts[5], L.count[5], ident, acc, b);
         else
-            r1_group<32, false, 8>(g, L.verts[4], L.count[4], init_state, 0, acc, b);
+            r1_group<32, false, 8>(g, L.verts[5], L.count[5], init_state, 0, acc, b);
+    } else if (b < L.bend[4]) {
+        r1_narrow<32, 8, 3, MASKED>(g, L.verts[4], L.count[4], init_state, ident, acc,
+                                    b - L.bend[5]);
     } else if (b < L.bend[3]) {
-        if (MASKED)
-            r1_group32<1>(g, L.verts[3], L.count[3], ident, acc, b - L.bend[4]);
-        else
-            r1_group<32, false, 4>(g, L.verts[3], L.count[3], init_state, 0, acc,
-                                   b - L.bend[4]);
+        r1_narrow<32, 8, 1, MASKED>(g, L.verts[3], L.count[3], init_state, ident, acc,
+                                    b - L.bend[4]);
     } else if (b < L.bend[2]) {
-        r1_group<16, MASKED, 1>(g, L.verts[2], L.count[2], init_state, ident, acc,
-                                b - L.bend[3]);
+        r1_narrow<16, 4, 1, MASKED>(g, L.verts[2], L.count[2], init_state, ident, acc,
+                                    b - L.bend[3]);
     } else if (b < L.bend[1]) {
-        r1_group<8, MASKED, 1>(g, L.verts[1], L.count[1], init_state, ident, acc, b - L.bend[2]);
+        r1_narrow<8, 2, 1, MASKED>(g, L.verts[1], L.count[1], init_state, ident, acc,
+                                   b - L.bend[2]);
     } else {
-        r1_group<4, MASKED, 1>(g, L.verts[0], L.count[0], init_state, ident, acc, b - L.bend[1]);
+        r1_narrow<4, 1, 1, MASKED>(g, L.verts[0], L.count[0], init_state, ident, acc,
+                                   b - L.bend[1]);
     }
 }
 
