@@ -269,7 +269,14 @@ struct MoveShape {
     uint32_t wave_vertices;  // bucket-6 kernel: vertices (one wave each) per workgroup: 1, 2, 4
     uint32_t block_threads;  // rows of 2049..8192 entries: 512 or 1024 threads per vertex
     uint32_t class8_slots;   // rows of 513..1024 entries: LDS table of 1280 or 2048 slots
+    // buckets 1-5: a class of at least this many vertices takes the narrow
+    // kernel (fewer lanes per vertex, more vertices per wave: faster once the
+    // class saturates the device), a smaller one the reference's lane count
+    // (shorter chain per vertex: faster while the device is not full)
+    uint32_t narrow_min;
 };
+// default of MoveShape::narrow_min
+constexpr uint32_t kNarrowMinVertices = 1u << 19;
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count, const MoveShape &shape);
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,

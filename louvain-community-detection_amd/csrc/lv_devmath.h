@@ -106,6 +106,14 @@ struct Log2<1> {
     static constexpr int value = 0;
 };
 
+// LDS writes of a wave's lanes visible to the wave's other lanes (a
+// wave-level barrier: for tables private to one wave, no workgroup barrier)
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // wave-level sum of u64
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);

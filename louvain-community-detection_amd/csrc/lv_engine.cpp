@@ -638,6 +638,14 @@ int pass_init(lv_graph *g, float *q0) {
     return LV_OK;
 }
 
+// classes of buckets 1-5 with at least this many vertices take the narrow
+// move kernels (LV_NARROW_MIN overrides the default; 0: always)
+uint32_t narrow_min() {
+    static const char *env = getenv("LV_NARROW_MIN");
+    static const uint32_t n = env ? (uint32_t)strtoul(env, nullptr, 10) : lv::kNarrowMinVertices;
+    return n;
+}
+
 // one bucket step: decide from the snapshot, then apply
 int step(lv_graph *g, int b) {
     lv::DevView v = view(g);
@@ -675,7 +683,7 @@ int step(lv_graph *g, int b) {
         else
             lv::launch_move_class(st, v, c, g->order + g->cls_off[c], g->cls_count[c],
                                   lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
-                                                g->tune.class8_slots});
+                                                g->tune.class8_slots, narrow_min()});
     }
     if (fork) {
         (void)hipEventRecord(g->ev_join, g->stream2);

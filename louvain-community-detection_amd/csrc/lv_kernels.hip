@@ -830,6 +830,141 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     }
 }
 
+// Buckets 1-5 with Q lanes per vertex instead of the reference's T: lane q
+// holds virtual lanes q, q + Q, ..., q + T - Q, and a wave holds 64/Q
+// vertices.  Round r inserts the entries at positions r T + l of every
+// virtual lane l into the vertex's LDS table, then (after a wave-level
+// barrier: a vertex's lanes share one wave) every virtual lane scores its
+// round-r entry on the sums of rounds <= r and keeps its first maximum —
+// exactly the T-lane kernel's per-lane state.  The winner over the virtual
+// lanes (gain desc, community asc) is a total order, so folding a lane's
+// virtual lanes before the shuffles changes nothing.  These kernels are bound
+// by the latency of their dependent loads, so more vertices per wave is
+// what makes them faster.
+template <int T, int Q, int ROUNDS, int CAP>
+__global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *verts,
+                                                     uint32_t count) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    static_assert(T % Q == 0 && 64 % Q == 0 && CAP > T * ROUNDS / 2, "lane shape");
+    constexpr int V = T / Q, GPB = 256 / Q, LC = Log2<CAP>::value;
+    __shared__ uint32_t skey[GPB * CAP];
+    __shared__ unsigned long long sval[GPB * CAP];
+    const int grp = threadIdx.x / Q, q = threadIdx.x % Q;
+    uint32_t *key = skey + grp * CAP;
+    unsigned long long *val = sval + grp * CAP;
+    for (int i = q; i < CAP; i += Q) {
+        key[i] = kEmpty;
+        val[i] = 0ull;
+    }
+    // groups past the end read the last vertex (count >= 1) and write nothing;
+    // the vertex state loads are unconditional so none of them waits on a branch
+    const uint32_t vi = blockIdx.x * GPB + grp;
+    const uint32_t v = verts[vi < count ? vi : count - 1u];
+    const bool active = vi < count && v_live(g, v);
+    // a workgroup with no live vertex (past the end, or only frozen graphs of
+    // a batch) skips the loads below; the barrier also orders the table
+    // clearing before the inserts
+    if (!__syncthreads_or(active)) return;
+    const float m = v_m(g, v);
+    const uint32_t base = g.row[v], rend = g.row[v + 1];
+    const uint32_t d = active ? rend - base : 0u;
+    const uint32_t cv = g.comm[v];
+    const uint32_t szv = g.comm_sz[cv];
+    const float kv = g.k[v];
+    const float acv = fix2f(g.A[cv], g.S);
+    float bg[V];
+    uint32_t bc[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        bg[i] = -INFINITY;
+        bc[i] = kEmpty;
+    }
+    unsigned long long cew = 0;
+    // every round's loads before the first insert, unconditionally
+    // (positions past the row end, or of an inactive vertex, load entry
+    // `base`, which exists: this class has entries)
+    uint32_t er[ROUNDS][V], cr[ROUNDS][V];
+    float wr[ROUNDS][V];
+    bool hs[ROUNDS][V];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const uint32_t j = (uint32_t)(r * T + q + Q * i);
+            hs[r][i] = active && j < d;
+            const uint32_t e = base + (hs[r][i] ? j : 0u);
+            er[r][i] = g.col[e];
+            wr[r][i] = g.w[e];
+        }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+#pragma unroll
+        for (int i = 0; i < V; ++i) cr[r][i] = g.comm[er[r][i]];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        uint32_t sl[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            sl[i] = 0;
+            if (!hs[r][i]) continue;
+            const uint32_t ce = cr[r][i];
+            const unsigned long long wf = f2fix(wr[r][i], g.scale);
+            uint32_t slot = hslot<LC>(ce);
+            for (int probe = 0; probe < CAP; ++probe) {
+                const uint32_t prev = atomicCAS(&key[slot], kEmpty, ce);
+                if (prev == kEmpty || prev == ce) break;
+                slot = (slot + 1) & (CAP - 1);
+            }
+            atomicAdd(&val[slot], wf);
+            if (ce == cv && er[r][i] != v) cew += wf;
+            sl[i] = slot;
+        }
+        wave_sync_lds();  // round r's inserts before its scores
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            if (!hs[r][i]) continue;
+            const uint32_t ce = cr[r][i];
+            // isLowerIdx (modularity.cu:85-93)
+            if (lower_idx_ok(g, szv, ce, cv)) {
+                const float gg =
+                    gain_of(fix2f(val[sl[i]], g.S), m, kv, acv, fix2f(g.A[ce], g.S));
+                if (gg > bg[i]) {  // the virtual lane's first maximum
+                    bg[i] = gg;
+                    bc[i] = ce;
+                }
+            }
+        }
+        if (ROUNDS > 1) wave_sync_lds();  // round r's scores before round r+1's inserts
+    }
+    float best_g = bg[0];
+    uint32_t best_c = bc[0];
+#pragma unroll
+    for (int i = 1; i < V; ++i)
+        if (better(bg[i], bc[i], best_g, best_c, g.lid)) {
+            best_g = bg[i];
+            best_c = bc[i];
+        }
+#pragma unroll
+    for (int o = Q / 2; o > 0; o >>= 1) {
+        const float og = __shfl_xor(best_g, o, Q);
+        const uint32_t oc = __shfl_xor(best_c, o, Q);
+        const unsigned long long ow = __shfl_xor(cew, o, Q);
+        if (better(og, oc, best_g, best_c, g.lid)) {
+            best_g = og;
+            best_c = oc;
+        }
+        cew += ow;
+    }
+    if (active && q == 0) {
+        uint32_t nc = cv;
+        if (best_g != -INFINITY) {
+            const float cewf = fix2f(cew, g.S);
+            if (best_g - (cewf / m) > 0.0f) nc = best_c;
+        }
+        g.new_comm[v] = nc;
+    }
+}
+
 // bucket 6 (84 < deg <= 319): one wave64 per vertex, LDS table per wave.
 // Physical lane p visits entries p, p+64, ...: even rounds belong to virtual
 // lane p, odd rounds to virtual lane p+64.
@@ -1122,25 +1257,48 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
                        uint32_t count, const MoveShape &shape) {
     if (!count) return;
     switch (cls) {
+        // large classes: T virtual lanes (the reference's) on Q lanes per
+        // vertex, 256/Q vertices per workgroup, 24.6 KB of LDS tables per
+        // workgroup; small classes: T lanes per vertex
         case 1:
-            hipLaunchKernelGGL((k_move_small<4, 1, 8>), dim3((count + 63) / 64), dim3(256), 0, s, g,
-                               verts, count);
+            if (count >= shape.narrow_min)
+                hipLaunchKernelGGL((k_move_narrow<4, 1, 1, 8>), dim3((count + 255) / 256),
+                                   dim3(256), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_small<4, 1, 8>), dim3((count + 63) / 64), dim3(256), 0,
+                                   s, g, verts, count);
             break;
         case 2:
-            hipLaunchKernelGGL((k_move_small<8, 1, 16>), dim3((count + 31) / 32), dim3(256), 0, s,
-                               g, verts, count);
+            if (count >= shape.narrow_min)
+                hipLaunchKernelGGL((k_move_narrow<8, 2, 1, 16>), dim3((count + 127) / 128),
+                                   dim3(256), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_small<8, 1, 16>), dim3((count + 31) / 32), dim3(256),
+                                   0, s, g, verts, count);
             break;
         case 3:
-            hipLaunchKernelGGL((k_move_small<16, 1, 32>), dim3((count + 15) / 16), dim3(256), 0, s,
-                               g, verts, count);
+            if (count >= shape.narrow_min)
+                hipLaunchKernelGGL((k_move_narrow<16, 4, 1, 32>), dim3((count + 63) / 64),
+                                   dim3(256), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_small<16, 1, 32>), dim3((count + 15) / 16), dim3(256),
+                                   0, s, g, verts, count);
             break;
         case 4:
-            hipLaunchKernelGGL((k_move_small<32, 1, 64>), dim3((count + 7) / 8), dim3(256), 0, s,
-                               g, verts, count);
+            if (count >= shape.narrow_min)
+                hipLaunchKernelGGL((k_move_narrow<32, 8, 1, 64>), dim3((count + 31) / 32),
+                                   dim3(256), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_small<32, 1, 64>), dim3((count + 7) / 8), dim3(256), 0,
+                                   s, g, verts, count);
             break;
         case 5:
-            hipLaunchKernelGGL((k_move_small<32, 3, LV_SMALLCAP5>), dim3((count + 7) / 8), dim3(256), 0, s,
-                               g, verts, count);
+            if (count >= shape.narrow_min)
+                hipLaunchKernelGGL((k_move_narrow<32, 16, 3, 128>), dim3((count + 15) / 16),
+                                   dim3(256), 0, s, g, verts, count);
+            else
+                hipLaunchKernelGGL((k_move_small<32, 3, LV_SMALLCAP5>), dim3((count + 7) / 8),
+                                   dim3(256), 0, s, g, verts, count);
             break;
         case 6:
             if (shape.wave_vertices == 1)
@@ -1665,11 +1823,6 @@ __global__ __launch_bounds__(P) void k_agg_merge_lds(DevView g, AggView a, const
 // merged in an open-addressing table with exact sums, and each distinct key
 // lands at its rank among the community's keys (the count of smaller ones),
 // which is the ascending order without a sort.
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 __global__ __launch_bounds__(256) void k_agg_merge_small(DevView g, AggView a,
                                                          const uint32_t *comms, uint32_t count) {
     constexpr int CAP = 256, LC = 8, N = (int)kAggSmallMax;

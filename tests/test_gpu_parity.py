@@ -378,6 +378,46 @@ def test_large_community_merge_paths(lv, orc, env):
     assert "merge paths ok" in r.stdout
 
 
+def test_narrow_move_kernels_everywhere(lv):
+    """Buckets 1-5 take the narrow move kernels (fewer lanes per vertex than
+    the reference's T) for classes of at least 2^19 vertices; LV_NARROW_MIN=0
+    sends every class there, so graphs with rows in every bucket, and a
+    batch, give the oracle's results through them -- in a child process so
+    the setting does not leak."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from loader import load_package; import oracle\n"
+        "lv = load_package()\n"
+        "qb = lambda q: int(np.float32(q).view(np.uint32))\n"
+        "graphs = [lv.gen_sbm(20000, 40, 20.0, 10.0, 61), lv.gen_fem(814, 25, 9, 62),\n"
+        "          lv.gen_chung_lu(60000, 9.4, 2.5, 420, 300, 20.0, 63),\n"
+        "          lv.gen_rmat(14, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 64)]\n"
+        "for r, c, w in graphs:\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    with lv.Graph(r, c, w) as g:\n"
+        "        res = g.run(1e-6); a = g.assignment()\n"
+        "    assert (qb(res.q_final), res.passes, res.total_iters) == \\\n"
+        "        (qb(ores.q_final), ores.passes, ores.total_iters)\n"
+        "    assert np.array_equal(a, o.init_comm())\n"
+        "small = [lv.gen_sbm(n, 8, 18.0, 6.0, 70 + i) for i, n in enumerate([300, 2000, 700, 5000])]\n"
+        "with lv.Batch(small) as b:\n"
+        "    per, tot = b.run(1e-6); assigns = b.assignments()\n"
+        "for i, (r, c, w) in enumerate(small):\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    assert (qb(per[i].q_final), per[i].passes) == (qb(ores.q_final), ores.passes)\n"
+        "    assert np.array_equal(assigns[i], o.init_comm())\n"
+        "print('narrow kernels ok')\n"
+    ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
+    print("[case] narrow move kernels for every class", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, LV_NARROW_MIN="0"),
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "narrow kernels ok" in r.stdout
+
+
 def _server_pool(lv):
     return [lv.gen_sbm(3000, 12, 14.0, 10.0, 31), lv.gen_sbm(20000, 40, 20.0, 10.0, 32),
             lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 33),
