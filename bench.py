@@ -85,12 +85,12 @@ FAMILY_PATTERNS = [
     ("relabel", r"k_relabel_(map|deg|rows|heavy)"),
     ("aggregate", r"k_agg_|k_aggp_|k_scatter_by|k_relabel_newid"),
     ("move_global", r"k_hp_|k_heavy_(insert|gain|lanes|decide|clean)"),
-    ("r1", r"k_r1<|k_r1v<|k_heavy_stage|k_heavy_r1"),
+    ("r1", r"k_r1_groups|k_heavy_stage|k_heavy_r1"),
     ("move_block", r"k_move_block"),
     ("move_wave", r"k_move_wave"),
-    ("move_small", r"k_move_small"),
+    ("move_small", r"k_move_small|k_move_narrow"),
     ("apply", r"k_apply"),
-    ("sumsq", r"k_sumsq"),
+    ("sumsq", r"k_sumsq|k_collapse_decide"),
     ("classify", r"k_classify_|k_heavy_caps|k_heavy_chunks|k_owner_fill"),
 ]
 

@@ -1082,7 +1082,23 @@ __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const ui
     float bg;
     uint32_t bc, bp;
     tree128(g0, c0, g1, c1, bg, bc, bp, g.lid);
-    if (active && p == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, cew);
+    // the destination is the community of the winning virtual lane's last
+    // entry (decide_lastedge); lane last % 64 gathered it already, as its
+    // entry last / 64, so a shuffle replaces the two dependent loads
+    bg = __shfl(bg, 0, 64);
+    bp = __shfl(bp, 0, 64);
+    const bool win = bg != -INFINITY && bp < d;
+    const uint32_t last = win ? last_entry(bp, d) : 0u;
+    uint32_t mine = ci[0];
+#pragma unroll
+    for (int i = 1; i < kWaveMaxPer; ++i)
+        if (last / 64u == (uint32_t)i) mine = ci[i];
+    const uint32_t dest = __shfl(mine, (int)(last & 63u), 64);
+    if (active && p == 0) {
+        uint32_t nc = cv;
+        if (win && bg - (fix2f(cew, g.S) / m) > 0.0f) nc = dest;
+        g.new_comm[v] = nc;
+    }
 }
 
 // Combine per-thread lane states into the 128 virtual lanes and run the
