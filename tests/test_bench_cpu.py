@@ -65,8 +65,8 @@ def test_pmc_summary_scales_by_the_copy_kernel(tmp_path):
     # as 512 KiB fetched (scale 2) and 1024 KiB written (scale 1)
     _pmc_db(f, "FETCH_SIZE", [("k_move_block<128, 1024>(x)", 100.0),
                               ("k_move_block<1280, 1024>(x)", 50.0),
-                              ("k_r1<4>(x)", 10.0), ("k_scan_block(x)", 7.0)])
-    _pmc_db(w, "WRITE_SIZE", [("k_move_block<128, 1024>(x)", 1.0), ("k_r1<4>(x)", 2.0)])
+                              ("k_r1_groups<true>(x)", 10.0), ("k_scan_block(x)", 7.0)])
+    _pmc_db(w, "WRITE_SIZE", [("k_move_block<128, 1024>(x)", 1.0), ("k_r1_groups<true>(x)", 2.0)])
     _pmc_db(cf, "FETCH_SIZE", [("small(x)", 1.0), ("copy(x)", 512.0)])
     _pmc_db(cw, "WRITE_SIZE", [("small(x)", 1.0), ("copy(x)", 1024.0)])
     r = _bench("--summarize-pmc", f, w, cf, cw, str(1 << 20))
