@@ -107,8 +107,8 @@ struct lv_graph {
     float *w = nullptr;
     uint32_t *row2 = nullptr, *col2 = nullptr;
     float *w2 = nullptr;
-    // pristine copy of the uploaded graph in pinned host memory
-    // (lv_graph_reset uploads it again); null on server and carved handles
+    // pristine copy of the uploaded graph in host memory (lv_graph_reset
+    // uploads it again); null on server and carved handles
     uint32_t *row0 = nullptr, *col0 = nullptr;
     float *w0 = nullptr;
     // pass state
@@ -1354,7 +1354,7 @@ void free_all(lv_graph *g) {
     dfree(g->col2);
     dfree(g->w2);
     for (void *p : {(void *)g->row0, (void *)g->col0, (void *)g->w0})
-        if (p) (void)hipHostFree(p);
+        free(p);
     g->row0 = g->col0 = nullptr;
     g->w0 = nullptr;
     dfree(g->k);
@@ -1656,14 +1656,11 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, bool pristine, lv_grap
         return LV_ERR_HIP;
     }
     int e = graph_buffers(g, n_cap, e_cap);
-    if (!e && pristine &&
-        (hipHostMalloc((void **)&g->row0, ((size_t)n_cap + 1) * sizeof(uint32_t)) != hipSuccess ||
-         hipHostMalloc((void **)&g->col0, std::max<size_t>(e_cap, 1) * sizeof(uint32_t)) !=
-             hipSuccess ||
-         hipHostMalloc((void **)&g->w0, std::max<size_t>(e_cap, 1) * sizeof(float)) !=
-             hipSuccess)) {
-        (void)hipGetLastError();
-        e = LV_ERR_NOMEM;
+    if (!e && pristine) {  // pageable: pinning gigabytes costs more than it saves a reset
+        g->row0 = static_cast<uint32_t *>(malloc(((size_t)n_cap + 1) * sizeof(uint32_t)));
+        g->col0 = static_cast<uint32_t *>(malloc(std::max<size_t>(e_cap, 1) * sizeof(uint32_t)));
+        g->w0 = static_cast<float *>(malloc(std::max<size_t>(e_cap, 1) * sizeof(float)));
+        if (!g->row0 || !g->col0 || !g->w0) e = LV_ERR_NOMEM;
     }
     if (!e && hipHostMalloc((void **)&g->pst_h, sizeof(lv::PassState)) != hipSuccess) {
         (void)hipGetLastError();
@@ -1721,13 +1718,18 @@ int graph_upload(lv_graph *g, uint32_t n, const uint32_t *row, const uint32_t *c
     g->scale = ldexpf(1.0f, S);
     g->inv_scale = ldexpf(1.0f, -S);
     if (g->row0) {
-        memcpy(g->row0, row, ((size_t)n + 1) * sizeof(uint32_t));
-        if (ne)
-            host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned) {
-                memcpy(g->col0 + a, col + a, (b - a) * sizeof(uint32_t));
-                memcpy(g->w0 + a, w + a, (b - a) * sizeof(float));
-            });
-        return lv_graph_reset(g);
+        // the pristine copy fills on host threads while the upload runs
+        std::thread copy([&] {
+            memcpy(g->row0, row, ((size_t)n + 1) * sizeof(uint32_t));
+            if (ne)
+                host_parallel(ne, [&](uint64_t a, uint64_t b, unsigned) {
+                    memcpy(g->col0 + a, col + a, (b - a) * sizeof(uint32_t));
+                    memcpy(g->w0 + a, w + a, (b - a) * sizeof(float));
+                });
+        });
+        const int rc = upload_working(g, row, col, w);
+        copy.join();
+        return rc;
     }
     return upload_working(g, row, col, w);
 }
