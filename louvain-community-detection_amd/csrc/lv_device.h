@@ -66,7 +66,7 @@ struct DevView {
     const float *w;         // ne
     uint32_t *comm;         // n
     uint32_t *comm_sz;      // n
-    uint32_t *new_comm;     // n
+    uint32_t *new_comm;     // n: a step's decisions (one copy of the state)
     unsigned long long *A;  // n: a_c * 2^S (exact)
     float *k;               // n
     uint32_t n;
@@ -88,6 +88,22 @@ struct DevView {
     // pass loop may have stopped on the device; when *halt is set the
     // kernels of the remaining captured sweeps return at entry.
     const uint32_t *halt;
+    // Two copies of the community state (small single graphs, where a
+    // separate apply launch per step costs more than its work; comm2 == null:
+    // one copy, new_comm and k_apply).  A bucket step reads its snapshot
+    // from (comm, comm_sz, A) and applies each of its moves at once to
+    // (comm2, comm_sz2, A2), which no kernel of the step reads; the next step
+    // reads that copy.  The copy a step writes still lacks the previous
+    // step's moves: the first cu_nblk workgroups of the step's first kernel
+    // apply them (the previous step's vertices cu_verts[0, cu_count), where
+    // the two copies can differ) and count them into cu_moved.  Pass init
+    // writes both copies (comm2 non-null); other kernels read one copy.
+    uint32_t *comm2;
+    uint32_t *comm_sz2;
+    unsigned long long *A2;
+    const uint32_t *cu_verts;
+    uint32_t cu_count, cu_nblk;
+    unsigned long long *cu_moved;
 };
 constexpr uint32_t kGraphActive = 1;
 constexpr uint32_t kGraphWait = 2;  // improved in this pass, waits for the contraction
@@ -277,8 +293,13 @@ struct MoveShape {
 };
 // default of MoveShape::narrow_min
 constexpr uint32_t kNarrowMinVertices = 1u << 19;
+// A class's move kernel; with g.cu_count set, catch-up workgroups for the
+// previous step's moves go in front of the class's workgroups.
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count, const MoveShape &shape);
+// The moves of g.cu_verts[0, g.cu_count) applied to the other copy on their own.
+void launch_catchup(hipStream_t s, const DevView &g);
+// One copy: the step's decisions (new_comm) of verts[0, count) applied in place.
 void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
                   unsigned long long *moved);
 void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts,

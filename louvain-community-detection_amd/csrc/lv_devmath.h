@@ -266,4 +266,84 @@ __device__ __forceinline__ void wave_add(unsigned long long *base, uint32_t key,
     }
 }
 
+// pass init (modularity.cu:459-463): v alone in its community, in both
+// copies of the community state when the view carries the second
+__device__ __forceinline__ void init_state_of(const DevView &g, uint32_t v,
+                                              unsigned long long fixed) {
+    g.comm[v] = v;
+    g.comm_sz[v] = 1u;
+    g.A[v] = fixed;
+    if (g.comm2) {
+        g.comm2[v] = v;
+        g.comm_sz2[v] = 1u;
+        g.A2[v] = fixed;
+    }
+}
+
+// ---- applying a step's moves (assignCommunity, modularity.cu:382-391, and
+// computeNewCommWeights, :393-398, as exact incremental updates) -------------
+
+// moves v (weighted degree kv) from community o to n in the copy (comm2,
+// comm_sz2, A2); integer atomics, so any order gives the same state
+__device__ __forceinline__ void apply_move2(const DevView &g, uint32_t v, float kv, uint32_t o,
+                                            uint32_t n) {
+    const unsigned long long K = f2fix(kv, g.scale);
+    atomicSub(&g.comm_sz2[o], 1u);
+    atomicAdd(&g.comm_sz2[n], 1u);
+    atomicAdd(&g.A2[o], (unsigned long long)(-(long long)K));
+    atomicAdd(&g.A2[n], K);
+    g.comm2[v] = n;
+}
+
+// A step's decision for v (community cv in the snapshot).  With two copies
+// (comm2 non-null) a move goes at once to the copy the step writes, which the
+// step's other vertices never read; otherwise the decision goes to new_comm
+// and k_apply applies the step's moves after it.
+__device__ __forceinline__ void commit_move(const DevView &g, uint32_t v, float kv, uint32_t cv,
+                                            uint32_t nc) {
+    if (!g.comm2)
+        g.new_comm[v] = nc;
+    else if (nc != cv)
+        apply_move2(g, v, kv, cv, nc);
+}
+
+// The previous step's moves, applied to the copy this step writes: that
+// copy holds the state before the previous step, the snapshot the state
+// after it, and only the previous step's vertices can differ between them.
+// Workgroup blk of the cu_nblk catch-up workgroups; every thread of the
+// workgroup calls it.  Counts the moves into cu_moved (the sweep's total).
+constexpr uint32_t kCatchUpPer = 4;  // vertices per thread and round
+__device__ inline void catch_up(const DevView &g, uint32_t blk) {
+    const uint32_t bd = blockDim.x, t = threadIdx.x, n = g.cu_count;
+    for (uint32_t b0 = blk * bd * kCatchUpPer; b0 < n; b0 += g.cu_nblk * bd * kCatchUpPer) {
+        // every load of the round is issued before the first use, none under
+        // a branch (positions past the end load the last vertex)
+        uint32_t vv[kCatchUpPer], o[kCatchUpPer], c[kCatchUpPer], gi[kCatchUpPer];
+        float kv[kCatchUpPer];
+#pragma unroll
+        for (uint32_t u = 0; u < kCatchUpPer; ++u) {
+            const uint32_t i = b0 + u * bd + t;
+            vv[u] = g.cu_verts[i < n ? i : n - 1u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kCatchUpPer; ++u) {
+            o[u] = g.comm2[vv[u]];
+            c[u] = g.comm[vv[u]];
+            kv[u] = g.k[vv[u]];
+            gi[u] = g.gid ? g.gid[vv[u]] : 0u;
+        }
+        unsigned long long moves = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kCatchUpPer; ++u) {
+            const bool mv = b0 + u * bd + t < n && o[u] != c[u];
+            if (mv) apply_move2(g, vv[u], kv[u], o[u], c[u]);
+            if (g.gid)
+                wave_add(g.cu_moved, gi[u], 1ull, mv);
+            else
+                moves += mv ? 1ull : 0ull;
+        }
+        if (!g.gid) block_add(g, g.cu_moved, 0u, moves, true);
+    }
+}
+
 }  // namespace lv

@@ -115,6 +115,11 @@ struct lv_graph {
     float *k = nullptr;
     uint32_t *comm = nullptr, *comm_sz = nullptr, *new_comm = nullptr;
     ull *A = nullptr;
+    // second copy of the community state (passes of small single graphs): a
+    // sweep's bucket steps read one copy and apply their moves to the other
+    // (both equal between sweeps)
+    uint32_t *comm2 = nullptr, *comm_sz2 = nullptr;
+    ull *A2 = nullptr;
     uint32_t *init_comm = nullptr;
     uint32_t *order = nullptr;
     // locality order of the current pass (position -> input id and inverse),
@@ -281,6 +286,12 @@ lv::DevView view(lv_graph *g) {
     v.G = g->G;
     v.lid = g->relabeled ? g->lid : nullptr;
     v.halt = !g->halt_on ? nullptr : g->G ? &g->bsum_d->halt : &g->pst->done;
+    v.comm2 = nullptr;
+    v.comm_sz2 = nullptr;
+    v.A2 = nullptr;
+    v.cu_verts = nullptr;
+    v.cu_count = v.cu_nblk = 0;
+    v.cu_moved = nullptr;
     return v;
 }
 
@@ -519,6 +530,11 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 // masked sums skip class 0: an empty row adds nothing.
 int r1_all(lv_graph *g, int masked, int init_state) {
     lv::DevView v = view(g);
+    if (init_state) {  // pass init sets both copies of the community state
+        v.comm2 = g->comm2;
+        v.comm_sz2 = g->comm_sz2;
+        v.A2 = g->A2;
+    }
     // the heavy rows' staged sums run on the second stream beside the class
     // groups (every group adds its exact total into the counters atomically)
     const bool fork = use_fork(g) && g->cls_count[lv::kClassHeavy] != 0 &&
@@ -646,23 +662,106 @@ uint32_t narrow_min() {
     return n;
 }
 
-// one bucket step: decide from the snapshot, then apply
-int step(lv_graph *g, int b) {
+// Whether the current pass sweeps with two copies of the community state
+// (moves applied inside the step kernels, no apply launch per step): single
+// graphs of at most 2^20 vertices, where a step's apply launch costs more
+// than its work (co-purchase graph 25.5 -> 22.5 ms per run).  Larger graphs
+// and batches keep one copy and k_apply: there the apply is real work that
+// a kernel of its own does faster (4096-graph batch +1.3 % with two copies).
+// LV_FUSED=0/1 forces one way (tests).
+bool fused_pass(const lv_graph *g) {
+    static const char *env = getenv("LV_FUSED");
+    if (env && *env) return atoi(env) != 0;
+    return !g->G && g->n <= (1u << 20);
+}
+
+// profiling family of each move class
+constexpr int kClassFamily[kNumClasses] = {
+    0,               LV_K_MOVE_SMALL, LV_K_MOVE_SMALL, LV_K_MOVE_SMALL, LV_K_MOVE_SMALL,
+    LV_K_MOVE_SMALL, LV_K_MOVE_WAVE,  LV_K_MOVE_BLOCK, LV_K_MOVE_BLOCK, LV_K_MOVE_BLOCK,
+    LV_K_MOVE_BLOCK, LV_K_MOVE_BLOCK, LV_K_MOVE_GLOBAL};
+
+// vertices of bucket step b in `order`: class b, or classes 7..heavy for b = 7
+void step_range(const lv_graph *g, int b, uint32_t *a, uint32_t *e) {
+    *a = g->cls_off[b];
+    *e = g->cls_off[(b == 7 ? lv::kClassHeavy : b) + 1];
+}
+
+// The view of a sweep's j-th non-empty bucket step: it reads copy j % 2 of
+// the community state (copy 0: comm, comm_sz, A) and applies its moves to
+// the other copy.
+lv::DevView step_view(lv_graph *g, uint32_t j) {
     lv::DevView v = view(g);
+    v.comm2 = g->comm2;
+    v.comm_sz2 = g->comm_sz2;
+    v.A2 = g->A2;
+    if (j & 1u) {
+        std::swap(v.comm, v.comm2);
+        std::swap(v.comm_sz, v.comm_sz2);
+        std::swap(v.A, v.A2);
+    }
+    v.cu_moved = moved_ptr(g);
+    return v;
+}
+
+// One bucket step, the sweep's j-th with vertices: every vertex decides from
+// the snapshot (copy j % 2) and its move goes to the other copy at once.
+// That copy still lacks the previous step's moves (vertices [pa, pe) of
+// `order`, none for the sweep's first step): workgroups in front of the
+// step's first class kernel apply them.
+int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
+    const lv::DevView v = step_view(g, j);
     const int c0 = b, c1 = (b == 7) ? lv::kClassHeavy : b;
-    static const int fam[kNumClasses] = {0,
-                                         LV_K_MOVE_SMALL,
-                                         LV_K_MOVE_SMALL,
-                                         LV_K_MOVE_SMALL,
-                                         LV_K_MOVE_SMALL,
-                                         LV_K_MOVE_SMALL,
-                                         LV_K_MOVE_WAVE,
-                                         LV_K_MOVE_BLOCK,
-                                         LV_K_MOVE_BLOCK,
-                                         LV_K_MOVE_BLOCK,
-                                         LV_K_MOVE_BLOCK,
-                                         LV_K_MOVE_BLOCK,
-                                         LV_K_MOVE_GLOBAL};
+    lv::DevView cu = v;  // carries the catch-up to the first class kernel
+    cu.cu_verts = g->order + pa;
+    cu.cu_count = pe - pa;
+    bool caught = pe == pa;
+    bool block_cls = false;
+    for (int c = c0; c < lv::kClassHeavy && c <= c1; ++c) block_cls |= g->cls_count[c] != 0;
+    // only the heavy class has vertices: its chain catches up on its own
+    if (!caught && !block_cls) {
+        ProfScope ps(g, LV_K_APPLY, 16.0 * (pe - pa));
+        lv::launch_catchup(g->stream, cu);
+        caught = true;
+    }
+    // the heavy rows' chain runs on the second stream beside the block
+    // classes of the same step (they only read the step-start snapshot, and
+    // their moves and the catch-up's are order-free atomic updates of the
+    // other copy)
+    const bool fork = c1 == lv::kClassHeavy && g->cls_count[lv::kClassHeavy] && block_cls &&
+                      use_fork(g);
+    if (fork) {
+        (void)hipEventRecord(g->ev_fork, g->stream);
+        (void)hipStreamWaitEvent(g->stream2, g->ev_fork, 0);
+    }
+    for (int c = c0; c <= c1; ++c) {
+        if (!g->cls_count[c]) continue;
+        hipStream_t st = (fork && c == lv::kClassHeavy) ? g->stream2 : g->stream;
+        const bool first = !caught && c != lv::kClassHeavy;
+        const double cub = first ? 16.0 * (pe - pa) : 0.0;
+        ProfScope ps(g, kClassFamily[c],
+                     12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c] + cub, st);
+        if (c == lv::kClassHeavy)
+            lv::launch_heavy_move(st, v, heavy_view(g));
+        else
+            lv::launch_move_class(st, first ? cu : v, c, g->order + g->cls_off[c],
+                                  g->cls_count[c],
+                                  lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
+                                                g->tune.class8_slots, narrow_min()});
+        caught |= first;
+    }
+    if (fork) {
+        (void)hipEventRecord(g->ev_join, g->stream2);
+        (void)hipStreamWaitEvent(g->stream, g->ev_join, 0);
+    }
+    return LV_OK;
+}
+
+// One bucket step on one copy of the state: the step's kernels write their
+// decisions to new_comm, then k_apply applies them in place.
+int step_apply(lv_graph *g, int b) {
+    const lv::DevView v = view(g);
+    const int c0 = b, c1 = (b == 7) ? lv::kClassHeavy : b;
     // the heavy rows' chain runs on the second stream beside the block
     // classes of the same step (they only read the step-start snapshot and
     // write new_comm of their own vertices)
@@ -677,7 +776,7 @@ int step(lv_graph *g, int b) {
     for (int c = c0; c <= c1; ++c) {
         if (!g->cls_count[c]) continue;
         hipStream_t st = (fork && c == lv::kClassHeavy) ? g->stream2 : g->stream;
-        ProfScope ps(g, fam[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c], st);
+        ProfScope ps(g, kClassFamily[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c], st);
         if (c == lv::kClassHeavy)
             lv::launch_heavy_move(st, v, heavy_view(g));
         else
@@ -697,11 +796,38 @@ int step(lv_graph *g, int b) {
     return LV_OK;
 }
 
+// A sweep's bucket steps 1-7 (those with vertices).  Two copies: each step
+// reads the copy its predecessor wrote, then the last step's moves are
+// applied to the copy it read, so both copies hold the sweep's result, as
+// they held its start.
+void sweep_steps(lv_graph *g) {
+    if (!fused_pass(g)) {
+        for (int b = 1; b <= 7; ++b) step_apply(g, b);
+        return;
+    }
+    uint32_t j = 0, pa = 0, pe = 0;
+    for (int b = 1; b <= 7; ++b) {
+        uint32_t a, e;
+        step_range(g, b, &a, &e);
+        if (e == a) continue;
+        step_fused(g, b, j, pa, pe);
+        pa = a;
+        pe = e;
+        ++j;
+    }
+    if (!j) return;
+    lv::DevView v = step_view(g, j);  // reads the copy the last step wrote
+    v.cu_verts = g->order + pa;
+    v.cu_count = pe - pa;
+    ProfScope ps(g, LV_K_APPLY, 16.0 * (pe - pa));
+    lv::launch_catchup(g->stream, v);
+}
+
 // the device work of one sweep, ending with the counters' D2H; no host sync
 // (this sequence is what a pass captures into a HIP graph)
 void sweep_enqueue(lv_graph *g) {
     zero_counters(g);
-    for (int b = 1; b <= 7; ++b) step(g, b);
+    sweep_steps(g);
     modularity_launch(g);
     counters_d2h(g);
 }
@@ -800,7 +926,7 @@ uint32_t sweeps_per_launch(const lv_graph *g) {
 // one sweep whose end applies the stop rule on the device (no host read)
 void sweep_enqueue_decide(lv_graph *g, float min_gain) {
     zero_counters(g);
-    for (int b = 1; b <= 7; ++b) step(g, b);
+    sweep_steps(g);
     modularity_launch(g);
     lv::SweepTail t;
     t.spread = g->dc->acc;
@@ -1181,7 +1307,7 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
             zero_counters(g);
             lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->bsum_d), 0u,
                                 lv::kBatchSummaryCounted);
-            for (int b = 1; b <= 7; ++b) step(g, b);
+            sweep_steps(g);
             modularity_launch(g);
             lv::launch_batch_decide(g->stream, G, g->gcnt, &g->dc->err, g->S, min_gain,
                                     kMaxIters, g->voff_d, g->gstate_d, g->gst_d, g->bsum_d);
@@ -1362,6 +1488,9 @@ void free_all(lv_graph *g) {
     dfree(g->comm_sz);
     dfree(g->new_comm);
     dfree(g->A);
+    dfree(g->comm2);
+    dfree(g->comm_sz2);
+    dfree(g->A2);
     dfree(g->init_comm);
     dfree(g->order);
     dfree(g->lid);
@@ -1576,6 +1705,7 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap) {
     A(&g->row2, N), A(&g->col2, E + 4), A(&g->w2, E + 4);
     A(&g->k, N), A(&g->comm, N), A(&g->comm_sz, N);
     A(&g->new_comm, N), A(&g->A, N), A(&g->init_comm, N);
+    A(&g->comm2, N), A(&g->comm_sz2, N), A(&g->A2, N);
     A(&g->order, N), A(&g->heavy_caps, N), A(&g->heavy_off, N + 1);
     A(&g->lid, N), A(&g->pmap, N), A(&g->rtmp, N), A(&g->rrank, N);
     A(&g->heavy_nch, N), A(&g->heavy_coff, N + 1);
@@ -1988,7 +2118,18 @@ int lv_step(lv_graph *g, int bucket, uint64_t *moved) {
     if (!g->pass_ready) return LV_ERR_STATE;
     LV_HIP(hipSetDevice(g->device));
     zero_counters(g);
-    step(g, bucket);
+    uint32_t a, e;
+    step_range(g, bucket, &a, &e);
+    if (!fused_pass(g)) {
+        step_apply(g, bucket);
+    } else if (e > a) {
+        // the step, then its moves applied to the copy it read
+        step_fused(g, bucket, 0, 0, 0);
+        lv::DevView v = step_view(g, 1);
+        v.cu_verts = g->order + a;
+        v.cu_count = e - a;
+        lv::launch_catchup(g->stream, v);
+    }
     int rc = sync_counters(g);
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;

@@ -264,7 +264,8 @@ __global__ __launch_bounds__(256) void k_heavy_decide(DevView g, HeavySplit h) {
     tree128(gl[0], cl[0], gl[1], cl[1], bg, bc, bp, g.lid);
     if (p == 0) {
         const uint32_t cv = g.comm[v];
-        g.new_comm[v] = decide_lastedge(g, v_m(g, v), base, d, cv, bg, bp, h.hcew[i]);
+        commit_move(g, v, g.k[v], cv,
+                    decide_lastedge(g, v_m(g, v), base, d, cv, bg, bp, h.hcew[i]));
     }
 }
 
@@ -842,9 +843,7 @@ __global__ __launch_bounds__(256) void k_heavy_r1(DevView g, HeavySplit h, const
             if (!masked) {
                 g.k[v] = p;
                 if (init_state) {
-                    g.comm[v] = v;
-                    g.comm_sz[v] = 1u;
-                    g.A[v] = fixed;
+                    init_state_of(g, v, fixed);
                 }
             }
             if (fixed) atomicAdd(&acc[acc_slot(g, v)], fixed);

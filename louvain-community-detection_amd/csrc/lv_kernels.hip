@@ -416,9 +416,7 @@ __device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts
             if (!MASKED) {
                 g.k[v] = p;
                 if (init_state) {
-                    g.comm[v] = v;
-                    g.comm_sz[v] = 1u;
-                    g.A[v] = fixed;
+                    init_state_of(g, v, fixed);
                 }
             }
         }
@@ -503,9 +501,7 @@ __device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *vert
             if (!MASKED) {
                 g.k[v] = p;
                 if (init_state) {
-                    g.comm[v] = v;
-                    g.comm_sz[v] = 1u;
-                    g.A[v] = fixed;
+                    init_state_of(g, v, fixed);
                 }
             }
         }
@@ -732,6 +728,8 @@ template <int T, int ROUNDS, int CAP>
 __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *verts,
                                                     uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     constexpr int GPB = 256 / T;
     constexpr int LC = Log2<CAP>::value;
     __shared__ uint32_t skey[GPB * CAP];
@@ -745,7 +743,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     }
     // groups past the end read the last vertex (count >= 1) and write nothing;
     // the vertex state loads are unconditional so none of them waits on a branch
-    const uint32_t vi = blockIdx.x * GPB + grp;
+    const uint32_t vi = bid * GPB + grp;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
@@ -826,7 +824,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
             float cewf = fix2f(cew, g.S);
             if (best_g - (cewf / m) > 0.0f) nc = best_c;
         }
-        g.new_comm[v] = nc;
+        commit_move(g, v, kv, cv, nc);
     }
 }
 
@@ -845,6 +843,8 @@ template <int T, int Q, int ROUNDS, int CAP>
 __global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *verts,
                                                      uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     static_assert(T % Q == 0 && 64 % Q == 0 && CAP > T * ROUNDS / 2, "lane shape");
     constexpr int V = T / Q, GPB = 256 / Q, LC = Log2<CAP>::value;
     __shared__ uint32_t skey[GPB * CAP];
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *
     }
     // groups past the end read the last vertex (count >= 1) and write nothing;
     // the vertex state loads are unconditional so none of them waits on a branch
-    const uint32_t vi = blockIdx.x * GPB + grp;
+    const uint32_t vi = bid * GPB + grp;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
@@ -961,7 +961,7 @@ __global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *
             const float cewf = fix2f(cew, g.S);
             if (best_g - (cewf / m) > 0.0f) nc = best_c;
         }
-        g.new_comm[v] = nc;
+        commit_move(g, v, kv, cv, nc);
     }
 }
 
@@ -978,6 +978,8 @@ template <int kWaveWpb>
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
                                                              uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     __shared__ uint32_t skey[kWaveWpb * kWaveCap];
     __shared__ unsigned long long sval[kWaveWpb * kWaveCap];
     __shared__ uint16_t sdist[kWaveWpb][320];
@@ -995,7 +997,7 @@ __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const ui
         scew[wv] = 0;
     }
     // waves past the end read the last vertex (count >= 1) and write nothing
-    const uint32_t vi = blockIdx.x * kWaveWpb + wv;
+    const uint32_t vi = bid * kWaveWpb + wv;
     const uint32_t v = verts[vi < count ? vi : count - 1u];
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
@@ -1097,7 +1099,7 @@ __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const ui
     if (active && p == 0) {
         uint32_t nc = cv;
         if (win && bg - (fix2f(cew, g.S) / m) > 0.0f) nc = dest;
-        g.new_comm[v] = nc;
+        commit_move(g, v, kv, cv, nc);
     }
 }
 
@@ -1147,6 +1149,8 @@ template <int P, int MAXDEG, int CAP, int MINB = 1>
 __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_t *verts,
                                                         uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     static_assert(CAP > MAXDEG && CAP <= 65536 && 3 * P * 4 <= CAP * 8, "table shape");
     constexpr int MAXPER = (MAXDEG + P - 1) / P;
     __shared__ unsigned long long val[CAP];
@@ -1166,7 +1170,7 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
         ndist = 0;
         scew = 0;
     }
-    const uint32_t v = verts[blockIdx.x];
+    const uint32_t v = verts[bid];
     if (!v_live(g, v)) return;  // uniform over the workgroup
     const float m = v_m(g, v);
     const uint32_t base = g.row[v];
@@ -1239,7 +1243,7 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
     uint32_t bc = kEmpty, bp = kEmpty;
     __syncthreads();  // the lane states overwrite the sums read above
     block_argmax<P>(tg, tc, tj, lg, lc, lj, bg, bc, bp, g.lid);
-    if (t == 0) g.new_comm[v] = decide_lastedge(g, m, base, d, cv, bg, bp, scew);
+    if (t == 0) commit_move(g, v, kv, cv, decide_lastedge(g, m, base, d, cv, bg, bp, scew));
 }
 
 #ifndef LV_CAP7
@@ -1269,94 +1273,81 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
 #ifndef LV_SMALLCAP5
 #define LV_SMALLCAP5 256
 #endif
+uint32_t catchup_blocks(uint32_t count, uint32_t threads);
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count, const MoveShape &shape) {
     if (!count) return;
+    // catch-up workgroups in front of the class's own
+    auto go = [&](auto kern, uint32_t grid, uint32_t threads) {
+        DevView gv = g;
+        gv.cu_nblk = g.cu_count ? catchup_blocks(g.cu_count, threads) : 0u;
+        hipLaunchKernelGGL(kern, dim3(grid + gv.cu_nblk), dim3(threads), 0, s, gv, verts, count);
+    };
     switch (cls) {
         // large classes: T virtual lanes (the reference's) on Q lanes per
         // vertex, 256/Q vertices per workgroup, 24.6 KB of LDS tables per
         // workgroup; small classes: T lanes per vertex
         case 1:
             if (count >= shape.narrow_min)
-                hipLaunchKernelGGL((k_move_narrow<4, 1, 1, 8>), dim3((count + 255) / 256),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_narrow<4, 1, 1, 8>, (count + 255) / 256, 256);
             else
-                hipLaunchKernelGGL((k_move_small<4, 1, 8>), dim3((count + 63) / 64), dim3(256), 0,
-                                   s, g, verts, count);
+                go(k_move_small<4, 1, 8>, (count + 63) / 64, 256);
             break;
         case 2:
             if (count >= shape.narrow_min)
-                hipLaunchKernelGGL((k_move_narrow<8, 2, 1, 16>), dim3((count + 127) / 128),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_narrow<8, 2, 1, 16>, (count + 127) / 128, 256);
             else
-                hipLaunchKernelGGL((k_move_small<8, 1, 16>), dim3((count + 31) / 32), dim3(256),
-                                   0, s, g, verts, count);
+                go(k_move_small<8, 1, 16>, (count + 31) / 32, 256);
             break;
         case 3:
             if (count >= shape.narrow_min)
-                hipLaunchKernelGGL((k_move_narrow<16, 4, 1, 32>), dim3((count + 63) / 64),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_narrow<16, 4, 1, 32>, (count + 63) / 64, 256);
             else
-                hipLaunchKernelGGL((k_move_small<16, 1, 32>), dim3((count + 15) / 16), dim3(256),
-                                   0, s, g, verts, count);
+                go(k_move_small<16, 1, 32>, (count + 15) / 16, 256);
             break;
         case 4:
             if (count >= shape.narrow_min)
-                hipLaunchKernelGGL((k_move_narrow<32, 8, 1, 64>), dim3((count + 31) / 32),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_narrow<32, 8, 1, 64>, (count + 31) / 32, 256);
             else
-                hipLaunchKernelGGL((k_move_small<32, 1, 64>), dim3((count + 7) / 8), dim3(256), 0,
-                                   s, g, verts, count);
+                go(k_move_small<32, 1, 64>, (count + 7) / 8, 256);
             break;
         case 5:
             if (count >= shape.narrow_min)
-                hipLaunchKernelGGL((k_move_narrow<32, 16, 3, 128>), dim3((count + 15) / 16),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_narrow<32, 16, 3, 128>, (count + 15) / 16, 256);
             else
-                hipLaunchKernelGGL((k_move_small<32, 3, LV_SMALLCAP5>), dim3((count + 7) / 8),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_small<32, 3, LV_SMALLCAP5>, (count + 7) / 8, 256);
             break;
         case 6:
             if (shape.wave_vertices == 1)
-                hipLaunchKernelGGL(k_move_wave<1>, dim3(count), dim3(64), 0, s, g, verts, count);
+                go(k_move_wave<1>, count, 64);
             else if (shape.wave_vertices == 4)
-                hipLaunchKernelGGL(k_move_wave<4>, dim3((count + 3) / 4), dim3(256), 0, s, g,
-                                   verts, count);
+                go(k_move_wave<4>, (count + 3) / 4, 256);
             else
-                hipLaunchKernelGGL(k_move_wave<2>, dim3((count + 1) / 2), dim3(128), 0, s, g,
-                                   verts, count);
+                go(k_move_wave<2>, (count + 1) / 2, 128);
             break;
         case 7:
-            hipLaunchKernelGGL((k_move_block<256, 512, LV_CAP7>), dim3(count), dim3(256), 0, s, g, verts,
-                               count);
+            go(k_move_block<256, 512, LV_CAP7>, count, 256);
             break;
         case 8:
             if (shape.class8_slots == 2048)
-                hipLaunchKernelGGL((k_move_block<256, 1024, 2048, 4>), dim3(count), dim3(256), 0, s,
-                                   g, verts, count);
+                go(k_move_block<256, 1024, 2048, 4>, count, 256);
             else
-                hipLaunchKernelGGL((k_move_block<256, 1024, LV_CAP8, LV_MINB8>), dim3(count),
-                                   dim3(256), 0, s, g, verts, count);
+                go(k_move_block<256, 1024, LV_CAP8, LV_MINB8>, count, 256);
             break;
         case 9:
-            hipLaunchKernelGGL((k_move_block<256, 2048, LV_CAP9>), dim3(count), dim3(256), 0, s, g, verts,
-                               count);
+            go(k_move_block<256, 2048, LV_CAP9>, count, 256);
             break;
         case 10:
             if (shape.block_threads == 512)
-                hipLaunchKernelGGL((k_move_block<512, 4096, LV_CAP10, LV_MINB10>), dim3(count),
-                                   dim3(512), 0, s, g, verts, count);
+                go(k_move_block<512, 4096, LV_CAP10, LV_MINB10>, count, 512);
             else
-                hipLaunchKernelGGL((k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>), dim3(count),
-                                   dim3(LV_P10), 0, s, g, verts, count);
+                go(k_move_block<LV_P10, 4096, LV_CAP10, LV_MINB10>, count, LV_P10);
             break;
         case 11:
             if (shape.block_threads == 512)
-                hipLaunchKernelGGL((k_move_block<512, 8192, LV_CAP11, 1>), dim3(count), dim3(512),
-                                   0, s, g, verts, count);
+                go(k_move_block<512, 8192, LV_CAP11, 1>, count, 512);
             else
-                hipLaunchKernelGGL((k_move_block<1024, 8192, LV_CAP11, 1>), dim3(count),
-                                   dim3(1024), 0, s, g, verts, count);
+                go(k_move_block<1024, 8192, LV_CAP11, 1>, count, 1024);
             break;
         default:
             break;
@@ -1380,9 +1371,10 @@ void launch_heavy_caps(hipStream_t s, const uint32_t *row, const uint32_t *verts
 }
 
 // ---------------------------------------------------------------------------
-// apply one step's moves together (assignCommunity, modularity.cu:382-391)
-// and update the community weights incrementally and exactly (equal to the
-// from-scratch recompute of computeNewCommWeights, modularity.cu:393-398)
+// apply one step's moves together in place (assignCommunity,
+// modularity.cu:382-391) and update the community weights incrementally and
+// exactly (equal to the from-scratch recompute of computeNewCommWeights,
+// modularity.cu:393-398): the step's kernels wrote their decisions to new_comm
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts, uint32_t count,
                                                unsigned long long *moved) {
@@ -1410,6 +1402,29 @@ void launch_apply(hipStream_t s, const DevView &g, const uint32_t *verts, uint32
     if (!count) return;
     hipLaunchKernelGGL(k_apply, dim3((count + 255) / 256), dim3(256), 0, s, g, verts, count,
                        moved);
+}
+
+// ---------------------------------------------------------------------------
+// a step's moves applied to the other copy of the community state on their
+// own (after a sweep's last step, or a single step): every workgroup catches
+// up (assignCommunity, modularity.cu:382-391, with the community weights
+// updated incrementally and exactly, equal to computeNewCommWeights'
+// recompute, :393-398)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_catchup(DevView g) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    catch_up(g, blockIdx.x);
+}
+uint32_t catchup_blocks(uint32_t count, uint32_t threads) {
+    const uint64_t per = (uint64_t)threads * kCatchUpPer;
+    return (uint32_t)std::min<uint64_t>((count + per - 1) / per, 2048);
+}
+void launch_catchup(hipStream_t s, const DevView &g) {
+    if (!g.cu_count) return;
+    DevView gv = g;
+    // a kernel of its own: one round per thread, as many workgroups as that takes
+    gv.cu_nblk = (g.cu_count + 256 * kCatchUpPer - 1) / (256 * kCatchUpPer);
+    hipLaunchKernelGGL(k_catchup, dim3(gv.cu_nblk), dim3(256), 0, s, gv);
 }
 
 // ---------------------------------------------------------------------------

@@ -195,7 +195,7 @@ def test_footprint_bound(lv):
     per_entry = (f(n, 2 * ne) - f(n, ne)) / ne
     per_vertex = (f(2 * n, ne) - f(n, ne)) / n
     assert 84 <= per_entry < 85, per_entry
-    assert per_vertex < 180, per_vertex
+    assert per_vertex < 200, per_vertex
     # the R-MAT scale-24 handle (16.8M vertices, 520.7M entries): under 48 GB
     assert f(1 << 24, 520_749_004) < 48e9
     assert f(100, 0) < 8 << 20  # fixed part: small arrays and slack

@@ -418,6 +418,59 @@ def test_narrow_move_kernels_everywhere(lv):
     assert "narrow kernels ok" in r.stdout
 
 
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_state_copies_either_way(lv, fused):
+    """Passes of small single graphs sweep with two copies of the community
+    state (a step's moves applied inside its kernels, the next step catching
+    up on them); larger graphs and batches keep one copy and an apply kernel
+    per step.  LV_FUSED forces one way for everything: the state after every
+    bucket step, full runs with heavy rows (R-MAT-18, heavy threshold 512)
+    and a batch give the oracle's results either way -- in a child process
+    so the setting does not leak."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from loader import load_package; import oracle\n"
+        "lv = load_package()\n"
+        "qb = lambda q: int(np.float32(q).view(np.uint32))\n"
+        "r, c, w = lv.gen_chung_lu(20000, 9.4, 2.5, 420, 100, 20.0, 81)\n"
+        "o = oracle.Oracle(r.astype(np.uint64), c, w)\n"
+        "with lv.Graph(r, c, w) as g:\n"
+        "    assert qb(g.pass_init()) == qb(o.pass_init())\n"
+        "    for it in range(3):\n"
+        "        for b in range(1, 8):\n"
+        "            assert g.step(b) == o.step(b), (it, b)\n"
+        "            k, comm, sz, a = g.state()\n"
+        "            assert np.array_equal(comm, o.comm()) and np.array_equal(sz, o.comm_sz()), (it, b)\n"
+        "            assert np.array_equal(a, o.a_fixed()), (it, b)\n"
+        "        assert qb(g.modularity()) == qb(o.modularity()), it\n"
+        "graphs = [(lv.gen_rmat(18, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 82), {'heavy_min_degree': 512}),\n"
+        "          (lv.gen_fem(814, 25, 9, 83), {}), (lv.gen_sbm(20000, 40, 20.0, 10.0, 84), {})]\n"
+        "for (r, c, w), tune in graphs:\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    with lv.Graph(r, c, w) as g:\n"
+        "        g.set_tuning(**tune); res = g.run(1e-6); a = g.assignment()\n"
+        "    assert (qb(res.q_final), res.passes, res.total_iters) == \\\n"
+        "        (qb(ores.q_final), ores.passes, ores.total_iters)\n"
+        "    assert np.array_equal(a, o.init_comm())\n"
+        "small = [lv.gen_sbm(n, 8, 18.0, 6.0, 90 + i) for i, n in enumerate([300, 2000, 700, 5000])]\n"
+        "with lv.Batch(small) as b:\n"
+        "    per, tot = b.run(1e-6); assigns = b.assignments()\n"
+        "for i, (r, c, w) in enumerate(small):\n"
+        "    o = oracle.Oracle(r.astype(np.uint64), c, w); ores = o.run(1e-6)\n"
+        "    assert (qb(per[i].q_final), per[i].passes) == (qb(ores.q_final), ores.passes)\n"
+        "    assert np.array_equal(assigns[i], o.init_comm())\n"
+        "print('state copies ok')\n"
+    ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
+    print(f"[case] state copies LV_FUSED={fused}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, LV_FUSED=fused),
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "state copies ok" in r.stdout
+
+
 def _server_pool(lv):
     return [lv.gen_sbm(3000, 12, 14.0, 10.0, 31), lv.gen_sbm(20000, 40, 20.0, 10.0, 32),
             lv.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0.5, 1.5, 33),
