@@ -78,7 +78,7 @@ typedef struct lv_result {
 enum {
     LV_K_CLASSIFY = 0,
     LV_K_R1 = 1,          /* k_v / s_v pinned-order sums */
-    LV_K_SUMSQ = 2,       /* sum of a_c^2 */
+    LV_K_SUMSQ = 2,       /* sum of a_c^2 (runs inside the LV_K_R1 masked-sum launch) */
     LV_K_MOVE_SMALL = 3,  /* buckets 1-5 */
     LV_K_MOVE_WAVE = 4,   /* bucket 6 */
     LV_K_MOVE_BLOCK = 5,  /* bucket 7, LDS hash */

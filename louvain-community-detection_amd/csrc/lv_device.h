@@ -178,12 +178,13 @@ struct HeavySplit {
 // limbs, tot[5] = moves), Q from them and the stop rule into *ps; *err is
 // the sweep's capacity-overflow flag.
 struct SweepTail {
-    const unsigned long long *spread;
+    unsigned long long *spread;
     unsigned long long *tot;
     const uint32_t *err;
     float m, min_gain;
     uint32_t max_iters;
     PassState *ps;
+    PassState *ps_host;  // device address of the host's mapped copy
 };
 void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S);
 // Batch pass init, after the k_v sums: each active graph's m (a graph
@@ -196,14 +197,16 @@ void launch_batch_m(hipStream_t s, uint32_t G, const unsigned long long *acc, in
 void launch_batch_init(hipStream_t s, uint32_t G, const unsigned long long *gcnt, int S,
                        const uint32_t *gstate, GraphState *gst);
 // After a batch sweep: every active graph's Q and stop rule; graphs that stop
-// wait (improved) or finish.  Writes the summary (zeroed by the caller).
-void launch_batch_decide(hipStream_t s, uint32_t G, const unsigned long long *gcnt,
+// wait (improved) or finish.  Counts into the summary (its recounted fields
+// zero before the sweep) and leaves every graph's counters zero.
+void launch_batch_decide(hipStream_t s, uint32_t G, unsigned long long *gcnt,
                          const uint32_t *err_flag, int S, float min_gain, uint32_t max_iters,
                          const uint32_t *voff, uint32_t *gstate, GraphState *gst,
                          BatchSummary *sum);
 // After launch_batch_decide: halt the remaining captured sweeps when no graph
-// is active.
-void launch_batch_halt(hipStream_t s, BatchSummary *sum);
+// is active; the summary to the host's mapped copy `host`, its recounted
+// fields cleared for the next sweep.
+void launch_batch_halt(hipStream_t s, BatchSummary *sum, BatchSummary *host);
 // After a batch contraction: waiting graphs become active for the next pass.
 void launch_batch_reactivate(hipStream_t s, uint32_t G, uint32_t *gstate, GraphState *gst);
 
@@ -270,13 +273,18 @@ struct R1Groups {
     const uint32_t *verts[kR1Groups];
     uint32_t count[kR1Groups];
     uint32_t bend[kR1Groups];
+    // masked sums: sq_blocks workgroups behind the groups add the sum of
+    // round32(a_c)^2 over the communities sq_verts[0, sq_count) into limbs
+    // (4 arrays of kSpread slots, or G in a batch)
+    const uint32_t *sq_verts;
+    uint32_t sq_count, sq_blocks;
+    unsigned long long *limbs;
 };
 void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
                       int init_state, unsigned long long *acc);
 // sum of round32(a_c)^2 over the communities in `verts`; limbs: 4 arrays of
 // kSpread (single graph) or G (batch) slots
-void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
-                  unsigned long long *limbs);
+uint32_t sumsq_blocks(uint32_t count);  // workgroups of the a_c^2 sum over count communities
 
 // classes 1-10 (the heavy class goes through launch_heavy_move)
 // Launch shapes the tuning selects among compiled variants of the move

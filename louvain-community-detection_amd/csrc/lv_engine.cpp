@@ -177,6 +177,7 @@ struct lv_graph {
     // halt_on is set (sweeps captured for a device-decided loop) kernels get
     // the halt flag in their view
     lv::PassState *pst = nullptr, *pst_h = nullptr;
+    lv::PassState *pst_hd = nullptr;  // device address of the mapped host copy
     bool halt_on = false;
     // scalars
     int S = 0;
@@ -199,6 +200,7 @@ struct lv_graph {
     // summary the host reads after each launch of captured sweeps
     lv::GraphState *gst_d = nullptr;
     lv::BatchSummary *bsum_d = nullptr, *bsum_h = nullptr;
+    lv::BatchSummary *bsum_hd = nullptr;  // device address of the mapped host copy
     std::vector<int> gstatus0;  // per graph: LV_ERR_RANGE if its weights do not fit
     // profiling
     bool prof_on = false;
@@ -565,6 +567,13 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         L.bend[i] = nb;
         bytes += (masked ? 12.0 : 8.0) * ent + 16.0 * (b - a);
     }
+    if (masked) {  // the sum of a_c^2 of the same snapshot rides in the same launch
+        L.sq_verts = g->order;
+        L.sq_count = g->cls_off[kNumClasses];  // every listed vertex (a batch: active graphs')
+        L.sq_blocks = lv::sumsq_blocks(L.sq_count);
+        L.limbs = limbs_ptr(g);
+        bytes += 8.0 * g->n;
+    }
     {
         ProfScope ps(g, LV_K_R1, bytes);
         lv::launch_r1_groups(g->stream, v, L, masked, init_state, acc_ptr(g));
@@ -591,13 +600,10 @@ float q_from_counters(const lv_graph *g) {
     return (s_in / (2.0f * m)) - (a2 / ((4.0f * m) * m));
 }
 
-// launches the S_in and sum a^2 reductions into zeroed counters
+// launches the S_in and sum a^2 reductions into zeroed counters (the sum of
+// a^2 in the masked sums' launch)
 void modularity_launch(lv_graph *g, bool singletons = false) {
     r1_all(g, singletons ? 2 : 1, 0);
-    lv::DevView v = view(g);
-    ProfScope ps(g, LV_K_SUMSQ, 8.0 * g->n);
-    // every listed vertex (a batch lists its active graphs' vertices only)
-    lv::launch_sumsq(g->stream, v, g->order, g->cls_off[kNumClasses], limbs_ptr(g));
 }
 
 // Locality order for the pass (single-graph mode): position x of the new
@@ -923,9 +929,10 @@ uint32_t sweeps_per_launch(const lv_graph *g) {
     return g->tune.sweeps_per_launch;
 }
 
-// one sweep whose end applies the stop rule on the device (no host read)
+// one sweep whose end applies the stop rule on the device and writes the
+// loop state to the host's mapped copy; the counters must be zero at its
+// start (the collapse leaves them so for the next sweep)
 void sweep_enqueue_decide(lv_graph *g, float min_gain) {
-    zero_counters(g);
     sweep_steps(g);
     modularity_launch(g);
     lv::SweepTail t;
@@ -936,6 +943,7 @@ void sweep_enqueue_decide(lv_graph *g, float min_gain) {
     t.min_gain = min_gain;
     t.max_iters = kMaxIters;
     t.ps = g->pst;
+    t.ps_host = g->pst_hd;
     lv::launch_collapse_decide(g->stream, t, g->S);
 }
 
@@ -957,11 +965,10 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
         *g->pst_h = lv::PassState{act, 0.0f, 0.0f, 0u, 1u, 1u, 0u, 0u};
         LV_HIP(hipMemcpyAsync(g->pst, g->pst_h, sizeof(lv::PassState), hipMemcpyHostToDevice,
                               g->stream));
+        zero_counters(g);
         g->halt_on = K > 1;  // kernels test the halt flag only when sweeps can follow the stop
         use_graph = capture(g, sg, [&] {
             for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain);
-            (void)hipMemcpyAsync(g->pst_h, g->pst, sizeof(lv::PassState), hipMemcpyDeviceToHost,
-                                 g->stream);
         });
         g->halt_on = false;
     }
@@ -1301,20 +1308,19 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
     if (rc) return rc;
     lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->bsum_d), 0u,
                         sizeof(lv::BatchSummary) / 4);
+    zero_counters(g);
     const uint32_t K = sweeps_per_launch(g);
+    // a sweep starts with zero counters and summary counts: the decide and
+    // halt kernels of the one before leave them so (and the halt writes the
+    // summary to the host's mapped copy)
     auto enqueue = [&] {
         for (uint32_t k = 0; k < K; ++k) {
-            zero_counters(g);
-            lv::launch_fill_u32(g->stream, reinterpret_cast<uint32_t *>(g->bsum_d), 0u,
-                                lv::kBatchSummaryCounted);
             sweep_steps(g);
             modularity_launch(g);
             lv::launch_batch_decide(g->stream, G, g->gcnt, &g->dc->err, g->S, min_gain,
                                     kMaxIters, g->voff_d, g->gstate_d, g->gst_d, g->bsum_d);
-            lv::launch_batch_halt(g->stream, g->bsum_d);
+            lv::launch_batch_halt(g->stream, g->bsum_d, g->bsum_hd);
         }
-        (void)hipMemcpyAsync(g->bsum_h, g->bsum_d, sizeof(lv::BatchSummary),
-                             hipMemcpyDeviceToHost, g->stream);
     };
     auto sg = std::make_unique<SweepGraph>();
     auto recapture = [&] {
@@ -1792,7 +1798,10 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, bool pristine, lv_grap
         g->w0 = static_cast<float *>(malloc(std::max<size_t>(e_cap, 1) * sizeof(float)));
         if (!g->row0 || !g->col0 || !g->w0) e = LV_ERR_NOMEM;
     }
-    if (!e && hipHostMalloc((void **)&g->pst_h, sizeof(lv::PassState)) != hipSuccess) {
+    // kernels write the pass loop state straight into this copy
+    if (!e && (hipHostMalloc((void **)&g->pst_h, sizeof(lv::PassState),
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+               hipHostGetDevicePointer((void **)&g->pst_hd, g->pst_h, 0) != hipSuccess)) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
     }
@@ -2354,7 +2363,9 @@ int lv_batch_create(int device, uint32_t ngraphs, const uint64_t *graph_off, con
     e |= dmalloc(&g->gid, (size_t)n + 1) | dmalloc(&g->gm_d, G) | dmalloc(&g->gstate_d, G);
     e |= dmalloc(&g->voff_d, (size_t)G + 1) | dmalloc(&g->vgather, (size_t)G + 1);
     e |= dmalloc(&g->gcnt, 6ull * G) | dmalloc(&g->gst_d, G) | dmalloc(&g->bsum_d, 1);
-    if (!e && hipHostMalloc((void **)&g->bsum_h, sizeof(lv::BatchSummary)) != hipSuccess) {
+    if (!e && (hipHostMalloc((void **)&g->bsum_h, sizeof(lv::BatchSummary),
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+               hipHostGetDevicePointer((void **)&g->bsum_hd, g->bsum_h, 0) != hipSuccess)) {
         (void)hipGetLastError();
         e = LV_ERR_NOMEM;
     }

@@ -43,11 +43,15 @@ static inline unsigned grid_for(size_t n, unsigned threads, unsigned cap = 65536
 }
 // sums `groups` arrays of kSpread slots into out[0..groups)
 // out[gi] = sum of the kSpread slots of group gi (one workgroup of 256)
-__device__ __forceinline__ void collapse_body(const unsigned long long *spread, int groups,
-                                              unsigned long long *out) {
+// (and with `clear`, leaves the slots zero for the next sweep)
+__device__ __forceinline__ void collapse_body(unsigned long long *spread, int groups,
+                                              unsigned long long *out, bool clear = false) {
     for (int gi = 0; gi < groups; ++gi) {
         unsigned long long x = 0;
-        for (uint32_t i = threadIdx.x; i < kSpread; i += 256) x += spread[gi * kSpread + i];
+        for (uint32_t i = threadIdx.x; i < kSpread; i += 256) {
+            x += spread[gi * kSpread + i];
+            if (clear) spread[gi * kSpread + i] = 0ull;
+        }
         x = wave_sum_u64(x);
         __shared__ unsigned long long part[4];
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
@@ -58,7 +62,7 @@ __device__ __forceinline__ void collapse_body(const unsigned long long *spread, 
 }
 __global__ __launch_bounds__(256) void k_collapse(const unsigned long long *spread, int groups,
                                                   unsigned long long *out) {
-    collapse_body(spread, groups, out);
+    collapse_body(const_cast<unsigned long long *>(spread), groups, out);
 }
 void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
                      unsigned long long *out) {
@@ -597,65 +601,17 @@ __device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *ver
     block_add(g, acc, slot, fixed, valid);  // exact, order-free
 }
 
-// Every class group's pinned sums in one launch, longest rows first:
-// workgroups [0, bend[5]) take group 5 (rows of more than 84 entries, 32
-// lanes), [bend[5], bend[4]) group 4 (33-84 entries), then groups 3 (17-32),
-// 2 (9-16), 1 (5-8) and 0 (at most 4 entries), so the longest workgroups
-// are dispatched first and the short ones fill in behind them.  Groups 0-4
-// take 1, 2, 4, 8 and 8 lanes per vertex (r1_narrow).  The groups read the
-// same snapshot and add into order-free exact counters, so running them
-// side by side in one grid changes no result; on small graphs it replaces
-// several dependent launches of a few microseconds each by one.
-template <bool MASKED>
-__global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int init_state,
-                                                   int ident, unsigned long long *acc) {
-    if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    const uint32_t b = blockIdx.x;
-    if (b < L.bend[5]) {
-        if (MASKED)
-            r1_group32<2>(g, L.verts[5], L.count[5], ident, acc, b);
-        else
-            r1_group<32, false, 8>(g, L.verts[5], L.count[5], init_state, 0, acc, b);
-    } else if (b < L.bend[4]) {
-        r1_narrow<32, 8, 3, MASKED>(g, L.verts[4], L.count[4], init_state, ident, acc,
-                                    b - L.bend[5]);
-    } else if (b < L.bend[3]) {
-        r1_narrow<32, 8, 1, MASKED>(g, L.verts[3], L.count[3], init_state, ident, acc,
-                                    b - L.bend[4]);
-    } else if (b < L.bend[2]) {
-        r1_narrow<16, 4, 1, MASKED>(g, L.verts[2], L.count[2], init_state, ident, acc,
-                                    b - L.bend[3]);
-    } else if (b < L.bend[1]) {
-        r1_narrow<8, 2, 1, MASKED>(g, L.verts[1], L.count[1], init_state, ident, acc,
-                                   b - L.bend[2]);
-    } else {
-        r1_narrow<4, 1, 1, MASKED>(g, L.verts[0], L.count[0], init_state, ident, acc,
-                                   b - L.bend[1]);
-    }
-}
-
-void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
-                      int init_state, unsigned long long *acc) {
-    const uint32_t nb = L.bend[0];
-    if (!nb) return;
-    if (masked)
-        hipLaunchKernelGGL(k_r1_groups<true>, dim3(nb), dim3(256), 0, s, g, L, init_state,
-                           masked == 2, acc);
-    else
-        hipLaunchKernelGGL(k_r1_groups<false>, dim3(nb), dim3(256), 0, s, g, L, init_state, 0,
-                           acc);
-}
-
 // ---------------------------------------------------------------------------
 // sum over communities of round32(a_c)^2, exact at scale 2^2S, as four
 // 32-bit limbs accumulated in u64 (modularity.cu:431-432, square<float>)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sumsq(DevView g, const uint32_t *verts, uint32_t count,
-                                               unsigned long long *limbs) {
-    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+// workgroup blk of nblk (256 threads)
+__device__ __forceinline__ void sumsq_body(const DevView &g, const uint32_t *verts,
+                                           uint32_t count, unsigned long long *limbs,
+                                           uint32_t blk, uint32_t nblk) {
     const int twoS = 2 * g.S;
     const uint32_t stride = acc_stride(g);
-    for (uint32_t base_i = blockIdx.x * 256; base_i < count; base_i += gridDim.x * 256) {
+    for (uint32_t base_i = blk * 256; base_i < count; base_i += nblk * 256) {
         const uint32_t i = base_i + threadIdx.x;
         unsigned long long l0 = 0, l1 = 0, l2 = 0, l3 = 0;
         uint32_t slot = 0;
@@ -701,12 +657,61 @@ __global__ __launch_bounds__(256) void k_sumsq(DevView g, const uint32_t *verts,
     }
 }
 
-void launch_sumsq(hipStream_t s, const DevView &g, const uint32_t *verts, uint32_t count,
-                  unsigned long long *limbs) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, g, verts, count,
-                       limbs);
+uint32_t sumsq_blocks(uint32_t count) { return count ? grid_for(count, 256, 8192) : 0u; }
+
+// Every class group's pinned sums in one launch, longest rows first:
+// workgroups [0, bend[5]) take group 5 (rows of more than 84 entries, 32
+// lanes), [bend[5], bend[4]) group 4 (33-84 entries), then groups 3 (17-32),
+// 2 (9-16), 1 (5-8) and 0 (at most 4 entries), so the longest workgroups
+// are dispatched first and the short ones fill in behind them.  Groups 0-4
+// take 1, 2, 4, 8 and 8 lanes per vertex (r1_narrow).  The groups read the
+// same snapshot and add into order-free exact counters, so running them
+// side by side in one grid changes no result; on small graphs it replaces
+// several dependent launches of a few microseconds each by one.
+template <bool MASKED>
+__global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int init_state,
+                                                   int ident, unsigned long long *acc) {
+    if (halted(g)) return;  // the pass loop stopped earlier in this launch
+    const uint32_t b = blockIdx.x;
+    if (b >= L.bend[0]) {  // behind the groups: the sum of a_c^2 of the same snapshot
+        sumsq_body(g, L.sq_verts, L.sq_count, L.limbs, b - L.bend[0], L.sq_blocks);
+        return;
+    }
+    if (b < L.bend[5]) {
+        if (MASKED)
+            r1_group32<2>(g, L.verts[5], L.count[5], ident, acc, b);
+        else
+            r1_group<32, false, 8>(g, L.verts[5], L.count[5], init_state, 0, acc, b);
+    } else if (b < L.bend[4]) {
+        r1_narrow<32, 8, 3, MASKED>(g, L.verts[4], L.count[4], init_state, ident, acc,
+                                    b - L.bend[5]);
+    } else if (b < L.bend[3]) {
+        r1_narrow<32, 8, 1, MASKED>(g, L.verts[3], L.count[3], init_state, ident, acc,
+                                    b - L.bend[4]);
+    } else if (b < L.bend[2]) {
+        r1_narrow<16, 4, 1, MASKED>(g, L.verts[2], L.count[2], init_state, ident, acc,
+                                    b - L.bend[3]);
+    } else if (b < L.bend[1]) {
+        r1_narrow<8, 2, 1, MASKED>(g, L.verts[1], L.count[1], init_state, ident, acc,
+                                   b - L.bend[2]);
+    } else {
+        r1_narrow<4, 1, 1, MASKED>(g, L.verts[0], L.count[0], init_state, ident, acc,
+                                   b - L.bend[1]);
+    }
 }
+
+void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
+                      int init_state, unsigned long long *acc) {
+    const uint32_t nb = L.bend[0] + (masked ? L.sq_blocks : 0u);
+    if (!nb) return;
+    if (masked)
+        hipLaunchKernelGGL(k_r1_groups<true>, dim3(nb), dim3(256), 0, s, g, L, init_state,
+                           masked == 2, acc);
+    else
+        hipLaunchKernelGGL(k_r1_groups<false>, dim3(nb), dim3(256), 0, s, g, L, init_state, 0,
+                           acc);
+}
+
 
 // ---------------------------------------------------------------------------
 // move kernels (computeMoveWarp / computeMoveBlock / computeMoveGlobal,
@@ -2778,9 +2783,16 @@ __device__ __forceinline__ void sweep_decide(const unsigned long long *tot,
 
 // After a sweep's sums: the spread counters (6 groups) collapsed into tot,
 // then the stop rule, in one workgroup (one launch instead of two).
+// The spread slots are left zero for the next sweep, and the loop state is
+// also written to the host's mapped copy, which the host reads after the
+// launch (no copy launch).
 __global__ __launch_bounds__(256) void k_collapse_decide(SweepTail t, int S) {
-    collapse_body(t.spread, 6, t.tot);
-    if (threadIdx.x == 0) sweep_decide(t.tot, t.err, S, t.m, t.min_gain, t.max_iters, t.ps);
+    collapse_body(t.spread, 6, t.tot, true);
+    if (threadIdx.x == 0) {
+        sweep_decide(t.tot, t.err, S, t.m, t.min_gain, t.max_iters, t.ps);
+        *t.ps_host = *t.ps;
+        __threadfence_system();
+    }
 }
 void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S) {
     hipLaunchKernelGGL(k_collapse_decide, dim3(1), dim3(256), 0, s, t, S);
@@ -2828,7 +2840,7 @@ void launch_batch_init(hipStream_t s, uint32_t G, const unsigned long long *gcnt
                        gst);
 }
 
-__global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, const unsigned long long *gcnt,
+__global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, unsigned long long *gcnt,
                                                       const uint32_t *err_flag, int S,
                                                       float min_gain, uint32_t max_iters,
                                                       const uint32_t *voff, uint32_t *gstate,
@@ -2852,6 +2864,8 @@ __global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, const unsigned
             b.q_final = qp;
             gst[gi] = b;
         }
+        // the graph's exact counters start the next sweep at zero
+        for (int f = 0; f < 6; ++f) gcnt[(size_t)f * G + gi] = 0ull;
     }
     const unsigned long long live =
         state == kGraphActive ? (unsigned long long)(voff[gi + 1] - voff[gi]) : 0ull;
@@ -2865,13 +2879,21 @@ __global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, const unsigned
     }
     if (gi == 0 && *err_flag) sum->err = 1u;
 }
-__global__ void k_batch_halt(BatchSummary *sum) {
+// No graph active: the remaining captured sweeps halt.  The summary goes to
+// the host's mapped copy, which the host reads after the launch, and its
+// recounted fields are cleared for the next sweep's decide.
+__global__ void k_batch_halt(BatchSummary *sum, BatchSummary *host) {
     if (sum->active == 0u) sum->halt = 1u;
+    *host = *sum;
+    __threadfence_system();
+    sum->active = 0u;
+    sum->waiting = 0u;
+    sum->live = 0ull;
 }
-void launch_batch_halt(hipStream_t s, BatchSummary *sum) {
-    hipLaunchKernelGGL(k_batch_halt, dim3(1), dim3(1), 0, s, sum);
+void launch_batch_halt(hipStream_t s, BatchSummary *sum, BatchSummary *host) {
+    hipLaunchKernelGGL(k_batch_halt, dim3(1), dim3(1), 0, s, sum, host);
 }
-void launch_batch_decide(hipStream_t s, uint32_t G, const unsigned long long *gcnt,
+void launch_batch_decide(hipStream_t s, uint32_t G, unsigned long long *gcnt,
                          const uint32_t *err_flag, int S, float min_gain, uint32_t max_iters,
                          const uint32_t *voff, uint32_t *gstate, GraphState *gst,
                          BatchSummary *sum) {
