@@ -392,8 +392,20 @@ void launch_aggp(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
                  const uint32_t *comms, uint32_t count, int phase);
 void launch_aggp_caps(hipStream_t s, AggView a, const AggPart &p, const uint32_t *comms,
                       uint32_t count, unsigned long long *caps);
+// Where a contraction writes the new CSR: the row of new vertex u (its
+// community's new id) starts at rows[map ? map[u] : u] and holds
+// rank_rows[u + 1] - rank_rows[u] entries (rank_rows: row pointers by new id),
+// its columns are map[new id] (or the new ids without a map).  With a map the
+// graph comes out in the next pass's locality order.
+struct EmitTarget {
+    const uint32_t *rows;
+    const uint32_t *rank_rows;
+    const uint32_t *map;
+    uint32_t *col;
+    float *w;
+};
 void launch_aggp_emit(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
-                      const uint32_t *comms, const uint32_t *row2, uint32_t *col2, float *w2);
+                      const uint32_t *comms, const EmitTarget &t);
 void launch_agg_prep(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_members(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_classify_count(hipStream_t s, const DevView &g, AggView a, uint32_t *counts);
@@ -410,12 +422,13 @@ void launch_agg_mdeg(hipStream_t s, const DevView &g, AggView a);
 void launch_agg_large_prep(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
                            uint32_t count);
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
-                     uint32_t count, const uint32_t *only, uint32_t *row2, uint32_t *col2,
-                     float *w2);
+                     uint32_t count, const uint32_t *only, const EmitTarget &t);
 // vnid[u] = new_id[comm[u]] for u < n: one gather per entry in the merges
 void launch_agg_vnid(hipStream_t s, const uint32_t *comm, const uint32_t *new_id, uint32_t n,
                      uint32_t *vnid);
+// init_comm <- new_id[comm[init_comm]], through map (new id -> vertex of the
+// next pass) when non-null
 void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
-                        uint32_t *init_comm, uint32_t n0);
+                        const uint32_t *map, uint32_t *init_comm, uint32_t n0);
 
 }  // namespace lv

@@ -126,6 +126,9 @@ struct lv_graph {
     // scratch for the renumbering under it
     uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
     bool relabeled = false;  // the current pass graph is in locality order
+    // the contraction wrote the graph in its locality order and classified it
+    bool prelabeled = false;
+    uint32_t cls_heavy_min = 0;  // heavy threshold of the current classes
     lv_tuning tune = {4096u, 512u, -1, 1u << 20, 2u, 1024u, 1280u, 384u, 1u};  // lv_graph_set_tuning
     // classification (host copies)
     uint32_t cls_count[kNumClasses] = {};
@@ -391,7 +394,9 @@ float fix2f_host(const lv_graph *g, int64_t x) { return (float)x * g->inv_scale;
 // ---------------------------------------------------------------------------
 // pass init (modularity.cu:455-463)
 // ---------------------------------------------------------------------------
-int classify(lv_graph *g) {
+// Degree classes of the current graph: counts, offsets and `order` (the
+// vertices grouped by class).
+int classify_order(lv_graph *g) {
     zero_counters(g);
     {
         ProfScope ps(g, LV_K_CLASSIFY, 8.0 * g->n);
@@ -423,8 +428,14 @@ int classify(lv_graph *g) {
                                     g->G ? g->gid : nullptr, g->gstate_d, run, g->dc->cursor,
                                     g->order);
     }
-    // heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk
-    // offsets of the split kernels, per-vertex scratch
+    g->cls_heavy_min = g->tune.heavy_min_degree;
+    return LV_OK;
+}
+
+// Heavy class: table arena (pow2 >= 2*deg slots per vertex), chunk offsets of
+// the split kernels, per-vertex scratch.  Per heavy index, so the same for
+// any numbering of the vertices.
+int classify_heavy(lv_graph *g) {
     const uint32_t nh = g->cls_count[lv::kClassHeavy];
     g->heavy_chunks = 0;
     if (nh) {
@@ -479,6 +490,11 @@ int classify(lv_graph *g) {
         }
     }
     return LV_OK;
+}
+
+int classify(lv_graph *g) {
+    int rc = classify_order(g);
+    return rc ? rc : classify_heavy(g);
 }
 
 lv::HeavySplit heavy_view(lv_graph *g) {
@@ -609,13 +625,13 @@ void modularity_launch(lv_graph *g, bool singletons = false) {
 // Locality order for the pass (single-graph mode): position x of the new
 // numbering holds input vertex order[x] of the class grouping, so the heavy
 // and block classes, whose state most entries gather, are contiguous.
-bool want_locality(const lv_graph *g) {
-    if (g->G || g->n < 2) return false;
+bool want_locality(const lv_graph *g, uint32_t n) {
+    if (g->G || n < 2) return false;
     static const char *env = getenv("LV_LOCALITY");
     const int mode = env ? atoi(env) : g->tune.locality;
     if (mode == 0) return false;
     if (mode == 1) return true;
-    return g->n >= g->tune.locality_min_vertices;
+    return n >= g->tune.locality_min_vertices;
 }
 
 void relabel(lv_graph *g) {
@@ -638,10 +654,16 @@ void relabel(lv_graph *g) {
 }
 
 int pass_init(lv_graph *g, float *q0) {
-    int rc = classify(g);
-    if (rc) return rc;
-    g->relabeled = false;
-    if (want_locality(g)) relabel(g);
+    int rc = LV_OK;
+    // a contraction that wrote the graph in its locality order classified it
+    // already (again if the heavy threshold changed since; a graph in a
+    // locality order keeps it)
+    if (!g->prelabeled || g->cls_heavy_min != g->tune.heavy_min_degree) {
+        rc = classify(g);
+        if (rc) return rc;
+        if (!g->relabeled && want_locality(g, g->n)) relabel(g);
+    }
+    g->prelabeled = false;
     zero_counters(g);
     r1_all(g, 0, 1);
     rc = sync_counters(g);
@@ -1231,24 +1253,61 @@ int aggregation(lv_graph *g) {
     lv::launch_scan_u32(s, g->cnt, 0, nn, g->row2, g->scan_tmp32);
     uint32_t ne2 = 0;
     LV_HIP(hipMemcpyAsync(&ne2, g->row2 + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    lv::EmitTarget t{g->row2, g->row2, nullptr, g->col2, g->w2};
+    // The next pass in a locality order: its classes and order come from the
+    // new degrees before anything is written, and the rows go straight to
+    // their positions (columns mapped), instead of a copy at pass init.  The
+    // old row pointers are free once the merges are done.
+    const bool pre = want_locality(g, nn);
+    uint32_t *const old_row = g->row;
+    const uint32_t old_n = g->n;
+    const uint64_t old_ne = g->ne;
+    if (pre) {
+        rc = sync_counters(g);  // the merges' overflow flag, before classify clears it
+        if (rc) return rc;
+        if (g->hc->err) return LV_ERR_OVERFLOW;
+        g->row = g->row2;  // by new id: the next pass's input numbering
+        g->n = nn;
+        g->ne = ne2;
+        rc = classify_order(g);
+        if (rc) {
+            g->row = old_row;
+            g->n = old_n;
+            g->ne = old_ne;
+            return rc;
+        }
+        lv::launch_relabel_map(s, g->order, nn, g->lid, g->pmap, g->row2, g->rtmp);
+        lv::launch_scan_u32(s, g->rtmp, 0, nn, old_row, g->scan_tmp32);
+        t.rows = old_row;
+        t.map = g->pmap;
+    }
     // communities without entries (class 0, first in aorder) have empty rows
     lv::launch_agg_emit(s, v, av, g->aorder + aoff[1], aoff[lv::kAggGlobalClass] - aoff[1],
-                        nullptr, g->row2, g->col2, g->w2);
+                        nullptr, t);
     if (nl) {
-        lv::launch_aggp_emit(s, v, av, ap, lc, g->row2, g->col2, g->w2);
-        lv::launch_agg_emit(s, v, av, lc, nl, g->ap_cflag, g->row2, g->col2, g->w2);
+        lv::launch_aggp_emit(s, v, av, ap, lc, t);
+        lv::launch_agg_emit(s, v, av, lc, nl, g->ap_cflag, t);
     }
-    lv::launch_agg_compose(s, g->comm, g->new_id, g->init_comm, g->n0);
+    lv::launch_agg_compose(s, g->comm, g->new_id, t.map, g->init_comm, g->n0);
     rc = sync_counters(g);
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;
-    std::swap(g->row, g->row2);
+    if (pre) {
+        g->row2 = g->row;  // the rows by new id become scratch
+        g->row = old_row;  // by position
+        rc = classify_heavy(g);
+        if (rc) return rc;
+    } else {
+        std::swap(g->row, g->row2);
+    }
     std::swap(g->col, g->col2);
     std::swap(g->w, g->w2);
     g->n = nn;
     g->ne = ne2;
     g->pass_ready = false;
-    g->relabeled = false;  // the contracted graph is in the input numbering
+    // in the input numbering, or in the next pass's locality order (lid)
+    g->relabeled = pre;
+    g->prelabeled = pre;
     return LV_OK;
 }
 
@@ -1840,6 +1899,7 @@ int upload_working(lv_graph *g, const uint32_t *row, const uint32_t *col, const 
     g->ne = g->ne0;
     g->pass_ready = false;
     g->relabeled = false;
+    g->prelabeled = false;
     return LV_OK;
 }
 

@@ -2614,8 +2614,7 @@ __global__ void k_aggp_caps(AggView a, AggPart p, const uint32_t *comms, uint32_
 // sorted distinct lists of the bucket outputs -> the new CSR (one workgroup
 // per bucket; buckets of flagged communities are emitted per community)
 __global__ __launch_bounds__(256) void k_aggp_emit(DevView g, AggView a, AggPart p,
-                                                   const uint32_t *comms, const uint32_t *row2,
-                                                   uint32_t *col2, float *w2) {
+                                                   const uint32_t *comms, EmitTarget t) {
     const uint32_t b = blockIdx.x;
     const uint32_t ci = p.bowner[b];
     if (p.cflag[ci]) return;
@@ -2623,10 +2622,12 @@ __global__ __launch_bounds__(256) void k_aggp_emit(DevView g, AggView a, AggPart
     if (!nd) return;
     const uint32_t c = comms[ci], first = p.bstart[ci];
     const uint32_t src = a.epos[c] + (p.bpos[b] - p.bpos[first]);
-    const uint32_t dst = row2[a.new_id[c]] + (p.boff[b] - p.boff[first]);
+    const uint32_t nid = a.new_id[c];
+    const uint32_t dst = t.rows[t.map ? t.map[nid] : nid] + (p.boff[b] - p.boff[first]);
     for (uint32_t i = threadIdx.x; i < nd; i += 256) {
-        col2[dst + i] = a.tkey[src + i];
-        w2[dst + i] = fix2f(a.tval[src + i], g.S);
+        const uint32_t k = a.tkey[src + i];
+        t.col[dst + i] = t.map ? t.map[k] : k;
+        t.w[dst + i] = fix2f(a.tval[src + i], g.S);
     }
 }
 
@@ -2660,34 +2661,35 @@ void launch_aggp_caps(hipStream_t s, AggView a, const AggPart &p, const uint32_t
                        count, caps);
 }
 void launch_aggp_emit(hipStream_t s, const DevView &g, AggView a, const AggPart &p,
-                      const uint32_t *comms, const uint32_t *row2, uint32_t *col2, float *w2) {
+                      const uint32_t *comms, const EmitTarget &t) {
     if (!p.NB) return;
-    hipLaunchKernelGGL(k_aggp_emit, dim3(p.NB), dim3(256), 0, s, g, a, p, comms, row2, col2, w2);
+    hipLaunchKernelGGL(k_aggp_emit, dim3(p.NB), dim3(256), 0, s, g, a, p, comms, t);
 }
 
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
 // sorted distinct list into the new CSR with renumbered ids and FP32 weights
 __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const uint32_t *comms,
                                                   uint32_t count, const uint32_t *only,
-                                                  uint32_t *row2, uint32_t *col2, float *w2) {
+                                                  EmitTarget t) {
     const uint32_t ci = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     if (ci >= count || (only && !only[ci])) return;
     const uint32_t c = comms[ci];
     const uint32_t nid = a.new_id[c];
-    const uint32_t o = row2[nid], nd = row2[nid + 1] - o;
+    const uint32_t o = t.rows[t.map ? t.map[nid] : nid];
+    const uint32_t nd = t.rank_rows[nid + 1] - t.rank_rows[nid];
     const uint32_t src = a.epos[c];
     for (uint32_t i = lane; i < nd; i += 64) {
-        col2[o + i] = a.tkey[src + i];  // already renumbered
-        w2[o + i] = fix2f(a.tval[src + i], g.S);
+        const uint32_t k = a.tkey[src + i];  // already renumbered
+        t.col[o + i] = t.map ? t.map[k] : k;
+        t.w[o + i] = fix2f(a.tval[src + i], g.S);
     }
 }
 void launch_agg_emit(hipStream_t s, const DevView &g, AggView a, const uint32_t *comms,
-                     uint32_t count, const uint32_t *only, uint32_t *row2, uint32_t *col2,
-                     float *w2) {
+                     uint32_t count, const uint32_t *only, const EmitTarget &t) {
     if (!count) return;
     hipLaunchKernelGGL(k_agg_emit, dim3((count + 3) / 4), dim3(256), 0, s, g, a, comms, count,
-                       only, row2, col2, w2);
+                       only, t);
 }
 
 // graphVertToComm + graphVertToNewComm (modularity.cu:51-56,
@@ -2702,15 +2704,18 @@ void launch_agg_vnid(hipStream_t s, const uint32_t *comm, const uint32_t *new_id
     if (!n) return;
     hipLaunchKernelGGL(k_agg_vnid, dim3((n + 255) / 256), dim3(256), 0, s, comm, new_id, n, vnid);
 }
-__global__ void k_agg_compose(const uint32_t *comm, const uint32_t *new_id, uint32_t *init_comm,
-                              uint32_t n0) {
+__global__ void k_agg_compose(const uint32_t *comm, const uint32_t *new_id, const uint32_t *map,
+                              uint32_t *init_comm, uint32_t n0) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n0) init_comm[i] = new_id[comm[init_comm[i]]];
+    if (i < n0) {
+        const uint32_t u = new_id[comm[init_comm[i]]];
+        init_comm[i] = map ? map[u] : u;
+    }
 }
 void launch_agg_compose(hipStream_t s, const uint32_t *comm, const uint32_t *new_id,
-                        uint32_t *init_comm, uint32_t n0) {
+                        const uint32_t *map, uint32_t *init_comm, uint32_t n0) {
     if (!n0) return;
-    hipLaunchKernelGGL(k_agg_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, comm, new_id,
+    hipLaunchKernelGGL(k_agg_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, comm, new_id, map,
                        init_comm, n0);
 }
 
