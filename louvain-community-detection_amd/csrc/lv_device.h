@@ -142,6 +142,7 @@ constexpr int kBatchSummaryCounted = 4;  // 32-bit words recounted per sweep
 // Heavy-vertex state for the chunked kernels (lv_heavy.hip).
 constexpr uint32_t kHeavyChunk = 2048;  // entries per workgroup; multiple of 256
 constexpr uint32_t kMaxBuckets = 1024;    // buckets per heavy vertex (max)
+constexpr uint32_t kHpGroup = 8;          // buckets per workgroup (min per vertex)
 struct HeavySplit {
     const uint32_t *verts;               // heavy-class vertices
     uint32_t nh;                         // their count

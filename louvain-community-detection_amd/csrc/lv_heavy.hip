@@ -492,6 +492,11 @@ __device__ __forceinline__ uint32_t hp_insert(uint32_t *key, unsigned long long 
     return kEmpty;
 }
 
+// One workgroup per kHpGroup consecutive buckets of one vertex (every
+// heavy vertex has a multiple of kHpGroup buckets): the buckets are merged
+// one after the other through the same LDS table, their per-lane maxima
+// gather in LDS, and the vertex's lanes get one global atomicMax per lane
+// per group instead of per bucket.
 template <int kBucketCap, int kDmax>
 __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
@@ -501,96 +506,102 @@ __global__ __launch_bounds__(256) void k_hp_bucket(DevView g, HeavySplit h) {
     __shared__ uint16_t dist[kDmax];
     __shared__ unsigned long long lanes[128];
     __shared__ uint32_t nd_s, over;
-    const uint32_t i = h.bvtx[blockIdx.x];
+    const uint32_t bg0 = blockIdx.x * kHpGroup;
+    const uint32_t i = h.bvtx[bg0];
     const uint32_t v = h.verts[i];
     if (!v_live(g, v)) return;
-    const uint32_t p0 = h.bpos[blockIdx.x], p1 = h.bpos[blockIdx.x + 1];
-    if (p0 == p1) return;
     // the vertex's own state, loaded now so its latency overlaps the inserts
     const float m = v_m(g, v);
     const uint32_t cv = g.comm[v];
     const uint32_t szv = g.comm_sz[cv];
     const float kv = g.k[v];
     const unsigned long long acvx = g.A[cv];
-    for (int s2 = threadIdx.x; s2 < kBucketCap; s2 += 256) {
-        key[s2] = kEmpty;
-        val[s2] = 0ull;
-    }
     if (threadIdx.x < 128) lanes[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) {
-        nd_s = 0;
-        over = 0;
-    }
     const uint32_t dmax = min(h.dmax, (uint32_t)kDmax);
-    __syncthreads();
-    constexpr uint32_t R = 4;
-    const bool cached = p1 - p0 <= 256u * R;  // uniform
-    uint32_t cs[R], js[R], ss[R];
-    if (cached) {
-        float ws[R];
-#pragma unroll
-        for (uint32_t u = 0; u < R; ++u) {  // all loads first, unconditional
-            const uint32_t qu = p0 + threadIdx.x + 256u * u;
-            const uint32_t q = qu < p1 ? qu : p0;
-            cs[u] = h.pc[q];
-            ws[u] = h.pw[q];
-            js[u] = h.pj[q];
+    for (uint32_t kb = 0; kb < kHpGroup; ++kb) {
+        const uint32_t p0 = h.bpos[bg0 + kb], p1 = h.bpos[bg0 + kb + 1];
+        if (p0 == p1) continue;  // uniform
+        for (int s2 = threadIdx.x; s2 < kBucketCap; s2 += 256) {
+            key[s2] = kEmpty;
+            val[s2] = 0ull;
         }
-#pragma unroll
-        for (uint32_t u = 0; u < R; ++u)
-            if (p0 + threadIdx.x + 256u * u >= p1) cs[u] = kEmpty;
-#pragma unroll
-        for (uint32_t u = 0; u < R; ++u)
-            ss[u] = cs[u] == kEmpty ? kEmpty
-                                    : hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, cs[u],
-                                                            f2fix(ws[u], g.scale), dmax);
-    } else {
-        for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256)
-            hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, h.pc[q], f2fix(h.pw[q], g.scale),
-                                  dmax);
-    }
-    __syncthreads();
-    if (over) {  // uniform: redo this vertex with the global-table kernels
-        if (threadIdx.x == 0) atomicOr(&h.hflag[i], 1u);
-        return;
-    }
-    const uint32_t nd = nd_s;
-    const float acv = fix2f(acvx, g.S);
-    distinct_gains(g, dist, nd, threadIdx.x, 256u, key, val, m, kv, acv, szv, cv);
-    __syncthreads();
-    auto lane_max = [&](uint32_t slot, uint32_t j) {
-        const float gg = __uint_as_float((uint32_t)val[slot]);
-        if (gg != -INFINITY) {
-            const unsigned long long packed =
-                ((unsigned long long)f32_ord(gg) << 32) | (unsigned long long)(0xffffffffu - j);
-            atomicMax(&lanes[j & 127u], packed);
+        if (threadIdx.x == 0) {
+            nd_s = 0;
+            over = 0;
         }
-    };
-    if (cached) {
+        __syncthreads();
+        constexpr uint32_t R = 4;
+        const bool cached = p1 - p0 <= 256u * R;  // uniform
+        uint32_t cs[R], js[R], ss[R];
+        if (cached) {
+            float ws[R];
 #pragma unroll
-        for (uint32_t u = 0; u < R; ++u)
-            if (ss[u] != kEmpty) lane_max(ss[u], js[u]);
-    } else {
-        for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
-            const uint32_t c = h.pc[q];
-            uint32_t slot = pslot_n<kBucketCap>(c);
-            while (key[slot] != c) slot = slot + 1 == (uint32_t)kBucketCap ? 0u : slot + 1;  // present
-            lane_max(slot, h.pj[q]);
+            for (uint32_t u = 0; u < R; ++u) {  // all loads first, unconditional
+                const uint32_t qu = p0 + threadIdx.x + 256u * u;
+                const uint32_t q = qu < p1 ? qu : p0;
+                cs[u] = h.pc[q];
+                ws[u] = h.pw[q];
+                js[u] = h.pj[q];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < R; ++u)
+                if (p0 + threadIdx.x + 256u * u >= p1) cs[u] = kEmpty;
+#pragma unroll
+            for (uint32_t u = 0; u < R; ++u)
+                ss[u] = cs[u] == kEmpty
+                            ? kEmpty
+                            : hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, cs[u],
+                                                    f2fix(ws[u], g.scale), dmax);
+        } else {
+            for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256)
+                hp_insert<kBucketCap>(key, val, dist, &nd_s, &over, h.pc[q],
+                                      f2fix(h.pw[q], g.scale), dmax);
         }
+        __syncthreads();
+        if (over) {  // uniform: redo this vertex with the global-table kernels
+            if (threadIdx.x == 0) atomicOr(&h.hflag[i], 1u);
+            return;
+        }
+        const uint32_t nd = nd_s;
+        const float acv = fix2f(acvx, g.S);
+        distinct_gains(g, dist, nd, threadIdx.x, 256u, key, val, m, kv, acv, szv, cv);
+        __syncthreads();
+        auto lane_max = [&](uint32_t slot, uint32_t j) {
+            const float gg = __uint_as_float((uint32_t)val[slot]);
+            if (gg != -INFINITY) {
+                const unsigned long long packed =
+                    ((unsigned long long)f32_ord(gg) << 32) | (unsigned long long)(0xffffffffu - j);
+                atomicMax(&lanes[j & 127u], packed);
+            }
+        };
+        if (cached) {
+#pragma unroll
+            for (uint32_t u = 0; u < R; ++u)
+                if (ss[u] != kEmpty) lane_max(ss[u], js[u]);
+        } else {
+            for (uint32_t q = p0 + threadIdx.x; q < p1; q += 256) {
+                const uint32_t c = h.pc[q];
+                uint32_t slot = pslot_n<kBucketCap>(c);
+                while (key[slot] != c)
+                    slot = slot + 1 == (uint32_t)kBucketCap ? 0u : slot + 1;  // present
+                lane_max(slot, h.pj[q]);
+            }
+        }
+        __syncthreads();  // the lane phase read the table the next bucket clears
     }
-    __syncthreads();
     if (threadIdx.x < 128 && lanes[threadIdx.x])
         atomicMax(&h.hlane[(size_t)i * 128 + threadIdx.x], lanes[threadIdx.x]);
 }
 
-// per heavy vertex: bucket count pow2 >= ceil(deg / target), at most kMaxBuckets
+// per heavy vertex: bucket count pow2 >= ceil(deg / target), at least
+// kHpGroup, at most kMaxBuckets
 __global__ void k_hp_nbuck(const uint32_t *row, const uint32_t *verts, uint32_t count,
                            uint32_t target, uint32_t *nbuck) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         const uint32_t v = verts[i];
         const uint32_t d = row[v + 1] - row[v];
-        uint32_t b = 1;
+        uint32_t b = kHpGroup;
         while (b < kMaxBuckets && (uint64_t)b * target < d) b <<= 1;
         nbuck[i] = b;
     }
@@ -673,11 +684,11 @@ void launch_heavy_move(hipStream_t s, const DevView &g, const HeavySplit &hin) {
                              hipMemcpyDeviceToDevice, s);
         hipLaunchKernelGGL(k_hp_scatter, grid, block, 0, s, g, h);
         // buckets of up to 512 target entries: 2048-slot tables; larger: 4096
+        const dim3 bgrid(h.nbuckets / kHpGroup);
         if (h.bucket_entries <= 512)
-            hipLaunchKernelGGL((k_hp_bucket<LV_HPCAP, LV_HPDMAX>), dim3(h.nbuckets), block, 0, s,
-                               g, h);
+            hipLaunchKernelGGL((k_hp_bucket<LV_HPCAP, LV_HPDMAX>), bgrid, block, 0, s, g, h);
         else
-            hipLaunchKernelGGL((k_hp_bucket<4096, 2048>), dim3(h.nbuckets), block, 0, s, g, h);
+            hipLaunchKernelGGL((k_hp_bucket<4096, 2048>), bgrid, block, 0, s, g, h);
         if (h.force_global == 2)
             hipLaunchKernelGGL(k_hp_flag_odd, dim3((h.nh + 255) / 256), dim3(256), 0, s, h);
     }
