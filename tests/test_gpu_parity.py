@@ -539,6 +539,37 @@ def test_tuning_never_changes_results(lv, orc, graphs, tune):
     assert np.array_equal(assign, o.init_comm())
 
 
+def test_tuning_change_between_passes(lv, orc, graphs):
+    """A contraction that writes the next pass in its locality order also
+    classifies it for the current heavy threshold; a threshold changed
+    before the next pass init makes that pass classify the ordered graph
+    again (without ordering it twice).  Passes and contractions stay equal
+    to the oracle's, the contracted graph is reported in the input
+    numbering, and a second pass init in a row changes nothing."""
+    row, col, w = graphs["rmat16"]
+    print("[case] tuning change between passes", flush=True)
+    o = orc.Oracle(row.astype(np.uint64), col, w)
+    gain = np.float32(1e-6)
+    with lv.Graph(_u32(row), col, w) as g:
+        g.set_locality(1)
+        for p, heavy in enumerate([4096, 512, 8192, 1024, 4096, 512, 2048]):
+            if p == 1:  # a second pass init in a row: same state, same Q
+                assert g.pass_init().view(np.uint32) == g.pass_init().view(np.uint32)
+            g.set_tuning(heavy_min_degree=heavy)
+            imp, qb, qa, it = g.modularity_optimization(gain)
+            oimp, oqp, oit = o.modularity_optimization(gain)
+            oqb, oqa = o.last_pass_q()
+            assert (imp, it) == (oimp, oit), f"pass {p}"
+            assert qa.view(np.uint32) == oqa.view(np.uint32), f"pass {p}"
+            _cmp_state(g, o, f"end of pass {p}")
+            if not imp:
+                break
+            g.aggregation()
+            o.aggregation()
+            _cmp_csr(g, o, f"after aggregation {p}")
+            assert np.array_equal(g.assignment(), o.init_comm()), f"assignment pass {p}"
+
+
 def test_autotune(lv, orc, graphs):
     """The autotuner times candidate launch shapes on the resident graph and
     keeps the fastest; a run afterwards still gives the oracle's result, and
