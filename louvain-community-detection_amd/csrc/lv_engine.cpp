@@ -692,15 +692,16 @@ uint32_t narrow_min() {
 
 // Whether the current pass sweeps with two copies of the community state
 // (moves applied inside the step kernels, no apply launch per step): single
-// graphs of at most 2^20 vertices, where a step's apply launch costs more
-// than its work (co-purchase graph 25.5 -> 22.5 ms per run).  Larger graphs
-// and batches keep one copy and k_apply: there the apply is real work that
-// a kernel of its own does faster (4096-graph batch +1.3 % with two copies).
-// LV_FUSED=0/1 forces one way (tests).
+// graphs of at most 2^22 vertices (co-purchase graph 25.5 -> 22.5 ms per run,
+// R-MAT scale 18-22 and SBMs of 0.2-1M vertices 0.6-5.5 % faster).  Larger
+// graphs and batches keep one copy and k_apply: a step's gathers then
+// alternate between two copies of state too large for the caches (R-MAT-24
+// +0.2 %, 4096-graph batch +1.3 % with two copies).  LV_FUSED=0/1 forces one
+// way (tests).
 bool fused_pass(const lv_graph *g) {
     static const char *env = getenv("LV_FUSED");
     if (env && *env) return atoi(env) != 0;
-    return !g->G && g->n <= (1u << 20);
+    return !g->G && g->n <= (1u << 22);
 }
 
 // profiling family of each move class
