@@ -104,6 +104,10 @@ struct DevView {
     const uint32_t *cu_verts;
     uint32_t cu_count, cu_nblk;
     unsigned long long *cu_moved;
+    // >= 0: the launch's vertex list is the identity run starting there (the
+    // class order of a graph in its locality order), so kernels compute
+    // vertex ids instead of loading them; -1: load them
+    int32_t vbase;
 };
 constexpr uint32_t kGraphActive = 1;
 constexpr uint32_t kGraphWait = 2;  // improved in this pass, waits for the contraction
@@ -272,12 +276,14 @@ constexpr int kR1Groups = 6;
 constexpr uint32_t kR1GroupVertices[kR1Groups] = {256, 128, 64, 32, 32, 8};
 struct R1Groups {
     const uint32_t *verts[kR1Groups];
+    int32_t vbase[kR1Groups];  // >= 0: the group's list is the identity run from there
     uint32_t count[kR1Groups];
     uint32_t bend[kR1Groups];
     // masked sums: sq_blocks workgroups behind the groups add the sum of
     // round32(a_c)^2 over the communities sq_verts[0, sq_count) into limbs
     // (4 arrays of kSpread slots, or G in a batch)
     const uint32_t *sq_verts;
+    int32_t sq_vbase;
     uint32_t sq_count, sq_blocks;
     unsigned long long *limbs;
 };

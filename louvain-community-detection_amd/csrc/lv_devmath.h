@@ -280,6 +280,11 @@ __device__ __forceinline__ void init_state_of(const DevView &g, uint32_t v,
     }
 }
 
+// vertex i of a launch's list (computed when the list is an identity run)
+__device__ __forceinline__ uint32_t vert_at(const DevView &g, const uint32_t *verts, uint32_t i) {
+    return g.vbase >= 0 ? (uint32_t)g.vbase + i : verts[i];
+}
+
 // ---- applying a step's moves (assignCommunity, modularity.cu:382-391, and
 // computeNewCommWeights, :393-398, as exact incremental updates) -------------
 

@@ -126,6 +126,7 @@ struct lv_graph {
     // scratch for the renumbering under it
     uint32_t *lid = nullptr, *pmap = nullptr, *rtmp = nullptr, *rrank = nullptr;
     bool relabeled = false;  // the current pass graph is in locality order
+    bool order_ident = false;  // `order` is the identity (classes in locality order)
     // the contraction wrote the graph in its locality order and classified it
     bool prelabeled = false;
     uint32_t cls_heavy_min = 0;  // heavy threshold of the current classes
@@ -297,6 +298,7 @@ lv::DevView view(lv_graph *g) {
     v.cu_verts = nullptr;
     v.cu_count = v.cu_nblk = 0;
     v.cu_moved = nullptr;
+    v.vbase = -1;
     return v;
 }
 
@@ -397,6 +399,7 @@ float fix2f_host(const lv_graph *g, int64_t x) { return (float)x * g->inv_scale;
 // Degree classes of the current graph: counts, offsets and `order` (the
 // vertices grouped by class).
 int classify_order(lv_graph *g) {
+    g->order_ident = false;
     zero_counters(g);
     {
         ProfScope ps(g, LV_K_CLASSIFY, 8.0 * g->n);
@@ -578,6 +581,7 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         for (int c = c0; c <= grp[i][1]; ++c) ent += (double)g->cls_deg[c];
         const uint32_t vpb = lv::kR1GroupVertices[i];
         L.verts[i] = g->order + a;
+        L.vbase[i] = g->order_ident ? (int32_t)a : -1;
         L.count[i] = b - a;
         nb += (b - a + vpb - 1) / vpb;
         L.bend[i] = nb;
@@ -585,6 +589,7 @@ int r1_all(lv_graph *g, int masked, int init_state) {
     }
     if (masked) {  // the sum of a_c^2 of the same snapshot rides in the same launch
         L.sq_verts = g->order;
+        L.sq_vbase = g->order_ident ? 0 : -1;
         L.sq_count = g->cls_off[kNumClasses];  // every listed vertex (a batch: active graphs')
         L.sq_blocks = lv::sumsq_blocks(L.sq_count);
         L.limbs = limbs_ptr(g);
@@ -639,6 +644,7 @@ void relabel(lv_graph *g) {
     const uint32_t n = g->n;
     ProfScope ps(g, LV_K_RELABEL, 20.0 * n + 16.0 * (double)g->ne);
     lv::launch_relabel_map(s, g->order, n, g->lid, g->pmap, g->row, g->rtmp);
+    g->order_ident = true;  // relabel_map leaves order as the identity
     lv::launch_scan_u32(s, g->rtmp, 0, n, g->row2, g->scan_tmp32);
     const lv::RelabelView r{g->row, g->col, g->w, g->row2, g->col2, g->w2, g->lid, g->pmap};
     lv::launch_relabel_rows(s, r, g->cls_off[1], g->cls_off[5], 8);
@@ -772,11 +778,13 @@ int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
                      12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c] + cub, st);
         if (c == lv::kClassHeavy)
             lv::launch_heavy_move(st, v, heavy_view(g));
-        else
-            lv::launch_move_class(st, first ? cu : v, c, g->order + g->cls_off[c],
-                                  g->cls_count[c],
+        else {
+            lv::DevView vc = first ? cu : v;
+            vc.vbase = g->order_ident ? (int32_t)g->cls_off[c] : -1;
+            lv::launch_move_class(st, vc, c, g->order + g->cls_off[c], g->cls_count[c],
                                   lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
                                                 g->tune.class8_slots, narrow_min()});
+        }
         caught |= first;
     }
     if (fork) {
@@ -806,12 +814,15 @@ int step_apply(lv_graph *g, int b) {
         if (!g->cls_count[c]) continue;
         hipStream_t st = (fork && c == lv::kClassHeavy) ? g->stream2 : g->stream;
         ProfScope ps(g, kClassFamily[c], 12.0 * (double)g->cls_deg[c] + 28.0 * g->cls_count[c], st);
-        if (c == lv::kClassHeavy)
+        if (c == lv::kClassHeavy) {
             lv::launch_heavy_move(st, v, heavy_view(g));
-        else
-            lv::launch_move_class(st, v, c, g->order + g->cls_off[c], g->cls_count[c],
+        } else {
+            lv::DevView vc = v;
+            vc.vbase = g->order_ident ? (int32_t)g->cls_off[c] : -1;
+            lv::launch_move_class(st, vc, c, g->order + g->cls_off[c], g->cls_count[c],
                                   lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
                                                 g->tune.class8_slots, narrow_min()});
+        }
     }
     if (fork) {
         (void)hipEventRecord(g->ev_join, g->stream2);
@@ -820,7 +831,9 @@ int step_apply(lv_graph *g, int b) {
     const uint32_t a = g->cls_off[c0], e = g->cls_off[c1 + 1];
     if (e > a) {
         ProfScope ps(g, LV_K_APPLY, 16.0 * (e - a));
-        lv::launch_apply(g->stream, v, g->order + a, e - a, moved_ptr(g));
+        lv::DevView va = v;
+        va.vbase = g->order_ident ? (int32_t)a : -1;
+        lv::launch_apply(g->stream, va, g->order + a, e - a, moved_ptr(g));
     }
     return LV_OK;
 }
@@ -1278,6 +1291,7 @@ int aggregation(lv_graph *g) {
             return rc;
         }
         lv::launch_relabel_map(s, g->order, nn, g->lid, g->pmap, g->row2, g->rtmp);
+        g->order_ident = true;
         lv::launch_scan_u32(s, g->rtmp, 0, nn, old_row, g->scan_tmp32);
         t.rows = old_row;
         t.map = g->pmap;

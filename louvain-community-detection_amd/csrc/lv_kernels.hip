@@ -364,16 +364,16 @@ void launch_classify_scatter(hipStream_t s, const uint32_t *row, uint32_t n, uin
 // One workgroup of 256 threads: vertices [bx * 256/G, (bx + 1) * 256/G) of
 // verts.
 template <int G, bool MASKED, int B>
-__device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts, uint32_t count,
-                                         int init_state, int ident, unsigned long long *acc,
-                                         uint32_t bx) {
+__device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts, int32_t vb,
+                                         uint32_t count, int init_state, int ident,
+                                         unsigned long long *acc, uint32_t bx) {
     constexpr int GPB = 256 / G;
     const int lane = threadIdx.x % G;
     const uint32_t vi = bx * GPB + threadIdx.x / G;
     unsigned long long fixed = 0;
     uint32_t slot = 0;
     bool valid = false;
-    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const uint32_t v = vi < count ? (vb >= 0 ? (uint32_t)vb + vi : verts[vi]) : 0u;
     if (vi < count && v_live(g, v)) {
         const uint32_t base = g.row[v];
         const uint32_t d = g.row[v + 1] - base;
@@ -439,9 +439,9 @@ __device__ __forceinline__ void r1_group(const DevView &g, const uint32_t *verts
 // vertices per wave in flight: these sums are bound by the latency of their
 // dependent loads (row, column, community, weight), not by their bytes.
 template <int L, int Q, int R, bool MASKED>
-__device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *verts, uint32_t count,
-                                          int init_state, int ident, unsigned long long *acc,
-                                          uint32_t bx) {
+__device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *verts, int32_t vb,
+                                          uint32_t count, int init_state, int ident,
+                                          unsigned long long *acc, uint32_t bx) {
     static_assert(L % Q == 0 && L <= 32 && (L & (L - 1)) == 0 && (R == 1 || L == 32),
                   "lane shape");
     constexpr int V = L / Q, VPB = 256 / Q;
@@ -450,7 +450,7 @@ __device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *vert
     unsigned long long fixed = 0;
     uint32_t slot = 0;
     bool valid = false;
-    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const uint32_t v = vi < count ? (vb >= 0 ? (uint32_t)vb + vi : verts[vi]) : 0u;
     if (vi < count && v_live(g, v)) {
         const uint32_t base = g.row[v];
         const uint32_t d = g.row[v + 1] - base;
@@ -526,7 +526,7 @@ __device__ __forceinline__ void r1_narrow(const DevView &g, const uint32_t *vert
 // the allocation.
 // One workgroup of 256 threads: vertices [8 bx, 8 bx + 8) of verts.
 template <int BK>
-__device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *verts,
+__device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *verts, int32_t vb,
                                            uint32_t count, int ident, unsigned long long *acc,
                                            uint32_t bx) {
     __shared__ float xs[8][BK * 128];
@@ -536,7 +536,7 @@ __device__ __forceinline__ void r1_group32(const DevView &g, const uint32_t *ver
     unsigned long long fixed = 0;
     uint32_t slot = 0;
     bool valid = false;
-    const uint32_t v = vi < count ? verts[vi] : 0u;
+    const uint32_t v = vi < count ? (vb >= 0 ? (uint32_t)vb + vi : verts[vi]) : 0u;
     if (vi < count && v_live(g, v)) {
         const uint32_t base = g.row[v], end = g.row[v + 1];
         const uint32_t cv = g.comm[v];
@@ -616,7 +616,7 @@ __device__ __forceinline__ void sumsq_body(const DevView &g, const uint32_t *ver
         unsigned long long l0 = 0, l1 = 0, l2 = 0, l3 = 0;
         uint32_t slot = 0;
         bool valid = false;
-        const uint32_t c = i < count ? verts[i] : 0u;
+        const uint32_t c = i < count ? vert_at(g, verts, i) : 0u;
         const unsigned long long a = (i < count && v_live(g, c)) ? g.A[c] : 0ull;
         if (a) {
         valid = true;
@@ -674,29 +674,32 @@ __global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int in
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const uint32_t b = blockIdx.x;
     if (b >= L.bend[0]) {  // behind the groups: the sum of a_c^2 of the same snapshot
-        sumsq_body(g, L.sq_verts, L.sq_count, L.limbs, b - L.bend[0], L.sq_blocks);
+        DevView gs = g;
+        gs.vbase = L.sq_vbase;
+        sumsq_body(gs, L.sq_verts, L.sq_count, L.limbs, b - L.bend[0], L.sq_blocks);
         return;
     }
     if (b < L.bend[5]) {
         if (MASKED)
-            r1_group32<2>(g, L.verts[5], L.count[5], ident, acc, b);
+            r1_group32<2>(g, L.verts[5], L.vbase[5], L.count[5], ident, acc, b);
         else
-            r1_group<32, false, 8>(g, L.verts[5], L.count[5], init_state, 0, acc, b);
+            r1_group<32, false, 8>(g, L.verts[5], L.vbase[5], L.count[5], init_state, 0, acc,
+                                   b);
     } else if (b < L.bend[4]) {
-        r1_narrow<32, 8, 3, MASKED>(g, L.verts[4], L.count[4], init_state, ident, acc,
-                                    b - L.bend[5]);
+        r1_narrow<32, 8, 3, MASKED>(g, L.verts[4], L.vbase[4], L.count[4], init_state, ident,
+                                    acc, b - L.bend[5]);
     } else if (b < L.bend[3]) {
-        r1_narrow<32, 8, 1, MASKED>(g, L.verts[3], L.count[3], init_state, ident, acc,
-                                    b - L.bend[4]);
+        r1_narrow<32, 8, 1, MASKED>(g, L.verts[3], L.vbase[3], L.count[3], init_state, ident,
+                                    acc, b - L.bend[4]);
     } else if (b < L.bend[2]) {
-        r1_narrow<16, 4, 1, MASKED>(g, L.verts[2], L.count[2], init_state, ident, acc,
-                                    b - L.bend[3]);
+        r1_narrow<16, 4, 1, MASKED>(g, L.verts[2], L.vbase[2], L.count[2], init_state, ident,
+                                    acc, b - L.bend[3]);
     } else if (b < L.bend[1]) {
-        r1_narrow<8, 2, 1, MASKED>(g, L.verts[1], L.count[1], init_state, ident, acc,
-                                   b - L.bend[2]);
+        r1_narrow<8, 2, 1, MASKED>(g, L.verts[1], L.vbase[1], L.count[1], init_state, ident,
+                                   acc, b - L.bend[2]);
     } else {
-        r1_narrow<4, 1, 1, MASKED>(g, L.verts[0], L.count[0], init_state, ident, acc,
-                                   b - L.bend[1]);
+        r1_narrow<4, 1, 1, MASKED>(g, L.verts[0], L.vbase[0], L.count[0], init_state, ident,
+                                   acc, b - L.bend[1]);
     }
 }
 
@@ -749,7 +752,7 @@ __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *v
     // groups past the end read the last vertex (count >= 1) and write nothing;
     // the vertex state loads are unconditional so none of them waits on a branch
     const uint32_t vi = bid * GPB + grp;
-    const uint32_t v = verts[vi < count ? vi : count - 1u];
+    const uint32_t v = vert_at(g, verts, vi < count ? vi : count - 1u);
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
     // a batch) skips the loads below
@@ -864,7 +867,7 @@ __global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *
     // groups past the end read the last vertex (count >= 1) and write nothing;
     // the vertex state loads are unconditional so none of them waits on a branch
     const uint32_t vi = bid * GPB + grp;
-    const uint32_t v = verts[vi < count ? vi : count - 1u];
+    const uint32_t v = vert_at(g, verts, vi < count ? vi : count - 1u);
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
     // a batch) skips the loads below; the barrier also orders the table
@@ -1003,7 +1006,7 @@ __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const ui
     }
     // waves past the end read the last vertex (count >= 1) and write nothing
     const uint32_t vi = bid * kWaveWpb + wv;
-    const uint32_t v = verts[vi < count ? vi : count - 1u];
+    const uint32_t v = vert_at(g, verts, vi < count ? vi : count - 1u);
     const bool active = vi < count && v_live(g, v);
     // a workgroup with no live vertex (past the end, or only frozen graphs of
     // a batch) skips the loads below
@@ -1175,7 +1178,7 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
         ndist = 0;
         scew = 0;
     }
-    const uint32_t v = verts[bid];
+    const uint32_t v = vert_at(g, verts, bid);
     if (!v_live(g, v)) return;  // uniform over the workgroup
     const float m = v_m(g, v);
     const uint32_t base = g.row[v];
@@ -1386,7 +1389,7 @@ __global__ __launch_bounds__(256) void k_apply(DevView g, const uint32_t *verts,
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long mv = 0;
-    const uint32_t v = i < count ? verts[i] : 0u;
+    const uint32_t v = i < count ? vert_at(g, verts, i) : 0u;
     const bool valid = i < count && v_live(g, v);
     if (valid) {
         const uint32_t o = g.comm[v], nc = g.new_comm[v];
