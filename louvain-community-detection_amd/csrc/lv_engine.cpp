@@ -549,8 +549,10 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 // (k_r1_groups).
 // masked: 0 k_v, 1 s_v, 2 s_v with singleton communities (pass init).  The
 // masked sums skip class 0: an empty row adds nothing.
-int r1_all(lv_graph *g, int masked, int init_state) {
-    lv::DevView v = view(g);
+// base: the view to read (two copies: the copy the sweep's last step wrote,
+// carrying that step's catch-up into the other copy), view(g) if null.
+int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nullptr) {
+    lv::DevView v = base ? *base : view(g);
     if (init_state) {  // pass init sets both copies of the community state
         v.comm2 = g->comm2;
         v.comm_sz2 = g->comm_sz2;
@@ -596,7 +598,7 @@ int r1_all(lv_graph *g, int masked, int init_state) {
         bytes += 8.0 * g->n;
     }
     {
-        ProfScope ps(g, LV_K_R1, bytes);
+        ProfScope ps(g, LV_K_R1, bytes + 16.0 * v.cu_count);
         lv::launch_r1_groups(g->stream, v, L, masked, init_state, acc_ptr(g));
     }
     if (fork) {
@@ -622,9 +624,10 @@ float q_from_counters(const lv_graph *g) {
 }
 
 // launches the S_in and sum a^2 reductions into zeroed counters (the sum of
-// a^2 in the masked sums' launch)
-void modularity_launch(lv_graph *g, bool singletons = false) {
-    r1_all(g, singletons ? 2 : 1, 0);
+// a^2 in the masked sums' launch); base as for r1_all
+void modularity_launch(lv_graph *g, bool singletons = false,
+                       const lv::DevView *base = nullptr) {
+    r1_all(g, singletons ? 2 : 1, 0, base);
 }
 
 // Locality order for the pass (single-graph mode): position x of the new
@@ -839,13 +842,15 @@ int step_apply(lv_graph *g, int b) {
 }
 
 // A sweep's bucket steps 1-7 (those with vertices).  Two copies: each step
-// reads the copy its predecessor wrote, then the last step's moves are
-// applied to the copy it read, so both copies hold the sweep's result, as
-// they held its start.
-void sweep_steps(lv_graph *g) {
+// reads the copy its predecessor wrote; the copy the last step wrote holds
+// the sweep's result, and the last step's moves still have to be applied to
+// the copy it read.  Returns the view of the sweep's result with that
+// catch-up attached, for the masked sums' launch (modularity_launch): after it both
+// copies hold the sweep's result, as they held its start.  One copy: view(g).
+lv::DevView sweep_steps(lv_graph *g) {
     if (!fused_pass(g)) {
         for (int b = 1; b <= 7; ++b) step_apply(g, b);
-        return;
+        return view(g);
     }
     uint32_t j = 0, pa = 0, pe = 0;
     for (int b = 1; b <= 7; ++b) {
@@ -857,20 +862,18 @@ void sweep_steps(lv_graph *g) {
         pe = e;
         ++j;
     }
-    if (!j) return;
     lv::DevView v = step_view(g, j);  // reads the copy the last step wrote
     v.cu_verts = g->order + pa;
     v.cu_count = pe - pa;
-    ProfScope ps(g, LV_K_APPLY, 16.0 * (pe - pa));
-    lv::launch_catchup(g->stream, v);
+    return v;
 }
 
 // the device work of one sweep, ending with the counters' D2H; no host sync
 // (this sequence is what a pass captures into a HIP graph)
 void sweep_enqueue(lv_graph *g) {
     zero_counters(g);
-    sweep_steps(g);
-    modularity_launch(g);
+    const lv::DevView v = sweep_steps(g);
+    modularity_launch(g, false, &v);
     counters_d2h(g);
 }
 
@@ -969,8 +972,8 @@ uint32_t sweeps_per_launch(const lv_graph *g) {
 // loop state to the host's mapped copy; the counters must be zero at its
 // start (the collapse leaves them so for the next sweep)
 void sweep_enqueue_decide(lv_graph *g, float min_gain) {
-    sweep_steps(g);
-    modularity_launch(g);
+    const lv::DevView v = sweep_steps(g);
+    modularity_launch(g, false, &v);
     lv::SweepTail t;
     t.spread = g->dc->acc;
     t.tot = g->dc->tot;
@@ -1389,8 +1392,8 @@ int batch_pass(lv_graph *g, float min_gain, bool *more) {
     // summary to the host's mapped copy)
     auto enqueue = [&] {
         for (uint32_t k = 0; k < K; ++k) {
-            sweep_steps(g);
-            modularity_launch(g);
+            const lv::DevView v = sweep_steps(g);
+            modularity_launch(g, false, &v);
             lv::launch_batch_decide(g->stream, G, g->gcnt, &g->dc->err, g->S, min_gain,
                                     kMaxIters, g->voff_d, g->gstate_d, g->gst_d, g->bsum_d);
             lv::launch_batch_halt(g->stream, g->bsum_d, g->bsum_hd);

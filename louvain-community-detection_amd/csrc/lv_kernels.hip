@@ -672,7 +672,10 @@ template <bool MASKED>
 __global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int init_state,
                                                    int ident, unsigned long long *acc) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    const uint32_t b = blockIdx.x;
+    // in front: the last bucket step's moves, applied to the copy of the
+    // state that the sums do not read (two copies)
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t b = blockIdx.x - g.cu_nblk;
     if (b >= L.bend[0]) {  // behind the groups: the sum of a_c^2 of the same snapshot
         DevView gs = g;
         gs.vbase = L.sq_vbase;
@@ -703,15 +706,18 @@ __global__ __launch_bounds__(256) void k_r1_groups(DevView g, R1Groups L, int in
     }
 }
 
+uint32_t catchup_blocks(uint32_t count, uint32_t threads);
 void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int masked,
                       int init_state, unsigned long long *acc) {
-    const uint32_t nb = L.bend[0] + (masked ? L.sq_blocks : 0u);
+    DevView gv = g;
+    gv.cu_nblk = g.cu_count ? catchup_blocks(g.cu_count, 256) : 0u;
+    const uint32_t nb = gv.cu_nblk + L.bend[0] + (masked ? L.sq_blocks : 0u);
     if (!nb) return;
     if (masked)
-        hipLaunchKernelGGL(k_r1_groups<true>, dim3(nb), dim3(256), 0, s, g, L, init_state,
+        hipLaunchKernelGGL(k_r1_groups<true>, dim3(nb), dim3(256), 0, s, gv, L, init_state,
                            masked == 2, acc);
     else
-        hipLaunchKernelGGL(k_r1_groups<false>, dim3(nb), dim3(256), 0, s, g, L, init_state, 0,
+        hipLaunchKernelGGL(k_r1_groups<false>, dim3(nb), dim3(256), 0, s, gv, L, init_state, 0,
                            acc);
 }
 
