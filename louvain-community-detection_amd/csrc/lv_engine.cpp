@@ -98,6 +98,8 @@ struct lv_graph {
     // the sweep graph as two branches)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // ends of the two captured sweep launches in flight (pipelined pass loop)
+    hipEvent_t ev_launch[2] = {nullptr, nullptr};
     uint32_t n = 0, n0 = 0;
     uint64_t ne = 0, ne0 = 0;
     uint32_t n_cap = 0;  // buffer capacities (lv_graph_create: the input's size)
@@ -961,6 +963,22 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
 // its completion wait are about 20 us), so one sweep per launch is the
 // default.  A profiled run always takes one, so every timed launch did its
 // work.
+// Whether the host keeps the next captured launch queued behind the one it
+// waits for.  The device then never idles through the host's wake-up, read
+// of the loop state and next submission (about 25-30 us per launch); the
+// queued launch after the stop returns at entry in every kernel (halt
+// flag).  That flag costs every kernel a dependent load, so only pass graphs
+// of at most 2^24 entries, whose sweeps take well under a millisecond, are
+// pipelined (FEM band -2 %, co-purchase -4 %; R-MAT-24 ran 0.4 % slower).
+// LV_PIPELINE=0/1 forces one way.  A profiled run waits for each launch: its
+// event pairs are re-recorded by every replay.
+bool pipelined(const lv_graph *g) {
+    if (g->prof_on) return false;
+    static const char *env = getenv("LV_PIPELINE");
+    if (env && *env) return atoi(env) != 0;
+    return g->ne <= (1ull << 24);
+}
+
 uint32_t sweeps_per_launch(const lv_graph *g) {
     if (g->prof_on) return 1;
     static const char *env = getenv("LV_SWEEPS_PER_LAUNCH");
@@ -1005,20 +1023,38 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
         LV_HIP(hipMemcpyAsync(g->pst, g->pst_h, sizeof(lv::PassState), hipMemcpyHostToDevice,
                               g->stream));
         zero_counters(g);
-        g->halt_on = K > 1;  // kernels test the halt flag only when sweeps can follow the stop
+        // kernels test the halt flag only when sweeps can follow the stop
+        g->halt_on = K > 1 || pipelined(g);
         use_graph = capture(g, sg, [&] {
             for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain);
         });
         g->halt_on = false;
     }
     if (use_graph) {
-        for (;;) {
+        const bool pipe = pipelined(g);
+        if (pipe) {
             LV_HIP(hipGraphLaunch(sg.exec, g->stream));
-            LV_HIP(hipStreamSynchronize(g->stream));
+            LV_HIP(hipEventRecord(g->ev_launch[0], g->stream));
+        }
+        for (uint32_t i = 0;; ++i) {
+            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            if (pipe) {
+                // launch i + 1 is queued: wait for launch i only
+                LV_HIP(hipEventRecord(g->ev_launch[(i + 1) & 1u], g->stream));
+                LV_HIP(hipEventSynchronize(g->ev_launch[i & 1u]));
+            } else {
+                LV_HIP(hipStreamSynchronize(g->stream));
+            }
             if (g->prof_on) sg.collect();
-            const lv::PassState p = *g->pst_h;
+            lv::PassState p = *g->pst_h;
+            if (!p.done && !p.err) continue;
+            // stopped: the queued launch returns at entry and leaves the
+            // loop state as it is; read it once nothing runs
+            if (pipe) {
+                LV_HIP(hipStreamSynchronize(g->stream));
+                p = *g->pst_h;
+            }
             if (p.err) return LV_ERR_OVERFLOW;
-            if (!p.done) continue;
             act = p.act;
             it = p.it;
             ret = p.ret != 0u;
@@ -1652,6 +1688,10 @@ void free_all(lv_graph *g) {
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
     g->ev_fork = g->ev_join = nullptr;
+    for (auto &e : g->ev_launch) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     if (g->stream2) (void)hipStreamDestroy(g->stream2);
     g->stream2 = nullptr;
     if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
@@ -1862,7 +1902,9 @@ int graph_new(int device, uint32_t n_cap, uint64_t e_cap, bool pristine, lv_grap
     g->stream = g->own_stream;
     if (hipStreamCreate(&g->stream2) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_launch[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_launch[1], hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         free_all(g);
         delete g;

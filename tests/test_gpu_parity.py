@@ -418,15 +418,17 @@ def test_narrow_move_kernels_everywhere(lv):
     assert "narrow kernels ok" in r.stdout
 
 
-@pytest.mark.parametrize("fused", ["0", "1"])
-def test_state_copies_either_way(lv, fused):
+@pytest.mark.parametrize("fused,pipe", [("0", "1"), ("1", "0")])
+def test_state_copies_either_way(lv, fused, pipe):
     """Passes of small single graphs sweep with two copies of the community
     state (a step's moves applied inside its kernels, the next step catching
     up on them); larger graphs and batches keep one copy and an apply kernel
     per step.  LV_FUSED forces one way for everything: the state after every
     bucket step, full runs with heavy rows (R-MAT-18, heavy threshold 512)
     and a batch give the oracle's results either way -- in a child process
-    so the setting does not leak."""
+    so the setting does not leak.  LV_PIPELINE crosses it: small graphs'
+    pass loops keep the next captured launch queued by default, large ones
+    do not; each copy mode runs here with the other choice."""
     import os
     import subprocess
     import sys
@@ -464,8 +466,9 @@ def test_state_copies_either_way(lv, fused):
         "    assert np.array_equal(assigns[i], o.init_comm())\n"
         "print('state copies ok')\n"
     ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
-    print(f"[case] state copies LV_FUSED={fused}", flush=True)
-    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, LV_FUSED=fused),
+    print(f"[case] state copies LV_FUSED={fused} LV_PIPELINE={pipe}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code],
+                       env=dict(os.environ, LV_FUSED=fused, LV_PIPELINE=pipe),
                        capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "state copies ok" in r.stdout
