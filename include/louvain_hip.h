@@ -146,8 +146,8 @@ typedef struct lv_tuning {
                                        1280 (default) or 2048 */
     uint32_t agg_bucket_entries;    /* contraction of large communities: target entries per
                                        key-range bucket, 256..512 (default 384) */
-    uint32_t sweeps_per_launch;     /* sweeps captured to one launch of a pass loop decided on
-                                       the device: 1 (default), 2 or 4 */
+    uint32_t sweeps_per_launch;     /* sweeps launched per wait of the host in a pass loop
+                                       decided on the device: 1 (default), 2 or 4 */
 } lv_tuning;
 int lv_tuning_default(lv_tuning *t);
 int lv_graph_set_tuning(lv_graph *g, const lv_tuning *t);  /* LV_ERR_ARG if out of range */
@@ -288,8 +288,8 @@ int lv_server_destroy(lv_server *server);
 /* ---- profiling ------------------------------------------------------------ */
 int lv_profile_enable(lv_graph *g, int on);
 /* Families that get event pairs while profiling is on (bit f = family f;
- * default all).  Event records captured into a sweep's HIP graph cost a few
- * microseconds each, so a timed run profiles only the family it reports. */
+ * default all).  An event record costs a few microseconds, so a timed run
+ * profiles only the family it reports. */
 int lv_profile_select(lv_graph *g, uint32_t family_mask);
 int lv_profile_get(lv_graph *g, lv_kernel_stats *out); /* synchronises */
 int lv_profile_reset(lv_graph *g);

@@ -60,6 +60,7 @@ struct Counters {
     ull acc[lv::kSpread];       // k/s fixed total
     ull limbs[4][lv::kSpread];  // sum a^2 limbs
     ull moved[lv::kSpread];
+    ull ktot;  // pass init's k total (host side; outside the head the sync copies)
 };
 constexpr size_t kCountersHead = offsetof(Counters, acc);
 static_assert(offsetof(Counters, limbs) == offsetof(Counters, acc) + lv::kSpread * 8, "layout");
@@ -418,10 +419,12 @@ int classify_order(lv_graph *g) {
         g->cls_deg[c] = g->hc->degsum[c];
         g->cls_off[c + 1] = g->cls_off[c] + g->cls_count[c];
     }
-    uint32_t cur[kNumClasses];
-    for (int c = 0; c < kNumClasses; ++c) cur[c] = g->cls_off[c];
-    LV_HIP(hipMemcpyAsync(g->dc->cursor, cur, sizeof(cur), hipMemcpyHostToDevice, g->stream));
-    LV_HIP(hipStreamSynchronize(g->stream));  // `cur` is pageable stack memory
+    // the scatter's cursors, staged in the host counters: the host writes
+    // them again only after a later synchronisation of the stream, so the
+    // copy needs no wait of its own
+    for (int c = 0; c < kNumClasses; ++c) g->hc->cursor[c] = g->cls_off[c];
+    LV_HIP(hipMemcpyAsync(g->dc->cursor, g->hc->cursor, sizeof(g->hc->cursor),
+                          hipMemcpyHostToDevice, g->stream));
     {
         ProfScope ps(g, LV_K_CLASSIFY, 12.0 * g->n);
         // one graph: runs of up to 256 vertices per thread while at least
@@ -677,17 +680,20 @@ int pass_init(lv_graph *g, float *q0) {
     g->prelabeled = false;
     zero_counters(g);
     r1_all(g, 0, 1);
-    rc = sync_counters(g);
-    if (rc) return rc;
-    g->m = fix2f_host(g, (int64_t)g->hc->tot[0]) / 2.0f;
-    if (!(g->m > 0.0f)) {
-        // zero total weight: the reference divides by zero and never stops
-        return LV_ERR_RANGE;
-    }
+    // the k total goes to its own host slot, and the singletons' modularity
+    // sums (which do not use m) run behind it: one round trip for both
+    lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
+    LV_HIP(hipMemcpyAsync(&g->hc->ktot, &g->dc->tot[0], sizeof(ull), hipMemcpyDeviceToHost,
+                          g->stream));
     zero_counters(g);
     modularity_launch(g, true);  // every community is a singleton here
     rc = sync_counters(g);
     if (rc) return rc;
+    g->m = fix2f_host(g, (int64_t)g->hc->ktot) / 2.0f;
+    if (!(g->m > 0.0f)) {
+        // zero total weight: the reference divides by zero and never stops
+        return LV_ERR_RANGE;
+    }
     g->pass_ready = true;
     if (q0) *q0 = q_from_counters(g);
     return LV_OK;
@@ -923,9 +929,30 @@ struct SweepGraph {
     }
 };
 
-// A carved handle launches eagerly: instantiating a HIP graph allocates.
+// Whether a pass's sweeps are captured once into a HIP graph and replayed
+// (LV_GRAPH=1) instead of launched kernel by kernel.  Measured on MI355X
+// with ROCm 7, replaying the captured sweep is slower than launching its
+// kernels from the host, by 0.6 % on R-MAT-24, 1.1 % on the 4096-graph
+// batch and 2-4 % on the FEM band and co-purchase graphs (every result
+// identical), so kernels are launched directly by default.  A carved handle
+// never captures: instantiating a HIP graph allocates.
 bool graphs_enabled(const lv_graph *g) {
-    static const bool off = getenv("LV_NO_GRAPH") != nullptr;
+    static const bool on = [] {
+        const char *e = getenv("LV_GRAPH");
+        return e && atoi(e) != 0;
+    }();
+    return on && !g->carve;
+}
+
+// Whether the pass loop's stop rule runs on the device (k_collapse_decide,
+// read by the host from mapped memory) rather than on the host after a copy
+// of the counters (LV_HOST_LOOP=1, and a carved handle, which has no mapped
+// host memory).
+bool device_loop(const lv_graph *g) {
+    static const bool off = [] {
+        const char *e = getenv("LV_HOST_LOOP");
+        return e && atoi(e) != 0;
+    }();
     return !off && !g->carve;
 }
 
@@ -1013,31 +1040,35 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
     bool first = true, ret = true;
     uint32_t it = 0;
     // The loop of modularity.cu:464-508 with its stop rule evaluated on the
-    // device after every sweep: the host launches a captured group of
-    // sweeps and reads the loop state once per group.
+    // device after every sweep: the host enqueues a group of sweeps (or
+    // replays it as a captured graph) and reads the loop state once per group.
     SweepGraph sg;
-    bool use_graph = false;
-    if (graphs_enabled(g)) {
+    if (device_loop(g)) {
         const uint32_t K = sweeps_per_launch(g);
+        const bool pipe = pipelined(g);
+        // kernels test the halt flag only when sweeps can follow the stop
+        const bool halt = K > 1 || pipe;
         *g->pst_h = lv::PassState{act, 0.0f, 0.0f, 0u, 1u, 1u, 0u, 0u};
         LV_HIP(hipMemcpyAsync(g->pst, g->pst_h, sizeof(lv::PassState), hipMemcpyHostToDevice,
                               g->stream));
         zero_counters(g);
-        // kernels test the halt flag only when sweeps can follow the stop
-        g->halt_on = K > 1 || pipelined(g);
-        use_graph = capture(g, sg, [&] {
+        auto enqueue = [&] {
+            g->halt_on = halt;
             for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain);
-        });
-        g->halt_on = false;
-    }
-    if (use_graph) {
-        const bool pipe = pipelined(g);
+            g->halt_on = false;
+        };
+        const bool use_graph = graphs_enabled(g) && capture(g, sg, enqueue);
+        auto launch = [&]() -> hipError_t {
+            if (use_graph) return hipGraphLaunch(sg.exec, g->stream);
+            enqueue();
+            return hipSuccess;
+        };
         if (pipe) {
-            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            LV_HIP(launch());
             LV_HIP(hipEventRecord(g->ev_launch[0], g->stream));
         }
         for (uint32_t i = 0;; ++i) {
-            LV_HIP(hipGraphLaunch(sg.exec, g->stream));
+            LV_HIP(launch());
             if (pipe) {
                 // launch i + 1 is queued: wait for launch i only
                 LV_HIP(hipEventRecord(g->ev_launch[(i + 1) & 1u], g->stream));
@@ -1045,7 +1076,7 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
             } else {
                 LV_HIP(hipStreamSynchronize(g->stream));
             }
-            if (g->prof_on) sg.collect();
+            if (use_graph && g->prof_on) sg.collect();
             lv::PassState p = *g->pst_h;
             if (!p.done && !p.err) continue;
             // stopped: the queued launch returns at entry and leaves the
@@ -1061,7 +1092,7 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
             if (!ret && q_print) *q_print = p.q_print;
             break;
         }
-    } else {  // eager launches, the stop rule on the host after each sweep
+    } else {  // the stop rule on the host after each sweep
         for (;;) {
             uint64_t moved = 0;
             old = act;
@@ -1265,9 +1296,10 @@ int aggregation(lv_graph *g) {
     for (int c = 0; c < kNumAggClasses; ++c) aoff[c + 1] = aoff[c] + g->hc->acounts[c];
     if (aoff[kNumAggClasses] != nn) return LV_ERR_OVERFLOW;
     g->agg_nn = nn;
-    LV_HIP(hipMemcpyAsync(g->dc->acursor, aoff, sizeof(uint32_t) * kNumAggClasses,
+    // staged in the host counters, as classify_order's cursors
+    for (int c = 0; c < kNumAggClasses; ++c) g->hc->acursor[c] = aoff[c];
+    LV_HIP(hipMemcpyAsync(g->dc->acursor, g->hc->acursor, sizeof(g->hc->acursor),
                           hipMemcpyHostToDevice, s));
-    LV_HIP(hipStreamSynchronize(s));  // `aoff` is pageable stack memory
     lv::launch_agg_classify_scatter(s, v, aview(g), g->dc->acursor, g->aorder);
     lv::launch_fill_u32(s, g->cnt, 0u, nn);  // edgeless communities keep 0
     int passes = 1;  // radix digits of a new id

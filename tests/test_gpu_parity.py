@@ -305,16 +305,18 @@ def test_batch_reset_repeatable(lv, orc, bcsstk16_text):
     assert all(np.array_equal(x, y) for x, y in zip(a1, a2))
 
 
-@pytest.mark.parametrize("mode", ["1", "2", "dmax", "nograph", "onestream"])
+@pytest.mark.parametrize("mode", ["1", "2", "dmax", "hostloop", "graph", "onestream"])
 def test_heavy_global_table_path(lv, orc, mode):
     """The global-table kernels that take over heavy vertices whose hash
     buckets overflow LDS: forced for every heavy vertex (LV_HEAVY_GLOBAL=1),
     for every other one after the partitioned path ran (=2), and through
     buckets that really overflow a reduced capacity (LV_HEAVY_DMAX) so both
-    paths serve one step; the eager launch path (LV_NO_GRAPH, the heavy
-    chain and the block classes on two live streams instead of graph
-    branches); and every chain on one stream (LV_ONE_STREAM, as profiler
-    runs use). Each in a child process so the setting does not leak."""
+    paths serve one step; the stop rule on the host after every sweep
+    (LV_HOST_LOOP, as carved handles run); the sweeps captured once per pass
+    and replayed as a HIP graph (LV_GRAPH, the heavy chain and the block
+    classes as graph branches instead of two live streams); and every chain
+    on one stream (LV_ONE_STREAM, as profiler runs use). Each in a child
+    process so the setting does not leak."""
     import os
     import subprocess
     import sys
@@ -331,7 +333,8 @@ def test_heavy_global_table_path(lv, orc, mode):
         "print('global-table path ok')\n"
     ) % (os.path.join(lv.HERE), os.path.dirname(lv.HERE))
     env = dict(os.environ, **({"LV_HEAVY_DMAX": "48"} if mode == "dmax" else
-                              {"LV_NO_GRAPH": "1"} if mode == "nograph" else
+                              {"LV_HOST_LOOP": "1"} if mode == "hostloop" else
+                              {"LV_GRAPH": "1"} if mode == "graph" else
                               {"LV_ONE_STREAM": "1"} if mode == "onestream" else
                               {"LV_HEAVY_GLOBAL": mode}))
     print(f"[case] heavy global tables mode {mode}", flush=True)
