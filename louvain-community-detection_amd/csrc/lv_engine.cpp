@@ -379,6 +379,13 @@ int sync_counters(lv_graph *g) {
     LV_HIP(hipStreamSynchronize(g->stream));
     return LV_OK;
 }
+// The counters' head (class counts, cursors, error flag) without collapsing
+// the spread sums first: for readers of those fields only (one launch fewer).
+int sync_head(lv_graph *g) {
+    LV_HIP(hipMemcpyAsync(g->hc, g->dc, kCountersHead, hipMemcpyDeviceToHost, g->stream));
+    LV_HIP(hipStreamSynchronize(g->stream));
+    return LV_OK;
+}
 
 int zero_counters(lv_graph *g) {
     // small fill kernels instead of memset nodes (safe under stream capture)
@@ -411,7 +418,7 @@ int classify_order(lv_graph *g) {
                                   g->G ? g->gid : nullptr, g->gstate_d, g->dc->counts,
                                   g->dc->degsum);
     }
-    int rc = sync_counters(g);
+    int rc = sync_head(g);
     if (rc) return rc;
     g->cls_off[0] = 0;
     for (int c = 0; c < kNumClasses; ++c) {
@@ -1289,7 +1296,7 @@ int aggregation(lv_graph *g) {
     lv::launch_agg_classify_count(s, v, aview(g), g->dc->acounts);
     uint32_t nn = 0;
     LV_HIP(hipMemcpyAsync(&nn, g->new_id + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    int rc = sync_counters(g);
+    int rc = sync_head(g);
     if (rc) return rc;
     uint32_t aoff[kNumAggClasses + 1];
     aoff[0] = 0;
@@ -1348,7 +1355,7 @@ int aggregation(lv_graph *g) {
     const uint32_t old_n = g->n;
     const uint64_t old_ne = g->ne;
     if (pre) {
-        rc = sync_counters(g);  // the merges' overflow flag, before classify clears it
+        rc = sync_head(g);  // the merges' overflow flag, before classify clears it
         if (rc) return rc;
         if (g->hc->err) return LV_ERR_OVERFLOW;
         g->row = g->row2;  // by new id: the next pass's input numbering
@@ -1375,7 +1382,7 @@ int aggregation(lv_graph *g) {
         lv::launch_agg_emit(s, v, av, lc, nl, g->ap_cflag, t);
     }
     lv::launch_agg_compose(s, g->comm, g->new_id, t.map, g->init_comm, g->n0);
-    rc = sync_counters(g);
+    rc = sync_head(g);
     if (rc) return rc;
     if (g->hc->err) return LV_ERR_OVERFLOW;
     if (pre) {

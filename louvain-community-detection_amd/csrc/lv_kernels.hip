@@ -1533,11 +1533,40 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const T *in, int ki
     }
 }
 
+// Short arrays (at most kScanOneTiles tiles): one workgroup scans them tile
+// by tile with a running carry, one launch instead of three -- the passes of
+// small graphs are bound by the host's launches, not by the scans' work.
+constexpr uint32_t kScanOneTiles = 4;
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_one(const T *in, int kind, uint32_t n,
+                                                           T *out) {
+    T carry = 0;
+    for (size_t t0 = 0; t0 < n; t0 += kScanTile) {
+        const size_t b = t0 + (size_t)threadIdx.x * kScanItems;
+        T x[kScanItems];
+        T sum = 0;
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            x[i] = (b + i < n) ? scan_in(in, kind, b + i) : (T)0;
+            sum += x[i];
+        }
+        T tot;
+        T e = block_excl_scan<T>(sum, tot) + carry;
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            if (b + i < n) out[b + i] = e;
+            e += x[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
 template <typename T>
 static void scan_impl(hipStream_t s, const T *in, int kind, uint32_t n, T *out, T *tmp) {
     const uint32_t nt = (uint32_t)(((size_t)n + kScanTile - 1) / kScanTile);
-    if (n == 0) {
-        hipLaunchKernelGGL((k_scan_sums<T>), dim3(1), dim3(kScanThreads), 0, s, tmp, 0u, out, 0u);
+    if (nt <= kScanOneTiles) {  // n == 0 included: out[0] = 0
+        hipLaunchKernelGGL((k_scan_one<T>), dim3(1), dim3(kScanThreads), 0, s, in, kind, n, out);
         return;
     }
     hipLaunchKernelGGL((k_scan_tiles<T>), dim3(nt), dim3(kScanThreads), 0, s, in, kind, n, tmp);
