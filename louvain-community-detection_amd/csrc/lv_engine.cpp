@@ -60,7 +60,6 @@ struct Counters {
     ull acc[lv::kSpread];       // k/s fixed total
     ull limbs[4][lv::kSpread];  // sum a^2 limbs
     ull moved[lv::kSpread];
-    ull ktot;  // pass init's k total (host side; outside the head the sync copies)
 };
 constexpr size_t kCountersHead = offsetof(Counters, acc);
 static_assert(offsetof(Counters, limbs) == offsetof(Counters, acc) + lv::kSpread * 8, "layout");
@@ -563,7 +562,10 @@ lv::HeavySplit heavy_view(lv_graph *g) {
 // masked sums skip class 0: an empty row adds nothing.
 // base: the view to read (two copies: the copy the sweep's last step wrote,
 // carrying that step's catch-up into the other copy), view(g) if null.
-int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nullptr) {
+// acc: the spread slots the sums go to (acc_ptr(g) if null).
+int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nullptr,
+           ull *acc = nullptr) {
+    if (!acc) acc = acc_ptr(g);
     lv::DevView v = base ? *base : view(g);
     if (init_state) {  // pass init sets both copies of the community state
         v.comm2 = g->comm2;
@@ -580,7 +582,7 @@ int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nu
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy],
                      g->stream2);
         lv::launch_heavy_r1(g->stream2, v, heavy_view(g), (float *)g->eslot, masked, init_state,
-                            acc_ptr(g));
+                            acc);
     }
     // classes of each group (contiguous in `order`)
     const int grp[lv::kR1Groups][2] = {{0, 1}, {2, 2}, {3, 3}, {4, 4}, {5, 5},
@@ -611,7 +613,7 @@ int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nu
     }
     {
         ProfScope ps(g, LV_K_R1, bytes + 16.0 * v.cu_count);
-        lv::launch_r1_groups(g->stream, v, L, masked, init_state, acc_ptr(g));
+        lv::launch_r1_groups(g->stream, v, L, masked, init_state, acc);
     }
     if (fork) {
         (void)hipEventRecord(g->ev_join, g->stream2);
@@ -619,7 +621,7 @@ int r1_all(lv_graph *g, int masked, int init_state, const lv::DevView *base = nu
     } else if (g->cls_count[lv::kClassHeavy]) {  // heavy rows: staged, LDS-streamed pinned sums
         ProfScope ps(g, LV_K_R1, (masked ? 16.0 : 8.0) * (double)g->cls_deg[lv::kClassHeavy]);
         lv::launch_heavy_r1(g->stream, v, heavy_view(g), (float *)g->eslot, masked, init_state,
-                            acc_ptr(g));
+                            acc);
     }
     return LV_OK;
 }
@@ -686,17 +688,14 @@ int pass_init(lv_graph *g, float *q0) {
     }
     g->prelabeled = false;
     zero_counters(g);
-    r1_all(g, 0, 1);
-    // the k total goes to its own host slot, and the singletons' modularity
-    // sums (which do not use m) run behind it: one round trip for both
-    lv::launch_collapse(g->stream, g->dc->acc, 6, g->dc->tot);
-    LV_HIP(hipMemcpyAsync(&g->hc->ktot, &g->dc->tot[0], sizeof(ull), hipMemcpyDeviceToHost,
-                          g->stream));
-    zero_counters(g);
+    // the k total into the spread slots of the move count (unused until the
+    // first sweep), and the singletons' modularity sums (which do not use m)
+    // behind it: one collapse and one round trip for both
+    r1_all(g, 0, 1, nullptr, moved_ptr(g));
     modularity_launch(g, true);  // every community is a singleton here
     rc = sync_counters(g);
     if (rc) return rc;
-    g->m = fix2f_host(g, (int64_t)g->hc->ktot) / 2.0f;
+    g->m = fix2f_host(g, (int64_t)g->hc->tot[5]) / 2.0f;
     if (!(g->m > 0.0f)) {
         // zero total weight: the reference divides by zero and never stops
         return LV_ERR_RANGE;
