@@ -218,6 +218,8 @@ void launch_batch_reactivate(hipStream_t s, uint32_t G, uint32_t *gstate, GraphS
 // ---- launchers (lv_kernels.hip) ---------------------------------------------
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n);
+void launch_zero3_u32(hipStream_t s, uint32_t *p0, size_t n0, uint32_t *p1, size_t n1,
+                      uint32_t *p2, size_t n2);
 void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
 void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
                      unsigned long long *out);

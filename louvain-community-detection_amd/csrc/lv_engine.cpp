@@ -681,13 +681,15 @@ int pass_init(lv_graph *g, float *q0) {
     // a contraction that wrote the graph in its locality order classified it
     // already (again if the heavy threshold changed since; a graph in a
     // locality order keeps it)
+    bool zeroed = false;  // classify_order left every counter but its own head fields zero
     if (!g->prelabeled || g->cls_heavy_min != g->tune.heavy_min_degree) {
         rc = classify(g);
         if (rc) return rc;
+        zeroed = true;
         if (!g->relabeled && want_locality(g, g->n)) relabel(g);
     }
     g->prelabeled = false;
-    zero_counters(g);
+    if (!zeroed) zero_counters(g);
     // the k total into the spread slots of the move count (unused until the
     // first sweep), and the singletons' modularity sums (which do not use m)
     // behind it: one collapse and one round trip for both
@@ -1277,9 +1279,14 @@ int aggregation(lv_graph *g) {
     lv::DevView v = view(g);
     const uint32_t n = g->n;
     ProfScope ps(g, LV_K_AGGREGATE, 12.0 * g->ne + 24.0 * n);
-    zero_counters(g);
-    lv::launch_fill_u32(s, g->comm_deg, 0u, n);
-    lv::launch_fill_u32(s, g->mcursor, 0u, n);
+    if (g->G) {  // a batch also clears its per-graph counters
+        zero_counters(g);
+        lv::launch_fill_u32(s, g->comm_deg, 0u, n);
+        lv::launch_fill_u32(s, g->mcursor, 0u, n);
+    } else {
+        lv::launch_zero3_u32(s, reinterpret_cast<uint32_t *>(g->dc), sizeof(Counters) / 4,
+                             g->comm_deg, n, g->mcursor, n);
+    }
     lv::launch_agg_prep(s, v, aview(g));
     if (g->relabeled)  // rank in the input numbering
         lv::launch_relabel_new_id(s, g->comm_sz, g->lid, n, g->rtmp, g->rrank, g->new_id,

@@ -35,6 +35,17 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, size_t n
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
+// three u32 arrays zeroed by one launch (a pass boundary of a small graph
+// is bound by the host's launches)
+__global__ void k_zero3_u32(uint32_t *p0, size_t n0, uint32_t *p1, size_t n1, uint32_t *p2,
+                            size_t n2) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n0 + n1 + n2; i += stride) {
+        if (i < n0) p0[i] = 0u;
+        else if (i < n0 + n1) p1[i - n0] = 0u;
+        else p2[i - n0 - n1] = 0u;
+    }
+}
 static inline unsigned grid_for(size_t n, unsigned threads, unsigned cap = 65536) {
     size_t g = (n + threads - 1) / threads;
     if (g < 1) g = 1;
@@ -72,6 +83,13 @@ void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_zero3_u32(hipStream_t s, uint32_t *p0, size_t n0, uint32_t *p1, size_t n1,
+                      uint32_t *p2, size_t n2) {
+    const size_t n = n0 + n1 + n2;
+    if (!n) return;
+    hipLaunchKernelGGL(k_zero3_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p0, n0, p1, n1, p2,
+                       n2);
 }
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n) {
     if (!n) return;
