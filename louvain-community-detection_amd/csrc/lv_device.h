@@ -220,6 +220,12 @@ void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, size_t n);
 void launch_zero3_u32(hipStream_t s, uint32_t *p0, size_t n0, uint32_t *p1, size_t n1,
                       uint32_t *p2, size_t n2);
+// whole contraction of a pass graph of at most kTinyN vertices and kTinyE
+// entries (single graph, input numbering) in one workgroup; sizes[0..1] <-
+// new vertex and entry counts
+constexpr uint32_t kTinyN = 1024, kTinyE = 4096, kTinyThreads = 1024;
+void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *init_comm,
+                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes);
 void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
 void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
                      unsigned long long *out);

@@ -1279,6 +1279,25 @@ int aggregation(lv_graph *g) {
     lv::DevView v = view(g);
     const uint32_t n = g->n;
     ProfScope ps(g, LV_K_AGGREGATE, 12.0 * g->ne + 24.0 * n);
+    // a tiny single graph in its input numbering: one workgroup does it all
+    if (!g->G && !g->relabeled && !want_locality(g, n) && n <= lv::kTinyN &&
+        g->ne <= lv::kTinyE) {
+        lv::launch_agg_tiny(s, v, g->n0, g->init_comm, g->row2, g->col2, g->w2, g->dc->acursor);
+        int rc = sync_head(g);
+        if (rc) return rc;
+        const uint32_t nn = g->hc->acursor[0], ne2 = g->hc->acursor[1];
+        if (nn > n || ne2 > g->ne) return LV_ERR_OVERFLOW;
+        std::swap(g->row, g->row2);
+        std::swap(g->col, g->col2);
+        std::swap(g->w, g->w2);
+        g->agg_nn = nn;
+        g->n = nn;
+        g->ne = ne2;
+        g->pass_ready = false;
+        g->relabeled = false;
+        g->prelabeled = false;
+        return LV_OK;
+    }
     if (g->G) {  // a batch also clears its per-graph counters
         zero_counters(g);
         lv::launch_fill_u32(s, g->comm_deg, 0u, n);

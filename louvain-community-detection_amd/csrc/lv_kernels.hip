@@ -2727,6 +2727,139 @@ void launch_aggp_emit(hipStream_t s, const DevView &g, AggView a, const AggPart 
     hipLaunchKernelGGL(k_aggp_emit, dim3(p.NB), dim3(256), 0, s, g, a, p, comms, t);
 }
 
+// ---------------------------------------------------------------------------
+// The whole contraction of a tiny pass graph (at most kTinyN vertices and
+// kTinyE entries) in one workgroup: new ids (non-empty communities in id
+// order, aggregation.cu:21-36,390-393), every entry keyed by (new id of its
+// row's community, new id of its column's community) and sorted in LDS,
+// exact sums per key, rows in ascending neighbour id (the order the merge
+// kernels give), and init_comm composed through comm
+// (aggregation.cu:380-385).  One launch instead of about thirty: the
+// boundaries between the passes of a small graph are bound by the host's
+// launches.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ uint32_t wg_excl_scan(uint32_t x, uint32_t &total, uint32_t *wsum) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += y;
+    }
+    if (lane == 63u) wsum[wv] = v;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t s = wsum[i];
+        if ((uint32_t)i < wv) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + v - x;
+}
+
+__global__ __launch_bounds__(kTinyThreads) void k_agg_tiny(DevView g, uint32_t n0,
+                                                           uint32_t *init_comm, uint32_t *orow,
+                                                           uint32_t *ocol, float *ow,
+                                                           uint32_t *sizes) {
+    constexpr uint32_t PER = kTinyE / kTinyThreads;
+    __shared__ uint32_t nid[kTinyN];
+    __shared__ uint32_t cnt[kTinyN];
+    __shared__ uint32_t key[kTinyE];
+    __shared__ uint16_t idx[kTinyE];
+    __shared__ unsigned long long val[kTinyE];
+    __shared__ uint32_t wsum[kTinyThreads / 64];
+    const uint32_t t = threadIdx.x, n = g.n;
+    // new ids: one community per thread
+    uint32_t nn;
+    {
+        const uint32_t f = (t < n && g.comm_sz[t] != 0u) ? 1u : 0u;
+        const uint32_t e = wg_excl_scan<kTinyThreads>(f, nn, wsum);
+        if (t < n) nid[t] = e;
+        cnt[t] = 0u;
+    }
+    __syncthreads();
+    // (row community, column community) of every entry: one vertex per thread
+    const uint32_t ne = g.row[n];
+    if (t < n) {
+        const uint32_t a = nid[g.comm[t]] * nn;
+        for (uint32_t j = g.row[t], je = g.row[t + 1]; j < je; ++j) {
+            key[j] = a + nid[g.comm[g.col[j]]];
+            val[j] = f2fix(g.w[j], g.scale);
+        }
+    }
+    uint32_t np2 = 1;
+    while (np2 < ne) np2 <<= 1;
+    for (uint32_t i = t; i < np2; i += kTinyThreads) {
+        idx[i] = (uint16_t)i;
+        if (i >= ne) key[i] = kEmpty;  // above every key (a key is below kTinyN^2)
+    }
+    __syncthreads();
+    // bitonic sort of the keys (with their entry index) in LDS
+    for (uint32_t k = 2; k <= np2; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < np2; i += kTinyThreads) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t ki = key[i], kl = key[l];
+                    if ((ki > kl) == ((i & k) == 0u)) {
+                        key[i] = kl;
+                        key[l] = ki;
+                        const uint16_t x = idx[i];
+                        idx[i] = idx[l];
+                        idx[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // one output entry per run of equal keys, in key order: rows by new id,
+    // columns ascending within a row
+    uint32_t h[PER], s = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t i = t * PER + q;
+        h[q] = (i < ne && (i == 0u || key[i] != key[i - 1u])) ? 1u : 0u;
+        s += h[q];
+    }
+    uint32_t ne2;
+    uint32_t pos = wg_excl_scan<kTinyThreads>(s, ne2, wsum);
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        if (!h[q]) continue;
+        const uint32_t i = t * PER + q, kk = key[i];
+        unsigned long long sum = 0;
+        for (uint32_t r = i; r < ne && key[r] == kk; ++r) sum += val[idx[r]];
+        const uint32_t a = kk / nn;
+        ocol[pos] = kk - a * nn;
+        ow[pos] = fix2f(sum, g.S);
+        atomicAdd(&cnt[a], 1u);
+        ++pos;
+    }
+    __syncthreads();
+    // row pointers; communities whose members have no entries get empty rows
+    {
+        uint32_t tot;
+        const uint32_t rp = wg_excl_scan<kTinyThreads>(t < nn ? cnt[t] : 0u, tot, wsum);
+        if (t < nn) orow[t] = rp;
+        if (t == 0) {
+            orow[nn] = tot;
+            sizes[0] = nn;
+            sizes[1] = tot;
+        }
+    }
+    // init_comm holds vertices of this pass: their community's new id
+    for (uint32_t i = t; i < n0; i += kTinyThreads) init_comm[i] = nid[g.comm[init_comm[i]]];
+}
+void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *init_comm,
+                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes) {
+    hipLaunchKernelGGL(k_agg_tiny, dim3(1), dim3(kTinyThreads), 0, s, g, n0, init_comm, orow,
+                       ocol, ow, sizes);
+}
+
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
 // sorted distinct list into the new CSR with renumbered ids and FP32 weights
 __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const uint32_t *comms,
