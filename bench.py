@@ -256,8 +256,8 @@ def bench_plain(lv, W, cfg, args):
     if args.warmup:
         wst = g.profile_get()
         fam = max(wst, key=lambda k: wst[k]["ms"])
-    # timed runs carry no events: an event record inside a captured sweep costs
-    # a few microseconds, a fifth of a run on the co-purchase graph
+    # timed runs carry no events: an event record costs a few microseconds,
+    # a fifth of a run on the co-purchase graph
     g.profile(False)
     g.profile_reset()
     times, edges, res = [], [], None

@@ -84,9 +84,10 @@ struct DevView {
     // that the algorithm makes (tie-breaks, isLowerIdx, renumbering) uses
     // lid, so results do not depend on the order.
     const uint32_t *lid;
-    // Single graph, sweeps captured several to a launch: non-null once the
+    // Sweeps queued ahead of the host's read of the loop state (several per
+    // wait, or the next group behind the one waited for): non-null once the
     // pass loop may have stopped on the device; when *halt is set the
-    // kernels of the remaining captured sweeps return at entry.
+    // kernels of the remaining queued sweeps return at entry.
     const uint32_t *halt;
     // Two copies of the community state (small single graphs, where a
     // separate apply launch per step costs more than its work; comm2 == null:
@@ -131,14 +132,14 @@ struct GraphState {
     uint32_t passes, status, pad0, pad1;
     unsigned long long ne, sum_edges, sum_visits, pad2;
 };
-// What the host reads after a batch's captured sweeps.  The first three
+// What the host reads after a batch's group of sweeps.  The first three
 // fields are recounted after every sweep (zeroed before it); halt and err
 // stay set for the rest of the batch pass.
 struct BatchSummary {
     uint32_t active;          // graphs still sweeping
     uint32_t waiting;         // graphs that improved and wait for the contraction
     unsigned long long live;  // vertices of the active graphs
-    uint32_t halt;            // no graph is active: later captured sweeps return at entry
+    uint32_t halt;            // no graph is active: later queued sweeps return at entry
     uint32_t err;             // a capacity overflow was flagged
 };
 constexpr int kBatchSummaryCounted = 4;  // 32-bit words recounted per sweep
@@ -208,7 +209,7 @@ void launch_batch_decide(hipStream_t s, uint32_t G, unsigned long long *gcnt,
                          const uint32_t *err_flag, int S, float min_gain, uint32_t max_iters,
                          const uint32_t *voff, uint32_t *gstate, GraphState *gst,
                          BatchSummary *sum);
-// After launch_batch_decide: halt the remaining captured sweeps when no graph
+// After launch_batch_decide: halt the remaining queued sweeps when no graph
 // is active; the summary to the host's mapped copy `host`, its recounted
 // fields cleared for the next sweep.
 void launch_batch_halt(hipStream_t s, BatchSummary *sum, BatchSummary *host);

@@ -206,7 +206,7 @@ __device__ __forceinline__ uint32_t decide_lastedge(const DevView &g, float m, u
 }
 
 // the pass loop has stopped on the device: kernels of the remaining
-// captured sweeps of a launch return at entry (uniform over the grid)
+// queued sweeps return at entry (uniform over the grid)
 __device__ __forceinline__ bool halted(const DevView &g) { return g.halt && *g.halt; }
 
 // ---- single-graph / batch mode --------------------------------------------

@@ -94,11 +94,11 @@ struct lv_graph {
     // non-null: device buffers come from a fixed region, no allocation
     Carver *carve = nullptr;
     // second stream of the handle: bucket step 7 runs the heavy rows' kernel
-    // chain beside the block classes (fork and join by events; captured into
-    // the sweep graph as two branches)
+    // chain beside the block classes (fork and join by events; two branches
+    // of the sweep graph when it is captured)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // ends of the two captured sweep launches in flight (pipelined pass loop)
+    // ends of the two groups of sweeps in flight (pipelined pass loop)
     hipEvent_t ev_launch[2] = {nullptr, nullptr};
     uint32_t n = 0, n0 = 0;
     uint64_t ne = 0, ne0 = 0;
@@ -203,7 +203,7 @@ struct lv_graph {
     std::vector<uint32_t> voff, voff0;  // host: current / original vertex offsets
     std::vector<uint32_t> gstate_h;     // host: initial graph states of a run
     // per graph on the device: pass loop variables and run totals; the
-    // summary the host reads after each launch of captured sweeps
+    // summary the host reads after each group of sweeps
     lv::GraphState *gst_d = nullptr;
     lv::BatchSummary *bsum_d = nullptr, *bsum_h = nullptr;
     lv::BatchSummary *bsum_hd = nullptr;  // device address of the mapped host copy
@@ -989,15 +989,14 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
     return true;
 }
 
-// Sweeps per launch of a pass loop decided on the device (the tuning's
-// sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).  With
-// more than one, the kernels of the sweeps captured after
-// the stop must return at entry, which costs each kernel of every sweep a
-// dependent load of the halt flag; measured on MI355X that load costs about
-// what the host round trip it saves does (a launch of a captured sweep and
-// its completion wait are about 20 us), so one sweep per launch is the
-// default.  A profiled run always takes one, so every timed launch did its
-// work.
+// Sweeps per wait of the host in a pass loop decided on the device (the
+// tuning's sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).
+// With more than one, the kernels of the sweeps queued after the stop must
+// return at entry, which costs each kernel of every sweep a dependent load
+// of the halt flag; measured on MI355X (with captured sweeps) that load
+// cost about what the host round trip it saves does, so one sweep per wait
+// is the default and the autotuner picks more per graph.  A profiled run
+// always takes one, so every timed sweep did its work.
 // Whether the host keeps the next captured launch queued behind the one it
 // waits for.  The device then never idles through the host's wake-up, read
 // of the loop state and next submission (about 25-30 us per launch); the
@@ -1476,8 +1475,8 @@ uint32_t compact_min() {
 // One pass for every active graph; returns in *more whether any graph
 // improved (and was contracted).  After every sweep a device kernel applies
 // each active graph's stop rule (modularity.cu:503-508) and counts the
-// graphs still active; the host reads that summary once per launch of
-// captured sweeps.
+// graphs still active; the host reads that summary once per group of
+// sweeps.
 int batch_pass(lv_graph *g, float min_gain, bool *more) {
     *more = false;
     const uint32_t G = g->G;

@@ -3078,8 +3078,8 @@ __global__ __launch_bounds__(256) void k_batch_decide(uint32_t G, unsigned long 
     }
     if (gi == 0 && *err_flag) sum->err = 1u;
 }
-// No graph active: the remaining captured sweeps halt.  The summary goes to
-// the host's mapped copy, which the host reads after the launch, and its
+// No graph active: the remaining queued sweeps halt.  The summary goes to
+// the host's mapped copy, which the host reads after the group, and its
 // recounted fields are cleared for the next sweep's decide.
 __global__ void k_batch_halt(BatchSummary *sum, BatchSummary *host) {
     if (sum->active == 0u) sum->halt = 1u;

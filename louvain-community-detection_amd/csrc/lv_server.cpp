@@ -213,7 +213,8 @@ extern "C" {
 
 uint64_t lv_graph_footprint(uint32_t n, uint64_t ne) {
     // every device buffer of a handle, the arenas that grow on demand at their
-    // worst case, plus allocation granularity and the captured sweep graphs
+    // worst case, plus allocation granularity (and a captured sweep graph
+    // with LV_GRAPH=1)
     return lv_host::footprint(n, ne) + (4ull << 20);
 }
 
