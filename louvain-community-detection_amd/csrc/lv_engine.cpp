@@ -861,8 +861,9 @@ int step_apply(lv_graph *g, int b) {
 // reads the copy its predecessor wrote; the copy the last step wrote holds
 // the sweep's result, and the last step's moves still have to be applied to
 // the copy it read.  Returns the view of the sweep's result with that
-// catch-up attached, for the masked sums' launch (modularity_launch): after it both
-// copies hold the sweep's result, as they held its start.  One copy: view(g).
+// catch-up attached, for the masked sums' launch (modularity_launch): after
+// it both copies hold the sweep's result, as they held its start.  One copy:
+// view(g).
 lv::DevView sweep_steps(lv_graph *g) {
     if (!fused_pass(g)) {
         for (int b = 1; b <= 7; ++b) step_apply(g, b);
@@ -885,7 +886,7 @@ lv::DevView sweep_steps(lv_graph *g) {
 }
 
 // the device work of one sweep, ending with the counters' D2H; no host sync
-// (this sequence is what a pass captures into a HIP graph)
+// (the host-decided loop: LV_HOST_LOOP=1, carved handles, lv_sweep)
 void sweep_enqueue(lv_graph *g) {
     zero_counters(g);
     const lv::DevView v = sweep_steps(g);
@@ -989,6 +990,22 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
     return true;
 }
 
+// Whether the host keeps the next group of sweeps queued behind the one it
+// waits for.  The device then never idles through the host's wake-up, read
+// of the loop state and next launches (about 25-30 us per group); the group
+// queued after the stop returns at entry in every kernel (halt flag).  That
+// flag costs every kernel a dependent load, so only pass graphs of at most
+// 2^24 entries, whose sweeps take well under a millisecond, are pipelined
+// (FEM band -2 %, co-purchase -4 %; R-MAT-24 ran 0.4 % slower).
+// LV_PIPELINE=0/1 forces one way.  A profiled run waits for each group: the
+// event pairs of a captured sweep are re-recorded by every replay.
+bool pipelined(const lv_graph *g) {
+    if (g->prof_on) return false;
+    static const char *env = getenv("LV_PIPELINE");
+    if (env && *env) return atoi(env) != 0;
+    return g->ne <= (1ull << 24);
+}
+
 // Sweeps per wait of the host in a pass loop decided on the device (the
 // tuning's sweeps_per_launch, default 1; LV_SWEEPS_PER_LAUNCH overrides it).
 // With more than one, the kernels of the sweeps queued after the stop must
@@ -997,22 +1014,6 @@ bool capture(lv_graph *g, SweepGraph &sg, F enqueue) {
 // cost about what the host round trip it saves does, so one sweep per wait
 // is the default and the autotuner picks more per graph.  A profiled run
 // always takes one, so every timed sweep did its work.
-// Whether the host keeps the next captured launch queued behind the one it
-// waits for.  The device then never idles through the host's wake-up, read
-// of the loop state and next submission (about 25-30 us per launch); the
-// queued launch after the stop returns at entry in every kernel (halt
-// flag).  That flag costs every kernel a dependent load, so only pass graphs
-// of at most 2^24 entries, whose sweeps take well under a millisecond, are
-// pipelined (FEM band -2 %, co-purchase -4 %; R-MAT-24 ran 0.4 % slower).
-// LV_PIPELINE=0/1 forces one way.  A profiled run waits for each launch: its
-// event pairs are re-recorded by every replay.
-bool pipelined(const lv_graph *g) {
-    if (g->prof_on) return false;
-    static const char *env = getenv("LV_PIPELINE");
-    if (env && *env) return atoi(env) != 0;
-    return g->ne <= (1ull << 24);
-}
-
 uint32_t sweeps_per_launch(const lv_graph *g) {
     if (g->prof_on) return 1;
     static const char *env = getenv("LV_SWEEPS_PER_LAUNCH");
