@@ -60,6 +60,7 @@ constexpr uint32_t kAggSmallMax = 128;
 constexpr uint32_t kAggMidMax = 2048;
 
 // View of one graph's device state, passed by value to kernels.
+struct SweepTail;
 struct DevView {
     const uint32_t *row;    // n+1
     const uint32_t *col;    // ne
@@ -109,6 +110,12 @@ struct DevView {
     // class order of a graph in its locality order), so kernels compute
     // vertex ids instead of loading them; -1: load them
     int32_t vbase;
+    // Non-null (a sweep's first move kernel, the pass loop decided on the
+    // device): workgroup 0 applies the previous sweep's stop rule first
+    // (k_collapse_decide's work, one launch fewer per sweep); the kernel's
+    // other workgroups run the step ahead of that decision, and the host
+    // drops their moves when it stops the pass.
+    const SweepTail *tail;
 };
 constexpr uint32_t kGraphActive = 1;
 constexpr uint32_t kGraphWait = 2;  // improved in this pass, waits for the contraction

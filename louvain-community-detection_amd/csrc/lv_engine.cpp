@@ -60,6 +60,7 @@ struct Counters {
     ull acc[lv::kSpread];       // k/s fixed total
     ull limbs[4][lv::kSpread];  // sum a^2 limbs
     ull moved[lv::kSpread];
+    lv::SweepTail tail;  // host staging of the device copy a folded pass loop reads
 };
 constexpr size_t kCountersHead = offsetof(Counters, acc);
 static_assert(offsetof(Counters, limbs) == offsetof(Counters, acc) + lv::kSpread * 8, "layout");
@@ -100,6 +101,8 @@ struct lv_graph {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // ends of the two groups of sweeps in flight (pipelined pass loop)
     hipEvent_t ev_launch[2] = {nullptr, nullptr};
+    // the pass loop's stop-rule arguments on the device (folded decide)
+    lv::SweepTail *tail_d = nullptr;
     uint32_t n = 0, n0 = 0;
     uint64_t ne = 0, ne0 = 0;
     uint32_t n_cap = 0;  // buffer capacities (lv_graph_create: the input's size)
@@ -301,6 +304,7 @@ lv::DevView view(lv_graph *g) {
     v.cu_count = v.cu_nblk = 0;
     v.cu_moved = nullptr;
     v.vbase = -1;
+    v.tail = nullptr;
     return v;
 }
 
@@ -763,7 +767,10 @@ lv::DevView step_view(lv_graph *g, uint32_t j) {
 // That copy still lacks the previous step's moves (vertices [pa, pe) of
 // `order`, none for the sweep's first step): workgroups in front of the
 // step's first class kernel apply them.
-int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
+// front: the previous sweep's stop rule on the device, carried by the
+// step's first kernel (null: none).
+int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe,
+               const lv::SweepTail *front = nullptr) {
     const lv::DevView v = step_view(g, j);
     const int c0 = b, c1 = (b == 7) ? lv::kClassHeavy : b;
     lv::DevView cu = v;  // carries the catch-up to the first class kernel
@@ -772,7 +779,12 @@ int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
     bool caught = pe == pa;
     bool block_cls = false;
     for (int c = c0; c < lv::kClassHeavy && c <= c1; ++c) block_cls |= g->cls_count[c] != 0;
-    // only the heavy class has vertices: its chain catches up on its own
+    // only the heavy class has vertices: its chain catches up on its own,
+    // and the stop rule gets a launch of its own
+    if (!block_cls && front) {
+        lv::launch_collapse_decide(g->stream, g->hc->tail, g->S);
+        front = nullptr;
+    }
     if (!caught && !block_cls) {
         ProfScope ps(g, LV_K_APPLY, 16.0 * (pe - pa));
         lv::launch_catchup(g->stream, cu);
@@ -800,6 +812,8 @@ int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
         else {
             lv::DevView vc = first ? cu : v;
             vc.vbase = g->order_ident ? (int32_t)g->cls_off[c] : -1;
+            vc.tail = front;
+            front = nullptr;
             lv::launch_move_class(st, vc, c, g->order + g->cls_off[c], g->cls_count[c],
                                   lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
                                                 g->tune.class8_slots, narrow_min()});
@@ -815,7 +829,7 @@ int step_fused(lv_graph *g, int b, uint32_t j, uint32_t pa, uint32_t pe) {
 
 // One bucket step on one copy of the state: the step's kernels write their
 // decisions to new_comm, then k_apply applies them in place.
-int step_apply(lv_graph *g, int b) {
+int step_apply(lv_graph *g, int b, const lv::SweepTail *front = nullptr) {
     const lv::DevView v = view(g);
     const int c0 = b, c1 = (b == 7) ? lv::kClassHeavy : b;
     // the heavy rows' chain runs on the second stream beside the block
@@ -829,6 +843,14 @@ int step_apply(lv_graph *g, int b) {
         (void)hipEventRecord(g->ev_fork, g->stream);
         (void)hipStreamWaitEvent(g->stream2, g->ev_fork, 0);
     }
+    if (front) {  // only the heavy class has vertices: the stop rule on its own
+        bool block_cls = false;
+        for (int c = c0; c < lv::kClassHeavy && c <= c1; ++c) block_cls |= g->cls_count[c] != 0;
+        if (!block_cls) {
+            lv::launch_collapse_decide(g->stream, g->hc->tail, g->S);
+            front = nullptr;
+        }
+    }
     for (int c = c0; c <= c1; ++c) {
         if (!g->cls_count[c]) continue;
         hipStream_t st = (fork && c == lv::kClassHeavy) ? g->stream2 : g->stream;
@@ -838,6 +860,8 @@ int step_apply(lv_graph *g, int b) {
         } else {
             lv::DevView vc = v;
             vc.vbase = g->order_ident ? (int32_t)g->cls_off[c] : -1;
+            vc.tail = front;
+            front = nullptr;
             lv::launch_move_class(st, vc, c, g->order + g->cls_off[c], g->cls_count[c],
                                   lv::MoveShape{g->tune.wave_vertices, g->tune.block_threads,
                                                 g->tune.class8_slots, narrow_min()});
@@ -864,9 +888,16 @@ int step_apply(lv_graph *g, int b) {
 // catch-up attached, for the masked sums' launch (modularity_launch): after
 // it both copies hold the sweep's result, as they held its start.  One copy:
 // view(g).
-lv::DevView sweep_steps(lv_graph *g) {
+// front: the previous sweep's stop rule, carried by the first step with
+// vertices (null: none).
+lv::DevView sweep_steps(lv_graph *g, const lv::SweepTail *front = nullptr) {
     if (!fused_pass(g)) {
-        for (int b = 1; b <= 7; ++b) step_apply(g, b);
+        for (int b = 1; b <= 7; ++b) {
+            uint32_t a, e;
+            step_range(g, b, &a, &e);
+            step_apply(g, b, e > a ? front : nullptr);
+            if (e > a) front = nullptr;
+        }
         return view(g);
     }
     uint32_t j = 0, pa = 0, pe = 0;
@@ -874,7 +905,7 @@ lv::DevView sweep_steps(lv_graph *g) {
         uint32_t a, e;
         step_range(g, b, &a, &e);
         if (e == a) continue;
-        step_fused(g, b, j, pa, pe);
+        step_fused(g, b, j, pa, pe, j == 0 ? front : nullptr);
         pa = a;
         pe = e;
         ++j;
@@ -1021,12 +1052,18 @@ uint32_t sweeps_per_launch(const lv_graph *g) {
     return g->tune.sweeps_per_launch;
 }
 
-// one sweep whose end applies the stop rule on the device and writes the
-// loop state to the host's mapped copy; the counters must be zero at its
-// start (the collapse leaves them so for the next sweep)
-void sweep_enqueue_decide(lv_graph *g, float min_gain) {
-    const lv::DevView v = sweep_steps(g);
-    modularity_launch(g, false, &v);
+// The first bucket step of a sweep with vertices (0: none).
+int lead_bucket(const lv_graph *g) {
+    for (int b = 1; b <= 7; ++b) {
+        uint32_t a, e;
+        step_range(g, b, &a, &e);
+        if (e > a) return b;
+    }
+    return 0;
+}
+
+// the stop rule's arguments for the pass
+lv::SweepTail sweep_tail(lv_graph *g, float min_gain) {
     lv::SweepTail t;
     t.spread = g->dc->acc;
     t.tot = g->dc->tot;
@@ -1036,7 +1073,19 @@ void sweep_enqueue_decide(lv_graph *g, float min_gain) {
     t.max_iters = kMaxIters;
     t.ps = g->pst;
     t.ps_host = g->pst_hd;
-    lv::launch_collapse_decide(g->stream, t, g->S);
+    return t;
+}
+
+// One sweep whose end applies the stop rule on the device and writes the
+// loop state to the host's mapped copy; the counters must be zero at its
+// start (the collapse leaves them so for the next sweep).  Folded: the stop
+// rule of this sweep runs in workgroup 0 of the next sweep's first kernel
+// (g->tail_d, staged in g->hc->tail), and this sweep's first kernel carries
+// the previous sweep's unless it is the pass's first.
+void sweep_enqueue_decide(lv_graph *g, float min_gain, bool fold = false, bool first = true) {
+    const lv::DevView v = sweep_steps(g, fold && !first ? g->tail_d : nullptr);
+    modularity_launch(g, false, &v);
+    if (!fold) lv::launch_collapse_decide(g->stream, sweep_tail(g, min_gain), g->S);
 }
 
 int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q_print,
@@ -1056,13 +1105,23 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
         const bool pipe = pipelined(g);
         // kernels test the halt flag only when sweeps can follow the stop
         const bool halt = K > 1 || pipe;
+        // pipelined direct launches fold each sweep's stop rule into the next
+        // sweep's first kernel: the kernels already test the halt flag, and
+        // the next sweep is queued anyway
+        const bool fold = pipe && !graphs_enabled(g) && lead_bucket(g) != 0;
         *g->pst_h = lv::PassState{act, 0.0f, 0.0f, 0u, 1u, 1u, 0u, 0u};
         LV_HIP(hipMemcpyAsync(g->pst, g->pst_h, sizeof(lv::PassState), hipMemcpyHostToDevice,
                               g->stream));
+        if (fold) {
+            g->hc->tail = sweep_tail(g, min_gain);
+            LV_HIP(hipMemcpyAsync(g->tail_d, &g->hc->tail, sizeof(lv::SweepTail),
+                                  hipMemcpyHostToDevice, g->stream));
+        }
         zero_counters(g);
+        uint32_t swept = 0;
         auto enqueue = [&] {
             g->halt_on = halt;
-            for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain);
+            for (uint32_t k = 0; k < K; ++k) sweep_enqueue_decide(g, min_gain, fold, swept++ == 0);
             g->halt_on = false;
         };
         const bool use_graph = graphs_enabled(g) && capture(g, sg, enqueue);
@@ -1099,6 +1158,18 @@ int modularity_optimization(lv_graph *g, float min_gain, int *improved, float *q
             ret = p.ret != 0u;
             if (!ret && q_print) *q_print = p.q_print;
             break;
+        }
+        // folded, two copies: the first step of the sweep after the stop moved
+        // its vertices in the second copy; put them back as the first copy
+        // has them, so both copies hold the pass result again (one copy: that
+        // step's apply returned at entry)
+        if (fold && fused_pass(g)) {
+            uint32_t a, e;
+            step_range(g, lead_bucket(g), &a, &e);
+            lv::DevView cv = step_view(g, 0);
+            cv.cu_verts = g->order + a;
+            cv.cu_count = e - a;
+            lv::launch_catchup(g->stream, cv);
         }
     } else {  // the stop rule on the host after each sweep
         for (;;) {
@@ -1735,6 +1806,7 @@ void free_all(lv_graph *g) {
     dfree(g->scan_tmp64);
     dfree(g->dc);
     dfree(g->pst);
+    dfree(g->tail_d);
     if (g->hc) (void)hipHostFree(g->hc);
     g->hc = nullptr;
     if (g->pst_h) (void)hipHostFree(g->pst_h);
@@ -1911,6 +1983,7 @@ int graph_buffers(lv_graph *g, uint32_t n_cap, uint64_t e_cap) {
     A(&g->scan_tmp64, lv::scan_tmp_elems(n_cap + 1) + 8);
     A(&g->dc, 1);
     A(&g->pst, 1);
+    A(&g->tail_d, 1);
     if (e || !g->carve) return e;
     const size_t hb = E / 16 + 1, hc = E / 256 + 1, slots = 2 * E + 1;
     A(&g->hp_bcnt, hb), A(&g->hp_bpos, hb + 1), A(&g->hp_bcur, hb);

@@ -759,14 +759,38 @@ void launch_r1_groups(hipStream_t s, const DevView &g, const R1Groups &L, int ma
 //   * move iff G - comm_e_w / m > 0 (FP32), else stay; no candidate: stay.
 // ---------------------------------------------------------------------------
 
+// The previous sweep's stop rule in workgroup 0 of a sweep's first move
+// kernel (any workgroup size up to 1024); defined with k_collapse_decide.
+__device__ void decide_front(const SweepTail &t, int S);
+
+// Workgroups in front of a move kernel's own: the previous sweep's stop rule
+// (g.tail), then the previous step's catch-up (cu_nblk).  True for those;
+// otherwise blk is the kernel's own workgroup index.
+__device__ __forceinline__ bool front_work(const DevView &g, uint32_t &blk) {
+    blk = blockIdx.x;
+    if (g.tail) {
+        if (blk == 0) {
+            decide_front(*g.tail, g.S);
+            return true;
+        }
+        --blk;
+    }
+    if (blk < g.cu_nblk) {
+        catch_up(g, blk);
+        return true;
+    }
+    blk -= g.cu_nblk;
+    return false;
+}
+
 // buckets 1-5: T lanes per vertex (T = reference lanes), ROUNDS rounds,
 // per-group LDS open-addressing table of CAP slots with exact u64 sums.
 template <int T, int ROUNDS, int CAP>
 __global__ __launch_bounds__(256) void k_move_small(DevView g, const uint32_t *verts,
                                                     uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
-    const uint32_t bid = blockIdx.x - g.cu_nblk;
+    uint32_t bid;
+    if (front_work(g, bid)) return;
     constexpr int GPB = 256 / T;
     constexpr int LC = Log2<CAP>::value;
     __shared__ uint32_t skey[GPB * CAP];
@@ -880,8 +904,8 @@ template <int T, int Q, int ROUNDS, int CAP>
 __global__ __launch_bounds__(256) void k_move_narrow(DevView g, const uint32_t *verts,
                                                      uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
-    const uint32_t bid = blockIdx.x - g.cu_nblk;
+    uint32_t bid;
+    if (front_work(g, bid)) return;
     static_assert(T % Q == 0 && 64 % Q == 0 && CAP > T * ROUNDS / 2, "lane shape");
     constexpr int V = T / Q, GPB = 256 / Q, LC = Log2<CAP>::value;
     __shared__ uint32_t skey[GPB * CAP];
@@ -1015,8 +1039,8 @@ template <int kWaveWpb>
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
                                                              uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
-    const uint32_t bid = blockIdx.x - g.cu_nblk;
+    uint32_t bid;
+    if (front_work(g, bid)) return;
     __shared__ uint32_t skey[kWaveWpb * kWaveCap];
     __shared__ unsigned long long sval[kWaveWpb * kWaveCap];
     __shared__ uint16_t sdist[kWaveWpb][320];
@@ -1186,8 +1210,8 @@ template <int P, int MAXDEG, int CAP, int MINB = 1>
 __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_t *verts,
                                                         uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
-    const uint32_t bid = blockIdx.x - g.cu_nblk;
+    uint32_t bid;
+    if (front_work(g, bid)) return;
     static_assert(CAP > MAXDEG && CAP <= 65536 && 3 * P * 4 <= CAP * 8, "table shape");
     constexpr int MAXPER = (MAXDEG + P - 1) / P;
     __shared__ unsigned long long val[CAP];
@@ -1318,7 +1342,8 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
     auto go = [&](auto kern, uint32_t grid, uint32_t threads) {
         DevView gv = g;
         gv.cu_nblk = g.cu_count ? catchup_blocks(g.cu_count, threads) : 0u;
-        hipLaunchKernelGGL(kern, dim3(grid + gv.cu_nblk), dim3(threads), 0, s, gv, verts, count);
+        hipLaunchKernelGGL(kern, dim3(grid + gv.cu_nblk + (g.tail ? 1u : 0u)), dim3(threads), 0,
+                           s, gv, verts, count);
     };
     switch (cls) {
         // large classes: T virtual lanes (the reference's) on Q lanes per
@@ -2985,13 +3010,33 @@ __device__ __forceinline__ void sweep_decide(const unsigned long long *tot,
 // The spread slots are left zero for the next sweep, and the loop state is
 // also written to the host's mapped copy, which the host reads after the
 // launch (no copy launch).
-__global__ __launch_bounds__(256) void k_collapse_decide(SweepTail t, int S) {
-    collapse_body(t.spread, 6, t.tot, true);
+__device__ void decide_front(const SweepTail &t, int S) {
+    __shared__ unsigned long long part[16];
+    const uint32_t nw = (blockDim.x + 63u) >> 6;
+    for (int gi = 0; gi < 6; ++gi) {
+        unsigned long long x = 0;
+        for (uint32_t i = threadIdx.x; i < kSpread; i += blockDim.x) {
+            x += t.spread[gi * kSpread + i];
+            t.spread[gi * kSpread + i] = 0ull;
+        }
+        x = wave_sum_u64(x);
+        if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long s = 0;
+            for (uint32_t w = 0; w < nw; ++w) s += part[w];
+            t.tot[gi] = s;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         sweep_decide(t.tot, t.err, S, t.m, t.min_gain, t.max_iters, t.ps);
         *t.ps_host = *t.ps;
         __threadfence_system();
     }
+}
+__global__ __launch_bounds__(256) void k_collapse_decide(SweepTail t, int S) {
+    decide_front(t, S);
 }
 void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S) {
     hipLaunchKernelGGL(k_collapse_decide, dim3(1), dim3(256), 0, s, t, S);
