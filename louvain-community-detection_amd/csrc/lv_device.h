@@ -234,6 +234,16 @@ void launch_zero3_u32(hipStream_t s, uint32_t *p0, size_t n0, uint32_t *p1, size
 constexpr uint32_t kTinyN = 1024, kTinyE = 4096, kTinyThreads = 1024;
 void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *init_comm,
                      uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes);
+// contraction of a pass graph of at most kFewN vertices into at most kFewNN
+// communities (single graph, input numbering) through a dense matrix of
+// exact sums (mat: kFewNN^2 sums and a kFewNN^2-bit presence map behind them,
+// new_id: n + 1); sizes[0] <- communities, and
+// when they are at most kFewNN, sizes[1] <- entries and the contracted CSR
+// and init_comm written; otherwise nothing else is written
+constexpr uint32_t kFewN = 65536, kFewNN = 64, kFewRows = 32;
+void launch_agg_few(hipStream_t s, const DevView &g, uint32_t *new_id, unsigned long long *mat,
+                    uint32_t n0, uint32_t *init_comm, uint32_t *orow, uint32_t *ocol, float *ow,
+                    uint32_t *sizes);
 void launch_iota_u32(hipStream_t s, uint32_t *p, size_t n);
 void launch_collapse(hipStream_t s, const unsigned long long *spread, int groups,
                      unsigned long long *out);

@@ -1369,6 +1369,31 @@ int aggregation(lv_graph *g) {
         g->prelabeled = false;
         return LV_OK;
     }
+    // a single graph that the pass left with few communities: a dense matrix
+    // of exact sums in three launches (the matrix in the sums scratch);
+    // more communities than the matrix holds fall through to the general path
+    if (!g->G && !g->relabeled && !want_locality(g, n) && n <= lv::kFewN &&
+        g->e_cap >= (uint64_t)lv::kFewNN * lv::kFewNN * 65 / 64) {
+        lv::launch_agg_few(s, v, g->new_id, g->tval, g->n0, g->init_comm, g->row2, g->col2,
+                           g->w2, g->dc->acursor);
+        int rc = sync_head(g);
+        if (rc) return rc;
+        const uint32_t nn = g->hc->acursor[0];
+        if (nn <= lv::kFewNN) {
+            const uint32_t ne2 = g->hc->acursor[1];
+            if (nn > n || ne2 > g->ne) return LV_ERR_OVERFLOW;
+            std::swap(g->row, g->row2);
+            std::swap(g->col, g->col2);
+            std::swap(g->w, g->w2);
+            g->agg_nn = nn;
+            g->n = nn;
+            g->ne = ne2;
+            g->pass_ready = false;
+            g->relabeled = false;
+            g->prelabeled = false;
+            return LV_OK;
+        }
+    }
     if (g->G) {  // a batch also clears its per-graph counters
         zero_counters(g);
         lv::launch_fill_u32(s, g->comm_deg, 0u, n);

@@ -2885,6 +2885,123 @@ void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *ini
                        ocol, ow, sizes);
 }
 
+// ---------------------------------------------------------------------------
+// Contraction into few communities (at most kFewNN non-empty ones, a pass
+// graph of at most kFewN vertices, single graph, input numbering): the
+// contracted graph is a dense kFewNN x kFewNN matrix of exact sums.  Three
+// launches instead of about thirty: new ids (one workgroup), the entries
+// added into per-workgroup LDS matrices and then into one global matrix,
+// and the CSR written from the matrix in ascending columns (one workgroup).
+// A pair of communities joined only by entries of weight zero still gets its
+// entry (as in the general path), so a presence bitmap follows the matrix.
+// More communities than kFewNN: every kernel returns at entry and the host
+// takes the general path.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_few_ids(DevView g, uint32_t *new_id,
+                                                  unsigned long long *mat, uint32_t *sizes) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t n = g.n;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 1024) {
+        const uint32_t c = c0 + threadIdx.x;
+        const uint32_t f = (c < n && g.comm_sz[c] != 0u) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan<1024>(f, tot, wsum);
+        if (c < n) new_id[c] = carry + e;
+        carry += tot;
+    }
+    // the matrix and, behind it, its presence bitmap
+    for (uint32_t i = threadIdx.x; i < kFewNN * kFewNN + kFewNN * kFewNN / 64; i += 1024)
+        mat[i] = 0ull;
+    if (threadIdx.x == 0) sizes[0] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_few_sums(DevView g, const uint32_t *new_id,
+                                                  unsigned long long *mat,
+                                                  const uint32_t *sizes) {
+    __shared__ unsigned long long lm[kFewNN * kFewNN];
+    __shared__ uint32_t lp[kFewNN * kFewNN / 32];
+    __shared__ uint32_t lrow[kFewRows + 1];
+    const uint32_t nn = sizes[0];
+    if (nn > kFewNN) return;  // uniform: the general path takes over
+    uint32_t *pres = reinterpret_cast<uint32_t *>(mat + kFewNN * kFewNN);
+    // the workgroup's rows [r0, r1): their offsets in LDS, their entries
+    // [lrow[0], lrow[r1 - r0]) shared out over the threads
+    const uint32_t r0 = blockIdx.x * kFewRows, r1 = min(g.n, r0 + kFewRows), nr = r1 - r0;
+    for (uint32_t i = threadIdx.x; i <= nr; i += 256) lrow[i] = g.row[r0 + i];
+    for (uint32_t i = threadIdx.x; i < nn * nn; i += 256) lm[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < kFewNN * kFewNN / 32; i += 256) lp[i] = 0u;
+    __syncthreads();
+    for (uint32_t j = lrow[0] + threadIdx.x; j < lrow[nr]; j += 256) {
+        uint32_t lo = 0, hi = nr;  // lrow[lo] <= j < lrow[hi]: the last such lo owns j
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lrow[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t a = new_id[g.comm[r0 + lo]], b = new_id[g.comm[g.col[j]]];
+        const uint32_t ab = a * nn + b;
+        atomicAdd(&lm[ab], f2fix(g.w[j], g.scale));
+        atomicOr(&lp[ab >> 5], 1u << (ab & 31u));
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nn * nn; i += 256)
+        if (lm[i]) atomicAdd(&mat[i], lm[i]);
+    for (uint32_t i = threadIdx.x; i < kFewNN * kFewNN / 32; i += 256)
+        if (lp[i]) atomicOr(&pres[i], lp[i]);
+}
+
+__global__ __launch_bounds__(1024) void k_few_emit(DevView g, const uint32_t *new_id,
+                                                   const unsigned long long *mat, uint32_t n0,
+                                                   uint32_t *init_comm, uint32_t *orow,
+                                                   uint32_t *ocol, float *ow, uint32_t *sizes) {
+    __shared__ uint32_t cnt[kFewNN];
+    __shared__ uint32_t start[kFewNN + 1];
+    const uint32_t nn = sizes[0];
+    if (nn > kFewNN) return;
+    const uint32_t *pres = reinterpret_cast<const uint32_t *>(mat + kFewNN * kFewNN);
+    auto present = [&](uint32_t ab) { return (pres[ab >> 5] >> (ab & 31u)) & 1u; };
+    const uint32_t t = threadIdx.x;
+    if (t < nn) {  // one row per thread: its columns with entries
+        uint32_t c = 0;
+        for (uint32_t b = 0; b < nn; ++b) c += present(t * nn + b);
+        cnt[t] = c;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t r = 0;
+        for (uint32_t a = 0; a < nn; ++a) {
+            start[a] = r;
+            r += cnt[a];
+        }
+        start[nn] = r;
+        sizes[1] = r;
+    }
+    __syncthreads();
+    if (t <= nn) orow[t] = start[t];
+    if (t < nn) {
+        uint32_t o = start[t];
+        for (uint32_t b = 0; b < nn; ++b) {
+            if (present(t * nn + b)) {
+                ocol[o] = b;
+                ow[o] = fix2f(mat[t * nn + b], g.S);
+                ++o;
+            }
+        }
+    }
+    for (uint32_t i = t; i < n0; i += 1024) init_comm[i] = new_id[g.comm[init_comm[i]]];
+}
+
+void launch_agg_few(hipStream_t s, const DevView &g, uint32_t *new_id, unsigned long long *mat,
+                    uint32_t n0, uint32_t *init_comm, uint32_t *orow, uint32_t *ocol, float *ow,
+                    uint32_t *sizes) {
+    hipLaunchKernelGGL(k_few_ids, dim3(1), dim3(1024), 0, s, g, new_id, mat, sizes);
+    hipLaunchKernelGGL(k_few_sums, dim3((g.n + kFewRows - 1) / kFewRows), dim3(256), 0, s, g,
+                       new_id, mat, sizes);
+    hipLaunchKernelGGL(k_few_emit, dim3(1), dim3(1024), 0, s, g, new_id, mat, n0, init_comm,
+                       orow, ocol, ow, sizes);
+}
+
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
 // sorted distinct list into the new CSR with renumbered ids and FP32 weights
 __global__ __launch_bounds__(256) void k_agg_emit(DevView g, AggView a, const uint32_t *comms,
