@@ -1039,8 +1039,8 @@ template <int kWaveWpb>
 __global__ __launch_bounds__(64 * kWaveWpb) void k_move_wave(DevView g, const uint32_t *verts,
                                                              uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    uint32_t bid;
-    if (front_work(g, bid)) return;
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     __shared__ uint32_t skey[kWaveWpb * kWaveCap];
     __shared__ unsigned long long sval[kWaveWpb * kWaveCap];
     __shared__ uint16_t sdist[kWaveWpb][320];
@@ -1210,8 +1210,8 @@ template <int P, int MAXDEG, int CAP, int MINB = 1>
 __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_t *verts,
                                                         uint32_t count) {
     if (halted(g)) return;  // the pass loop stopped earlier in this launch
-    uint32_t bid;
-    if (front_work(g, bid)) return;
+    if (blockIdx.x < g.cu_nblk) return catch_up(g, blockIdx.x);
+    const uint32_t bid = blockIdx.x - g.cu_nblk;
     static_assert(CAP > MAXDEG && CAP <= 65536 && 3 * P * 4 <= CAP * 8, "table shape");
     constexpr int MAXPER = (MAXDEG + P - 1) / P;
     __shared__ unsigned long long val[CAP];
@@ -1335,14 +1335,21 @@ __global__ __launch_bounds__(P, MINB) void k_move_block(DevView g, const uint32_
 #define LV_SMALLCAP5 256
 #endif
 uint32_t catchup_blocks(uint32_t count, uint32_t threads);
+__global__ __launch_bounds__(256) void k_decide_from(const SweepTail *t, int S);
 void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t *verts,
                        uint32_t count, const MoveShape &shape) {
     if (!count) return;
-    // catch-up workgroups in front of the class's own
-    auto go = [&](auto kern, uint32_t grid, uint32_t threads) {
+    // catch-up workgroups in front of the class's own, and the previous
+    // sweep's stop rule in workgroup 0 of the kernels of buckets 1-5 (a
+    // launch of its own before the others: their entry stays as it was)
+    auto go = [&](auto kern, uint32_t grid, uint32_t threads, bool front = false) {
         DevView gv = g;
+        if (g.tail && !front) {
+            hipLaunchKernelGGL(k_decide_from, dim3(1), dim3(256), 0, s, g.tail, g.S);
+            gv.tail = nullptr;
+        }
         gv.cu_nblk = g.cu_count ? catchup_blocks(g.cu_count, threads) : 0u;
-        hipLaunchKernelGGL(kern, dim3(grid + gv.cu_nblk + (g.tail ? 1u : 0u)), dim3(threads), 0,
+        hipLaunchKernelGGL(kern, dim3(grid + gv.cu_nblk + (gv.tail ? 1u : 0u)), dim3(threads), 0,
                            s, gv, verts, count);
     };
     switch (cls) {
@@ -1351,33 +1358,33 @@ void launch_move_class(hipStream_t s, const DevView &g, int cls, const uint32_t 
         // workgroup; small classes: T lanes per vertex
         case 1:
             if (count >= shape.narrow_min)
-                go(k_move_narrow<4, 1, 1, 8>, (count + 255) / 256, 256);
+                go(k_move_narrow<4, 1, 1, 8>, (count + 255) / 256, 256, true);
             else
-                go(k_move_small<4, 1, 8>, (count + 63) / 64, 256);
+                go(k_move_small<4, 1, 8>, (count + 63) / 64, 256, true);
             break;
         case 2:
             if (count >= shape.narrow_min)
-                go(k_move_narrow<8, 2, 1, 16>, (count + 127) / 128, 256);
+                go(k_move_narrow<8, 2, 1, 16>, (count + 127) / 128, 256, true);
             else
-                go(k_move_small<8, 1, 16>, (count + 31) / 32, 256);
+                go(k_move_small<8, 1, 16>, (count + 31) / 32, 256, true);
             break;
         case 3:
             if (count >= shape.narrow_min)
-                go(k_move_narrow<16, 4, 1, 32>, (count + 63) / 64, 256);
+                go(k_move_narrow<16, 4, 1, 32>, (count + 63) / 64, 256, true);
             else
-                go(k_move_small<16, 1, 32>, (count + 15) / 16, 256);
+                go(k_move_small<16, 1, 32>, (count + 15) / 16, 256, true);
             break;
         case 4:
             if (count >= shape.narrow_min)
-                go(k_move_narrow<32, 8, 1, 64>, (count + 31) / 32, 256);
+                go(k_move_narrow<32, 8, 1, 64>, (count + 31) / 32, 256, true);
             else
-                go(k_move_small<32, 1, 64>, (count + 7) / 8, 256);
+                go(k_move_small<32, 1, 64>, (count + 7) / 8, 256, true);
             break;
         case 5:
             if (count >= shape.narrow_min)
-                go(k_move_narrow<32, 16, 3, 128>, (count + 15) / 16, 256);
+                go(k_move_narrow<32, 16, 3, 128>, (count + 15) / 16, 256, true);
             else
-                go(k_move_small<32, 3, LV_SMALLCAP5>, (count + 7) / 8, 256);
+                go(k_move_small<32, 3, LV_SMALLCAP5>, (count + 7) / 8, 256, true);
             break;
         case 6:
             if (shape.wave_vertices == 1)
@@ -3154,6 +3161,9 @@ __device__ void decide_front(const SweepTail &t, int S) {
 }
 __global__ __launch_bounds__(256) void k_collapse_decide(SweepTail t, int S) {
     decide_front(t, S);
+}
+__global__ __launch_bounds__(256) void k_decide_from(const SweepTail *t, int S) {
+    decide_front(*t, S);
 }
 void launch_collapse_decide(hipStream_t s, const SweepTail &t, int S) {
     hipLaunchKernelGGL(k_collapse_decide, dim3(1), dim3(256), 0, s, t, S);
