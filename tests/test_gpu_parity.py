@@ -83,6 +83,61 @@ def test_stepwise_first_pass(lv, orc, graphs, name, loc):
             assert q.view(np.uint32) == oq.view(np.uint32), (it, q, oq)
 
 
+def _with_zero_links(row, col, w, pairs):
+    """The CSR with undirected links of weight zero added (rows kept sorted)."""
+    row = np.asarray(row).astype(np.int64)
+    n = row.size - 1
+    rows = [list(zip(col[row[v]:row[v + 1]].tolist(), w[row[v]:row[v + 1]].tolist()))
+            for v in range(n)]
+    for a, b in pairs:
+        rows[a].append((b, 0.0))
+        rows[b].append((a, 0.0))
+    for v in {x for p in pairs for x in p}:
+        rows[v].sort(key=lambda t: t[0])
+    nrow = np.concatenate([[0], np.cumsum([len(x) for x in rows])]).astype(np.uint32)
+    ncol = np.array([t[0] for x in rows for t in x], np.uint32)
+    nw = np.array([t[1] for x in rows for t in x], np.float32)
+    return nrow, ncol, nw
+
+
+def test_small_contraction_paths(lv, orc, graphs):
+    """Contractions of small pass graphs take one of three paths: one
+    workgroup for a tiny graph (at most 1024 vertices and 4096 entries), a
+    dense matrix of exact sums when at most 64 communities remain of at most
+    65,536 vertices, the general path otherwise.  Every contraction gives the
+    oracle's CSR, weights bit for bit, and assignment.  The FEM graph gets
+    three long links of weight zero: its first contraction (23 communities,
+    the matrix path) keeps them as entries of weight zero, as the general
+    path does; bcsstk16 carries 1,492 entries of weight zero through the
+    general and the tiny path."""
+    fem = _with_zero_links(*graphs["fem"], [(0, 4883), (100, 4000), (2000, 4500)])
+    cases = {"fem_zero_links": fem, "bcsstk16": graphs["bcsstk16"], "sbm_5k": graphs["sbm_5k"]}
+    gain = np.float32(1e-6)
+    seen, zero_kept = set(), None
+    for name, (row, col, w) in cases.items():
+        print(f"[case] small contractions {name}", flush=True)
+        o = orc.Oracle(np.asarray(row).astype(np.uint64), col, w)
+        with lv.Graph(_u32(row), col, w) as g:
+            g.set_locality(0)
+            for p in range(20):
+                imp = g.modularity_optimization(gain)[0]
+                assert imp == o.modularity_optimization(gain)[0], f"{name} pass {p}"
+                if not imp:
+                    break
+                n, ne, _ = g.size()
+                g.aggregation()
+                o.aggregation()
+                nn = g.size()[0]
+                seen.add("tiny" if n <= 1024 and ne <= 4096 else
+                         "few" if n <= 65536 and nn <= 64 else "general")
+                _cmp_csr(g, o, f"{name} after aggregation {p}")
+                assert np.array_equal(g.assignment(), o.init_comm()), f"{name} pass {p}"
+                if name == "fem_zero_links" and p == 0:
+                    zero_kept = int((g.csr()[2] == 0).sum())
+    assert seen == {"tiny", "few", "general"}, seen
+    assert zero_kept == 6, zero_kept
+
+
 # locality 0: input numbering on the device; 1: every pass graph in locality
 # order (ids compared through lid) -- results must be identical either way
 @pytest.mark.parametrize("loc", [0, 1])
