@@ -230,10 +230,11 @@ void launch_zero3_u32(hipStream_t s, uint32_t *p0, size_t n0, uint32_t *p1, size
                       uint32_t *p2, size_t n2);
 // whole contraction of a pass graph of at most kTinyN vertices and kTinyE
 // entries (single graph, input numbering) in one workgroup; sizes[0..1] <-
-// new vertex and entry counts
+// new vertex and entry counts, new_id <- the new ids (n entries)
 constexpr uint32_t kTinyN = 1024, kTinyE = 4096, kTinyThreads = 1024;
 void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *init_comm,
-                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes);
+                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes,
+                     uint32_t *new_id);
 // contraction of a pass graph of at most kFewN vertices into at most kFewNN
 // communities (single graph, input numbering) through a dense matrix of
 // exact sums (mat: kFewNN^2 sums and a kFewNN^2-bit presence map behind them,

@@ -1353,7 +1353,8 @@ int aggregation(lv_graph *g) {
     // a tiny single graph in its input numbering: one workgroup does it all
     if (!g->G && !g->relabeled && !want_locality(g, n) && n <= lv::kTinyN &&
         g->ne <= lv::kTinyE) {
-        lv::launch_agg_tiny(s, v, g->n0, g->init_comm, g->row2, g->col2, g->w2, g->dc->acursor);
+        lv::launch_agg_tiny(s, v, g->n0, g->init_comm, g->row2, g->col2, g->w2, g->dc->acursor,
+                            g->new_id);
         int rc = sync_head(g);
         if (rc) return rc;
         const uint32_t nn = g->hc->acursor[0], ne2 = g->hc->acursor[1];

@@ -2796,7 +2796,7 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t x, uint32_t &total, ui
 __global__ __launch_bounds__(kTinyThreads) void k_agg_tiny(DevView g, uint32_t n0,
                                                            uint32_t *init_comm, uint32_t *orow,
                                                            uint32_t *ocol, float *ow,
-                                                           uint32_t *sizes) {
+                                                           uint32_t *sizes, uint32_t *new_id) {
     constexpr uint32_t PER = kTinyE / kTinyThreads;
     __shared__ uint32_t nid[kTinyN];
     __shared__ uint32_t cnt[kTinyN];
@@ -2810,7 +2810,7 @@ __global__ __launch_bounds__(kTinyThreads) void k_agg_tiny(DevView g, uint32_t n
     {
         const uint32_t f = (t < n && g.comm_sz[t] != 0u) ? 1u : 0u;
         const uint32_t e = wg_excl_scan<kTinyThreads>(f, nn, wsum);
-        if (t < n) nid[t] = e;
+        if (t < n) nid[t] = new_id[t] = e;
         cnt[t] = 0u;
     }
     __syncthreads();
@@ -2883,13 +2883,24 @@ __global__ __launch_bounds__(kTinyThreads) void k_agg_tiny(DevView g, uint32_t n
             sizes[1] = tot;
         }
     }
-    // init_comm holds vertices of this pass: their community's new id
+    // init_comm holds vertices of this pass: their community's new id (n0
+    // is 0 when the input graph is large: a launch of its own composes it)
     for (uint32_t i = t; i < n0; i += kTinyThreads) init_comm[i] = nid[g.comm[init_comm[i]]];
 }
+// an input graph larger than this composes init_comm in a launch of its own
+// (one workgroup would walk it alone)
+constexpr uint32_t kComposeInKernel = 8192;
+__global__ void k_agg_compose(const uint32_t *comm, const uint32_t *new_id, const uint32_t *map,
+                              uint32_t *init_comm, uint32_t n0);
 void launch_agg_tiny(hipStream_t s, const DevView &g, uint32_t n0, uint32_t *init_comm,
-                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes) {
-    hipLaunchKernelGGL(k_agg_tiny, dim3(1), dim3(kTinyThreads), 0, s, g, n0, init_comm, orow,
-                       ocol, ow, sizes);
+                     uint32_t *orow, uint32_t *ocol, float *ow, uint32_t *sizes,
+                     uint32_t *new_id) {
+    const bool in_kernel = n0 <= kComposeInKernel;
+    hipLaunchKernelGGL(k_agg_tiny, dim3(1), dim3(kTinyThreads), 0, s, g, in_kernel ? n0 : 0u,
+                       init_comm, orow, ocol, ow, sizes, new_id);
+    if (!in_kernel)
+        hipLaunchKernelGGL(k_agg_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, g.comm, new_id,
+                           nullptr, init_comm, n0);
 }
 
 // ---------------------------------------------------------------------------
@@ -2999,14 +3010,25 @@ __global__ __launch_bounds__(1024) void k_few_emit(DevView g, const uint32_t *ne
     for (uint32_t i = t; i < n0; i += 1024) init_comm[i] = new_id[g.comm[init_comm[i]]];
 }
 
+// init_comm of a large input graph, when the pass left at most kFewNN
+// communities
+__global__ void k_few_compose(const uint32_t *comm, const uint32_t *new_id, uint32_t *init_comm,
+                              uint32_t n0, const uint32_t *sizes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sizes[0] <= kFewNN && i < n0) init_comm[i] = new_id[comm[init_comm[i]]];
+}
 void launch_agg_few(hipStream_t s, const DevView &g, uint32_t *new_id, unsigned long long *mat,
                     uint32_t n0, uint32_t *init_comm, uint32_t *orow, uint32_t *ocol, float *ow,
                     uint32_t *sizes) {
+    const bool in_kernel = n0 <= kComposeInKernel;
     hipLaunchKernelGGL(k_few_ids, dim3(1), dim3(1024), 0, s, g, new_id, mat, sizes);
     hipLaunchKernelGGL(k_few_sums, dim3((g.n + kFewRows - 1) / kFewRows), dim3(256), 0, s, g,
                        new_id, mat, sizes);
-    hipLaunchKernelGGL(k_few_emit, dim3(1), dim3(1024), 0, s, g, new_id, mat, n0, init_comm,
-                       orow, ocol, ow, sizes);
+    hipLaunchKernelGGL(k_few_emit, dim3(1), dim3(1024), 0, s, g, new_id, mat,
+                       in_kernel ? n0 : 0u, init_comm, orow, ocol, ow, sizes);
+    if (!in_kernel)
+        hipLaunchKernelGGL(k_few_compose, dim3((n0 + 255) / 256), dim3(256), 0, s, g.comm, new_id,
+                           init_comm, n0, sizes);
 }
 
 // compressEdges (aggregation.cu:359-378): one wave per community copies its
